@@ -1,0 +1,129 @@
+/*
+ * oracle.h -- CPU restatement of PAWSIM's rigid-lid timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the parity oracle for the HIP
+ * implementation under pawsim_amd/ and the timed CPU baseline in bench.py.
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may
+ * build, load or run it; the product never links it.
+ *
+ * It restates the serial (PAWSIM_USE_MPI unset) algorithm of the reference:
+ * the same stage order, the same ghosted layouts (data[i*sy+j], ghost width
+ * 3), the same halo-fill order and the same FP64 expression association, so
+ * that compiled with -ffp-contract=off it reproduces the reference outputs
+ * bit for bit.  Each function cites the reference file:line it follows.
+ *
+ * Pinning: every function here is checked against golden vectors written by
+ * the reference serial executable (tests/golden/), and the multigrid
+ * transfer operators additionally against closed forms (tests/).
+ */
+#ifndef PAWSIM_ORACLE_H
+#define PAWSIM_ORACLE_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdio.h>
+
+typedef double real;
+typedef unsigned int uint;
+
+/* Scheme identifiers (defs.h:14-47). */
+enum { OR_AB1 = 3, OR_AB2 = 4, OR_AB3 = 5 };
+enum { OR_MOM_AL81 = 0, OR_MOM_HK83 = 1, OR_MOM_S75E = 2, OR_MOM_TW81 = 3 };
+enum { OR_THK_AL81 = 0 };
+enum { OR_SOLVER_SOR = 0, OR_SOLVER_MG = 1 };
+
+/* Parsed parameter file (config.c:42-122 defaults, 172-282 keys). */
+typedef struct or_config {
+  uint Nlay, Nx, Ny, Nt, tsScheme, momScheme, thkScheme, maxiters, solver, tauNrecs;
+  real Lx, Ly, dt, tmin, tmax, savefrequency, h0, hsml, hbbl, A2, A4, A2smag, A4smag;
+  real linDrag, quadDrag, linDragSurf, quadDragSurf, pi_tol, SOR_rp, omega_MG;
+  bool wallEW, wallNS, useRL, useFullMG, pressureTiming, writeInit, useWind;
+  bool unsupported;           /* some active feature outside the restated path */
+  char unsupported_what[128];
+  char inputDir[512];
+  char hInit[256], uInit[256], vInit[256], hbFile[256], hsFile[256];
+  char omegaFile[256], gFile[256], tauxFile[256], tauyFile[256];
+} or_config;
+
+/* Ghosted field, column-major in x: element (i,j) at d[i*sy+j] (field.c:37-71). */
+typedef struct or_field {
+  uint nx, ny, g, sx, sy;
+  real *d;
+} or_field;
+
+#define OF(f, i, j) ((f)->d[(size_t)(i) * (f)->sy + (size_t)(j)])
+
+enum { OR_HALO_SCALAR, OR_HALO_Q, OR_HALO_U, OR_HALO_V };
+
+/* Model context (context.h:16-83 restricted to the restated path). */
+typedef struct or_model {
+  or_config cfg;
+  uint L, g;
+  bool per_x, per_y;
+  real dx, dy, t;
+  real *gg, *geff;
+  /* state and AB history */
+  or_field *u, *v, *h;              /* state */
+  or_field *tu, *tv, *th;           /* tendency, tendency_1, tendency_2 as [3][L] */
+  or_field hhb, hhs, omz, taux, tauy;
+  /* work arrays (work.h:93-193) */
+  or_field *uw, *vw, *hw, *dtu, *dtv, *dth, *hwest, *hsouth;
+  or_field *hFsurf_w, *hFsurf_s, *hFbot_w, *hFbot_s;
+  or_field *hhq, *zeta, *qq, *al, *be, *ga, *de, *ep, *ph, *la, *mu;
+  or_field *pp, *KE, *MM, *eta, *huu, *hvv;
+  or_field hhs_w, hhb_w, hhs_west, hhs_south, hhb_west, hhb_south, Hc, Hw, Hs;
+  or_field taux_w, tauy_w, usq_bot, vsq_bot, uabs_bot;
+  or_field DD_T, DD_S, DD4_T, DD4_S, A2_h, A2_q, A4s_h, A4s_q, UU4, VV4;
+  or_field pi, rhs;
+  /* pressure diagnostics (context.h:52-61) */
+  real res_max, res_rms, div_max, div_rms;
+  uint solves, iters_total, n_saves;
+  double solve_seconds;             /* wall time inside the solver only */
+  double correct_seconds;           /* wall time of whole corrections */
+  int last_iters;
+} or_model;
+
+/* ---- oracle_io.c ---- */
+void or_config_defaults(or_config *c);
+bool or_config_read(const char *fname, or_config *c, FILE *err);
+bool or_config_validate(const or_config *c, FILE *err);
+bool or_read_matrix(const char *path, real *a, uint m, uint n, uint layers, FILE *err);
+bool or_write_field(const char *path, const or_field *f);
+
+/* ---- pawsim_oracle.c ---- */
+bool or_field_alloc(or_field *f, uint nx, uint ny, uint g);
+void or_field_free(or_field *f);
+void or_field_zero(or_field *f);
+void or_fill(or_field *f, const or_model *m, int kind);
+void or_fill_internal(or_field *f, const or_model *m);
+
+bool or_model_init(or_model *m, const char *infile, FILE *err);
+void or_model_free(or_model *m);
+void or_prepare_work(or_model *m);
+void or_forcing_update(or_model *m);
+void or_tendency_core(or_model *m);
+void or_tendency_eval(or_model *m, int slot);
+bool or_step(or_model *m, uint n);
+uint or_pressure_correct(or_model *m);
+bool or_write_state(or_model *m, const char *outdir, uint n);
+bool or_write_init_diagnostics(or_model *m, const char *outdir);
+
+/* ---- oracle_mg.c: multigrid.c restatement ---- */
+bool or_mg_init(uint Nx, uint Ny, real Lx, real Ly, const real *Hc, bool wallEW, bool wallNS,
+                bool fullMG, real tol, uint maxiters, real omega);
+uint or_mg_solve(real *pi, real *rhs);
+void or_mg_free(void);
+
+/* Per-function entry points used by the unit tests (flat no-ghost arrays,
+ * element (i,j) at a[i*Ny+j], the multigrid.c layout). */
+int  or_mg_nlevels(void);
+void or_mg_level_size(int m, uint *Nx, uint *Ny);
+void or_mg_relax(int m, real *pi, const real *rhs);
+void or_mg_residual(int m, const real *pi, const real *rhs, real *res);
+void or_mg_restrict(int m_fine, const real *fine, real *coarse);
+void or_mg_interpolate(int m_fine, const real *coarse, real *fine);
+real or_mg_max_residual(int m, const real *pi, const real *rhs);
+void or_mg_vcycle(real *pi, real *rhs);
+void or_mg_operators(int m, real *Ow, real *Os, real *Osum);
+
+#endif

@@ -1,0 +1,194 @@
+/*
+ * oracle_io.c -- parameter file and raw-field I/O for the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Follows config.c (defaults 42-122, key parsing 129-314, validation
+ * 320-459) and the raw float64 layouts of defs.c (printMatrix 21-37,
+ * readMatrix 191-211, readMatrix3 291-309).
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+/* Defaults of the keys the restated path reads (config.c:42-122). */
+void or_config_defaults(or_config *c)
+{
+  memset(c, 0, sizeof(*c));
+  c->tsScheme = OR_AB3;
+  c->momScheme = OR_MOM_TW81;
+  c->thkScheme = OR_THK_AL81;
+  c->tmin = -1;
+  c->pi_tol = 1e-10;
+  c->SOR_rp = 1.0;
+  c->omega_MG = 2.0 / 3.0;
+  c->tauNrecs = 1;
+  c->maxiters = 10000;
+  c->useRL = true;
+  strcpy(c->inputDir, ".");
+}
+
+static void flag_unsupported(or_config *c, const char *what)
+{
+  if (!c->unsupported) {
+    c->unsupported = true;
+    snprintf(c->unsupported_what, sizeof(c->unsupported_what), "%s", what);
+  }
+}
+
+static void copy_name(char *dst, const char *src)
+{
+  size_t n = strlen(src);
+  if (n > 255) n = 255;
+  memcpy(dst, src, n);
+  dst[n] = '\0';
+}
+
+/* Whitespace key/value reader; unknown keys are tolerated (config.c:172-282).
+ * Keys that switch on machinery outside the restated timestep set
+ * c->unsupported so that callers can refuse the input explicitly. */
+bool or_config_read(const char *fname, or_config *c, FILE *err)
+{
+  char key[256], val[256];
+  bool solverGiven = false, useMG = false;
+  const char *slash;
+  FILE *f = fopen(fname, "r");
+  if (!f) {
+    if (err) fprintf(err, "ERROR: Could not open input parameter file %s\n", fname);
+    return false;
+  }
+  slash = strrchr(fname, '/');
+  if (slash) {
+    size_t n = (size_t)(slash - fname);
+    if (n >= sizeof(c->inputDir)) n = sizeof(c->inputDir) - 1;
+    memcpy(c->inputDir, fname, n);
+    c->inputDir[n] = '\0';
+  } else {
+    strcpy(c->inputDir, ".");
+  }
+  while (fscanf(f, "%255s", key) != EOF) {
+    if (fscanf(f, "%255s", val) == EOF) {
+      if (err) fprintf(err, "ERROR: Missing value for parameter %s\n", key);
+      fclose(f);
+      return false;
+    }
+#define U(k, dst) else if (!strcmp(key, k)) dst = (uint)strtoul(val, NULL, 10)
+#define R(k, dst) else if (!strcmp(key, k)) dst = strtod(val, NULL)
+#define B(k, dst) else if (!strcmp(key, k)) dst = (atoi(val) != 0)
+#define S(k, dst) else if (!strcmp(key, k)) copy_name(dst, val)
+    if (0) {}
+    U("Nlay", c->Nlay); U("Nx", c->Nx); U("Ny", c->Ny); U("Nt", c->Nt);
+    U("timeSteppingScheme", c->tsScheme); U("momentumScheme", c->momScheme);
+    U("thicknessScheme", c->thkScheme); U("maxiters", c->maxiters); U("tauNrecs", c->tauNrecs);
+    R("Lx", c->Lx); R("Ly", c->Ly); R("dt", c->dt); R("tmin", c->tmin); R("tmax", c->tmax);
+    R("savefrequency", c->savefrequency); R("h0", c->h0); R("hsml", c->hsml); R("hbbl", c->hbbl);
+    R("A2", c->A2); R("A4", c->A4); R("A2smag", c->A2smag); R("A4smag", c->A4smag);
+    R("linDragCoeff", c->linDrag); R("quadDragCoeff", c->quadDrag);
+    R("linDragSurf", c->linDragSurf); R("quadDragSurf", c->quadDragSurf);
+    R("tol", c->pi_tol); R("SOR_rp", c->SOR_rp); R("SOR_rp_min", c->SOR_rp); R("omega_MG", c->omega_MG);
+    B("useWallEW", c->wallEW); B("useWallNS", c->wallNS); B("useRL", c->useRL);
+    B("use_fullMG", c->useFullMG); B("pressureTiming", c->pressureTiming);
+    B("writeInitDiagnostics", c->writeInit); B("use_MG", useMG);
+    S("hInitFile", c->hInit); S("uInitFile", c->uInit); S("vInitFile", c->vInit);
+    S("hbFile", c->hbFile); S("hsFile", c->hsFile); S("OmegazFile", c->omegaFile);
+    S("gFile", c->gFile); S("tauxFile", c->tauxFile); S("tauyFile", c->tauyFile);
+    else if (!strcmp(key, "pressureSolver")) { c->solver = (uint)strtoul(val, NULL, 10); solverGiven = true; }
+    else if (!strcmp(key, "useTracer") || !strcmp(key, "useBuoyancy") || !strcmp(key, "restart")
+             || !strcmp(key, "windFeedback") || !strcmp(key, "useRandomForcing") || !strcmp(key, "useTrad")) {
+      if (atoi(val) != 0) flag_unsupported(c, key);
+    }
+    else if (!strcmp(key, "FbaroXFile") || !strcmp(key, "FbaroYFile") || !strcmp(key, "wDiaFile")
+             || !strcmp(key, "uTimeFile") || !strcmp(key, "vTimeFile") || !strcmp(key, "hTimeFile")
+             || !strcmp(key, "eTimeFile") || !strcmp(key, "diaViscUFile") || !strcmp(key, "diaViscVFile")
+             || !strcmp(key, "bFluxFile") || !strcmp(key, "diaDiffFile")) {
+      flag_unsupported(c, key);
+    }
+    else if (!strcmp(key, "convDiff") || !strcmp(key, "savefreqAvg") || !strcmp(key, "savefreqEZ")
+             || !strcmp(key, "savefreqUMom") || !strcmp(key, "savefreqVMom") || !strcmp(key, "savefreqThic")
+             || !strcmp(key, "savefreqEnergy") || !strcmp(key, "savefreqTracer")) {
+      if (strtod(val, NULL) != 0) flag_unsupported(c, key);
+    }
+#undef U
+#undef R
+#undef B
+#undef S
+  }
+  fclose(f);
+  /* use_MG <-> pressureSolver mapping (config.c:289-297). */
+  if (!solverGiven) c->solver = useMG ? OR_SOLVER_MG : OR_SOLVER_SOR;
+  c->useWind = (c->tauxFile[0] != '\0') || (c->tauyFile[0] != '\0');
+  /* Features outside the restated path. */
+  if (c->thkScheme != OR_THK_AL81) flag_unsupported(c, "thicknessScheme");
+  if (c->hsml > 0) flag_unsupported(c, "hsml");
+  if (c->hbbl > 0) flag_unsupported(c, "hbbl");
+  if (c->A2smag > 0 || c->A4smag > 0) flag_unsupported(c, "Smagorinsky");
+  if (c->linDragSurf > 0 || c->quadDragSurf > 0) flag_unsupported(c, "surface drag");
+  if (c->tauNrecs != 1) flag_unsupported(c, "tauNrecs");
+  if (!c->useRL) flag_unsupported(c, "useRL 0");
+  if (c->solver > OR_SOLVER_MG) flag_unsupported(c, "pressureSolver");
+  if (c->tsScheme < OR_AB1 || c->tsScheme > OR_AB3) flag_unsupported(c, "timeSteppingScheme");
+  return true;
+}
+
+/* Validation subset of config.c:320-459. */
+bool or_config_validate(const or_config *c, FILE *err)
+{
+  if (c->Nlay == 0 || c->Nx == 0 || c->Ny == 0 || c->Ly <= 0) {
+    if (err) fprintf(err, "ERROR: Nlay, Nx, Ny and Ly must be positive\n");
+    return false;
+  }
+  if (c->momScheme > OR_MOM_TW81) {
+    if (err) fprintf(err, "ERROR: Unknown or unsupported numerical scheme identifier\n");
+    return false;
+  }
+  if (c->h0 < 0 || c->A2 < 0 || c->A4 < 0 || c->linDrag < 0 || c->quadDrag < 0 || c->tauNrecs == 0) {
+    if (err) fprintf(err, "ERROR: Invalid wind forcing parameters\n");
+    return false;
+  }
+  if (c->pi_tol <= 0 || c->SOR_rp <= 0 || c->SOR_rp >= 2 || c->omega_MG <= 0 || c->maxiters == 0) {
+    if (err) fprintf(err, "ERROR: Invalid pressure solver parameters\n");
+    return false;
+  }
+  if (c->unsupported) {
+    if (err) fprintf(err, "ERROR: oracle does not restate '%s'\n", c->unsupported_what);
+    return false;
+  }
+  return true;
+}
+
+/* Read m x n (x fastest) or layers x m x n (layer fastest) float64 values
+ * into a[(layer*m + i)*n + j] (defs.c:191-211, 291-309). */
+bool or_read_matrix(const char *path, real *a, uint m, uint n, uint layers, FILE *err)
+{
+  uint i, j, k;
+  FILE *f = fopen(path, "rb");
+  if (!f) {
+    if (err) fprintf(err, "ERROR: Could not open %s\n", path);
+    return false;
+  }
+  for (j = 0; j < n; j++)
+    for (i = 0; i < m; i++)
+      for (k = 0; k < layers; k++) {
+        if (fread(&a[((size_t)k * m + i) * n + j], sizeof(real), 1, f) != 1) {
+          if (err) fprintf(err, "ERROR: short read in %s\n", path);
+          fclose(f);
+          return false;
+        }
+      }
+  fclose(f);
+  return true;
+}
+
+/* Write the owned Nx x Ny cells with x fastest (io.c:17-129, defs.c:21-37). */
+bool or_write_field(const char *path, const or_field *fld)
+{
+  uint i, j;
+  FILE *f = fopen(path, "wb");
+  if (!f) return false;
+  for (j = 0; j < fld->ny; j++)
+    for (i = 0; i < fld->nx; i++)
+      fwrite(&OF(fld, i + fld->g, j + fld->g), sizeof(real), 1, f);
+  fclose(f);
+  return true;
+}

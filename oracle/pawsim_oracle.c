@@ -1,0 +1,1238 @@
+/*
+ * pawsim_oracle.c -- CPU restatement of the PAWSIM timestep (serial build).
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h): parity oracle and timed CPU
+ * baseline; never linked into the product.
+ *
+ * Stage order, index ranges, halo kinds and FP64 association follow the
+ * reference files cited at each function.  The work arrays are kept as
+ * separate ghosted fields, exactly as the reference keeps them, so that
+ * ghost-cell contents (including the order of the serial halo fills) are
+ * reproduced.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#define SQ(x) ((x) * (x))
+
+static double now_seconds(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ------------------------------------------------------------------ */
+/* fields and halo fills (field.c:37-71, 121-241, 416-449, 528-531)    */
+/* ------------------------------------------------------------------ */
+
+bool or_field_alloc(or_field *f, uint nx, uint ny, uint g)
+{
+  f->nx = nx;
+  f->ny = ny;
+  f->g = g;
+  f->sx = nx + 2 * g;
+  f->sy = ny + 2 * g;
+  f->d = calloc((size_t)f->sx * f->sy, sizeof(real));
+  return f->d != NULL;
+}
+
+void or_field_free(or_field *f)
+{
+  free(f->d);
+  f->d = NULL;
+}
+
+void or_field_zero(or_field *f)
+{
+  memset(f->d, 0, (size_t)f->sx * f->sy * sizeof(real));
+}
+
+static void copy_interior(or_field *dst, const or_field *src)
+{
+  uint i, j;
+  for (i = 0; i < dst->nx; i++)
+    for (j = 0; j < dst->ny; j++)
+      OF(dst, i + dst->g, j + dst->g) = OF(src, i + src->g, j + src->g);
+}
+
+/* Serial periodic copy in y over owned columns (field.c:121-134). */
+static void periodic_y(or_field *f)
+{
+  uint i, j, g = f->g;
+  for (i = g; i < g + f->nx; i++)
+    for (j = 0; j < g; j++) {
+      OF(f, i, j) = OF(f, i, f->ny + j);
+      OF(f, i, g + f->ny + j) = OF(f, i, g + j);
+    }
+}
+
+/* Serial periodic copy in x over full columns, corners included (field.c:137-150). */
+static void periodic_x(or_field *f)
+{
+  uint i, j, g = f->g;
+  for (j = 0; j < f->sy; j++)
+    for (i = 0; i < g; i++) {
+      OF(f, i, j) = OF(f, f->nx + i, j);
+      OF(f, g + f->nx + i, j) = OF(f, g + i, j);
+    }
+}
+
+/* South/north wall ghosts (field.c:158-198). */
+static void wall_y(or_field *f, int kind)
+{
+  uint i, j, g = f->g;
+  uint i0 = (kind == OR_HALO_Q) ? 0 : g;
+  uint i1 = (kind == OR_HALO_Q) ? f->sx : g + f->nx;
+  for (i = i0; i < i1; i++) {
+    if (kind == OR_HALO_V) {
+      OF(f, i, g) = 0;
+      OF(f, i, g + f->ny) = 0;
+    }
+    for (j = 0; j < g; j++) {
+      if (kind == OR_HALO_Q) {
+        OF(f, i, j) = OF(f, i, g);
+        OF(f, i, g + f->ny + j) = OF(f, i, g + f->ny);
+      } else if (kind == OR_HALO_V) {
+        OF(f, i, j) = -OF(f, i, 2 * g - j);
+        OF(f, i, g + f->ny + j) = -OF(f, i, g + f->ny - j);
+      } else {
+        OF(f, i, j) = OF(f, i, 2 * g - j - 1);
+        OF(f, i, g + f->ny + j) = OF(f, i, g + f->ny - j - 1);
+      }
+    }
+  }
+}
+
+/* West/east wall ghosts over all rows (field.c:204-241). */
+static void wall_x(or_field *f, int kind)
+{
+  uint i, j, g = f->g;
+  for (j = 0; j < f->sy; j++) {
+    if (kind == OR_HALO_U) {
+      OF(f, g, j) = 0;
+      OF(f, g + f->nx, j) = 0;
+    }
+    for (i = 0; i < g; i++) {
+      if (kind == OR_HALO_Q) {
+        OF(f, i, j) = OF(f, g, j);
+        OF(f, g + f->nx + i, j) = OF(f, g + f->nx, j);
+      } else if (kind == OR_HALO_U) {
+        OF(f, i, j) = -OF(f, 2 * g - i, j);
+        OF(f, g + f->nx + i, j) = -OF(f, g + f->nx - i, j);
+      } else {
+        OF(f, i, j) = OF(f, 2 * g - i - 1, j);
+        OF(f, g + f->nx + i, j) = OF(f, g + f->nx - i - 1, j);
+      }
+    }
+  }
+}
+
+/* Serial "internal" exchange: periodic copies only (field.c:416-426). */
+void or_fill_internal(or_field *f, const or_model *m)
+{
+  if (m->per_y) periodic_y(f);
+  if (m->per_x) periodic_x(f);
+}
+
+/* Serial exchange + physical boundary fill (field.c:441-449, 528-531). */
+void or_fill(or_field *f, const or_model *m, int kind)
+{
+  or_fill_internal(f, m);
+  if (!m->per_y) wall_y(f, kind);
+  if (!m->per_x) wall_x(f, kind);
+}
+
+static void fill_prognostic(or_model *m, or_field *u, or_field *v, or_field *h)
+{
+  uint k;
+  for (k = 0; k < m->L; k++) {
+    or_fill(&u[k], m, OR_HALO_U);
+    or_fill(&v[k], m, OR_HALO_V);
+    or_fill(&h[k], m, OR_HALO_SCALAR);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* model setup (pawsim.c:118-196, 355-470; state.c:957-1100)           */
+/* ------------------------------------------------------------------ */
+
+static or_field *alloc_layers(uint n, uint nx, uint ny, uint g)
+{
+  uint k;
+  or_field *f = calloc(n, sizeof(or_field));
+  for (k = 0; k < n; k++) or_field_alloc(&f[k], nx, ny, g);
+  return f;
+}
+
+static void free_layers(or_field *f, uint n)
+{
+  uint k;
+  if (!f) return;
+  for (k = 0; k < n; k++) or_field_free(&f[k]);
+  free(f);
+}
+
+static void resolve(const or_config *c, const char *name, char *out, size_t n)
+{
+  if (name[0] == '/') snprintf(out, n, "%s", name);
+  else snprintf(out, n, "%s/%s", c->inputDir, name);
+}
+
+/* Read a 2-D (layers==1) or layer-stacked input into owned cells. */
+static bool read_into(or_model *m, const char *name, or_field *f, uint layers, FILE *err)
+{
+  char path[1024];
+  uint k, i, j, nx = m->cfg.Nx, ny = m->cfg.Ny;
+  real *buf;
+  if (name[0] == '\0') return true;
+  resolve(&m->cfg, name, path, sizeof(path));
+  buf = malloc((size_t)layers * nx * ny * sizeof(real));
+  if (!or_read_matrix(path, buf, nx, ny, layers, err)) {
+    free(buf);
+    return false;
+  }
+  for (k = 0; k < layers; k++)
+    for (i = 0; i < nx; i++)
+      for (j = 0; j < ny; j++)
+        OF(&f[k], i + m->g, j + m->g) = buf[((size_t)k * nx + i) * ny + j];
+  free(buf);
+  return true;
+}
+
+/* Resolve "two of {Nt, tmax, dt}" (pawsim.c:139-196). */
+static bool set_time(or_model *m, FILE *err)
+{
+  or_config *c = &m->cfg;
+  uint cnt = (c->Nt > 0) + (c->tmax > 0) + (c->dt > 0);
+  if (cnt != 2) {
+    if (err) fprintf(err, "Must specify exactly two of {Nt, tmax, dt}\n");
+    return false;
+  }
+  if (c->tmin < 0) c->tmin = 0;
+  if (c->Nt == 0) {
+    if (c->tmax <= c->tmin) return false;
+    c->Nt = (uint)ceil((c->tmax - c->tmin) / c->dt);
+  }
+  if (c->dt == 0) {
+    if (c->tmax <= c->tmin) return false;
+    c->dt = (c->tmax - c->tmin) / c->Nt;
+  }
+  if (c->savefrequency == 0) c->savefrequency = c->dt;
+  if (c->tmax == 0) c->tmax = c->tmin + c->dt * c->Nt;
+  if (c->tmin >= c->tmax) return false;
+  m->t = c->tmin;
+  return true;
+}
+
+bool or_model_init(or_model *m, const char *infile, FILE *err)
+{
+  uint k, i, j, nx, ny, L, g = 3;
+  or_config *c = &m->cfg;
+  memset(m, 0, sizeof(*m));
+  or_config_defaults(c);
+  if (!or_config_read(infile, c, err) || !or_config_validate(c, err)) return false;
+  if (c->Lx <= 0) c->Lx = c->Ly * c->Nx / c->Ny;
+  m->dx = c->Lx / c->Nx;
+  m->dy = c->Ly / c->Ny;
+  if (!set_time(m, err)) return false;
+  nx = c->Nx;
+  ny = c->Ny;
+  L = c->Nlay;
+  m->L = L;
+  m->g = g;
+  m->per_x = !c->wallEW;
+  m->per_y = !c->wallNS;
+
+  m->u = alloc_layers(L, nx, ny, g);
+  m->v = alloc_layers(L, nx, ny, g);
+  m->h = alloc_layers(L, nx, ny, g);
+  m->tu = alloc_layers(3 * L, nx, ny, g);
+  m->tv = alloc_layers(3 * L, nx, ny, g);
+  m->th = alloc_layers(3 * L, nx, ny, g);
+  or_field_alloc(&m->hhb, nx, ny, g);
+  or_field_alloc(&m->hhs, nx, ny, g);
+  or_field_alloc(&m->omz, nx, ny, g);
+  or_field_alloc(&m->taux, nx, ny, g);
+  or_field_alloc(&m->tauy, nx, ny, g);
+#define WL(f) m->f = alloc_layers(L, nx, ny, g)
+  WL(uw); WL(vw); WL(hw); WL(dtu); WL(dtv); WL(dth); WL(hwest); WL(hsouth);
+  WL(hFsurf_w); WL(hFsurf_s); WL(hFbot_w); WL(hFbot_s);
+  WL(hhq); WL(zeta); WL(qq); WL(al); WL(be); WL(ga); WL(de); WL(ep); WL(ph); WL(la); WL(mu);
+  WL(pp); WL(KE); WL(MM); WL(huu); WL(hvv);
+#undef WL
+  m->eta = alloc_layers(L + 1, nx, ny, g);
+#define W2(f) or_field_alloc(&m->f, nx, ny, g)
+  W2(hhs_w); W2(hhb_w); W2(hhs_west); W2(hhs_south); W2(hhb_west); W2(hhb_south);
+  W2(Hc); W2(Hw); W2(Hs); W2(taux_w); W2(tauy_w); W2(usq_bot); W2(vsq_bot); W2(uabs_bot);
+  W2(DD_T); W2(DD_S); W2(DD4_T); W2(DD4_S); W2(A2_h); W2(A2_q); W2(A4s_h); W2(A4s_q);
+  W2(UU4); W2(VV4); W2(pi); W2(rhs);
+#undef W2
+  m->gg = calloc(L, sizeof(real));
+  m->geff = calloc(L, sizeof(real));
+  for (k = 0; k < L; k++) m->gg[k] = 9.81;
+
+  /* Inputs (state.c:1012-1047). */
+  if (!read_into(m, c->hsFile, &m->hhs, 1, err) || !read_into(m, c->hbFile, &m->hhb, 1, err))
+    return false;
+  if (c->omegaFile[0]) {
+    /* (Nx+1) x (Ny+1) q grid; the extra east/north line goes to the first
+     * ghost column/row on wall domains (state.c:207-238, 440-477). */
+    char path[1024];
+    uint qx = c->wallEW ? 1 : 0, qy = c->wallNS ? 1 : 0;
+    real *buf = malloc((size_t)(nx + 1) * (ny + 1) * sizeof(real));
+    resolve(c, c->omegaFile, path, sizeof(path));
+    if (!or_read_matrix(path, buf, nx + 1, ny + 1, 1, err)) {
+      free(buf);
+      return false;
+    }
+    for (i = 0; i < nx + qx; i++)
+      for (j = 0; j < ny + qy; j++)
+        OF(&m->omz, i + g, j + g) = buf[(size_t)i * (ny + 1) + j];
+    free(buf);
+  }
+  if (c->gFile[0]) {
+    char path[1024];
+    FILE *f;
+    resolve(c, c->gFile, path, sizeof(path));
+    f = fopen(path, "rb");
+    if (!f || fread(m->gg, sizeof(real), L, f) != L) {
+      if (err) fprintf(err, "ERROR: Could not read %s\n", path);
+      if (f) fclose(f);
+      return false;
+    }
+    fclose(f);
+  }
+  if (!read_into(m, c->uInit, m->u, L, err) || !read_into(m, c->vInit, m->v, L, err)
+      || !read_into(m, c->hInit, m->h, L, err) || !read_into(m, c->tauxFile, &m->taux, 1, err)
+      || !read_into(m, c->tauyFile, &m->tauy, 1, err))
+    return false;
+
+  fill_prognostic(m, m->u, m->v, m->h);
+  or_fill(&m->hhb, m, OR_HALO_SCALAR);
+  or_fill(&m->hhs, m, OR_HALO_SCALAR);
+  or_fill(&m->omz, m, OR_HALO_Q);
+  or_fill(&m->taux, m, OR_HALO_SCALAR);
+  or_fill(&m->tauy, m, OR_HALO_SCALAR);
+  /* geff = cumulative gg (state.c:1082-1086). */
+  m->geff[0] = m->gg[0];
+  for (k = 1; k < L; k++) m->geff[k] = m->geff[k - 1] + m->gg[k];
+  for (k = 0; k < L; k++)
+    if (m->gg[k] < 0) return false;
+  return true;
+}
+
+void or_model_free(or_model *m)
+{
+  uint L = m->L;
+  free_layers(m->u, L); free_layers(m->v, L); free_layers(m->h, L);
+  free_layers(m->tu, 3 * L); free_layers(m->tv, 3 * L); free_layers(m->th, 3 * L);
+  or_field_free(&m->hhb); or_field_free(&m->hhs); or_field_free(&m->omz);
+  or_field_free(&m->taux); or_field_free(&m->tauy);
+  free_layers(m->uw, L); free_layers(m->vw, L); free_layers(m->hw, L);
+  free_layers(m->dtu, L); free_layers(m->dtv, L); free_layers(m->dth, L);
+  free_layers(m->hwest, L); free_layers(m->hsouth, L);
+  free_layers(m->hFsurf_w, L); free_layers(m->hFsurf_s, L);
+  free_layers(m->hFbot_w, L); free_layers(m->hFbot_s, L);
+  free_layers(m->hhq, L); free_layers(m->zeta, L); free_layers(m->qq, L);
+  free_layers(m->al, L); free_layers(m->be, L); free_layers(m->ga, L); free_layers(m->de, L);
+  free_layers(m->ep, L); free_layers(m->ph, L); free_layers(m->la, L); free_layers(m->mu, L);
+  free_layers(m->pp, L); free_layers(m->KE, L); free_layers(m->MM, L);
+  free_layers(m->huu, L); free_layers(m->hvv, L); free_layers(m->eta, L + 1);
+  or_field_free(&m->hhs_w); or_field_free(&m->hhb_w); or_field_free(&m->hhs_west);
+  or_field_free(&m->hhs_south); or_field_free(&m->hhb_west); or_field_free(&m->hhb_south);
+  or_field_free(&m->Hc); or_field_free(&m->Hw); or_field_free(&m->Hs);
+  or_field_free(&m->taux_w); or_field_free(&m->tauy_w); or_field_free(&m->usq_bot);
+  or_field_free(&m->vsq_bot); or_field_free(&m->uabs_bot);
+  or_field_free(&m->DD_T); or_field_free(&m->DD_S); or_field_free(&m->DD4_T); or_field_free(&m->DD4_S);
+  or_field_free(&m->A2_h); or_field_free(&m->A2_q); or_field_free(&m->A4s_h); or_field_free(&m->A4s_q);
+  or_field_free(&m->UU4); or_field_free(&m->VV4); or_field_free(&m->pi); or_field_free(&m->rhs);
+  free(m->gg);
+  free(m->geff);
+  or_mg_free();
+}
+
+/* ------------------------------------------------------------------ */
+/* work arrays (work.c:538-559, 566-611, 1124-1224)                    */
+/* ------------------------------------------------------------------ */
+
+/* Centred face averages over [1,sx-1) x [1,sy-1) (work.c:588-611). */
+static void face_avg(const or_field *hh, or_field *hw, or_field *hs)
+{
+  uint i, j;
+  for (i = 1; i < hh->sx - 1; i++)
+    for (j = 1; j < hh->sy - 1; j++) {
+      OF(hw, i, j) = 0.5 * (OF(hh, i, j) + OF(hh, i - 1, j));
+      OF(hs, i, j) = 0.5 * (OF(hh, i, j) + OF(hh, i, j - 1));
+    }
+}
+
+/* pawsim_work_load_state (work.c:538-559). */
+static void load_state(or_model *m)
+{
+  uint k;
+  for (k = 0; k < m->L; k++) {
+    copy_interior(&m->uw[k], &m->u[k]);
+    copy_interior(&m->vw[k], &m->v[k]);
+    copy_interior(&m->hw[k], &m->h[k]);
+    or_fill(&m->uw[k], m, OR_HALO_U);
+    or_fill(&m->vw[k], m, OR_HALO_V);
+    or_fill(&m->hw[k], m, OR_HALO_SCALAR);
+  }
+}
+
+/* pawsim_work_init_static_geometry (work.c:1124-1167). */
+static void static_geometry(or_model *m)
+{
+  uint i, j, g = m->g;
+  copy_interior(&m->hhs_w, &m->hhs);
+  copy_interior(&m->hhb_w, &m->hhb);
+  or_fill(&m->hhs_w, m, OR_HALO_SCALAR);
+  or_fill(&m->hhb_w, m, OR_HALO_SCALAR);
+  face_avg(&m->hhs_w, &m->hhs_west, &m->hhs_south);
+  face_avg(&m->hhb_w, &m->hhb_west, &m->hhb_south);
+  or_fill(&m->hhs_west, m, OR_HALO_SCALAR);
+  or_fill(&m->hhs_south, m, OR_HALO_SCALAR);
+  or_fill(&m->hhb_west, m, OR_HALO_SCALAR);
+  or_fill(&m->hhb_south, m, OR_HALO_SCALAR);
+  for (i = 0; i < m->cfg.Nx; i++)
+    for (j = 0; j < m->cfg.Ny; j++)
+      OF(&m->Hc, i + g, j + g) = OF(&m->hhs, i + g, j + g) - OF(&m->hhb, i + g, j + g);
+  or_fill(&m->Hc, m, OR_HALO_SCALAR);
+  face_avg(&m->Hc, &m->Hw, &m->Hs);
+  or_fill(&m->Hw, m, OR_HALO_SCALAR);
+  or_fill(&m->Hs, m, OR_HALO_SCALAR);
+}
+
+void or_prepare_work(or_model *m)
+{
+  load_state(m);
+  static_geometry(m);
+}
+
+/* pawsim_forcing_update with a single wind record (forcing.c:30-89). */
+void or_forcing_update(or_model *m)
+{
+  uint i, j, g = m->g;
+  const real w0 = 1, w1 = 0;
+  or_field_zero(&m->taux_w);
+  or_field_zero(&m->tauy_w);
+  if (!m->cfg.useWind) return;
+  for (i = 0; i < m->cfg.Nx; i++)
+    for (j = 0; j < m->cfg.Ny; j++) {
+      OF(&m->taux_w, i + g, j + g) = w0 * OF(&m->taux, i + g, j + g) + w1 * OF(&m->taux, i + g, j + g);
+      OF(&m->tauy_w, i + g, j + g) = w0 * OF(&m->tauy, i + g, j + g) + w1 * OF(&m->tauy, i + g, j + g);
+    }
+  or_fill(&m->taux_w, m, OR_HALO_SCALAR);
+  or_fill(&m->tauy_w, m, OR_HALO_SCALAR);
+}
+
+/* ------------------------------------------------------------------ */
+/* tendency core (tendency.c:49-75)                                     */
+/* ------------------------------------------------------------------ */
+
+/* work.c:1206-1224 with the AL81 branch of 566-611. */
+static void calc_face_thickness(or_model *m)
+{
+  uint k;
+  for (k = 0; k < m->L; k++) {
+    face_avg(&m->hw[k], &m->hwest[k], &m->hsouth[k]);
+    or_fill(&m->hwest[k], m, OR_HALO_SCALAR);
+    or_fill(&m->hsouth[k], m, OR_HALO_SCALAR);
+  }
+}
+
+/* work.c:1173-1203: interface heights built upward from the bottom. */
+static void calc_eta(or_model *m)
+{
+  int k;
+  uint i, j, L = m->L;
+  for (k = (int)L; k >= 0; k--) {
+    or_field *e = &m->eta[k];
+    for (i = 0; i < e->sx; i++)
+      for (j = 0; j < e->sy; j++) {
+        if ((uint)k == L) OF(e, i, j) = OF(&m->hhb_w, i, j);
+        else OF(e, i, j) = OF(&m->eta[k + 1], i, j) + OF(&m->hw[k], i, j);
+      }
+    or_fill(e, m, OR_HALO_SCALAR);
+  }
+}
+
+/* work.c:1231-1288: zeta, q-point thickness and PV with Coriolis. */
+static void calc_pv(or_model *m)
+{
+  uint k, i, j;
+  const real dx = m->dx, dy = m->dy;
+  for (k = 0; k < m->L; k++) {
+    or_field *u = &m->uw[k], *v = &m->vw[k], *h = &m->hw[k];
+    uint imax = h->sx - 2, jmax = h->sy - 2;
+    or_field_zero(&m->zeta[k]);
+    or_field_zero(&m->hhq[k]);
+    or_field_zero(&m->qq[k]);
+    for (i = 1; i <= imax; i++)
+      for (j = 1; j <= jmax; j++) {
+        OF(&m->zeta[k], i, j) = (OF(u, i, j - 1) - OF(u, i, j)) / dy + (OF(v, i, j) - OF(v, i - 1, j)) / dx;
+        OF(&m->hhq[k], i, j) = 0.25 * (OF(h, i, j) + OF(h, i - 1, j - 1) + OF(h, i - 1, j) + OF(h, i, j - 1));
+        OF(&m->qq[k], i, j) = (2 * OF(&m->omz, i, j) + OF(&m->zeta[k], i, j)) / OF(&m->hhq[k], i, j);
+      }
+    or_fill_internal(&m->zeta[k], m);
+    or_fill_internal(&m->hhq[k], m);
+    or_fill_internal(&m->qq[k], m);
+  }
+}
+
+/* work.c:1294-1404: PV interpolation coefficients. */
+static void calc_pv_coefficients(or_model *m)
+{
+  uint k, i, j;
+  const uint sch = m->cfg.momScheme;
+  for (k = 0; k < m->L; k++) {
+    const or_field *q = &m->qq[k];
+    or_field *A = &m->al[k], *B = &m->be[k], *G = &m->ga[k], *D = &m->de[k];
+    or_field *E = &m->ep[k], *P = &m->ph[k], *La = &m->la[k], *Mu = &m->mu[k];
+    or_field_zero(A); or_field_zero(B); or_field_zero(G); or_field_zero(D);
+    or_field_zero(E); or_field_zero(P); or_field_zero(La); or_field_zero(Mu);
+    for (i = 1; i < q->sx - 1; i++)
+      for (j = 1; j < q->sy - 1; j++) {
+        if (sch == OR_MOM_AL81 || sch == OR_MOM_HK83) {
+          OF(A, i, j) = (2 * OF(q, i + 1, j + 1) + OF(q, i, j + 1) + 2 * OF(q, i, j) + OF(q, i + 1, j)) / 24;
+          OF(B, i, j) = (OF(q, i, j + 1) + 2 * OF(q, i - 1, j + 1) + OF(q, i - 1, j) + 2 * OF(q, i, j)) / 24;
+          OF(G, i, j) = (2 * OF(q, i, j + 1) + OF(q, i - 1, j + 1) + 2 * OF(q, i - 1, j) + OF(q, i, j)) / 24;
+          OF(D, i, j) = (OF(q, i + 1, j + 1) + 2 * OF(q, i, j + 1) + OF(q, i, j) + 2 * OF(q, i + 1, j)) / 24;
+          OF(E, i, j) = (OF(q, i + 1, j + 1) + OF(q, i, j + 1) - OF(q, i, j) - OF(q, i + 1, j)) / 24;
+          OF(P, i, j) = (-OF(q, i + 1, j + 1) + OF(q, i, j + 1) + OF(q, i, j) - OF(q, i + 1, j)) / 24;
+        } else if (sch == OR_MOM_TW81 && j < q->sy - 2) {
+          OF(A, i, j) = (2 * OF(q, i + 1, j + 1) + 3 * OF(q, i, j + 1) + 2 * OF(q, i, j) + OF(q, i + 1, j)
+                         - OF(q, i, j + 2) - OF(q, i - 1, j + 1)) / 24;
+          OF(B, i, j) = (3 * OF(q, i, j + 1) + 2 * OF(q, i - 1, j + 1) + OF(q, i - 1, j) + 2 * OF(q, i, j)
+                         - OF(q, i + 1, j + 1) - OF(q, i, j + 2)) / 24;
+          OF(G, i, j) = (2 * OF(q, i, j + 1) + OF(q, i - 1, j + 1) + 2 * OF(q, i - 1, j) + 3 * OF(q, i, j)
+                         - OF(q, i, j - 1) - OF(q, i + 1, j)) / 24;
+          OF(D, i, j) = (OF(q, i + 1, j + 1) + 2 * OF(q, i, j + 1) + 3 * OF(q, i, j) + 2 * OF(q, i + 1, j)
+                         - OF(q, i - 1, j) - OF(q, i, j - 1)) / 24;
+          OF(E, i, j) = (OF(q, i + 1, j + 1) + OF(q, i, j + 1) - OF(q, i, j) - OF(q, i + 1, j)) / 24;
+          OF(P, i, j) = (-OF(q, i + 1, j + 1) + OF(q, i, j + 1) + OF(q, i, j) - OF(q, i + 1, j)) / 24;
+          OF(La, i, j) = (OF(q, i + 1, j) - OF(q, i - 1, j)) / 24;
+          OF(Mu, i, j) = (OF(q, i, j - 1) - OF(q, i, j + 1)) / 24;
+        }
+      }
+    or_fill_internal(A, m); or_fill_internal(B, m); or_fill_internal(G, m); or_fill_internal(D, m);
+    or_fill_internal(E, m); or_fill_internal(P, m); or_fill_internal(La, m); or_fill_internal(Mu, m);
+  }
+}
+
+/* work.c:1410-1512: Bernoulli pressure, kinetic energy, Montgomery potential. */
+static void calc_bernoulli(or_model *m)
+{
+  uint k, i, j;
+  const uint sch = m->cfg.momScheme;
+  const real h0 = m->cfg.h0;
+  for (k = 0; k < m->L; k++) {
+    or_field *u = &m->uw[k], *v = &m->vw[k], *h = &m->hw[k];
+    or_field *pp = &m->pp[k], *KE = &m->KE[k], *MM = &m->MM[k];
+    or_field_zero(pp);
+    or_field_zero(KE);
+    or_field_zero(MM);
+    for (i = 1; i < h->sx - 1; i++)
+      for (j = 1; j < h->sy - 1; j++) {
+        if (k == 0) OF(pp, i, j) = 0; /* rigid lid: barotropic pressure solved separately */
+        else OF(pp, i, j) = OF(&m->pp[k - 1], i, j) + m->gg[k] * OF(&m->eta[k], i, j);
+        if (sch == OR_MOM_HK83) {
+          OF(KE, i, j) = (SQ(OF(u, i + 1, j)) + SQ(OF(u, i, j))
+                          + 0.25 * (SQ(OF(u, i + 1, j + 1)) + SQ(OF(u, i, j + 1)) + SQ(OF(u, i + 1, j - 1)) + SQ(OF(u, i, j - 1)))
+                          + SQ(OF(v, i, j + 1)) + SQ(OF(v, i, j))
+                          + 0.25 * (SQ(OF(v, i + 1, j + 1)) + SQ(OF(v, i + 1, j)) + SQ(OF(v, i - 1, j + 1)) + SQ(OF(v, i - 1, j))))
+                         / 6;
+        } else if (sch == OR_MOM_TW81) {
+          if (i < h->sx - 2 && j < h->sy - 2)
+            OF(KE, i, j) = 0.25 * ((2.0 / 3.0) * SQ(OF(u, i, j)) + (1.0 / 3.0) * SQ(0.5 * (OF(u, i - 1, j) + OF(u, i + 1, j)))
+                                   + (2.0 / 3.0) * SQ(OF(u, i + 1, j)) + (1.0 / 3.0) * SQ(0.5 * (OF(u, i, j) + OF(u, i + 2, j)))
+                                   + (2.0 / 3.0) * SQ(OF(v, i, j)) + (1.0 / 3.0) * SQ(0.5 * (OF(v, i, j - 1) + OF(v, i, j + 1)))
+                                   + (2.0 / 3.0) * SQ(OF(v, i, j + 1)) + (1.0 / 3.0) * SQ(0.5 * (OF(v, i, j) + OF(v, i, j + 2))));
+        } else {
+          OF(KE, i, j) = (SQ(OF(u, i + 1, j)) + SQ(OF(u, i, j))) / 4 + (SQ(OF(v, i, j + 1)) + SQ(OF(v, i, j))) / 4;
+        }
+        OF(MM, i, j) = OF(pp, i, j);
+        if (h0 != 0)
+          OF(MM, i, j) -= m->geff[k] * (h0 * h0 * h0 * h0) / (OF(h, i, j) * OF(h, i, j) * OF(h, i, j)) / 3;
+      }
+    or_fill(pp, m, OR_HALO_SCALAR);
+    or_fill(KE, m, OR_HALO_SCALAR);
+    or_fill(MM, m, OR_HALO_SCALAR);
+  }
+}
+
+/* work.c:162-186: per-layer mass fluxes. */
+static void calc_mass_fluxes(or_model *m, uint k)
+{
+  uint i, j;
+  or_field *huu = &m->huu[k], *hvv = &m->hvv[k];
+  or_field_zero(huu);
+  or_field_zero(hvv);
+  for (i = 1; i < huu->sx - 1; i++)
+    for (j = 1; j < huu->sy - 1; j++) {
+      OF(huu, i, j) = OF(&m->uw[k], i, j) * OF(&m->hwest[k], i, j);
+      OF(hvv, i, j) = OF(&m->vw[k], i, j) * OF(&m->hsouth[k], i, j);
+    }
+  or_fill_internal(huu, m);
+  or_fill_internal(hvv, m);
+}
+
+/* work.c:1520-1857: momentum RHS (constant A2/A4; no Smagorinsky, no buoyancy). */
+static void calc_momentum(or_model *m)
+{
+  uint k, i, j, g = m->g;
+  const real dx = m->dx, dy = m->dy;
+  const bool useA2 = m->cfg.A2 > 0, useA4 = m->cfg.A4 > 0;
+  const real A4c = sqrt(m->cfg.A4);
+  const uint sch = m->cfg.momScheme;
+  for (k = 0; k < m->L; k++) {
+    or_field *u = &m->uw[k], *v = &m->vw[k], *h = &m->hw[k], *hq = &m->hhq[k];
+    or_field *hW = &m->hwest[k], *hS = &m->hsouth[k], *huu = &m->huu[k], *hvv = &m->hvv[k];
+    or_field *A = &m->al[k], *B = &m->be[k], *G = &m->ga[k], *D = &m->de[k];
+    or_field *E = &m->ep[k], *P = &m->ph[k], *La = &m->la[k], *Mu = &m->mu[k], *q = &m->qq[k];
+    or_field *MM = &m->MM[k], *KE = &m->KE[k];
+    or_field_zero(&m->dtu[k]);
+    or_field_zero(&m->dtv[k]);
+    calc_mass_fluxes(m, k);
+    if (useA2 || useA4) {
+      or_field_zero(&m->DD_T); or_field_zero(&m->DD_S); or_field_zero(&m->A2_h); or_field_zero(&m->A2_q);
+      or_field_zero(&m->A4s_h); or_field_zero(&m->A4s_q); or_field_zero(&m->UU4); or_field_zero(&m->VV4);
+      or_field_zero(&m->DD4_T); or_field_zero(&m->DD4_S);
+      for (i = 1; i < m->DD_T.sx - 1; i++)
+        for (j = 1; j < m->DD_T.sy - 1; j++) {
+          OF(&m->DD_T, i, j) = (OF(u, i + 1, j) - OF(u, i, j)) / dx - (OF(v, i, j + 1) - OF(v, i, j)) / dy;
+          OF(&m->DD_S, i, j) = (OF(u, i, j) - OF(u, i, j - 1)) / dy + (OF(v, i, j) - OF(v, i - 1, j)) / dx;
+        }
+      for (i = 1; i < m->DD_T.sx - 1; i++)
+        for (j = 1; j < m->DD_T.sy - 1; j++) {
+          if (useA2) {
+            OF(&m->A2_q, i, j) = m->cfg.A2;
+            OF(&m->A2_h, i, j) = m->cfg.A2;
+          }
+          if (useA4) {
+            OF(&m->A4s_q, i, j) = A4c;
+            OF(&m->A4s_h, i, j) = A4c;
+          }
+        }
+      if (useA4) {
+        or_field *DT = &m->DD_T, *DS = &m->DD_S, *Ah = &m->A4s_h, *Aq = &m->A4s_q;
+        for (i = 1; i < m->UU4.sx - 1; i++)
+          for (j = 1; j < m->UU4.sy - 1; j++) {
+            OF(&m->UU4, i, j) = ((OF(Ah, i, j) * OF(h, i, j) * OF(DT, i, j) - OF(Ah, i - 1, j) * OF(h, i - 1, j) * OF(DT, i - 1, j)) / dx
+                                 + (OF(Aq, i, j + 1) * OF(hq, i, j + 1) * OF(DS, i, j + 1) - OF(Aq, i, j) * OF(hq, i, j) * OF(DS, i, j)) / dy)
+                                / OF(hW, i, j);
+            OF(&m->VV4, i, j) = ((OF(Aq, i + 1, j) * OF(hq, i + 1, j) * OF(DS, i + 1, j) - OF(Aq, i, j) * OF(hq, i, j) * OF(DS, i, j)) / dx
+                                 - (OF(Ah, i, j) * OF(h, i, j) * OF(DT, i, j) - OF(Ah, i, j - 1) * OF(h, i, j - 1) * OF(DT, i, j - 1)) / dy)
+                                / OF(hS, i, j);
+          }
+        for (i = 1; i < m->DD4_T.sx - 1; i++)
+          for (j = 1; j < m->DD4_T.sy - 1; j++) {
+            OF(&m->DD4_T, i, j) = (OF(&m->UU4, i + 1, j) - OF(&m->UU4, i, j)) / dx - (OF(&m->VV4, i, j + 1) - OF(&m->VV4, i, j)) / dy;
+            OF(&m->DD4_S, i, j) = (OF(&m->UU4, i, j) - OF(&m->UU4, i, j - 1)) / dy + (OF(&m->VV4, i, j) - OF(&m->VV4, i - 1, j)) / dx;
+          }
+      }
+    }
+    for (i = g; i < g + m->cfg.Nx; i++)
+      for (j = g; j < g + m->cfg.Ny; j++) {
+        real hwest = OF(hW, i, j), hsouth = OF(hS, i, j);
+        real gM, gK, qf, a2 = 0, a4 = 0, buoy = 0, rhs;
+        gM = -(OF(MM, i, j) - OF(MM, i - 1, j)) / dx;
+        gK = -(OF(KE, i, j) - OF(KE, i - 1, j)) / dx;
+        qf = 0;
+        if (sch == OR_MOM_AL81 || sch == OR_MOM_HK83)
+          qf += OF(A, i, j) * OF(hvv, i, j + 1) + OF(B, i, j) * OF(hvv, i - 1, j + 1) + OF(G, i, j) * OF(hvv, i - 1, j)
+                + OF(D, i, j) * OF(hvv, i, j) - OF(E, i, j) * OF(huu, i + 1, j) + OF(E, i - 1, j) * OF(huu, i - 1, j);
+        else if (sch == OR_MOM_TW81)
+          qf += OF(A, i, j) * OF(hvv, i, j + 1) + OF(B, i, j) * OF(hvv, i - 1, j + 1) + OF(G, i, j) * OF(hvv, i - 1, j)
+                + OF(D, i, j) * OF(hvv, i, j) - OF(E, i, j) * OF(huu, i + 1, j) + OF(E, i - 1, j) * OF(huu, i - 1, j)
+                - OF(La, i, j + 1) * OF(huu, i, j) + OF(La, i, j) * OF(huu, i, j - 1);
+        else
+          qf += 0.25 * OF(q, i, j) * (OF(hvv, i, j) + OF(hvv, i - 1, j))
+                + 0.25 * OF(q, i, j + 1) * (OF(hvv, i, j + 1) + OF(hvv, i - 1, j + 1));
+        if (useA2)
+          a2 = ((OF(&m->A2_h, i, j) * OF(h, i, j) * OF(&m->DD_T, i, j) - OF(&m->A2_h, i - 1, j) * OF(h, i - 1, j) * OF(&m->DD_T, i - 1, j)) / dx
+                + (OF(&m->A2_q, i, j + 1) * OF(hq, i, j + 1) * OF(&m->DD_S, i, j + 1) - OF(&m->A2_q, i, j) * OF(hq, i, j) * OF(&m->DD_S, i, j)) / dy);
+        if (useA4)
+          a4 = -((OF(&m->A4s_h, i, j) * OF(h, i, j) * OF(&m->DD4_T, i, j) - OF(&m->A4s_h, i - 1, j) * OF(h, i - 1, j) * OF(&m->DD4_T, i - 1, j)) / dx
+                 + (OF(&m->A4s_q, i, j + 1) * OF(hq, i, j + 1) * OF(&m->DD4_S, i, j + 1) - OF(&m->A4s_q, i, j) * OF(hq, i, j) * OF(&m->DD4_S, i, j)) / dy);
+        rhs = gM + gK + qf + ((hwest != 0) ? (a2 + a4 + buoy) / hwest : 0);
+        OF(&m->dtu[k], i, j) = rhs;
+
+        gM = -(OF(MM, i, j) - OF(MM, i, j - 1)) / dy;
+        gK = -(OF(KE, i, j) - OF(KE, i, j - 1)) / dy;
+        qf = 0;
+        a2 = 0;
+        a4 = 0;
+        if (sch == OR_MOM_AL81 || sch == OR_MOM_HK83)
+          qf += OF(P, i, j - 1) * OF(hvv, i, j - 1) - OF(P, i, j) * OF(hvv, i, j + 1) - OF(G, i + 1, j) * OF(huu, i + 1, j)
+                - OF(D, i, j) * OF(huu, i, j) - OF(A, i, j - 1) * OF(huu, i, j - 1) - OF(B, i + 1, j - 1) * OF(huu, i + 1, j - 1);
+        else if (sch == OR_MOM_TW81)
+          qf += OF(P, i, j - 1) * OF(hvv, i, j - 1) - OF(P, i, j) * OF(hvv, i, j + 1) - OF(G, i + 1, j) * OF(huu, i + 1, j)
+                - OF(D, i, j) * OF(huu, i, j) - OF(A, i, j - 1) * OF(huu, i, j - 1) - OF(B, i + 1, j - 1) * OF(huu, i + 1, j - 1)
+                - OF(Mu, i + 1, j) * OF(hvv, i, j) + OF(Mu, i, j) * OF(hvv, i - 1, j);
+        else
+          qf -= 0.25 * OF(q, i, j) * (OF(huu, i, j) + OF(huu, i, j - 1))
+                + 0.25 * OF(q, i + 1, j) * (OF(huu, i + 1, j) + OF(huu, i + 1, j - 1));
+        if (useA2)
+          a2 = ((OF(&m->A2_q, i + 1, j) * OF(hq, i + 1, j) * OF(&m->DD_S, i + 1, j) - OF(&m->A2_q, i, j) * OF(hq, i, j) * OF(&m->DD_S, i, j)) / dx
+                - (OF(&m->A2_h, i, j) * OF(h, i, j) * OF(&m->DD_T, i, j) - OF(&m->A2_h, i, j - 1) * OF(h, i, j - 1) * OF(&m->DD_T, i, j - 1)) / dy);
+        if (useA4)
+          a4 = -((OF(&m->A4s_q, i + 1, j) * OF(hq, i + 1, j) * OF(&m->DD4_S, i + 1, j) - OF(&m->A4s_q, i, j) * OF(hq, i, j) * OF(&m->DD4_S, i, j)) / dx
+                 - (OF(&m->A4s_h, i, j) * OF(h, i, j) * OF(&m->DD4_T, i, j) - OF(&m->A4s_h, i, j - 1) * OF(h, i, j - 1) * OF(&m->DD4_T, i, j - 1)) / dy);
+        rhs = gM + gK + qf + ((hsouth != 0) ? (a2 + a4 + buoy) / hsouth : 0);
+        OF(&m->dtv[k], i, j) = rhs;
+      }
+    or_fill(&m->dtu[k], m, OR_HALO_U);
+    or_fill(&m->dtv[k], m, OR_HALO_V);
+  }
+}
+
+/* forcing.c:149-395 with hsml<=0, hbbl<=0 (work.c:2151-2340 top/bottom
+ * layer indicator) and no surface drag or barotropic forcing. */
+static void forcing_apply(or_model *m)
+{
+  uint k, i, j, g = m->g, L = m->L;
+  const bool useBot = (m->cfg.linDrag > 0) || (m->cfg.quadDrag > 0);
+  if (m->cfg.useWind) {
+    for (k = 0; k < L; k++) {
+      or_field_zero(&m->hFsurf_w[k]);
+      or_field_zero(&m->hFsurf_s[k]);
+    }
+    for (i = 0; i < m->hFsurf_w[0].sx; i++)
+      for (j = 0; j < m->hFsurf_w[0].sy; j++) {
+        OF(&m->hFsurf_w[0], i, j) = 1;
+        OF(&m->hFsurf_s[0], i, j) = 1;
+      }
+    for (k = 0; k < L; k++) {
+      or_fill(&m->hFsurf_w[k], m, OR_HALO_U);
+      or_fill(&m->hFsurf_s[k], m, OR_HALO_V);
+    }
+  }
+  if (useBot) {
+    for (k = 0; k < L; k++) {
+      or_field_zero(&m->hFbot_w[k]);
+      or_field_zero(&m->hFbot_s[k]);
+    }
+    for (i = 0; i < m->hFbot_w[L - 1].sx; i++)
+      for (j = 0; j < m->hFbot_w[L - 1].sy; j++) {
+        OF(&m->hFbot_w[L - 1], i, j) = 1;
+        OF(&m->hFbot_s[L - 1], i, j) = 1;
+      }
+    for (k = 0; k < L; k++) {
+      or_fill(&m->hFbot_w[k], m, OR_HALO_U);
+      or_fill(&m->hFbot_s[k], m, OR_HALO_V);
+    }
+  }
+  if (m->cfg.quadDrag > 0) {
+    for (i = 0; i < m->usq_bot.sx; i++)
+      for (j = 0; j < m->usq_bot.sy; j++) {
+        real ub = OF(&m->uw[L - 1], i, j), vb = OF(&m->vw[L - 1], i, j);
+        OF(&m->usq_bot, i, j) = ub * ub;
+        OF(&m->vsq_bot, i, j) = vb * vb;
+      }
+    for (i = 0; i < m->uabs_bot.sx - 1; i++)
+      for (j = 0; j < m->uabs_bot.sy - 1; j++)
+        OF(&m->uabs_bot, i, j) = sqrt(0.5 * (OF(&m->usq_bot, i, j) + OF(&m->usq_bot, i + 1, j))
+                                      + 0.5 * (OF(&m->vsq_bot, i, j) + OF(&m->vsq_bot, i, j + 1)));
+  }
+  if (!m->cfg.useWind && !useBot) return;
+  for (k = 0; k < L; k++) {
+    for (i = 0; i < m->cfg.Nx; i++) {
+      uint il = i + g;
+      for (j = 0; j < m->cfg.Ny; j++) {
+        uint jl = j + g;
+        real hwest = OF(&m->hwest[k], il, jl), hsouth = OF(&m->hsouth[k], il, jl);
+        real *du = &OF(&m->dtu[k], il, jl), *dv = &OF(&m->dtv[k], il, jl);
+        if (m->cfg.useWind) {
+          real r = OF(&m->taux_w, il, jl) * OF(&m->hFsurf_w[k], il, jl);
+          if (hwest != 0) *du += r / hwest;
+          r = OF(&m->tauy_w, il, jl) * OF(&m->hFsurf_s[k], il, jl);
+          if (hsouth != 0) *dv += r / hsouth;
+        }
+        if (m->cfg.linDrag > 0 && hwest != 0) {
+          real r = -m->cfg.linDrag * OF(&m->uw[k], il, jl) * OF(&m->hFbot_w[k], il, jl);
+          *du += r / hwest;
+        }
+        if (m->cfg.linDrag > 0 && hsouth != 0) {
+          real r = -m->cfg.linDrag * OF(&m->vw[k], il, jl) * OF(&m->hFbot_s[k], il, jl);
+          *dv += r / hsouth;
+        }
+        if (m->cfg.quadDrag > 0 && hwest != 0) {
+          real ua = 0.5 * (OF(&m->uabs_bot, il, jl) + OF(&m->uabs_bot, il - 1, jl));
+          *du -= m->cfg.quadDrag * ua * OF(&m->uw[k], il, jl) * OF(&m->hFbot_w[k], il, jl) / hwest;
+        }
+        if (m->cfg.quadDrag > 0 && hsouth != 0) {
+          real ua = 0.5 * (OF(&m->uabs_bot, il, jl) + OF(&m->uabs_bot, il, jl - 1));
+          *dv -= m->cfg.quadDrag * ua * OF(&m->vw[k], il, jl) * OF(&m->hFbot_s[k], il, jl) / hsouth;
+        }
+      }
+    }
+    or_fill(&m->dtu[k], m, OR_HALO_U);
+    or_fill(&m->dtv[k], m, OR_HALO_V);
+  }
+}
+
+/* work.c:1864-1941: thickness tendency; the diapycnal velocity is zero
+ * (restoring.c:39-47), so rhs_relax = -(0 - 0). */
+static void calc_thickness(or_model *m)
+{
+  uint k, i, j, g = m->g;
+  const real dx = m->dx, dy = m->dy;
+  const real wk = 0, wk1 = 0;
+  for (k = 0; k < m->L; k++) {
+    or_field *huu = &m->huu[k], *hvv = &m->hvv[k];
+    or_field_zero(&m->dth[k]);
+    for (i = g; i < g + m->cfg.Nx; i++)
+      for (j = g; j < g + m->cfg.Ny; j++) {
+        real adv = (OF(huu, i, j) - OF(huu, i + 1, j)) / dx + (OF(hvv, i, j) - OF(hvv, i, j + 1)) / dy;
+        real rel = -(wk - wk1);
+        OF(&m->dth[k], i, j) = adv + rel;
+      }
+    or_fill(&m->dth[k], m, OR_HALO_SCALAR);
+  }
+}
+
+/* tendency.c:49-75 (restoring, diapycnal and tracer stages are no-ops
+ * for the restated configurations). */
+void or_tendency_core(or_model *m)
+{
+  calc_face_thickness(m);
+  calc_eta(m);
+  calc_pv(m);
+  calc_pv_coefficients(m);
+  calc_bernoulli(m);
+  calc_momentum(m);
+  forcing_apply(m);
+  calc_thickness(m);
+}
+
+/* tendency.c:81-115: evaluate into tendency slot `slot` (0 = tendency). */
+void or_tendency_eval(or_model *m, int slot)
+{
+  uint k;
+  or_forcing_update(m);
+  for (k = 0; k < m->L; k++) {
+    or_field_zero(&m->tu[slot * m->L + k]);
+    or_field_zero(&m->tv[slot * m->L + k]);
+    or_field_zero(&m->th[slot * m->L + k]);
+  }
+  load_state(m);
+  static_geometry(m);
+  or_tendency_core(m);
+  for (k = 0; k < m->L; k++) {
+    copy_interior(&m->tu[slot * m->L + k], &m->dtu[k]);
+    copy_interior(&m->tv[slot * m->L + k], &m->dtv[k]);
+    copy_interior(&m->th[slot * m->L + k], &m->dth[k]);
+  }
+  fill_prognostic(m, &m->tu[slot * m->L], &m->tv[slot * m->L], &m->th[slot * m->L]);
+}
+
+/* ------------------------------------------------------------------ */
+/* time integration (integrator.c:15-198)                               */
+/* ------------------------------------------------------------------ */
+
+static void ab_field(or_field *x, const or_field *f0, const or_field *f1, const or_field *f2, int order, real dt)
+{
+  uint i, j, g = x->g;
+  for (i = 0; i < x->nx; i++)
+    for (j = 0; j < x->ny; j++) {
+      real *p = &OF(x, i + g, j + g);
+      if (order == 1) *p += dt * OF(f0, i + g, j + g);
+      else if (order == 2) *p += (3 * dt / 2) * OF(f0, i + g, j + g) - (dt / 2) * OF(f1, i + g, j + g);
+      else *p += (23 * dt / 12) * OF(f0, i + g, j + g) - (4 * dt / 3) * OF(f1, i + g, j + g) + (5 * dt / 12) * OF(f2, i + g, j + g);
+    }
+}
+
+static void ab_update(or_model *m, int order)
+{
+  uint k, L = m->L;
+  for (k = 0; k < L; k++) {
+    ab_field(&m->u[k], &m->tu[k], &m->tu[L + k], &m->tu[2 * L + k], order, m->cfg.dt);
+    ab_field(&m->v[k], &m->tv[k], &m->tv[L + k], &m->tv[2 * L + k], order, m->cfg.dt);
+    ab_field(&m->h[k], &m->th[k], &m->th[L + k], &m->th[2 * L + k], order, m->cfg.dt);
+  }
+}
+
+static void copy_full(or_field *dst, const or_field *src)
+{
+  memcpy(dst->d, src->d, (size_t)dst->sx * dst->sy * sizeof(real));
+}
+
+/* shift_tendency_history (integrator.c:127-132). */
+static void shift_history(or_model *m)
+{
+  uint k, L = m->L;
+  for (k = 0; k < L; k++) {
+    copy_full(&m->tu[2 * L + k], &m->tu[L + k]);
+    copy_full(&m->tv[2 * L + k], &m->tv[L + k]);
+    copy_full(&m->th[2 * L + k], &m->th[L + k]);
+    copy_full(&m->tu[L + k], &m->tu[k]);
+    copy_full(&m->tv[L + k], &m->tv[k]);
+    copy_full(&m->th[L + k], &m->th[k]);
+  }
+}
+
+/* pawsim_integrator_step (integrator.c:138-198), AB3 startup quirk included:
+ * at n==1 AB3 takes an AB1 step without rotating the history. */
+bool or_step(or_model *m, uint n)
+{
+  uint s = m->cfg.tsScheme;
+  or_tendency_eval(m, 0);
+  if (s == OR_AB1) {
+    ab_update(m, 1);
+    shift_history(m);
+  } else if (s == OR_AB2) {
+    ab_update(m, n == 1 ? 1 : 2);
+    shift_history(m);
+  } else if (s == OR_AB3) {
+    if (n == 1) {
+      ab_update(m, 1);
+    } else if (n == 2) {
+      ab_update(m, 2);
+      shift_history(m);
+    } else {
+      ab_update(m, 3);
+      shift_history(m);
+    }
+  } else {
+    return false;
+  }
+  m->t += m->cfg.dt;
+  fill_prognostic(m, m->u, m->v, m->h);
+  return true;
+}
+
+/* ------------------------------------------------------------------ */
+/* rigid-lid pressure correction (pressure.c:55-88, 525-814, 2182-2300) */
+/* ------------------------------------------------------------------ */
+
+/* Serial-order mean of the owned cells (pressure.c:55-72). */
+static real field_mean(const or_field *f, const or_model *m)
+{
+  uint i, j, g = f->g;
+  real s = 0;
+  for (i = 0; i < f->nx; i++)
+    for (j = 0; j < f->ny; j++) s += OF(f, i + g, j + g);
+  return s / ((real)m->cfg.Nx * (real)m->cfg.Ny);
+}
+
+static void subtract_mean(or_field *f, const or_model *m)
+{
+  uint i, j, g = f->g;
+  real mean = field_mean(f, m);
+  for (i = 0; i < f->nx; i++)
+    for (j = 0; j < f->ny; j++) OF(f, i + g, j + g) -= mean;
+}
+
+/* pressure.c:525-561. */
+static void correct_thickness(or_model *m)
+{
+  uint i, j, k, g = m->g;
+  for (i = 0; i < m->cfg.Nx; i++)
+    for (j = 0; j < m->cfg.Ny; j++) {
+      real tot = 0, herr;
+      for (k = 0; k < m->L; k++) tot += OF(&m->h[k], i + g, j + g);
+      if (tot == 0) continue;
+      herr = tot - OF(&m->Hc, i + g, j + g);
+      for (k = 0; k < m->L; k++) OF(&m->h[k], i + g, j + g) *= (1 - herr / tot);
+    }
+  fill_prognostic(m, m->u, m->v, m->h);
+}
+
+/* work.c:865-896 + 997-1118: AL81 face thickness with modular wrap on the
+ * no-ghost global arrays, scattered back and scalar-filled. */
+static void face_thickness_no_ghost(or_model *m)
+{
+  uint k, i, j, g = m->g, Nx = m->cfg.Nx, Ny = m->cfg.Ny;
+  for (k = 0; k < m->L; k++) {
+    or_field *hh = &m->hw[k], *hW = &m->hwest[k], *hS = &m->hsouth[k];
+    or_field_zero(hW);
+    or_field_zero(hS);
+    for (i = 0; i < Nx; i++) {
+      uint im1 = (i + Nx - 1) % Nx;
+      for (j = 0; j < Ny; j++) {
+        uint jm1 = (j + Ny - 1) % Ny;
+        OF(hW, i + g, j + g) = 0.5 * (OF(hh, i + g, j + g) + OF(hh, im1 + g, j + g));
+        OF(hS, i + g, j + g) = 0.5 * (OF(hh, i + g, j + g) + OF(hh, i + g, jm1 + g));
+      }
+    }
+    or_fill(hW, m, OR_HALO_SCALAR);
+    or_fill(hS, m, OR_HALO_SCALAR);
+  }
+}
+
+/* pressure.c:570-606. */
+static void build_rhs(or_model *m)
+{
+  uint i, j, k, g = m->g;
+  or_field_zero(&m->rhs);
+  load_state(m);
+  face_thickness_no_ghost(m);
+  for (k = 0; k < m->L; k++)
+    for (i = 0; i < m->cfg.Nx; i++) {
+      uint il = i + g;
+      for (j = 0; j < m->cfg.Ny; j++) {
+        uint jl = j + g;
+        real hu_w = OF(&m->u[k], il, jl) * OF(&m->hwest[k], il, jl);
+        real hu_e = OF(&m->u[k], il + 1, jl) * OF(&m->hwest[k], il + 1, jl);
+        real hv_s = OF(&m->v[k], il, jl) * OF(&m->hsouth[k], il, jl);
+        real hv_n = OF(&m->v[k], il, jl + 1) * OF(&m->hsouth[k], il, jl + 1);
+        OF(&m->rhs, il, jl) += (hu_e - hu_w) / (m->dx * m->cfg.dt) + (hv_n - hv_s) / (m->dy * m->cfg.dt);
+      }
+    }
+  or_fill(&m->rhs, m, OR_HALO_SCALAR);
+  subtract_mean(&m->rhs, m);
+  or_fill(&m->rhs, m, OR_HALO_SCALAR);
+}
+
+/* pressure.c:609-637: wall faces drop out of the stencil. */
+static real west_op(const or_model *m, uint il, uint jl)
+{
+  uint gi = il - m->g;
+  if (m->cfg.wallEW && (gi == 0 || gi == m->cfg.Nx)) return 0;
+  return OF(&m->Hw, il, jl) / SQ(m->dx);
+}
+
+static real south_op(const or_model *m, uint il, uint jl)
+{
+  uint gj = jl - m->g;
+  if (m->cfg.wallNS && (gj == 0 || gj == m->cfg.Ny)) return 0;
+  return OF(&m->Hs, il, jl) / SQ(m->dy);
+}
+
+/* Red-black SOR (pressure.c:743-814). */
+static uint solve_sor(or_model *m)
+{
+  uint iters = 0, i, j, color, g = m->g;
+  const real w = m->cfg.SOR_rp;
+  real gmax = m->cfg.pi_tol + 1;
+  while (gmax > m->cfg.pi_tol && iters < m->cfg.maxiters) {
+    real itmax = 0;
+    for (color = 0; color < 2; color++) {
+      real cmax = 0;
+      or_fill(&m->pi, m, OR_HALO_SCALAR);
+      for (i = 0; i < m->cfg.Nx; i++) {
+        uint il = i + g;
+        for (j = 0; j < m->cfg.Ny; j++) {
+          uint jl = j + g;
+          real Ow = west_op(m, il, jl), Oe = west_op(m, il + 1, jl);
+          real Os = south_op(m, il, jl), On = south_op(m, il, jl + 1);
+          real Osum = Ow + Oe + Os + On, prev, nw;
+          if (((i + j) & 1U) != color) continue;
+          if (Osum == 0) continue;
+          prev = OF(&m->pi, il, jl);
+          nw = (1 - w) * prev + w / Osum * (On * OF(&m->pi, il, jl + 1) + Os * OF(&m->pi, il, jl - 1)
+                                              + Oe * OF(&m->pi, il + 1, jl) + Ow * OF(&m->pi, il - 1, jl)
+                                              - OF(&m->rhs, il, jl));
+          OF(&m->pi, il, jl) = nw;
+          cmax = fmax(cmax, fabs(nw - prev));
+        }
+      }
+      itmax = fmax(itmax, cmax);
+    }
+    gmax = itmax;
+    iters++;
+  }
+  subtract_mean(&m->pi, m);
+  or_fill(&m->pi, m, OR_HALO_SCALAR);
+  return iters;
+}
+
+/* Gathered multigrid (pressure.c:820-914): hierarchy built once from Hc,
+ * no mean removal. */
+static uint solve_mg(or_model *m)
+{
+  static bool inited = false;
+  uint i, j, g = m->g, Nx = m->cfg.Nx, Ny = m->cfg.Ny, iters;
+  real *gpi = malloc((size_t)Nx * Ny * sizeof(real)), *grhs = malloc((size_t)Nx * Ny * sizeof(real));
+  if (!inited) {
+    real *gHc = malloc((size_t)Nx * Ny * sizeof(real));
+    for (i = 0; i < Nx; i++)
+      for (j = 0; j < Ny; j++) gHc[(size_t)i * Ny + j] = OF(&m->Hc, i + g, j + g);
+    or_mg_init(Nx, Ny, m->cfg.Lx, m->cfg.Ly, gHc, m->cfg.wallEW, m->cfg.wallNS, m->cfg.useFullMG,
+               m->cfg.pi_tol, m->cfg.maxiters, m->cfg.omega_MG);
+    free(gHc);
+    inited = true;
+  }
+  for (i = 0; i < Nx; i++)
+    for (j = 0; j < Ny; j++) {
+      gpi[(size_t)i * Ny + j] = OF(&m->pi, i + g, j + g);
+      grhs[(size_t)i * Ny + j] = OF(&m->rhs, i + g, j + g);
+    }
+  iters = or_mg_solve(gpi, grhs);
+  for (i = 0; i < Nx; i++)
+    for (j = 0; j < Ny; j++) OF(&m->pi, i + g, j + g) = gpi[(size_t)i * Ny + j];
+  free(gpi);
+  free(grhs);
+  or_fill(&m->pi, m, OR_HALO_SCALAR);
+  return iters;
+}
+
+/* pressure.c:652-690: max and rms of rhs - L(pi). */
+static void residual_norms(or_model *m)
+{
+  uint i, j, g = m->g;
+  real mx = 0, ss = 0;
+  or_fill(&m->pi, m, OR_HALO_SCALAR);
+  for (i = 0; i < m->cfg.Nx; i++) {
+    uint il = i + g;
+    for (j = 0; j < m->cfg.Ny; j++) {
+      uint jl = j + g;
+      real Ow = west_op(m, il, jl), Oe = west_op(m, il + 1, jl);
+      real Os = south_op(m, il, jl), On = south_op(m, il, jl + 1);
+      real pc = OF(&m->pi, il, jl);
+      real Lpi = Oe * (OF(&m->pi, il + 1, jl) - pc) + Ow * (OF(&m->pi, il - 1, jl) - pc)
+                 + On * (OF(&m->pi, il, jl + 1) - pc) + Os * (OF(&m->pi, il, jl - 1) - pc);
+      real r = OF(&m->rhs, il, jl) - Lpi;
+      mx = fmax(mx, fabs(r));
+      ss += r * r;
+    }
+  }
+  m->res_max = mx;
+  m->res_rms = sqrt(ss / ((real)m->cfg.Nx * (real)m->cfg.Ny));
+}
+
+/* pressure.c:693-737: barotropic divergence after the projection. */
+static void divergence_norms(or_model *m)
+{
+  uint i, j, k, g = m->g;
+  real mx = 0, ss = 0;
+  load_state(m);
+  face_thickness_no_ghost(m);
+  for (i = 0; i < m->cfg.Nx; i++) {
+    uint il = i + g;
+    for (j = 0; j < m->cfg.Ny; j++) {
+      uint jl = j + g;
+      real dv = 0;
+      for (k = 0; k < m->L; k++) {
+        real hu_w = OF(&m->u[k], il, jl) * OF(&m->hwest[k], il, jl);
+        real hu_e = OF(&m->u[k], il + 1, jl) * OF(&m->hwest[k], il + 1, jl);
+        real hv_s = OF(&m->v[k], il, jl) * OF(&m->hsouth[k], il, jl);
+        real hv_n = OF(&m->v[k], il, jl + 1) * OF(&m->hsouth[k], il, jl + 1);
+        dv += (hu_e - hu_w) / m->dx + (hv_n - hv_s) / m->dy;
+      }
+      mx = fmax(mx, fabs(dv));
+      ss += dv * dv;
+    }
+  }
+  m->div_max = mx;
+  m->div_rms = sqrt(ss / ((real)m->cfg.Nx * (real)m->cfg.Ny));
+}
+
+/* pressure.c:2182-2248. */
+static void apply_correction(or_model *m)
+{
+  uint i, j, k, g = m->g;
+  for (k = 0; k < m->L; k++) {
+    for (i = 0; i < m->cfg.Nx; i++) {
+      uint il = i + g;
+      if (m->cfg.wallEW && i == 0) continue;
+      for (j = 0; j < m->cfg.Ny; j++) {
+        uint jl = j + g;
+        real r = -m->cfg.dt * (OF(&m->pi, il, jl) - OF(&m->pi, il - 1, jl)) / m->dx;
+        OF(&m->u[k], il, jl) += r;
+      }
+    }
+    for (j = 0; j < m->cfg.Ny; j++) {
+      uint jl = j + g;
+      if (m->cfg.wallNS && j == 0) continue;
+      for (i = 0; i < m->cfg.Nx; i++) {
+        uint il = i + g;
+        real r = -m->cfg.dt * (OF(&m->pi, il, jl) - OF(&m->pi, il, jl - 1)) / m->dy;
+        OF(&m->v[k], il, jl) += r;
+      }
+    }
+  }
+  fill_prognostic(m, m->u, m->v, m->h);
+}
+
+/* pawsim_pressure_correct_rigid_lid (pressure.c:2254-2300). */
+uint or_pressure_correct(or_model *m)
+{
+  uint iters;
+  double t0 = now_seconds(), t1, t2;
+  static_geometry(m);
+  correct_thickness(m);
+  build_rhs(m);
+  t1 = now_seconds();
+  if (m->cfg.solver == OR_SOLVER_MG) iters = solve_mg(m);
+  else iters = solve_sor(m);
+  t2 = now_seconds();
+  if (m->cfg.pressureTiming) residual_norms(m);
+  apply_correction(m);
+  if (m->cfg.pressureTiming) divergence_norms(m);
+  m->solve_seconds += t2 - t1;
+  m->correct_seconds += now_seconds() - t0;
+  m->solves++;
+  m->iters_total += iters;
+  m->last_iters = (int)iters;
+  return iters;
+}
+
+/* ------------------------------------------------------------------ */
+/* output (diagnostics.c:84-92, 874-972)                                */
+/* ------------------------------------------------------------------ */
+
+static bool write_named(const char *dir, const char *name, const or_field *f)
+{
+  char path[1024];
+  snprintf(path, sizeof(path), "%s/%s", dir, name);
+  return or_write_field(path, f);
+}
+
+bool or_write_state(or_model *m, const char *outdir, uint n)
+{
+  uint k;
+  char name[256];
+  or_field zero;
+  bool ok = true;
+  or_field_alloc(&zero, m->cfg.Nx, m->cfg.Ny, m->g);
+  for (k = 0; k < m->L && ok; k++) {
+    snprintf(name, sizeof(name), "U%u_n=%u.dat", k, n);
+    ok = ok && write_named(outdir, name, &m->u[k]);
+    snprintf(name, sizeof(name), "V%u_n=%u.dat", k, n);
+    ok = ok && write_named(outdir, name, &m->v[k]);
+    snprintf(name, sizeof(name), "H%u_n=%u.dat", k, n);
+    ok = ok && write_named(outdir, name, &m->h[k]);
+    snprintf(name, sizeof(name), "W%u_n=%u.dat", k, n); /* diapycnal velocity: identically zero */
+    ok = ok && write_named(outdir, name, &zero);
+  }
+  snprintf(name, sizeof(name), "W%u_n=%u.dat", m->L, n);
+  ok = ok && write_named(outdir, name, &zero);
+  snprintf(name, sizeof(name), "P_n=%u.dat", n);
+  ok = ok && write_named(outdir, name, &m->pi);
+  or_field_free(&zero);
+  return ok;
+}
+
+bool or_write_init_diagnostics(or_model *m, const char *outdir)
+{
+  uint k;
+  char name[256];
+  bool ok = write_named(outdir, "init_Hc.dat", &m->Hc) && write_named(outdir, "init_Hw.dat", &m->Hw)
+            && write_named(outdir, "init_Hs.dat", &m->Hs);
+  for (k = 0; k < m->L + 1 && ok; k++) {
+    snprintf(name, sizeof(name), "init_eta%u_n=0.dat", k);
+    ok = write_named(outdir, name, &m->eta[k]);
+  }
+  for (k = 0; k < m->L && ok; k++) {
+    const struct { const char *p; or_field *f; } t[] = {
+      {"init_hwest", &m->hwest[k]}, {"init_hsouth", &m->hsouth[k]}, {"init_dudt", &m->dtu[k]},
+      {"init_dvdt", &m->dtv[k]}, {"init_dhdt", &m->dth[k]}, {"init_zeta", &m->zeta[k]},
+      {"init_hhq", &m->hhq[k]}, {"init_qq", &m->qq[k]}, {"init_pp", &m->pp[k]}, {"init_KE", &m->KE[k]},
+      {"init_MM", &m->MM[k]}, {"init_alpha", &m->al[k]}, {"init_beta", &m->be[k]},
+      {"init_gamma", &m->ga[k]}, {"init_delta", &m->de[k]}, {"init_epsilon", &m->ep[k]},
+      {"init_phi", &m->ph[k]}, {"init_lambda", &m->la[k]}, {"init_mu", &m->mu[k]}};
+    size_t q;
+    for (q = 0; q < sizeof(t) / sizeof(t[0]) && ok; q++) {
+      snprintf(name, sizeof(name), "%s%u_n=0.dat", t[q].p, k);
+      ok = write_named(outdir, name, t[q].f);
+    }
+  }
+  return ok;
+}

@@ -1,0 +1,302 @@
+"""Deterministic PAWSIM input generator and raw-file I/O.
+
+Writes the AWSIM/PAWSIM key/value parameter file and the raw native-endian
+float64 input fields in the reference layouts:
+
+* 2-D fields: Nx x Ny values, x index fastest (defs.c:191-211 readMatrix).
+* Omega_z: (Nx+1) x (Ny+1) q-grid values, x fastest (state.c:440-477).
+* 3-D fields [Nlay, Nx, Ny]: layer index fastest, then x, then y
+  (defs.c:291-309 readMatrix3).
+* gFile: Nlay values.
+
+Outputs written by the model (U{k}_n=N.dat etc.) are Nx x Ny with x fastest
+(defs.c:21-37 printMatrix, io.c:17-129).
+
+Random numbers come from our own splitmix64 stream and Box-Muller, so the
+inputs are reproducible on any machine from the seed alone.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+
+_MASK = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """Return n consecutive outputs of the splitmix64 generator (uint64)."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, n: int) -> np.ndarray:
+    """Uniform doubles in (0, 1) from the top 53 bits of splitmix64."""
+    z = splitmix64(seed, n) >> np.uint64(11)
+    return (z.astype(np.float64) + 0.5) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed: int, n: int) -> np.ndarray:
+    """Standard normal deviates by Box-Muller on splitmix64 uniforms."""
+    m = (n + 1) // 2
+    u = uniform01(seed, 2 * m)
+    u1, u2 = u[0::2], u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    out = np.empty(2 * m)
+    out[0::2] = r * np.cos(2.0 * np.pi * u2)
+    out[1::2] = r * np.sin(2.0 * np.pi * u2)
+    return out[:n]
+
+
+# --------------------------------------------------------------------------
+# raw file I/O in the reference layouts
+# --------------------------------------------------------------------------
+
+def write_2d(path: str, a: np.ndarray) -> None:
+    """a[i, j] (Nx x Ny) -> file with x fastest."""
+    np.ascontiguousarray(np.asarray(a, dtype=np.float64).T).tofile(path)
+
+
+def read_2d(path: str, nx: int, ny: int) -> np.ndarray:
+    """File with x fastest -> a[i, j] (Nx x Ny)."""
+    return np.fromfile(path, dtype=np.float64, count=nx * ny).reshape(ny, nx).T.copy()
+
+
+def write_3d(path: str, a: np.ndarray) -> None:
+    """a[k, i, j] -> file with k fastest, then x, then y."""
+    np.ascontiguousarray(np.asarray(a, dtype=np.float64).transpose(2, 1, 0)).tofile(path)
+
+
+def read_3d(path: str, nl: int, nx: int, ny: int) -> np.ndarray:
+    return np.fromfile(path, dtype=np.float64, count=nl * nx * ny).reshape(ny, nx, nl).transpose(2, 1, 0).copy()
+
+
+# --------------------------------------------------------------------------
+# model setups
+# --------------------------------------------------------------------------
+
+@dataclass
+class Setup:
+    """A complete model input: parameter-file keys plus field arrays."""
+    params: Dict[str, object]
+    hhb: np.ndarray                 # [Nx, Ny] bottom elevation (negative)
+    hhs: Optional[np.ndarray]       # [Nx, Ny] surface elevation (0 under a rigid lid)
+    omegaz: np.ndarray              # [Nx+1, Ny+1]
+    gg: np.ndarray                  # [Nlay]
+    h: np.ndarray                   # [Nlay, Nx, Ny]
+    u: Optional[np.ndarray] = None  # [Nlay, Nx, Ny]
+    v: Optional[np.ndarray] = None
+    taux: Optional[np.ndarray] = None  # [Nx, Ny], kinematic (tau / rho0)
+    tauy: Optional[np.ndarray] = None
+    name: str = "case"
+    notes: Dict[str, object] = field(default_factory=dict)
+
+    @property
+    def nx(self) -> int:
+        return int(self.params["Nx"])
+
+    @property
+    def ny(self) -> int:
+        return int(self.params["Ny"])
+
+    @property
+    def nlay(self) -> int:
+        return int(self.params["Nlay"])
+
+    def write(self, directory: str) -> str:
+        """Write all input files plus '<name>_in' and return its path."""
+        os.makedirs(directory, exist_ok=True)
+        p = dict(self.params)
+        write_2d(os.path.join(directory, "hhb.dat"), self.hhb)
+        p["hbFile"] = "hhb.dat"
+        if self.hhs is not None:
+            write_2d(os.path.join(directory, "hhs.dat"), self.hhs)
+            p["hsFile"] = "hhs.dat"
+        write_2d(os.path.join(directory, "Omegaz.dat"), self.omegaz)
+        p["OmegazFile"] = "Omegaz.dat"
+        np.asarray(self.gg, dtype=np.float64).tofile(os.path.join(directory, "gg.dat"))
+        p["gFile"] = "gg.dat"
+        write_3d(os.path.join(directory, "hInit.dat"), self.h)
+        p["hInitFile"] = "hInit.dat"
+        if self.u is not None:
+            write_3d(os.path.join(directory, "uInit.dat"), self.u)
+            p["uInitFile"] = "uInit.dat"
+        if self.v is not None:
+            write_3d(os.path.join(directory, "vInit.dat"), self.v)
+            p["vInitFile"] = "vInit.dat"
+        if self.taux is not None:
+            write_2d(os.path.join(directory, "taux.dat"), self.taux)
+            p["tauxFile"] = "taux.dat"
+        if self.tauy is not None:
+            write_2d(os.path.join(directory, "tauy.dat"), self.tauy)
+            p["tauyFile"] = "tauy.dat"
+        path = os.path.join(directory, self.name + "_in")
+        with open(path, "w") as f:
+            for k, val in p.items():
+                if isinstance(val, float):
+                    f.write(f"{k} {val!r}\n")
+                else:
+                    f.write(f"{k} {val}\n")
+        return path
+
+
+def _interface_noise(seed: int, nlay: int, nx: int, ny: int, sigma: float) -> np.ndarray:
+    """Perturbations of the nlay-1 internal interfaces, eps[k] for k=1..nlay-1."""
+    eps = np.zeros((nlay + 1, nx, ny))
+    if nlay > 1 and sigma > 0:
+        eps[1:nlay] = sigma * normal(seed, (nlay - 1) * nx * ny).reshape(nlay - 1, nx, ny)
+    return eps
+
+
+def _layers_from_interfaces(H_rest, eps, bottom_adjust=None) -> np.ndarray:
+    """h_k = H_k - eps_k + eps_{k+1}: raising interface k thins layer k-1."""
+    nlay = len(H_rest)
+    nx, ny = eps.shape[1:]
+    h = np.empty((nlay, nx, ny))
+    for k in range(nlay):
+        h[k] = H_rest[k] + eps[k] - eps[k + 1]
+    if bottom_adjust is not None:
+        h[nlay - 1] += bottom_adjust
+    return h
+
+
+def _base_params(nx, ny, nlay, Ly, dt, nt, walls_ew, walls_ns, solver, tol, **extra):
+    p = {
+        "Nlay": nlay, "Nx": nx, "Ny": ny, "Ly": float(Ly), "Lx": float(Ly) * nx / ny,
+        "dt": float(dt), "Nt": int(nt), "savefrequency": float(dt),
+        "timeSteppingScheme": 5, "momentumScheme": 3, "thicknessScheme": 0,
+        "useWallEW": int(walls_ew), "useWallNS": int(walls_ns), "useRL": 1,
+        "pressureSolver": int(solver), "tol": float(tol), "maxiters": 100000,
+    }
+    p.update(extra)
+    return p
+
+
+def gyre(nx=128, ny=128, nlay=2, dx=20e3, nt=1000, dt=300.0, solver=0, tol=1e-10,
+         seed=1, sigma=1.0, tau0=0.1, rho0=1025.0, H_rest=(500.0, 3500.0),
+         rho=(1025.0, 1027.0), f0=1e-4, beta=2e-11, name="gyre", **extra) -> Setup:
+    """Closed wind-driven double gyre on a beta plane (2-layer gyre config).
+
+    Omega_z = f/2 on the q grid (setparams_gyre.m:285); wind is kinematic
+    tau/rho0 (setparams_gyre.m:295); gg = [g, g'] (state.c:986).
+    """
+    Ly = ny * dx
+    Lx = nx * dx
+    H = float(sum(H_rest))
+    extra.setdefault("SOR_rp", 1.9)
+    p = _base_params(nx, ny, nlay, Ly, dt, nt, 1, 1, solver, tol, **extra)
+    hhb = -H * np.ones((nx, ny))
+    yq = np.arange(ny + 1) * dx
+    omegaz = 0.5 * (f0 + beta * (yq - Ly / 2.0))[None, :] * np.ones((nx + 1, 1))
+    g = 9.81
+    gg = np.empty(nlay)
+    gg[0] = g
+    for k in range(1, nlay):
+        gg[k] = g * (rho[k] - rho[k - 1]) / rho0
+    eps = _interface_noise(seed, nlay, nx, ny, sigma)
+    h = _layers_from_interfaces(list(H_rest), eps)
+    yu = (np.arange(ny) + 0.5) * dx
+    taux = (tau0 * np.cos(2.0 * np.pi * yu / Ly) / rho0)[None, :] * np.ones((nx, 1))
+    del Lx
+    return Setup(p, hhb, None, omegaz, gg, h, taux=taux, name=name,
+                 notes={"seed": seed, "sigma": sigma, "tau0": tau0})
+
+
+def channel(nx=512, ny=256, nlay=3, dx=10e3, nt=2000, dt=200.0, solver=1, tol=1e-10,
+            seed=2, sigma=0.5, tau0=0.2, rho0=1027.0, H_rest=(400.0, 1100.0, 2500.0),
+            ridge_h=1500.0, ridge_w=100e3, depth=4000.0, f0=-1e-4, beta=1.5e-11,
+            gprime=(9.81, 0.01, 0.005), name="channel", **extra) -> Setup:
+    """Re-entrant channel, periodic in x with walls in y, Gaussian ridge.
+
+    The bottom layer absorbs the topography: h_bottom = H_rest[-1] + (hb + depth)
+    with hb = -depth + ridge (setparams_ACC.m:136-137 style ridge).
+    """
+    Ly = ny * dx
+    Lx = nx * dx
+    p = _base_params(nx, ny, nlay, Ly, dt, nt, 0, 1, solver, tol, **extra)
+    x = (np.arange(nx) + 0.5) * dx
+    ridge = ridge_h * np.exp(-((x - Lx / 2.0) / ridge_w) ** 2)
+    hhb = (-depth + ridge)[:, None] * np.ones((1, ny))
+    yq = np.arange(ny + 1) * dx
+    omegaz = 0.5 * (f0 + beta * (yq - Ly / 2.0))[None, :] * np.ones((nx + 1, 1))
+    gg = np.array(gprime[:nlay], dtype=np.float64)
+    eps = _interface_noise(seed, nlay, nx, ny, sigma)
+    h = _layers_from_interfaces(list(H_rest), eps, bottom_adjust=-ridge[:, None] * np.ones((1, ny)))
+    yu = (np.arange(ny) + 0.5) * dx
+    taux = (tau0 * np.sin(np.pi * yu / Ly) ** 2 / rho0)[None, :] * np.ones((nx, 1))
+    return Setup(p, hhb, None, omegaz, gg, h, taux=taux, name=name,
+                 notes={"seed": seed, "sigma": sigma, "tau0": tau0})
+
+
+def jets(nx=1024, ny=1024, nlay=4, dx=5e3, nt=500, dt=300.0, solver=3, tol=1e-8,
+         seed=3, sigma=1.0, tau0=0.1, rho0=1025.0, H_rest=(250.0, 500.0, 1250.0, 2000.0),
+         topo_rms=200.0, nmodes=8, f0=1e-4, beta=2e-11,
+         gprime=(9.81, 0.02, 0.01, 0.005), name="jets", **extra) -> Setup:
+    """Doubly bounded zonal-jets domain with random-phase smooth topography.
+
+    Topography: the 8 lowest cosine modes (kx, ky) != (0, 0) ordered by
+    kx^2+ky^2 then kx, with seeded random phases, scaled to the given rms.
+    The bottom layer absorbs it.  Wind: four alternating zonal jets.
+    """
+    Ly = ny * dx
+    Lx = nx * dx
+    p = _base_params(nx, ny, nlay, Ly, dt, nt, 1, 1, solver, tol, **extra)
+    modes = sorted(((kx, ky) for kx in range(4) for ky in range(4) if (kx, ky) != (0, 0)),
+                   key=lambda m: (m[0] ** 2 + m[1] ** 2, m[0]))[:nmodes]
+    ph = 2.0 * np.pi * uniform01(seed + 1000, 2 * nmodes)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    topo = np.zeros((nx, ny))
+    for m, (kx, ky) in enumerate(modes):
+        topo += np.cos(np.pi * kx * x + ph[2 * m])[:, None] * np.cos(np.pi * ky * y + ph[2 * m + 1])[None, :]
+    topo *= topo_rms / np.sqrt(np.mean(topo ** 2))
+    depth = float(sum(H_rest))
+    hhb = -depth + topo
+    yq = np.arange(ny + 1) * dx
+    omegaz = 0.5 * (f0 + beta * (yq - Ly / 2.0))[None, :] * np.ones((nx + 1, 1))
+    gg = np.array(gprime[:nlay], dtype=np.float64)
+    eps = _interface_noise(seed, nlay, nx, ny, sigma)
+    h = _layers_from_interfaces(list(H_rest), eps, bottom_adjust=-topo)
+    yu = (np.arange(ny) + 0.5) * dx
+    taux = (tau0 * np.sin(4.0 * np.pi * yu / Ly) / rho0)[None, :] * np.ones((nx, 1))
+    del Lx
+    return Setup(p, hhb, None, omegaz, gg, h, taux=taux, name=name,
+                 notes={"seed": seed, "sigma": sigma, "tau0": tau0, "modes": modes})
+
+
+def periodic_box(nx=16, ny=16, nlay=2, dx=20e3, nt=10, dt=300.0, solver=0, tol=1e-10,
+                 seed=4, sigma=2.0, name="periodic", **extra) -> Setup:
+    """Doubly periodic f-plane box with a sheared initial velocity (no wind)."""
+    Ly = ny * dx
+    H_rest = [1000.0 * (k + 1) for k in range(nlay)]
+    p = _base_params(nx, ny, nlay, Ly, dt, nt, 0, 0, solver, tol, **extra)
+    hhb = -float(sum(H_rest)) * np.ones((nx, ny))
+    omegaz = 0.5e-4 * np.ones((nx + 1, ny + 1))
+    gg = np.array([9.81] + [0.02] * (nlay - 1))
+    eps = _interface_noise(seed, nlay, nx, ny, sigma)
+    h = _layers_from_interfaces(H_rest, eps)
+    yy = (np.arange(ny) + 0.5) / ny
+    xx = (np.arange(nx) + 0.5) / nx
+    u = np.zeros((nlay, nx, ny))
+    v = np.zeros((nlay, nx, ny))
+    for k in range(nlay):
+        u[k] = 0.1 * np.sin(2 * np.pi * yy)[None, :] * (k + 1) / nlay
+        v[k] = 0.05 * np.cos(2 * np.pi * xx)[:, None] * np.ones((1, ny))
+    return Setup(p, hhb, None, omegaz, gg, h, u=u, v=v, name=name, notes={"seed": seed})
+
+
+def read_outputs(outdir: str, nx: int, ny: int, nlay: int, n: int, with_p: bool = True) -> Dict[str, np.ndarray]:
+    """Load U/V/H (and P) for save index n as [Nlay, Nx, Ny] arrays."""
+    out = {}
+    for name in ("U", "V", "H"):
+        out[name] = np.stack([read_2d(os.path.join(outdir, f"{name}{k}_n={n}.dat"), nx, ny) for k in range(nlay)])
+    if with_p:
+        out["P"] = read_2d(os.path.join(outdir, f"P_n={n}.dat"), nx, ny)
+    return out
