@@ -1,0 +1,138 @@
+/*
+ * pawsim_gpu.h -- C ABI of the MI355X PAWSIM timestep library (libpawsim_gpu.so).
+ *
+ * The reference is one monolithic C executable with no plugin or FFI layer;
+ * its internal seams for the hot path are the functions cited on each entry
+ * point below.  This header exposes those seams as plain extern "C"
+ * functions on an opaque handle: host arrays in, host arrays out, no
+ * library-specific types beyond the structs declared here.
+ *
+ * Host array layout: the reference's a[i][j] view with j (y) contiguous,
+ * i.e. element (i, j) of an Nx x Ny field at index i*Ny + j; layered fields
+ * are [Nlay][Nx][Ny].  Omega_z is (Nx+1) x (Ny+1) at index i*(Ny+1) + j.
+ *
+ * Return codes: 0 on success, negative on error (see PG_E*); the message of
+ * the last error on a handle is available from pg_last_error().  A pressure
+ * solve that reaches maxiters returns PG_ENOCONV and still fills its stats
+ * (pawsim.c:487-491 warns and continues).
+ *
+ * Threading: a handle owns one HIP stream; calls on one handle must be
+ * serialised by the caller.  Different handles are independent.
+ */
+#ifndef PAWSIM_GPU_H
+#define PAWSIM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_OK 0
+#define PG_EARG (-1)        /* invalid argument or unsupported configuration */
+#define PG_EHIP (-2)        /* HIP runtime error */
+#define PG_ENOCONV (-3)     /* pressure solve hit maxiters */
+#define PG_ENODEV (-4)      /* no usable GPU */
+
+/* Pressure solver identifiers.  0 and 1 follow config.h:13-15; the GPU
+ * multigrid variants are additions with no reference counterpart. */
+#define PG_SOLVER_SOR 0        /* red-black SOR, pressure.c:743-814 (bitwise replica) */
+#define PG_SOLVER_MG_REPLICA 1 /* gathered multigrid.c V-cycle (bitwise replica) */
+#define PG_SOLVER_GPU_MG 3     /* red-black GS V(nu1,nu2) FP64, relative-residual stop */
+#define PG_SOLVER_GPU_MG_MIXED 4 /* FP32 V-cycles inside FP64 iterative refinement */
+
+typedef struct pg_handle pg_handle;
+
+/* Parameter-file values the timestep reads (config.c:42-122 defaults,
+ * config.h:17-120 fields).  Zero-initialise, then set. */
+typedef struct pg_config {
+  uint32_t Nx, Ny, Nlay;
+  double Lx, Ly, dt;
+  uint32_t timeSteppingScheme; /* 3 AB1, 4 AB2, 5 AB3 (defs.h:18-24) */
+  uint32_t momentumScheme;     /* 0 AL81, 1 HK83, 2 S75e, 3 TW81 (defs.h:27-30) */
+  uint32_t thicknessScheme;    /* 0 AL81 (only centred face thickness is supported) */
+  int32_t useWallEW, useWallNS;
+  double h0, A2, A4, linDragCoeff, quadDragCoeff;
+  uint32_t pressureSolver;     /* PG_SOLVER_* */
+  double pi_tol;               /* SOR/replica-MG absolute tolerance, GPU-MG relative tolerance */
+  double SOR_rp, omega_MG;
+  uint32_t maxiters;
+  int32_t use_fullMG;
+  int32_t pressureTiming;      /* compute residual/divergence norms per solve */
+  uint32_t mg_nu1, mg_nu2;     /* GPU MG smoothing counts (0 -> 2, 2) */
+} pg_config;
+
+/* Static inputs (state.c:1012-1047).  NULL hhs means a flat lid at 0;
+ * NULL taux/tauy means no wind. */
+typedef struct pg_static {
+  const double *hhb;     /* [Nx][Ny] bottom elevation */
+  const double *hhs;     /* [Nx][Ny] surface elevation or NULL */
+  const double *omega_z; /* [Nx+1][Ny+1] */
+  const double *gg;      /* [Nlay] reduced gravities */
+  const double *taux;    /* [Nx][Ny] kinematic wind stress or NULL */
+  const double *tauy;
+} pg_static;
+
+/* Per-solve statistics (context.h:52-61 plus the relative residual). */
+typedef struct pg_solve_stats {
+  uint32_t iters;
+  uint32_t solver;
+  uint32_t fallback;         /* solver fell back to plain FP64 MG cycles */
+  double residual_max;       /* max |rhs - L pi| (pressure.c:652-690) */
+  double residual_rms;
+  double divergence_max;     /* max |sum_k div(h u)| after correction (pressure.c:693-737) */
+  double divergence_rms;
+  double rel_residual;       /* ||rhs - L pi||_2 / ||rhs||_2 */
+  double solve_ms;           /* device time of the solve alone */
+} pg_solve_stats;
+
+/* Build a handle: allocate device state, upload statics, build the static
+ * geometry (work.c:1124-1167) and, for MG solvers, the level hierarchy
+ * (multigrid.c:196-446).  Replaces the allocation/initialisation sequence of
+ * pawsim.c:391-446 and state.c:957-1100. */
+int pg_create(const pg_config *cfg, const pg_static *st, pg_handle **out);
+void pg_destroy(pg_handle *h);
+const char *pg_last_error(const pg_handle *h);
+
+/* Prognostic state transfer; upload also refreshes halos
+ * (pawsim_state_exchange_prognostic_halos, state.c:929-950). */
+int pg_upload_state(pg_handle *h, const double *u, const double *v, const double *hh);
+int pg_download_state(pg_handle *h, double *u, double *v, double *hh);
+int pg_upload_pi(pg_handle *h, const double *pi);
+int pg_download_pi(pg_handle *h, double *pi);
+
+/* Evaluate the tendency of the current state (pawsim_tendency_eval,
+ * tendency.h:20-22) into host arrays; a validation hook. */
+int pg_tendency(pg_handle *h, double *du, double *dv, double *dh);
+
+/* One explicit step: tendency + AB1/AB2/AB3 update with the reference's
+ * startup sequence, t += dt, halo refresh (pawsim_integrator_step,
+ * integrator.h:20, integrator.c:138-198). */
+int pg_step(pg_handle *h, uint32_t n);
+
+/* Rigid-lid correction: thickness fix, RHS, solve, velocity projection,
+ * optional norms (pawsim_pressure_correct_rigid_lid, pressure.h:13-16,
+ * pressure.c:2254-2300). */
+int pg_pressure_correct(pg_handle *h, pg_solve_stats *stats);
+
+/* Standalone pressure solve of L pi = rhs on host arrays [Nx][Ny] with the
+ * handle's operator; pi is the initial guess and the result
+ * (solve_MG, multigrid.h:24; solve_pressure_sor, pressure.c:743). */
+int pg_solve_pressure(pg_handle *h, uint32_t solver, double *pi, const double *rhs, pg_solve_stats *stats);
+
+/* Model time (ctx->t) and stream control. */
+double pg_time(const pg_handle *h);
+int pg_set_stream(pg_handle *h, void *hip_stream); /* NULL restores the handle's own stream */
+int pg_synchronize(pg_handle *h);
+
+/* Benchmark hooks: launch one hot kernel `reps` times on the handle's stream
+ * without synchronising.  which: 0 tendency, 1 AB update, 2 RHS build,
+ * 3 velocity correction, 4 one GPU-MG V-cycle, 5 one RB-GS sweep on the
+ * finest level, 6 one FP32 V-cycle. */
+int pg_bench_kernel(pg_handle *h, int which, int reps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,440 @@
+// pg_api.hip -- extern "C" entry points of libpawsim_gpu.so.
+//
+// Sequencing follows the reference driver: pawsim_integrator_step
+// (integrator.c:138-198) for pg_step and pawsim_pressure_correct_rigid_lid
+// (pressure.c:2254-2300) for pg_pressure_correct.  All numerical work runs
+// in HIP kernels on the handle's stream; the host only marshals arrays
+// between the caller's no-ghost layout and the ghosted device layout.
+#include "pg_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+
+namespace {
+
+constexpr int G_WIDTH = 3;  // ghost width of model fields (pawsim.c:28)
+
+int fail(pg_handle* h, int code, const std::string& msg)
+{
+  if (h) h->err = msg;
+  return code;
+}
+
+// Host [Nx][Ny] (row stride ld) <-> ghosted device field (owned cells only).
+int upload_owned(pg_handle* h, double* dst, const double* src, int ld, std::vector<double>& stage)
+{
+  const PgGrid& G = h->G;
+  stage.assign(h->fsz, 0.0);
+  for (int i = 0; i < G.nx; i++)
+    for (int j = 0; j < G.ny; j++) stage[pg_at(G, i + G.g, j + G.g)] = src[(size_t)i * ld + j];
+  PG_CHECK(h, hipMemcpyAsync(dst, stage.data(), h->fsz * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
+
+int download_owned(pg_handle* h, double* dst, const double* src, std::vector<double>& stage)
+{
+  const PgGrid& G = h->G;
+  stage.resize(h->fsz);
+  PG_CHECK(h, hipMemcpyAsync(stage.data(), src, h->fsz * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  for (int i = 0; i < G.nx; i++)
+    for (int j = 0; j < G.ny; j++) dst[(size_t)i * G.ny + j] = stage[pg_at(G, i + G.g, j + G.g)];
+  return PG_OK;
+}
+
+int validate(const pg_config* c, std::string& why)
+{
+  if (c->Nx < 4 || c->Ny < 4) return why = "Nx and Ny must be at least 4", PG_EARG;
+  if (c->Nlay == 0 || c->Nlay > PG_MAXL) return why = "Nlay must be in [1, 16]", PG_EARG;
+  if (c->Ly <= 0 || c->dt <= 0) return why = "Ly and dt must be positive", PG_EARG;
+  if (c->timeSteppingScheme < 3 || c->timeSteppingScheme > 5) return why = "only AB1, AB2 and AB3 are supported", PG_EARG;
+  if (c->momentumScheme > 3) return why = "unknown momentumScheme", PG_EARG;
+  if (c->thicknessScheme != 0) return why = "only the AL81 thickness scheme is supported", PG_EARG;
+  if (c->h0 < 0 || c->A2 < 0 || c->A4 < 0 || c->linDragCoeff < 0 || c->quadDragCoeff < 0)
+    return why = "negative physical coefficient", PG_EARG;
+  if (c->pi_tol <= 0 || c->SOR_rp <= 0 || c->SOR_rp >= 2 || c->omega_MG <= 0 || c->maxiters == 0)
+    return why = "invalid pressure solver parameters", PG_EARG;
+  const uint32_t s = c->pressureSolver;
+  if (s != PG_SOLVER_SOR && s != PG_SOLVER_MG_REPLICA && s != PG_SOLVER_GPU_MG && s != PG_SOLVER_GPU_MG_MIXED)
+    return why = "unknown pressureSolver", PG_EARG;
+  return PG_OK;
+}
+
+int ensure_solver(pg_handle* h, uint32_t solver)
+{
+  if (solver == PG_SOLVER_SOR) {
+    if (!h->sor_c) {
+      PG_CHECK(h, hipMalloc(&h->sor_c, 5 * h->fsz * sizeof(double)));
+      PG_CHECK(h, hipMemsetAsync(h->sor_c, 0, 5 * h->fsz * sizeof(double), h->stream));
+      PG_CHECK(h, pg_launch_sor_setup(h));
+    }
+    return PG_OK;
+  }
+  const bool need32 = solver == PG_SOLVER_GPU_MG_MIXED;
+  if (h->mg_built && need32 && !h->lev.back().pi32) pg_mg_free(h);
+  if (!h->mg_built) {
+    const uint32_t keep = h->cfg.pressureSolver;
+    h->cfg.pressureSolver = solver;
+    int rc = pg_mg_build(h);
+    h->cfg.pressureSolver = keep;
+    return rc;
+  }
+  return PG_OK;
+}
+
+// Run the selected solver on h->pi / h->rhs (rhs mean already removed).
+int run_solver(pg_handle* h, uint32_t solver, pg_solve_stats* st)
+{
+  int rc = ensure_solver(h, solver);
+  if (rc) return rc;
+  int iters = 0;
+  double rel = 0;
+  PG_CHECK(h, hipEventRecord(h->ev0, h->stream));
+  if (solver == PG_SOLVER_SOR) rc = pg_sor_solve(h, &iters);
+  else if (solver == PG_SOLVER_MG_REPLICA) rc = pg_mg_replica_solve(h, &iters);
+  else rc = pg_gpu_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel);
+  if (rc) return rc;
+  PG_CHECK(h, hipEventRecord(h->ev1, h->stream));
+  PG_CHECK(h, hipEventSynchronize(h->ev1));
+  float ms = 0;
+  PG_CHECK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (st) {
+    st->iters = (uint32_t)iters;
+    st->solver = solver;
+    st->fallback = 0;
+    st->rel_residual = rel;
+    st->solve_ms = ms;
+  }
+  return (uint32_t)iters >= h->cfg.maxiters ? PG_ENOCONV : PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_create(const pg_config* cfg, const pg_static* st, pg_handle** out)
+{
+  if (!out) return PG_EARG;
+  *out = nullptr;
+  if (!cfg || !st || !st->hhb || !st->omega_z || !st->gg) return PG_EARG;
+  std::string why;
+  if (validate(cfg, why)) {
+    fprintf(stderr, "pg_create: %s\n", why.c_str());
+    return PG_EARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    fprintf(stderr, "pg_create: no HIP device available\n");
+    return PG_ENODEV;
+  }
+  pg_handle* h = new (std::nothrow) pg_handle();
+  if (!h) return PG_EARG;
+  h->cfg = *cfg;
+  pg_config& c = h->cfg;
+  if (c.Lx <= 0) c.Lx = c.Ly * c.Nx / c.Ny;  // pawsim.c:129-132
+  PgGrid& G = h->G;
+  G.nx = (int)c.Nx;
+  G.ny = (int)c.Ny;
+  G.g = G_WIDTH;
+  G.sx = G.nx + 2 * G.g;
+  G.sy = G.ny + 2 * G.g;
+  G.wallx = c.useWallEW ? 1 : 0;
+  G.wally = c.useWallNS ? 1 : 0;
+  G.L = (int)c.Nlay;
+  G.dx = c.Lx / c.Nx;
+  G.dy = c.Ly / c.Ny;
+  PgPhys& P = h->P;
+  P.mom = (int)c.momentumScheme;
+  P.useA2 = c.A2 > 0;
+  P.useA4 = c.A4 > 0;
+  P.useWind = (st->taux != nullptr) || (st->tauy != nullptr);
+  P.useLin = c.linDragCoeff > 0;
+  P.useQuad = c.quadDragCoeff > 0;
+  P.h0 = c.h0;
+  P.A2 = c.A2;
+  P.A4c = std::sqrt(c.A4);
+  P.lin = c.linDragCoeff;
+  P.quad = c.quadDragCoeff;
+  for (int k = 0; k < G.L; k++) P.gg[k] = st->gg[k];
+  P.geff[0] = P.gg[0];  // state.c:1082-1086
+  for (int k = 1; k < G.L; k++) P.geff[k] = P.geff[k - 1] + P.gg[k];
+  for (int k = 0; k < G.L; k++)
+    if (P.gg[k] < 0) {
+      delete h;
+      fprintf(stderr, "pg_create: gg may contain only non-negative values\n");
+      return PG_EARG;
+    }
+  h->fsz = (size_t)G.sx * G.sy;
+  *out = h;
+  PG_CHECK(h, hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
+  h->stream = h->own_stream;
+  PG_CHECK(h, hipEventCreate(&h->ev0));
+  PG_CHECK(h, hipEventCreate(&h->ev1));
+  const size_t lf = (size_t)G.L * h->fsz * sizeof(double);
+  double** layered[] = {&h->u, &h->v, &h->h, &h->fu[0], &h->fu[1], &h->fu[2], &h->fv[0], &h->fv[1], &h->fv[2],
+                        &h->fh[0], &h->fh[1], &h->fh[2]};
+  for (double** p : layered) {
+    PG_CHECK(h, hipMalloc(p, lf));
+    PG_CHECK(h, hipMemsetAsync(*p, 0, lf, h->stream));
+  }
+  double** single[] = {&h->hhb, &h->hhs, &h->omz, &h->taux, &h->tauy, &h->Hc, &h->Hw, &h->Hs, &h->pi, &h->rhs};
+  for (double** p : single) {
+    PG_CHECK(h, hipMalloc(p, h->fsz * sizeof(double)));
+    PG_CHECK(h, hipMemsetAsync(*p, 0, h->fsz * sizeof(double), h->stream));
+  }
+  // reduction partials, followed by room for up to 64 level descriptors
+  h->npartials = ((G.nx + 3) / 4) * ((G.ny + 63) / 64) * 2 + 4096;
+  PG_CHECK(h, hipMalloc(&h->partials, h->npartials * sizeof(double) + 64 * sizeof(MgLevel)));
+  PG_CHECK(h, hipMalloc(&h->ctl, sizeof(PgCtl)));
+  PG_CHECK(h, hipMemsetAsync(h->ctl, 0, sizeof(PgCtl), h->stream));
+  PG_CHECK(h, hipHostMalloc(&h->ctl_host, sizeof(PgCtl), hipHostMallocDefault));
+
+  // Static inputs with their halo kinds (state.c:1049-1055).
+  std::vector<double> stage;
+  int rc = upload_owned(h, h->hhb, st->hhb, G.ny, stage);
+  if (!rc && st->hhs) rc = upload_owned(h, h->hhs, st->hhs, G.ny, stage);
+  if (!rc && st->taux) rc = upload_owned(h, h->taux, st->taux, G.ny, stage);
+  if (!rc && st->tauy) rc = upload_owned(h, h->tauy, st->tauy, G.ny, stage);
+  if (rc) return rc;
+  {
+    // Omega_z: (Nx+1) x (Ny+1) q grid; the extra east/north line occupies the
+    // first ghost column/row on wall directions (state.c:207-238).
+    const int qx = G.wallx, qy = G.wally;
+    stage.assign(h->fsz, 0.0);
+    for (int i = 0; i < G.nx + qx; i++)
+      for (int j = 0; j < G.ny + qy; j++) stage[pg_at(G, i + G.g, j + G.g)] = st->omega_z[(size_t)i * (G.ny + 1) + j];
+    PG_CHECK(h, hipMemcpyAsync(h->omz, stage.data(), h->fsz * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    PG_CHECK(h, hipStreamSynchronize(h->stream));
+  }
+  PG_CHECK(h, pg_fill(h, h->hhb, 1, PG_HALO_SCALAR));
+  PG_CHECK(h, pg_fill(h, h->hhs, 1, PG_HALO_SCALAR));
+  PG_CHECK(h, pg_fill(h, h->omz, 1, PG_HALO_Q));
+  PG_CHECK(h, pg_fill(h, h->taux, 1, PG_HALO_SCALAR));
+  PG_CHECK(h, pg_fill(h, h->tauy, 1, PG_HALO_SCALAR));
+  PG_CHECK(h, pg_launch_static_geometry(h));
+  rc = ensure_solver(h, c.pressureSolver);
+  if (rc) return rc;
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
+
+void pg_destroy(pg_handle* h)
+{
+  if (!h) return;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  pg_mg_free(h);
+  double* bufs[] = {h->u, h->v, h->h, h->fu[0], h->fu[1], h->fu[2], h->fv[0], h->fv[1], h->fv[2], h->fh[0], h->fh[1],
+                    h->fh[2], h->hhb, h->hhs, h->omz, h->taux, h->tauy, h->Hc, h->Hw, h->Hs, h->pi, h->rhs,
+                    h->sor_c, h->partials};
+  for (double* p : bufs) hipFree(p);
+  hipFree(h->ctl);
+  if (h->ctl_host) hipHostFree(h->ctl_host);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->own_stream) hipStreamDestroy(h->own_stream);
+  delete h;
+}
+
+const char* pg_last_error(const pg_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int pg_upload_state(pg_handle* h, const double* u, const double* v, const double* hh)
+{
+  if (!h || !u || !v || !hh) return PG_EARG;
+  std::vector<double> stage;
+  const size_t n = (size_t)h->G.nx * h->G.ny;
+  for (int k = 0; k < h->G.L; k++) {
+    int rc = upload_owned(h, h->u + pg_lay(h->G, k), u + k * n, h->G.ny, stage);
+    if (!rc) rc = upload_owned(h, h->v + pg_lay(h->G, k), v + k * n, h->G.ny, stage);
+    if (!rc) rc = upload_owned(h, h->h + pg_lay(h->G, k), hh + k * n, h->G.ny, stage);
+    if (rc) return rc;
+  }
+  PG_CHECK(h, pg_fill(h, h->u, h->G.L, PG_HALO_U));
+  PG_CHECK(h, pg_fill(h, h->v, h->G.L, PG_HALO_V));
+  PG_CHECK(h, pg_fill(h, h->h, h->G.L, PG_HALO_SCALAR));
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
+
+int pg_download_state(pg_handle* h, double* u, double* v, double* hh)
+{
+  if (!h) return PG_EARG;
+  std::vector<double> stage;
+  const size_t n = (size_t)h->G.nx * h->G.ny;
+  for (int k = 0; k < h->G.L; k++) {
+    int rc = PG_OK;
+    if (u) rc = download_owned(h, u + k * n, h->u + pg_lay(h->G, k), stage);
+    if (!rc && v) rc = download_owned(h, v + k * n, h->v + pg_lay(h->G, k), stage);
+    if (!rc && hh) rc = download_owned(h, hh + k * n, h->h + pg_lay(h->G, k), stage);
+    if (rc) return rc;
+  }
+  return PG_OK;
+}
+
+int pg_upload_pi(pg_handle* h, const double* pi)
+{
+  if (!h || !pi) return PG_EARG;
+  std::vector<double> stage;
+  int rc = upload_owned(h, h->pi, pi, h->G.ny, stage);
+  if (rc) return rc;
+  PG_CHECK(h, pg_fill(h, h->pi, 1, PG_HALO_SCALAR));
+  return PG_OK;
+}
+
+int pg_download_pi(pg_handle* h, double* pi)
+{
+  if (!h || !pi) return PG_EARG;
+  std::vector<double> stage;
+  return download_owned(h, pi, h->pi, stage);
+}
+
+int pg_tendency(pg_handle* h, double* du, double* dv, double* dh)
+{
+  if (!h) return PG_EARG;
+  const int s = h->slot[0];
+  PG_CHECK(h, pg_launch_tendency(h, h->fu[s], h->fv[s], h->fh[s]));
+  std::vector<double> stage;
+  const size_t n = (size_t)h->G.nx * h->G.ny;
+  for (int k = 0; k < h->G.L; k++) {
+    int rc = PG_OK;
+    if (du) rc = download_owned(h, du + k * n, h->fu[s] + pg_lay(h->G, k), stage);
+    if (!rc && dv) rc = download_owned(h, dv + k * n, h->fv[s] + pg_lay(h->G, k), stage);
+    if (!rc && dh) rc = download_owned(h, dh + k * n, h->fh[s] + pg_lay(h->G, k), stage);
+    if (rc) return rc;
+  }
+  return PG_OK;
+}
+
+int pg_step(pg_handle* h, uint32_t n)
+{
+  if (!h || n == 0) return PG_EARG;
+  const int s = h->slot[0];
+  PG_CHECK(h, pg_launch_tendency(h, h->fu[s], h->fv[s], h->fh[s]));
+  const uint32_t scheme = h->cfg.timeSteppingScheme;
+  bool shift = true;
+  int order;
+  if (scheme == 3) {
+    order = 1;
+  } else if (scheme == 4) {
+    order = (n == 1) ? 1 : 2;
+  } else {
+    // AB3 startup: AB1 without history rotation at n == 1 (integrator.c:168-172)
+    order = (n == 1) ? 1 : (n == 2 ? 2 : 3);
+    shift = (n != 1);
+  }
+  PG_CHECK(h, pg_launch_ab(h, order));
+  if (shift) {  // shift_tendency_history (integrator.c:127-132) as a buffer rotation
+    const int t0 = h->slot[0], t1 = h->slot[1], t2 = h->slot[2];
+    h->slot[0] = t2;
+    h->slot[1] = t0;
+    h->slot[2] = t1;
+  }
+  h->t += h->cfg.dt;
+  PG_CHECK(h, pg_fill(h, h->u, h->G.L, PG_HALO_U));
+  PG_CHECK(h, pg_fill(h, h->v, h->G.L, PG_HALO_V));
+  PG_CHECK(h, pg_fill(h, h->h, h->G.L, PG_HALO_SCALAR));
+  return PG_OK;
+}
+
+int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
+{
+  if (!h) return PG_EARG;
+  pg_solve_stats local{};
+  if (!st) st = &local;
+  *st = pg_solve_stats{};
+  const uint32_t solver = h->cfg.pressureSolver;
+  // thickness constraint, then halo refresh (pressure.c:525-561)
+  PG_CHECK(h, pg_launch_thickness(h));
+  PG_CHECK(h, pg_fill(h, h->h, h->G.L, PG_HALO_SCALAR));
+  // RHS with mean removal (pressure.c:570-606)
+  PG_CHECK(h, pg_launch_rhs(h));
+  PG_CHECK(h, pg_launch_serial_mean(h, h->rhs));
+  PG_CHECK(h, pg_launch_sub_mean(h, h->rhs));
+  PG_CHECK(h, pg_fill(h, h->rhs, 1, PG_HALO_SCALAR));
+  int rc = run_solver(h, solver, st);
+  if (rc && rc != PG_ENOCONV) return rc;
+  if (h->cfg.pressureTiming) PG_CHECK(h, pg_launch_norms(h, 0));
+  PG_CHECK(h, pg_launch_correct(h));
+  PG_CHECK(h, pg_fill(h, h->u, h->G.L, PG_HALO_U));
+  PG_CHECK(h, pg_fill(h, h->v, h->G.L, PG_HALO_V));
+  if (h->cfg.pressureTiming) {
+    PG_CHECK(h, pg_launch_norms(h, 1));
+    PG_CHECK(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(PgCtl), hipMemcpyDeviceToHost, h->stream));
+    PG_CHECK(h, hipStreamSynchronize(h->stream));
+    const double N = (double)h->G.nx * (double)h->G.ny;
+    st->residual_max = h->ctl_host->norm_max[0];
+    st->residual_rms = std::sqrt(h->ctl_host->norm_ss[0] / N);
+    st->divergence_max = h->ctl_host->norm_max[1];
+    st->divergence_rms = std::sqrt(h->ctl_host->norm_ss[1] / N);
+  }
+  return rc;
+}
+
+int pg_solve_pressure(pg_handle* h, uint32_t solver, double* pi, const double* rhs, pg_solve_stats* st)
+{
+  if (!h || !pi || !rhs) return PG_EARG;
+  if (solver != PG_SOLVER_SOR && solver != PG_SOLVER_MG_REPLICA && solver != PG_SOLVER_GPU_MG &&
+      solver != PG_SOLVER_GPU_MG_MIXED)
+    return fail(h, PG_EARG, "unknown solver");
+  std::vector<double> stage;
+  int rc = upload_owned(h, h->pi, pi, h->G.ny, stage);
+  if (!rc) rc = upload_owned(h, h->rhs, rhs, h->G.ny, stage);
+  if (rc) return rc;
+  PG_CHECK(h, pg_fill(h, h->pi, 1, PG_HALO_SCALAR));
+  PG_CHECK(h, pg_fill(h, h->rhs, 1, PG_HALO_SCALAR));
+  pg_solve_stats local{};
+  if (!st) st = &local;
+  *st = pg_solve_stats{};
+  rc = run_solver(h, solver, st);
+  if (rc && rc != PG_ENOCONV) return rc;
+  PG_CHECK(h, pg_launch_norms(h, 0));
+  PG_CHECK(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(PgCtl), hipMemcpyDeviceToHost, h->stream));
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  st->residual_max = h->ctl_host->norm_max[0];
+  st->residual_rms = std::sqrt(h->ctl_host->norm_ss[0] / ((double)h->G.nx * (double)h->G.ny));
+  int rc2 = download_owned(h, pi, h->pi, stage);
+  return rc2 ? rc2 : rc;
+}
+
+double pg_time(const pg_handle* h) { return h ? h->t : 0.0; }
+
+int pg_set_stream(pg_handle* h, void* stream)
+{
+  if (!h) return PG_EARG;
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  h->stream = stream ? (hipStream_t)stream : h->own_stream;
+  return PG_OK;
+}
+
+int pg_synchronize(pg_handle* h)
+{
+  if (!h) return PG_EARG;
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
+
+int pg_bench_kernel(pg_handle* h, int which, int reps)
+{
+  if (!h || reps < 0) return PG_EARG;
+  if ((which == 4 || which == 5) && ensure_solver(h, PG_SOLVER_GPU_MG)) return PG_EHIP;
+  if (which == 6 && ensure_solver(h, PG_SOLVER_GPU_MG_MIXED)) return PG_EHIP;
+  const int s = h->slot[0];
+  for (int r = 0; r < reps; r++) {
+    hipError_t e;
+    switch (which) {
+      case 0: e = pg_launch_tendency(h, h->fu[s], h->fv[s], h->fh[s]); break;
+      case 1: e = pg_launch_ab(h, 3); break;
+      case 2: e = pg_launch_rhs(h); break;
+      case 3: e = pg_launch_correct(h); break;
+      case 4: e = pg_gpu_mg_vcycle(h); break;
+      case 5: e = pg_gpu_mg_sweep(h); break;
+      case 6: e = pg_gpu_mg_vcycle32(h); break;
+      default: return fail(h, PG_EARG, "unknown kernel id");
+    }
+    PG_CHECK(h, e);
+  }
+  return PG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// pg_internal.h -- device-side handle and kernel launchers of libpawsim_gpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "pawsim_gpu.h"
+#include "pg_point.h"
+
+enum { PG_HALO_SCALAR = 0, PG_HALO_Q = 1, PG_HALO_U = 2, PG_HALO_V = 3 };
+
+// One multigrid level: no-ghost Nx x Ny arrays, element (i,j) at i*Ny + j
+// (the multigrid.c layout).  Coefficients follow multigrid.c:311-442.
+struct MgLevel {
+  int Nx, Ny, pow2;      // pow2: this level transfers 2:1 to the next coarser one
+  double dx, dy;
+  double *pi, *tmp, *rhs, *Hc, *Ow, *Os, *Osum, *rOsum, *wmm, *wmp, *wpm, *wpp;
+  float *pi32, *tmp32, *rhs32, *Ow32, *Os32, *rOsum32;
+};
+
+// Device control block of iterative solvers (single allocation).
+struct PgCtl {
+  double maxres;         // last max-norm criterion value
+  double sumsq;          // last sum of squared residuals
+  double rhs_sumsq;      // sum of squared rhs
+  double mean;           // last serial-order mean
+  double norm_max[2];    // residual / divergence max
+  double norm_ss[2];     // residual / divergence sum of squares
+  unsigned long long maxbits; // bit pattern of a non-negative double max
+  int iters;
+  int done;
+};
+
+struct pg_handle {
+  pg_config cfg;
+  PgGrid G;
+  PgPhys P;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  size_t fsz = 0;                      // sx * sy
+  double *u = nullptr, *v = nullptr, *h = nullptr;   // [L][fsz] state, halos filled
+  double *fu[3] = {}, *fv[3] = {}, *fh[3] = {};      // AB history buffers
+  int slot[3] = {0, 1, 2};             // buffers holding tendency, tendency_1, tendency_2
+  double *hhb = nullptr, *hhs = nullptr, *omz = nullptr, *taux = nullptr, *tauy = nullptr;
+  double *Hc = nullptr, *Hw = nullptr, *Hs = nullptr;  // ghosted static geometry
+  double *pi = nullptr, *rhs = nullptr;                // ghosted
+  double *sor_c = nullptr;             // [5][fsz] SOR coefficients: W, E, S, N, w/Osum
+  double *partials = nullptr;          // reduction scratch
+  int npartials = 0;
+  PgCtl *ctl = nullptr;                // device control block
+  PgCtl *ctl_host = nullptr;           // pinned mirror
+  std::vector<MgLevel> lev;            // [0] = coarsest
+  bool mg_built = false;
+  double t = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+};
+
+#define PG_CHECK(h, call)                                                           \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+      return PG_EHIP;                                                               \
+    }                                                                               \
+  } while (0)
+
+// ---- launchers (pg_kernels.hip) ----
+hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind);
+hipError_t pg_fill1(pg_handle* h, double* f);
+hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh);
+hipError_t pg_launch_ab(pg_handle* h, int order);
+hipError_t pg_launch_thickness(pg_handle* h);
+hipError_t pg_launch_rhs(pg_handle* h);
+hipError_t pg_launch_serial_mean(pg_handle* h, double* f);
+hipError_t pg_launch_sub_mean(pg_handle* h, double* f);
+hipError_t pg_launch_correct(pg_handle* h);
+hipError_t pg_launch_norms(pg_handle* h, int which);
+hipError_t pg_launch_static_geometry(pg_handle* h);
+hipError_t pg_launch_sor_setup(pg_handle* h);
+int pg_sor_solve(pg_handle* h, int* iters);
+
+// ---- multigrid (pg_mg.hip) ----
+int pg_mg_build(pg_handle* h);
+void pg_mg_free(pg_handle* h);
+int pg_mg_replica_solve(pg_handle* h, int* iters);
+int pg_gpu_mg_solve(pg_handle* h, int mixed, int* iters, double* rel);
+hipError_t pg_gpu_mg_vcycle(pg_handle* h);
+hipError_t pg_gpu_mg_vcycle32(pg_handle* h);
+hipError_t pg_gpu_mg_sweep(pg_handle* h);
+hipError_t pg_gather_to_mg(pg_handle* h);
+hipError_t pg_scatter_from_mg(pg_handle* h);

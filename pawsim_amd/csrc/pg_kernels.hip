@@ -1,0 +1,603 @@
+// pg_kernels.hip -- stencil, update, projection and SOR kernels for gfx950.
+//
+// All fields use the reference's ghosted layout (element (i,j) of a field
+// at i*sy + j, j contiguous, ghost width 3), so threads of a wavefront run
+// along j and every load/store is a coalesced row segment.  The code is
+// compiled with -ffp-contract=off: each FP64 expression is evaluated with
+// the association of the reference line it cites, so results are bitwise
+// equal to the serial build.
+#include "pg_internal.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int TJ = 64;  // threads along j (one wavefront)
+constexpr int TI = 4;   // rows of i per workgroup
+
+// ---------------------------------------------------------------------------
+// Halo fills in the serial order of field.c:416-449/528-531: periodic y,
+// periodic x (full columns, corners included), wall y (owned columns; all
+// columns for q-kind), wall x (all rows).  One workgroup per field; each
+// phase is independent per column (y phases) or per row (x phases), and a
+// thread performs its column/row in the reference's loop order.
+// ---------------------------------------------------------------------------
+__global__ void k_fill(double* base, size_t stride, PgGrid G, int kind)
+{
+  double* f = base + (size_t)blockIdx.x * stride;
+  const int g = G.g, nx = G.nx, ny = G.ny, sx = G.sx, sy = G.sy;
+  auto F = [&](int i, int j) -> double& { return f[(size_t)i * sy + j]; };
+  if (!G.wally) {  // periodic y over owned columns (field.c:121-134)
+    for (int i = g + threadIdx.x; i < g + nx; i += blockDim.x)
+      for (int j = 0; j < g; j++) {
+        F(i, j) = F(i, ny + j);
+        F(i, g + ny + j) = F(i, g + j);
+      }
+    __syncthreads();
+  }
+  if (!G.wallx) {  // periodic x over full columns (field.c:137-150)
+    for (int j = threadIdx.x; j < sy; j += blockDim.x)
+      for (int i = 0; i < g; i++) {
+        F(i, j) = F(nx + i, j);
+        F(g + nx + i, j) = F(g + i, j);
+      }
+    __syncthreads();
+  }
+  if (G.wally) {  // wall y (field.c:158-198)
+    const int i0 = (kind == PG_HALO_Q) ? 0 : g, i1 = (kind == PG_HALO_Q) ? sx : g + nx;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+      if (kind == PG_HALO_V) {
+        F(i, g) = 0;
+        F(i, g + ny) = 0;
+      }
+      for (int j = 0; j < g; j++) {
+        if (kind == PG_HALO_Q) {
+          F(i, j) = F(i, g);
+          F(i, g + ny + j) = F(i, g + ny);
+        } else if (kind == PG_HALO_V) {
+          F(i, j) = -F(i, 2 * g - j);
+          F(i, g + ny + j) = -F(i, g + ny - j);
+        } else {
+          F(i, j) = F(i, 2 * g - j - 1);
+          F(i, g + ny + j) = F(i, g + ny - j - 1);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (G.wallx) {  // wall x (field.c:204-241)
+    for (int j = threadIdx.x; j < sy; j += blockDim.x) {
+      if (kind == PG_HALO_U) {
+        F(g, j) = 0;
+        F(g + nx, j) = 0;
+      }
+      for (int i = 0; i < g; i++) {
+        if (kind == PG_HALO_Q) {
+          F(i, j) = F(g, j);
+          F(g + nx + i, j) = F(g + nx, j);
+        } else if (kind == PG_HALO_U) {
+          F(i, j) = -F(2 * g - i, j);
+          F(g + nx + i, j) = -F(g + nx - i, j);
+        } else {
+          F(i, j) = F(2 * g - i - 1, j);
+          F(g + nx + i, j) = F(g + nx - i - 1, j);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tendency: one thread per (layer, owned point); the stage results of
+// tendency.c:49-75 are evaluated on the fly by pg_tendency_point.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TJ* TI) void k_tendency(PgGrid G, PgPhys P, PgIn I, double* du, double* dv, double* dh)
+{
+  const int j = G.g + blockIdx.x * TJ + threadIdx.x;
+  const int i = G.g + blockIdx.y * TI + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= G.g + G.nx || j >= G.g + G.ny) return;
+  double a, b, c;
+  pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
+  const size_t p = pg_lay(G, k) + pg_at(G, i, j);
+  du[p] = a;
+  dv[p] = b;
+  dh[p] = c;
+}
+
+// ---------------------------------------------------------------------------
+// Adams-Bashforth update of u, v, h in place (integrator.c:15-70).
+// ---------------------------------------------------------------------------
+struct AbArgs {
+  double* x[3];
+  const double* f0[3];
+  const double* f1[3];
+  const double* f2[3];
+  double c0, c1, c2;
+  int order;
+};
+
+__global__ __launch_bounds__(TJ* TI) void k_ab(PgGrid G, AbArgs A)
+{
+  const int j = G.g + blockIdx.x * TJ + threadIdx.x;
+  const int i = G.g + blockIdx.y * TI + threadIdx.y;
+  const int k = blockIdx.z % G.L, fld = blockIdx.z / G.L;
+  if (i >= G.g + G.nx || j >= G.g + G.ny) return;
+  const size_t p = pg_lay(G, k) + pg_at(G, i, j);
+  double x = A.x[fld][p];
+  if (A.order == 1) x += A.c0 * A.f0[fld][p];
+  else if (A.order == 2) x += A.c0 * A.f0[fld][p] - A.c1 * A.f1[fld][p];
+  else x += A.c0 * A.f0[fld][p] - A.c1 * A.f1[fld][p] + A.c2 * A.f2[fld][p];
+  A.x[fld][p] = x;
+}
+
+// ---------------------------------------------------------------------------
+// Rigid-lid thickness fix (pressure.c:525-561): rescale every layer so that
+// the column sum equals Hc.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TJ* TI) void k_thickness(PgGrid G, double* h, const double* Hc)
+{
+  const int j = G.g + blockIdx.x * TJ + threadIdx.x;
+  const int i = G.g + blockIdx.y * TI + threadIdx.y;
+  if (i >= G.g + G.nx || j >= G.g + G.ny) return;
+  const size_t p = pg_at(G, i, j);
+  double tot = 0;
+  for (int k = 0; k < G.L; k++) tot += h[pg_lay(G, k) + p];
+  if (tot == 0) return;
+  const double herr = tot - Hc[p];
+  for (int k = 0; k < G.L; k++) h[pg_lay(G, k) + p] *= (1 - herr / tot);
+}
+
+// No-ghost face thickness with modular wrap (work.c:865-896), read back
+// through the scalar halo fill of work.c:1112-1113 for the one ghost face
+// (i = Nx or j = Ny) the divergence stencil touches.
+__device__ __forceinline__ double face_w_ng(const PgGrid& G, const double* h, int k, int i, int j)
+{
+  // i, j: owned indices 0..Nx (i == Nx is the east ghost face)
+  if (i == G.nx) i = G.wallx ? G.nx - 1 : 0;
+  const int im1 = (i + G.nx - 1) % G.nx;
+  const size_t L = pg_lay(G, k);
+  return 0.5 * (h[L + pg_at(G, i + G.g, j + G.g)] + h[L + pg_at(G, im1 + G.g, j + G.g)]);
+}
+__device__ __forceinline__ double face_s_ng(const PgGrid& G, const double* h, int k, int i, int j)
+{
+  if (j == G.ny) j = G.wally ? G.ny - 1 : 0;
+  const int jm1 = (j + G.ny - 1) % G.ny;
+  const size_t L = pg_lay(G, k);
+  return 0.5 * (h[L + pg_at(G, i + G.g, j + G.g)] + h[L + pg_at(G, i + G.g, jm1 + G.g)]);
+}
+
+// Barotropic divergence sum_k div(h u) at owned point (i,j): the RHS form
+// of pressure.c:585-597 (scale = dx*dt, dy*dt) or the diagnostic form of
+// pressure.c:716-726 (scale = dx, dy).
+__device__ __forceinline__ double divergence(const PgGrid& G, const double* u, const double* v, const double* h,
+                                             int i, int j, double sx_, double sy_)
+{
+  const int il = i + G.g, jl = j + G.g;
+  double acc = 0;
+  for (int k = 0; k < G.L; k++) {
+    const size_t L = pg_lay(G, k);
+    const double hu_w = u[L + pg_at(G, il, jl)] * face_w_ng(G, h, k, i, j);
+    const double hu_e = u[L + pg_at(G, il + 1, jl)] * face_w_ng(G, h, k, i + 1, j);
+    const double hv_s = v[L + pg_at(G, il, jl)] * face_s_ng(G, h, k, i, j);
+    const double hv_n = v[L + pg_at(G, il, jl + 1)] * face_s_ng(G, h, k, i, j + 1);
+    acc += (hu_e - hu_w) / sx_ + (hv_n - hv_s) / sy_;
+  }
+  return acc;
+}
+
+// RHS of the pressure equation (pressure.c:570-606) before mean removal.
+__global__ __launch_bounds__(TJ* TI) void k_rhs(PgGrid G, const double* u, const double* v, const double* h,
+                                                double* rhs, double dxdt, double dydt)
+{
+  const int j = blockIdx.x * TJ + threadIdx.x;
+  const int i = blockIdx.y * TI + threadIdx.y;
+  if (i >= G.nx || j >= G.ny) return;
+  rhs[pg_at(G, i + G.g, j + G.g)] = 0 + divergence(G, u, v, h, i, j, dxdt, dydt);
+}
+
+// Serial-order mean of the owned cells (pressure.c:55-72): one thread, the
+// same i-outer j-inner accumulation as the reference, so the rounding of
+// the sum is reproduced exactly.
+__global__ void k_serial_mean(PgGrid G, const double* f, PgCtl* ctl)
+{
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0;
+  for (int i = 0; i < G.nx; i++) {
+    const double* col = f + pg_at(G, i + G.g, G.g);
+    for (int j = 0; j < G.ny; j++) s += col[j];
+  }
+  ctl->mean = s / ((double)G.nx * (double)G.ny);
+}
+
+__global__ __launch_bounds__(TJ* TI) void k_sub_mean(PgGrid G, double* f, const PgCtl* ctl)
+{
+  const int j = blockIdx.x * TJ + threadIdx.x;
+  const int i = blockIdx.y * TI + threadIdx.y;
+  if (i >= G.nx || j >= G.ny) return;
+  f[pg_at(G, i + G.g, j + G.g)] -= ctl->mean;
+}
+
+// Velocity projection u -= dt dpi/dx, v -= dt dpi/dy, wall faces skipped
+// (pressure.c:2182-2248).
+__global__ __launch_bounds__(TJ* TI) void k_correct(PgGrid G, double* u, double* v, const double* pi, double dt)
+{
+  const int j = G.g + blockIdx.x * TJ + threadIdx.x;
+  const int i = G.g + blockIdx.y * TI + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= G.g + G.nx || j >= G.g + G.ny) return;
+  const size_t p = pg_at(G, i, j), L = pg_lay(G, k);
+  const double pc = pi[p];
+  if (!(G.wallx && i == G.g)) u[L + p] += -dt * (pc - pi[pg_at(G, i - 1, j)]) / G.dx;
+  if (!(G.wally && j == G.g)) v[L + p] += -dt * (pc - pi[pg_at(G, i, j - 1)]) / G.dy;
+}
+
+// Pressure operator coefficients (pressure.c:609-637).
+__device__ __forceinline__ double west_op(const PgGrid& G, const double* Hw, int il, int jl)
+{
+  const int gi = il - G.g;
+  if (G.wallx && (gi == 0 || gi == G.nx)) return 0;
+  return Hw[pg_at(G, il, jl)] / (G.dx * G.dx);
+}
+__device__ __forceinline__ double south_op(const PgGrid& G, const double* Hs, int il, int jl)
+{
+  const int gj = jl - G.g;
+  if (G.wally && (gj == 0 || gj == G.ny)) return 0;
+  return Hs[pg_at(G, il, jl)] / (G.dy * G.dy);
+}
+
+// Block-level max / sum reduction helpers (wave64 shuffles, then LDS).
+__device__ __forceinline__ void block_max_sum(double& mx, double& ss)
+{
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+    ss += __shfl_xor(ss, o, 64);
+  }
+  __shared__ double smx[16], sss[16];
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int w = tid >> 6, nw = (blockDim.x * blockDim.y + 63) >> 6;
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    smx[w] = mx;
+    sss[w] = ss;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int q = 1; q < nw; q++) {
+      mx = fmax(mx, smx[q]);
+      ss += sss[q];
+    }
+  }
+}
+
+// Residual norms of rhs - L(pi) (pressure.c:652-690) [which = 0] and
+// barotropic divergence norms after the projection (pressure.c:693-737)
+// [which = 1]: per-block partial max / sum of squares.
+__global__ __launch_bounds__(TJ* TI) void k_norm_partials(PgGrid G, int which, const double* pi, const double* rhs,
+                                                          const double* Hw, const double* Hs, const double* u,
+                                                          const double* v, const double* h, double* part)
+{
+  const int j = blockIdx.x * TJ + threadIdx.x;
+  const int i = blockIdx.y * TI + threadIdx.y;
+  double mx = 0, ss = 0;
+  if (i < G.nx && j < G.ny) {
+    double r;
+    if (which == 0) {
+      const int il = i + G.g, jl = j + G.g;
+      const double Ow = west_op(G, Hw, il, jl), Oe = west_op(G, Hw, il + 1, jl);
+      const double Os = south_op(G, Hs, il, jl), On = south_op(G, Hs, il, jl + 1);
+      const double pc = pi[pg_at(G, il, jl)];
+      const double Lpi = Oe * (pi[pg_at(G, il + 1, jl)] - pc) + Ow * (pi[pg_at(G, il - 1, jl)] - pc)
+                         + On * (pi[pg_at(G, il, jl + 1)] - pc) + Os * (pi[pg_at(G, il, jl - 1)] - pc);
+      r = rhs[pg_at(G, il, jl)] - Lpi;
+    } else {
+      r = divergence(G, u, v, h, i, j, G.dx, G.dy);
+    }
+    mx = fabs(r);
+    ss = r * r;
+  }
+  block_max_sum(mx, ss);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int b = blockIdx.x + blockIdx.y * gridDim.x;
+    part[2 * b] = mx;
+    part[2 * b + 1] = ss;
+  }
+}
+
+__global__ void k_norm_final(int nb, const double* part, PgCtl* ctl, int which)
+{
+  double mx = 0, ss = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    mx = fmax(mx, part[2 * b]);
+    ss += part[2 * b + 1];
+  }
+  // single wavefront-sized block
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
+    ss += __shfl_xor(ss, o, 64);
+  }
+  if (threadIdx.x == 0) {
+    ctl->norm_max[which] = mx;
+    ctl->norm_ss[which] = ss;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Static geometry (work.c:1124-1167): Hc = hhs - hhb on owned cells, then
+// centred face averages Hw, Hs over [1, s-1).
+// ---------------------------------------------------------------------------
+__global__ void k_hc(PgGrid G, const double* hhs, const double* hhb, double* Hc)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= G.ny) return;
+  const size_t p = pg_at(G, i + G.g, j + G.g);
+  Hc[p] = hhs[p] - hhb[p];
+}
+
+__global__ void k_face_avg(PgGrid G, const double* hh, double* hw, double* hs)
+{
+  const int j = 1 + blockIdx.x * blockDim.x + threadIdx.x, i = 1 + blockIdx.y;
+  if (j >= G.sy - 1 || i >= G.sx - 1) return;
+  hw[pg_at(G, i, j)] = 0.5 * (hh[pg_at(G, i, j)] + hh[pg_at(G, i - 1, j)]);
+  hs[pg_at(G, i, j)] = 0.5 * (hh[pg_at(G, i, j)] + hh[pg_at(G, i, j - 1)]);
+}
+
+// Per-cell SOR coefficients (pressure.c:768-796): W, E, S, N and w/Osum.
+// A cell with Osum == 0 is marked with a zero factor and skipped.
+__global__ void k_sor_setup(PgGrid G, const double* Hw, const double* Hs, double w, double* c, size_t fsz)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= G.ny) return;
+  const int il = i + G.g, jl = j + G.g;
+  const size_t p = pg_at(G, il, jl);
+  const double Ow = west_op(G, Hw, il, jl), Oe = west_op(G, Hw, il + 1, jl);
+  const double Os = south_op(G, Hs, il, jl), On = south_op(G, Hs, il, jl + 1);
+  const double Osum = Ow + Oe + Os + On;
+  c[p] = Ow;
+  c[fsz + p] = Oe;
+  c[2 * fsz + p] = Os;
+  c[3 * fsz + p] = On;
+  c[4 * fsz + p] = (Osum == 0) ? 0.0 : w / Osum;
+  if (Osum == 0) c[3 * fsz + p] = __longlong_as_double(0x7ff8000000000001LL);  // flag: skip cell
+}
+
+// ---------------------------------------------------------------------------
+// Red-black SOR (pressure.c:743-814), whole solve in one workgroup.
+//
+// The iterate lives in LDS with a one-cell ghost ring (the 5-point stencil
+// reads no deeper ghost).  Before each colour sweep the ring is refilled
+// with the scalar-kind rule -- mirror on walls (field.c:188-193/230-236),
+// periodic copy otherwise -- so a periodic neighbour of the same colour
+// (odd periodic size) is read with its pre-sweep value exactly as in the
+// reference's exchange-then-sweep order.  Cells of one colour never read
+// each other inside the sweep, so the in-place update is race free.  The
+// convergence test is max |delta| over both colours (exact, order free) and
+// the mean is removed with the serial-order sum.
+// ---------------------------------------------------------------------------
+template <bool LDS>
+__global__ __launch_bounds__(1024) void k_sor_solve(PgGrid G, double* pi_g, const double* rhs, const double* c,
+                                                   size_t fsz, double one_minus_w, double tol, int maxiters,
+                                                   PgCtl* ctl)
+{
+  extern __shared__ double smem[];
+  __shared__ double red[16];
+  __shared__ double s_gmax;
+  const int nx = G.nx, ny = G.ny, W = ny + 2;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  // element (i,j), i in [-1, nx], j in [-1, ny] of the one-ring copy
+  double* P = LDS ? smem : nullptr;
+  auto at1 = [&](int i, int j) -> double& {
+    if (LDS) return P[(size_t)(i + 1) * W + (j + 1)];
+    return pi_g[pg_at(G, i + G.g, j + G.g)];
+  };
+  if (LDS) {
+    for (int idx = tid; idx < nx * ny; idx += nt) {
+      const int i = idx / ny, j = idx % ny;
+      at1(i, j) = pi_g[pg_at(G, i + G.g, j + G.g)];
+    }
+  }
+  __syncthreads();
+  int iters = 0;
+  double gmax = tol + 1;
+  while (gmax > tol && iters < maxiters) {
+    double itmax = 0;
+    for (int color = 0; color < 2; color++) {
+      // refresh the ghost ring: y direction over owned columns, then x over all rows
+      for (int i = tid; i < nx; i += nt) {
+        at1(i, -1) = G.wally ? at1(i, 0) : at1(i, ny - 1);
+        at1(i, ny) = G.wally ? at1(i, ny - 1) : at1(i, 0);
+      }
+      __syncthreads();
+      for (int j = tid - 1; j < ny + 1; j += nt) {
+        at1(-1, j) = G.wallx ? at1(0, j) : at1(nx - 1, j);
+        at1(nx, j) = G.wallx ? at1(nx - 1, j) : at1(0, j);
+      }
+      __syncthreads();
+      double cmax = 0;
+      // cells of this colour: (i + j) & 1 == color
+      const int half = (nx * ny + 1) / 2;
+      for (int idx = tid; idx < half + 1; idx += nt) {
+        const int lin = 2 * idx;  // enumerate pairs along the row-major order, pick the right parity
+        for (int q = 0; q < 2; q++) {
+          const int l = lin + q;
+          if (l >= nx * ny) break;
+          const int i = l / ny, j = l % ny;
+          if (((i + j) & 1) != color) continue;
+          const size_t p = pg_at(G, i + G.g, j + G.g);
+          const double On = c[3 * fsz + p];
+          if (On != On) continue;  // Osum == 0
+          const double Ow = c[p], Oe = c[fsz + p], Os = c[2 * fsz + p], fac = c[4 * fsz + p];
+          const double prev = at1(i, j);
+          const double nw = one_minus_w * prev
+                            + fac * (On * at1(i, j + 1) + Os * at1(i, j - 1) + Oe * at1(i + 1, j) + Ow * at1(i - 1, j)
+                                     - rhs[p]);
+          at1(i, j) = nw;
+          cmax = fmax(cmax, fabs(nw - prev));
+        }
+      }
+      itmax = fmax(itmax, cmax);
+      __syncthreads();
+    }
+    // workgroup max of |delta|
+    for (int o = 32; o > 0; o >>= 1) itmax = fmax(itmax, __shfl_xor(itmax, o, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = itmax;
+    __syncthreads();
+    if (tid == 0) {
+      double m = red[0];
+      for (int q = 1; q < (nt + 63) / 64; q++) m = fmax(m, red[q]);
+      s_gmax = m;
+    }
+    __syncthreads();
+    gmax = s_gmax;
+    iters++;
+  }
+  // serial-order mean (pressure.c:55-72) and removal (pressure.c:811)
+  if (tid == 0) {
+    double s = 0;
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++) s += at1(i, j);
+    s_gmax = s / ((double)nx * (double)ny);
+    ctl->iters = iters;
+    ctl->maxres = gmax;
+  }
+  __syncthreads();
+  const double mean = s_gmax;
+  for (int idx = tid; idx < nx * ny; idx += nt) {
+    const int i = idx / ny, j = idx % ny;
+    const double val = at1(i, j) - mean;
+    if (LDS) pi_g[pg_at(G, i + G.g, j + G.g)] = val;
+    else at1(i, j) = val;
+  }
+}
+
+dim3 owned_grid(const PgGrid& G, int z) { return dim3((G.ny + TJ - 1) / TJ, (G.nx + TI - 1) / TI, z); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind)
+{
+  hipLaunchKernelGGL(k_fill, dim3(nfields), dim3(256), 0, h->stream, base, h->fsz, h->G, kind);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh)
+{
+  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
+  hipLaunchKernelGGL(k_tendency, owned_grid(h->G, h->G.L), dim3(TJ, TI), 0, h->stream, h->G, h->P, I, du, dv, dh);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_ab(pg_handle* h, int order)
+{
+  const double dt = h->cfg.dt;
+  AbArgs A;
+  double* xs[3] = {h->u, h->v, h->h};
+  double** fs[3] = {h->fu, h->fv, h->fh};
+  for (int f = 0; f < 3; f++) {
+    A.x[f] = xs[f];
+    A.f0[f] = fs[f][h->slot[0]];
+    A.f1[f] = fs[f][h->slot[1]];
+    A.f2[f] = fs[f][h->slot[2]];
+  }
+  A.order = order;
+  // integrator.c:26, 44-45, 64-66 coefficient expressions
+  if (order == 1) {
+    A.c0 = dt; A.c1 = 0; A.c2 = 0;
+  } else if (order == 2) {
+    A.c0 = 3 * dt / 2; A.c1 = dt / 2; A.c2 = 0;
+  } else {
+    A.c0 = 23 * dt / 12; A.c1 = 4 * dt / 3; A.c2 = 5 * dt / 12;
+  }
+  hipLaunchKernelGGL(k_ab, owned_grid(h->G, 3 * h->G.L), dim3(TJ, TI), 0, h->stream, h->G, A);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_thickness(pg_handle* h)
+{
+  hipLaunchKernelGGL(k_thickness, owned_grid(h->G, 1), dim3(TJ, TI), 0, h->stream, h->G, h->h, h->Hc);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_rhs(pg_handle* h)
+{
+  const double dxdt = h->G.dx * h->cfg.dt, dydt = h->G.dy * h->cfg.dt;
+  hipLaunchKernelGGL(k_rhs, owned_grid(h->G, 1), dim3(TJ, TI), 0, h->stream, h->G, h->u, h->v, h->h, h->rhs, dxdt, dydt);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_serial_mean(pg_handle* h, double* f)
+{
+  hipLaunchKernelGGL(k_serial_mean, dim3(1), dim3(64), 0, h->stream, h->G, f, h->ctl);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_sub_mean(pg_handle* h, double* f)
+{
+  hipLaunchKernelGGL(k_sub_mean, owned_grid(h->G, 1), dim3(TJ, TI), 0, h->stream, h->G, f, h->ctl);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_correct(pg_handle* h)
+{
+  hipLaunchKernelGGL(k_correct, owned_grid(h->G, h->G.L), dim3(TJ, TI), 0, h->stream, h->G, h->u, h->v, h->pi, h->cfg.dt);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_norms(pg_handle* h, int which)
+{
+  dim3 grid = owned_grid(h->G, 1);
+  const int nb = grid.x * grid.y;
+  hipLaunchKernelGGL(k_norm_partials, grid, dim3(TJ, TI), 0, h->stream, h->G, which, h->pi, h->rhs, h->Hw, h->Hs,
+                     h->u, h->v, h->h, h->partials);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, h->stream, nb, h->partials, h->ctl, which);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_static_geometry(pg_handle* h)
+{
+  const PgGrid& G = h->G;
+  hipLaunchKernelGGL(k_hc, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->hhs, h->hhb, h->Hc);
+  hipError_t e = pg_fill(h, h->Hc, 1, PG_HALO_SCALAR);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_face_avg, dim3((G.sy - 2 + 255) / 256, G.sx - 2), dim3(256), 0, h->stream, G, h->Hc, h->Hw, h->Hs);
+  e = pg_fill(h, h->Hw, 1, PG_HALO_SCALAR);
+  if (e != hipSuccess) return e;
+  return pg_fill(h, h->Hs, 1, PG_HALO_SCALAR);
+}
+
+hipError_t pg_launch_sor_setup(pg_handle* h)
+{
+  const PgGrid& G = h->G;
+  hipLaunchKernelGGL(k_sor_setup, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->Hw, h->Hs,
+                     h->cfg.SOR_rp, h->sor_c, h->fsz);
+  return hipGetLastError();
+}
+
+int pg_sor_solve(pg_handle* h, int* iters)
+{
+  const PgGrid& G = h->G;
+  const size_t lds = (size_t)(G.nx + 2) * (G.ny + 2) * sizeof(double);
+  const double omw = 1 - h->cfg.SOR_rp;
+  if (lds <= 150 * 1024) {
+    static bool attr_set = false;
+    if (!attr_set) {
+      PG_CHECK(h, hipFuncSetAttribute((const void*)k_sor_solve<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      150 * 1024));
+      attr_set = true;
+    }
+    hipLaunchKernelGGL(k_sor_solve<true>, dim3(1), dim3(1024), lds, h->stream, G, h->pi, h->rhs, h->sor_c, h->fsz,
+                       omw, h->cfg.pi_tol, (int)h->cfg.maxiters, h->ctl);
+  } else {
+    hipLaunchKernelGGL(k_sor_solve<false>, dim3(1), dim3(1024), 0, h->stream, G, h->pi, h->rhs, h->sor_c, h->fsz,
+                       omw, h->cfg.pi_tol, (int)h->cfg.maxiters, h->ctl);
+  }
+  PG_CHECK(h, hipGetLastError());
+  PG_CHECK(h, pg_fill(h, h->pi, 1, PG_HALO_SCALAR));
+  PG_CHECK(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(PgCtl), hipMemcpyDeviceToHost, h->stream));
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  *iters = h->ctl_host->iters;
+  return PG_OK;
+}

@@ -1,0 +1,740 @@
+// pg_mg.hip -- multigrid pressure solvers for gfx950.
+//
+// Hierarchy: the multigrid.c construction (levels to 1 x 1 with (N+1)/2
+// coarsening, Hc restricted level by level, operators Ow = Hw/dx^2 and
+// Os = Hs/dy^2 zeroed on walls, bilinear weights with wall overrides;
+// multigrid.c:196-446), built on the device.
+//
+// Two solvers share it:
+//  * the replica (PG_SOLVER_MG_REPLICA) reproduces multigrid.c bit for bit:
+//    its smoother is an in-place lexicographic Gauss-Seidel sweep with
+//    modular neighbours (multigrid.c:536-568), whose only exact parallel
+//    schedule is anti-diagonal wavefronts; one workgroup runs a whole
+//    V-cycle with a barrier per wavefront.
+//  * the GPU multigrid (PG_SOLVER_GPU_MG / _MIXED) is a new algorithm on the
+//    same operator: red-black Gauss-Seidel V(nu1,nu2) with multi-workgroup
+//    kernels on the fine levels (fused residual+restriction and
+//    prolongation+correction), the coarse tail in one workgroup, and a
+//    relative-residual stopping test ||L pi - rhs||_2 / ||rhs||_2.  The
+//    mixed variant runs the V-cycles in FP32 on the defect of an FP64
+//    outer iteration (iterative refinement).
+#include "pg_internal.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int BJ = 64, BI = 4;
+constexpr int COARSE_CELLS = 4096;  // levels at or below this size run in one workgroup
+
+__host__ __device__ inline int coarsen(int N) { return N > 1 ? (N + 1) / 2 : 1; }
+__device__ __forceinline__ int im1(int i, int N) { return (i + N - 1) % N; }
+__device__ __forceinline__ int ip1(int i, int N) { return (i + N + 1) % N; }
+
+template <typename T> struct LvPtr;
+template <> struct LvPtr<double> {
+  __device__ static double* pi(const MgLevel& l) { return l.pi; }
+  __device__ static double* tmp(const MgLevel& l) { return l.tmp; }
+  __device__ static double* rhs(const MgLevel& l) { return l.rhs; }
+  __device__ static const double* Ow(const MgLevel& l) { return l.Ow; }
+  __device__ static const double* Os(const MgLevel& l) { return l.Os; }
+  __device__ static const double* rOsum(const MgLevel& l) { return l.rOsum; }
+};
+template <> struct LvPtr<float> {
+  __device__ static float* pi(const MgLevel& l) { return l.pi32; }
+  __device__ static float* tmp(const MgLevel& l) { return l.tmp32; }
+  __device__ static float* rhs(const MgLevel& l) { return l.rhs32; }
+  __device__ static const float* Ow(const MgLevel& l) { return l.Ow32; }
+  __device__ static const float* Os(const MgLevel& l) { return l.Os32; }
+  __device__ static const float* rOsum(const MgLevel& l) { return l.rOsum32; }
+};
+
+// ------------------------------------------------------------------------
+// hierarchy construction (multigrid.c:196-446)
+// ------------------------------------------------------------------------
+
+// restrict_power2_MG / restrict_general_MG (multigrid.c:651-767), one coarse
+// cell per thread; `f` fine, `c` coarse arrays.
+template <typename T>
+__device__ void restrict_cell(const MgLevel& F, const T* f, const MgLevel& C, T* c, int ic, int jc)
+{
+  if (F.pow2) {
+    c[(size_t)ic * C.Ny + jc] = (T)0.25 * (f[(size_t)(2 * ic) * F.Ny + 2 * jc] + f[(size_t)(2 * ic + 1) * F.Ny + 2 * jc]
+                                           + f[(size_t)(2 * ic) * F.Ny + 2 * jc + 1] + f[(size_t)(2 * ic + 1) * F.Ny + 2 * jc + 1]);
+    return;
+  }
+  const double x0 = ((double)ic) * F.Nx / C.Nx, x1 = ((double)(ic + 1)) * F.Nx / C.Nx;
+  const double y0 = ((double)jc) * F.Ny / C.Ny, y1 = ((double)(jc + 1)) * F.Ny / C.Ny;
+  T acc = 0;
+  double wsum = 0;
+  for (unsigned ifn = (unsigned)floor(x0); ifn < (unsigned)ceil(x1); ifn++) {
+    const double wx = fmin(x1, (double)(ifn + 1)) - fmax(x0, (double)ifn);
+    if (wx <= 0) continue;
+    for (unsigned jfn = (unsigned)floor(y0); jfn < (unsigned)ceil(y1); jfn++) {
+      const double wy = fmin(y1, (double)(jfn + 1)) - fmax(y0, (double)jfn);
+      if (wy <= 0) continue;
+      const double w = wx * wy;
+      acc += (T)w * f[(size_t)ifn * F.Ny + jfn];
+      wsum += w;
+    }
+  }
+  if (wsum > 0) acc /= (T)wsum;
+  c[(size_t)ic * C.Ny + jc] = acc;
+}
+
+__global__ void k_restrict_hc(const MgLevel* lv, int m)
+{
+  const MgLevel& C = lv[m];
+  const MgLevel& F = lv[m + 1];
+  const int jc = blockIdx.x * blockDim.x + threadIdx.x, ic = blockIdx.y;
+  if (jc >= C.Ny) return;
+  restrict_cell<double>(F, F.Hc, C, C.Hc, ic, jc);
+}
+
+// calcFaceThickness_MG + operators + weights for one level (multigrid.c:311-442).
+__global__ void k_level_coeffs(const MgLevel* lv, int m, int wallEW, int wallNS)
+{
+  const MgLevel& l = lv[m];
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int X = l.Nx, Y = l.Ny;
+  if (j >= Y) return;
+  const size_t p = (size_t)i * Y + j;
+  auto Hc = [&](int a, int b) { return l.Hc[(size_t)a * Y + b]; };
+  const double Hw = 0.5 * (Hc(i, j) + Hc(im1(i, X), j));
+  const double Hs = 0.5 * (Hc(i, j) + Hc(i, im1(j, Y)));
+  const double Hw_p = 0.5 * (Hc(ip1(i, X), j) + Hc(i, j));          // Hw at (i+1, j)
+  const double Hs_p = 0.5 * (Hc(i, ip1(j, Y)) + Hc(i, j));          // Hs at (i, j+1)
+  double Os = Hs / (l.dy * l.dy), Ow = Hw / (l.dx * l.dx);
+  double Os_p = Hs_p / (l.dy * l.dy), Ow_p = Hw_p / (l.dx * l.dx);
+  if (wallNS && j == 0) Os = 0;
+  if (wallNS && ip1(j, Y) == 0) Os_p = 0;
+  if (wallEW && i == 0) Ow = 0;
+  if (wallEW && ip1(i, X) == 0) Ow_p = 0;
+  l.Os[p] = Os;
+  l.Ow[p] = Ow;
+  const double Osum = Ow + Ow_p + Os + Os_p;
+  l.Osum[p] = Osum;
+  l.rOsum[p] = 1 / Osum;
+  if (l.Ow32) {
+    l.Os32[p] = (float)Os;
+    l.Ow32[p] = (float)Ow;
+    l.rOsum32[p] = (Osum != 0) ? (float)(1 / Osum) : 0.0f;
+  }
+  double wmm = ((i % 2 == 1) ? 0.75 : 0.25) * ((j % 2 == 1) ? 0.75 : 0.25);
+  double wmp = ((i % 2 == 1) ? 0.75 : 0.25) * ((j % 2 == 1) ? 0.25 : 0.75);
+  double wpm = ((i % 2 == 1) ? 0.25 : 0.75) * ((j % 2 == 1) ? 0.75 : 0.25);
+  double wpp = ((i % 2 == 1) ? 0.25 : 0.75) * ((j % 2 == 1) ? 0.25 : 0.75);
+  // Wall overrides, applied in the reference's order (N/S rows, then E/W
+  // columns, then corners); later assignments win.
+  if (wallNS) {
+    if (j == 0) { wmm = 0; wpm = 0; wmp = (i % 2 == 1) ? 0.75 : 0.25; wpp = (i % 2 == 1) ? 0.25 : 0.75; }
+    if (j == Y - 1) { wmp = 0; wpp = 0; wmm = (i % 2 == 1) ? 0.75 : 0.25; wpm = (i % 2 == 1) ? 0.25 : 0.75; }
+  }
+  if (wallEW) {
+    if (i == 0) { wmm = 0; wmp = 0; wpm = (j % 2 == 1) ? 0.75 : 0.25; wpp = (j % 2 == 1) ? 0.25 : 0.75; }
+    if (i == X - 1) { wpm = 0; wpp = 0; wmm = (j % 2 == 1) ? 0.75 : 0.25; wmp = (j % 2 == 1) ? 0.25 : 0.75; }
+  }
+  if (wallNS && wallEW) {
+    if (i == 0 && j == 0) { wmm = 0; wmp = 0; wpm = 0; wpp = 1; }
+    if (i == 0 && j == Y - 1) { wmm = 0; wmp = 0; wpm = 1; wpp = 0; }
+    if (i == X - 1 && j == 0) { wmm = 0; wmp = 1; wpm = 0; wpp = 0; }
+    if (i == X - 1 && j == Y - 1) { wmm = 1; wmp = 0; wpm = 0; wpp = 0; }
+  }
+  l.wmm[p] = wmm;
+  l.wmp[p] = wmp;
+  l.wpm[p] = wpm;
+  l.wpp[p] = wpp;
+}
+
+// Finest-level gather/scatter between the ghosted pressure fields and the
+// no-ghost multigrid arrays (pressure.c:888-912).
+__global__ void k_gather(PgGrid G, const double* gf, double* f)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= G.ny) return;
+  f[(size_t)i * G.ny + j] = gf[pg_at(G, i + G.g, j + G.g)];
+}
+__global__ void k_scatter(PgGrid G, const double* f, double* gf)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= G.ny) return;
+  gf[pg_at(G, i + G.g, j + G.g)] = f[(size_t)i * G.ny + j];
+}
+
+// ------------------------------------------------------------------------
+// workgroup-cooperative building blocks (tid, nt = this workgroup)
+// ------------------------------------------------------------------------
+
+// In-place lexicographic damped GS sweep (multigrid.c:552-567) scheduled by
+// anti-diagonals d = i + j: every neighbour that precedes (i,j) in the
+// reference's i-outer/j-inner order, including the modular wrap at
+// i = Nx-1 / j = Ny-1, lies on an earlier diagonal, and every neighbour that
+// follows it lies on a later one; cells of one diagonal are independent.
+template <typename T>
+__device__ void wg_relax_lex(const MgLevel& l, T* pi, const T* rhs, T omega, int tid, int nt)
+{
+  const int X = l.Nx, Y = l.Ny;
+  const T* Os = LvPtr<T>::Os(l);
+  const T* Ow = LvPtr<T>::Ow(l);
+  const T* rO = LvPtr<T>::rOsum(l);
+  for (int d = 0; d <= X + Y - 2; d++) {
+    const int ilo = d - (Y - 1) > 0 ? d - (Y - 1) : 0;
+    const int ihi = d < X - 1 ? d : X - 1;
+    for (int i = ilo + tid; i <= ihi; i += nt) {
+      const int j = d - i;
+      const int im = im1(i, X), ip = ip1(i, X), jm = im1(j, Y), jp = ip1(j, Y);
+      const size_t p = (size_t)i * Y + j;
+      const T v = omega * rO[p]
+                      * (Os[(size_t)i * Y + jp] * pi[(size_t)i * Y + jp] + Os[p] * pi[(size_t)i * Y + jm]
+                         + Ow[(size_t)ip * Y + j] * pi[(size_t)ip * Y + j] + Ow[p] * pi[(size_t)im * Y + j] - rhs[p])
+                  + (1 - omega) * pi[p];
+      pi[p] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// residual_MG (multigrid.c:599-627): res = L(pi) - rhs.
+template <typename T>
+__device__ __forceinline__ T point_residual(const MgLevel& l, const T* pi, const T* rhs, int i, int j)
+{
+  const int X = l.Nx, Y = l.Ny;
+  const T* Os = LvPtr<T>::Os(l);
+  const T* Ow = LvPtr<T>::Ow(l);
+  const int im = im1(i, X), ip = ip1(i, X), jm = im1(j, Y), jp = ip1(j, Y);
+  const size_t p = (size_t)i * Y + j;
+  T osum;
+  if constexpr (sizeof(T) == 8) osum = l.Osum[p];
+  else osum = (T)l.Osum[p];
+  return Os[(size_t)i * Y + jp] * pi[(size_t)i * Y + jp] + Os[p] * pi[(size_t)i * Y + jm]
+         + Ow[(size_t)ip * Y + j] * pi[(size_t)ip * Y + j] + Ow[p] * pi[(size_t)im * Y + j] - osum * pi[p] - rhs[p];
+}
+
+template <typename T>
+__device__ void wg_residual(const MgLevel& l, const T* pi, const T* rhs, T* res, int tid, int nt)
+{
+  for (int idx = tid; idx < l.Nx * l.Ny; idx += nt) {
+    const int i = idx / l.Ny, j = idx % l.Ny;
+    res[idx] = point_residual<T>(l, pi, rhs, i, j);
+  }
+}
+
+template <typename T>
+__device__ void wg_restrict(const MgLevel& F, const T* f, const MgLevel& C, T* c, int tid, int nt)
+{
+  for (int idx = tid; idx < C.Nx * C.Ny; idx += nt) restrict_cell<T>(F, f, C, c, idx / C.Ny, idx % C.Ny);
+}
+
+__device__ __forceinline__ unsigned interp_index(int idx, unsigned N, bool wall)
+{
+  if (N <= 1) return 0;
+  if (wall) {
+    if (idx < 0) return 0;
+    if (idx >= (int)N) return N - 1;
+    return (unsigned)idx;
+  }
+  return (unsigned)((idx + (int)N) % (int)N);
+}
+
+// interpolate_power2_MG / interpolate_general_MG (multigrid.c:827-932) at one fine cell.
+template <typename T>
+__device__ __forceinline__ T interp_point(const MgLevel& F, const MgLevel& C, const T* c, int i, int j, int wallEW, int wallNS)
+{
+  if (F.pow2) {
+    const unsigned hx = F.Nx / 2, hy = F.Ny / 2;
+    const unsigned ip = (((i + 1) - (i + 1) % 2) / 2 + hx) % hx;
+    const unsigned im = (ip - 1 + hx) % hx;
+    const unsigned jp = (((j + 1) - (j + 1) % 2) / 2 + hy) % hy;
+    const unsigned jm = (jp - 1 + hy) % hy;
+    const size_t p = (size_t)i * F.Ny + j;
+    return (T)F.wmm[p] * c[(size_t)im * C.Ny + jm] + (T)F.wpm[p] * c[(size_t)ip * C.Ny + jm]
+           + (T)F.wmp[p] * c[(size_t)im * C.Ny + jp] + (T)F.wpp[p] * c[(size_t)ip * C.Ny + jp];
+  }
+  const double x = (((double)i) + 0.5) * C.Nx / F.Nx - 0.5;
+  double tx;
+  int imr;
+  if (wallEW && x <= 0) { imr = 0; tx = 0; }
+  else if (wallEW && x >= (unsigned)(C.Nx - 1)) { imr = C.Nx - 1; tx = 0; }
+  else { imr = (int)floor(x); tx = x - imr; }
+  const unsigned im = interp_index(imr, C.Nx, wallEW), ip = interp_index(imr + 1, C.Nx, wallEW);
+  const double y = (((double)j) + 0.5) * C.Ny / F.Ny - 0.5;
+  double ty;
+  int jmr;
+  if (wallNS && y <= 0) { jmr = 0; ty = 0; }
+  else if (wallNS && y >= (unsigned)(C.Ny - 1)) { jmr = C.Ny - 1; ty = 0; }
+  else { jmr = (int)floor(y); ty = y - jmr; }
+  const unsigned jm = interp_index(jmr, C.Ny, wallNS), jp = interp_index(jmr + 1, C.Ny, wallNS);
+  return (T)((1 - tx) * (1 - ty)) * c[(size_t)im * C.Ny + jm] + (T)(tx * (1 - ty)) * c[(size_t)ip * C.Ny + jm]
+         + (T)((1 - tx) * ty) * c[(size_t)im * C.Ny + jp] + (T)(tx * ty) * c[(size_t)ip * C.Ny + jp];
+}
+
+// max_residual (multigrid.c:1123-1156).
+__device__ double wg_max_residual(const MgLevel& l, int tid, int nt)
+{
+  __shared__ double red[16];
+  const int X = l.Nx, Y = l.Ny;
+  double mx = 0;
+  for (int idx = tid; idx < X * Y; idx += nt) {
+    const int i = idx / Y, j = idx % Y;
+    const int im = im1(i, X), ip = ip1(i, X), jm = im1(j, Y), jp = ip1(j, Y);
+    const size_t p = idx;
+    const double r = l.pi[p]
+                     - l.rOsum[p] * (l.Os[(size_t)i * Y + jp] * l.pi[(size_t)i * Y + jp] + l.Os[p] * l.pi[(size_t)i * Y + jm]
+                                     + l.Ow[(size_t)ip * Y + j] * l.pi[(size_t)ip * Y + j] + l.Ow[p] * l.pi[(size_t)im * Y + j]
+                                     - l.rhs[p]);
+    mx = fmax(mx, fabs(r));
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  if ((tid & 63) == 0) red[tid >> 6] = mx;
+  __syncthreads();
+  if (tid == 0)
+    for (int q = 1; q < (nt + 63) / 64; q++) mx = fmax(mx, red[q]);
+  __syncthreads();
+  return mx;  // valid on tid 0
+}
+
+// Replica V-cycle from level `top` down to 1 x 1 and back (vcycle_MG,
+// multigrid.c:1023-1096), then max_residual on `top`.
+__global__ __launch_bounds__(1024) void k_vcycle_replica(const MgLevel* lv, int top, double omega, int wallEW, int wallNS,
+                                                         PgCtl* ctl)
+{
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int n = top; n >= 1; n--) {
+    const MgLevel& l = lv[n];
+    const MgLevel& c = lv[n - 1];
+    wg_relax_lex<double>(l, l.pi, l.rhs, omega, tid, nt);
+    wg_residual<double>(l, l.pi, l.rhs, l.tmp, tid, nt);
+    __syncthreads();
+    wg_restrict<double>(l, l.tmp, c, c.rhs, tid, nt);
+    for (int idx = tid; idx < c.Nx * c.Ny; idx += nt) c.pi[idx] = 0;
+    __syncthreads();
+  }
+  if (tid == 0 && lv[0].Nx == 1 && lv[0].Ny == 1) lv[0].pi[0] = 0;  // exactSolve_MG (multigrid.c:472-476)
+  __syncthreads();
+  for (int n = 1; n <= top; n++) {
+    const MgLevel& l = lv[n];
+    const MgLevel& c = lv[n - 1];
+    for (int idx = tid; idx < l.Nx * l.Ny; idx += nt) l.tmp[idx] = interp_point<double>(l, c, c.pi, idx / l.Ny, idx % l.Ny, wallEW, wallNS);
+    __syncthreads();
+    for (int idx = tid; idx < l.Nx * l.Ny; idx += nt) l.pi[idx] -= l.tmp[idx];
+    __syncthreads();
+    wg_relax_lex<double>(l, l.pi, l.rhs, omega, tid, nt);
+  }
+  const double mr = wg_max_residual(lv[top], tid, nt);
+  if (tid == 0) ctl->maxres = mr;
+}
+
+// full_MG helpers (multigrid.c:1236-1270): restrict the rhs to every level,
+// and interpolate level n-1's solution into level n's iterate.
+__global__ __launch_bounds__(1024) void k_fmg_restrict_rhs(const MgLevel* lv, int top)
+{
+  for (int n = top; n > 0; n--) {
+    wg_restrict<double>(lv[n], lv[n].rhs, lv[n - 1], lv[n - 1].rhs, threadIdx.x, blockDim.x);
+    __syncthreads();
+  }
+}
+__global__ __launch_bounds__(1024) void k_fmg_interp(const MgLevel* lv, int n, int wallEW, int wallNS)
+{
+  const MgLevel& l = lv[n];
+  for (int idx = threadIdx.x; idx < l.Nx * l.Ny; idx += blockDim.x)
+    l.pi[idx] = interp_point<double>(l, lv[n - 1], lv[n - 1].pi, idx / l.Ny, idx % l.Ny, wallEW, wallNS);
+}
+
+// ------------------------------------------------------------------------
+// GPU multigrid (new algorithm on the same operator)
+// ------------------------------------------------------------------------
+
+// Red-black GS half sweep on a fine level: cells with (i+j)&1 == color.
+template <typename T>
+__global__ __launch_bounds__(BJ* BI) void k_rb(MgLevel l, int color)
+{
+  const int j = blockIdx.x * BJ + threadIdx.x, i = blockIdx.y * BI + threadIdx.y;
+  if (i >= l.Nx || j >= l.Ny || ((i + j) & 1) != color) return;
+  T* pi = LvPtr<T>::pi(l);
+  const T* rhs = LvPtr<T>::rhs(l);
+  const T* Os = LvPtr<T>::Os(l);
+  const T* Ow = LvPtr<T>::Ow(l);
+  const T* rO = LvPtr<T>::rOsum(l);
+  const int X = l.Nx, Y = l.Ny;
+  const int im = im1(i, X), ip = ip1(i, X), jm = im1(j, Y), jp = ip1(j, Y);
+  const size_t p = (size_t)i * Y + j;
+  const T s = Os[(size_t)i * Y + jp] * pi[(size_t)i * Y + jp] + Os[p] * pi[(size_t)i * Y + jm]
+              + Ow[(size_t)ip * Y + j] * pi[(size_t)ip * Y + j] + Ow[p] * pi[(size_t)im * Y + j] - rhs[p];
+  const T r = rO[p];
+  if (r == r && r != (T)INFINITY) pi[p] = r * s;
+}
+
+// Residual + 2:1 restriction into the next level's rhs; zeroes its iterate.
+template <typename T>
+__global__ __launch_bounds__(BJ* BI) void k_resid_restrict(MgLevel f, MgLevel c)
+{
+  const int jc = blockIdx.x * BJ + threadIdx.x, ic = blockIdx.y * BI + threadIdx.y;
+  if (ic >= c.Nx || jc >= c.Ny) return;
+  const T* pi = LvPtr<T>::pi(f);
+  const T* rhs = LvPtr<T>::rhs(f);
+  const T r00 = point_residual<T>(f, pi, rhs, 2 * ic, 2 * jc);
+  const T r10 = point_residual<T>(f, pi, rhs, 2 * ic + 1, 2 * jc);
+  const T r01 = point_residual<T>(f, pi, rhs, 2 * ic, 2 * jc + 1);
+  const T r11 = point_residual<T>(f, pi, rhs, 2 * ic + 1, 2 * jc + 1);
+  const size_t q = (size_t)ic * c.Ny + jc;
+  LvPtr<T>::rhs(c)[q] = (T)0.25 * (r00 + r10 + r01 + r11);
+  LvPtr<T>::pi(c)[q] = 0;
+}
+
+// Bilinear prolongation of the coarse correction and its subtraction.
+template <typename T>
+__global__ __launch_bounds__(BJ* BI) void k_prolong_correct(MgLevel f, MgLevel c, int wallEW, int wallNS)
+{
+  const int j = blockIdx.x * BJ + threadIdx.x, i = blockIdx.y * BI + threadIdx.y;
+  if (i >= f.Nx || j >= f.Ny) return;
+  const T e = interp_point<T>(f, c, LvPtr<T>::pi(c), i, j, wallEW, wallNS);
+  LvPtr<T>::pi(f)[(size_t)i * f.Ny + j] -= e;
+}
+
+// Coarse tail in one workgroup: V(nu1,nu2) from level `top` with
+// lexicographic GS (wavefront schedule; race free for every size and
+// boundary type) and the reference's transfers.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_vcycle_coarse(const MgLevel* lv, int top, int nu1, int nu2, int wallEW, int wallNS)
+{
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int n = top; n >= 1; n--) {
+    const MgLevel& l = lv[n];
+    const MgLevel& c = lv[n - 1];
+    for (int s = 0; s < nu1; s++) wg_relax_lex<T>(l, LvPtr<T>::pi(l), LvPtr<T>::rhs(l), (T)1, tid, nt);
+    wg_residual<T>(l, LvPtr<T>::pi(l), LvPtr<T>::rhs(l), LvPtr<T>::tmp(l), tid, nt);
+    __syncthreads();
+    wg_restrict<T>(l, LvPtr<T>::tmp(l), c, LvPtr<T>::rhs(c), tid, nt);
+    for (int idx = tid; idx < c.Nx * c.Ny; idx += nt) LvPtr<T>::pi(c)[idx] = 0;
+    __syncthreads();
+  }
+  if (tid == 0 && lv[0].Nx * lv[0].Ny == 1) LvPtr<T>::pi(lv[0])[0] = 0;
+  __syncthreads();
+  for (int n = 1; n <= top; n++) {
+    const MgLevel& l = lv[n];
+    const MgLevel& c = lv[n - 1];
+    for (int idx = tid; idx < l.Nx * l.Ny; idx += nt)
+      LvPtr<T>::pi(l)[idx] -= interp_point<T>(l, c, LvPtr<T>::pi(c), idx / l.Ny, idx % l.Ny, wallEW, wallNS);
+    __syncthreads();
+    for (int s = 0; s < nu2; s++) wg_relax_lex<T>(l, LvPtr<T>::pi(l), LvPtr<T>::rhs(l), (T)1, tid, nt);
+  }
+}
+
+// ||L pi - rhs||^2 partial sums on the finest level (FP64).
+__global__ __launch_bounds__(BJ* BI) void k_resnorm(MgLevel l, double* part)
+{
+  const int j = blockIdx.x * BJ + threadIdx.x, i = blockIdx.y * BI + threadIdx.y;
+  double ss = 0;
+  if (i < l.Nx && j < l.Ny) {
+    const double r = point_residual<double>(l, l.pi, l.rhs, i, j);
+    ss = r * r;
+  }
+  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+  __shared__ double red[BI * BJ / 64];
+  const int tid = threadIdx.x + threadIdx.y * BJ;
+  if ((tid & 63) == 0) red[tid >> 6] = ss;
+  __syncthreads();
+  if (tid == 0) {
+    for (int q = 1; q < BI * BJ / 64; q++) ss += red[q];
+    part[blockIdx.x + blockIdx.y * gridDim.x] = ss;
+  }
+}
+
+// FP64 defect d = rhs - L pi of the finest level, stored as the FP32 rhs of
+// a correction solve (sign: the V-cycle solves L e = -d with the
+// reference's residual convention, so store L pi - rhs), FP32 iterate
+// zeroed; partial sums of d^2.
+__global__ __launch_bounds__(BJ* BI) void k_defect32(MgLevel l, double* part)
+{
+  const int j = blockIdx.x * BJ + threadIdx.x, i = blockIdx.y * BI + threadIdx.y;
+  double ss = 0;
+  if (i < l.Nx && j < l.Ny) {
+    const double r = point_residual<double>(l, l.pi, l.rhs, i, j);
+    ss = r * r;
+    const size_t p = (size_t)i * l.Ny + j;
+    l.rhs32[p] = (float)r;
+    l.pi32[p] = 0.0f;
+  }
+  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+  __shared__ double red[BI * BJ / 64];
+  const int tid = threadIdx.x + threadIdx.y * BJ;
+  if ((tid & 63) == 0) red[tid >> 6] = ss;
+  __syncthreads();
+  if (tid == 0) {
+    for (int q = 1; q < BI * BJ / 64; q++) ss += red[q];
+    part[blockIdx.x + blockIdx.y * gridDim.x] = ss;
+  }
+}
+
+// pi -= e (the FP32 correction solves L e = L pi - rhs).
+__global__ __launch_bounds__(BJ* BI) void k_apply32(MgLevel l)
+{
+  const int j = blockIdx.x * BJ + threadIdx.x, i = blockIdx.y * BI + threadIdx.y;
+  if (i >= l.Nx || j >= l.Ny) return;
+  const size_t p = (size_t)i * l.Ny + j;
+  l.pi[p] -= (double)l.pi32[p];
+}
+
+__global__ void k_sum_partials(int nb, const double* part, double* out)
+{
+  double ss = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) ss += part[b];
+  __shared__ double red[16];
+  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < (int)(blockDim.x + 63) / 64; q++) ss += red[q];
+    *out = ss;
+  }
+}
+
+__global__ void k_sumsq_rhs(MgLevel l, double* part)
+{
+  double ss = 0;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < (size_t)l.Nx * l.Ny; p += (size_t)gridDim.x * blockDim.x)
+    ss += l.rhs[p] * l.rhs[p];
+  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+dim3 level_grid(const MgLevel& l) { return dim3((l.Ny + BJ - 1) / BJ, (l.Nx + BI - 1) / BI); }
+
+// Level index from which the single-workgroup coarse tail takes over.
+int coarse_top(const pg_handle* h)
+{
+  const int top = (int)h->lev.size() - 1;
+  int m = top;
+  while (m > 0) {
+    const MgLevel& l = h->lev[m];
+    const bool odd_periodic = (!h->G.wallx && (l.Nx & 1)) || (!h->G.wally && (l.Ny & 1));
+    if (!l.pow2 || (long)l.Nx * l.Ny <= COARSE_CELLS || odd_periodic) break;
+    m--;
+  }
+  return m;
+}
+
+template <typename T>
+hipError_t vcycle_t(pg_handle* h, int nu1, int nu2)
+{
+  const int top = (int)h->lev.size() - 1;
+  const int mc = coarse_top(h);
+  const int wx = h->G.wallx, wy = h->G.wally;
+  for (int n = top; n > mc; n--) {
+    const MgLevel& l = h->lev[n];
+    for (int s = 0; s < nu1; s++) {
+      hipLaunchKernelGGL(k_rb<T>, level_grid(l), dim3(BJ, BI), 0, h->stream, l, 0);
+      hipLaunchKernelGGL(k_rb<T>, level_grid(l), dim3(BJ, BI), 0, h->stream, l, 1);
+    }
+    hipLaunchKernelGGL(k_resid_restrict<T>, level_grid(h->lev[n - 1]), dim3(BJ, BI), 0, h->stream, l, h->lev[n - 1]);
+  }
+  const MgLevel* dlev = reinterpret_cast<const MgLevel*>(h->partials + h->npartials);
+  hipLaunchKernelGGL(k_vcycle_coarse<T>, dim3(1), dim3(1024), 0, h->stream, dlev, mc, nu1, nu2, wx, wy);
+  for (int n = mc + 1; n <= top; n++) {
+    const MgLevel& l = h->lev[n];
+    hipLaunchKernelGGL(k_prolong_correct<T>, level_grid(l), dim3(BJ, BI), 0, h->stream, l, h->lev[n - 1], wx, wy);
+    for (int s = 0; s < nu2; s++) {
+      hipLaunchKernelGGL(k_rb<T>, level_grid(l), dim3(BJ, BI), 0, h->stream, l, 1);
+      hipLaunchKernelGGL(k_rb<T>, level_grid(l), dim3(BJ, BI), 0, h->stream, l, 0);
+    }
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------
+
+void pg_mg_free(pg_handle* h)
+{
+  for (MgLevel& l : h->lev) {
+    double* d[] = {l.tmp, l.Hc, l.Ow, l.Os, l.Osum, l.rOsum, l.wmm, l.wmp, l.wpm, l.wpp, l.pi, l.rhs};
+    for (double* p : d) hipFree(p);
+    float* f[] = {l.pi32, l.tmp32, l.rhs32, l.Ow32, l.Os32, l.rOsum32};
+    for (float* p : f) hipFree(p);
+  }
+  h->lev.clear();
+  h->mg_built = false;
+}
+
+int pg_mg_build(pg_handle* h)
+{
+  if (h->mg_built) return PG_OK;
+  const int Nx = h->G.nx, Ny = h->G.ny;
+  const bool mixed = h->cfg.pressureSolver == PG_SOLVER_GPU_MG_MIXED;
+  int nlev = 1, x = Nx, y = Ny;
+  while (x > 1 || y > 1) {
+    x = coarsen(x);
+    y = coarsen(y);
+    nlev++;
+  }
+  h->lev.assign(nlev, MgLevel{});
+  x = Nx;
+  y = Ny;
+  for (int m = nlev - 1; m >= 0; m--) {
+    MgLevel& l = h->lev[m];
+    l.Nx = x;
+    l.Ny = y;
+    l.dx = h->cfg.Lx / x;
+    l.dy = h->cfg.Ly / y;
+    x = coarsen(x);
+    y = coarsen(y);
+  }
+  for (int m = nlev - 1; m > 0; m--)
+    h->lev[m].pow2 = (h->lev[m].Nx == 2 * h->lev[m - 1].Nx) && (h->lev[m].Ny == 2 * h->lev[m - 1].Ny);
+  for (int m = 0; m < nlev; m++) {
+    MgLevel& l = h->lev[m];
+    const size_t n = (size_t)l.Nx * l.Ny;
+    double** d[] = {&l.tmp, &l.Hc, &l.Ow, &l.Os, &l.Osum, &l.rOsum, &l.wmm, &l.wmp, &l.wpm, &l.wpp, &l.pi, &l.rhs};
+    for (double** p : d) PG_CHECK(h, hipMalloc(p, n * sizeof(double)));
+    PG_CHECK(h, hipMemsetAsync(l.pi, 0, n * sizeof(double), h->stream));
+    if (mixed) {
+      float** f[] = {&l.pi32, &l.tmp32, &l.rhs32, &l.Ow32, &l.Os32, &l.rOsum32};
+      for (float** p : f) PG_CHECK(h, hipMalloc(p, n * sizeof(float)));
+      PG_CHECK(h, hipMemsetAsync(l.pi32, 0, n * sizeof(float), h->stream));
+    }
+  }
+  // level descriptors are stored on the device right after the reduction partials
+  PG_CHECK(h, hipMemcpyAsync(h->partials + h->npartials, h->lev.data(), nlev * sizeof(MgLevel), hipMemcpyHostToDevice, h->stream));
+  const MgLevel* dlev = reinterpret_cast<const MgLevel*>(h->partials + h->npartials);
+  MgLevel& top = h->lev[nlev - 1];
+  hipLaunchKernelGGL(k_gather, dim3((Ny + 255) / 256, Nx), dim3(256), 0, h->stream, h->G, h->Hc, top.Hc);
+  for (int m = nlev - 2; m >= 0; m--)
+    hipLaunchKernelGGL(k_restrict_hc, dim3((h->lev[m].Ny + 255) / 256, h->lev[m].Nx), dim3(256), 0, h->stream, dlev, m);
+  for (int m = nlev - 1; m >= 0; m--)
+    hipLaunchKernelGGL(k_level_coeffs, dim3((h->lev[m].Ny + 255) / 256, h->lev[m].Nx), dim3(256), 0, h->stream, dlev, m,
+                       (int)h->cfg.useWallEW, (int)h->cfg.useWallNS);
+  PG_CHECK(h, hipGetLastError());
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  h->mg_built = true;
+  return PG_OK;
+}
+
+hipError_t pg_gather_to_mg(pg_handle* h)
+{
+  MgLevel& top = h->lev.back();
+  const PgGrid& G = h->G;
+  hipLaunchKernelGGL(k_gather, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->pi, top.pi);
+  hipLaunchKernelGGL(k_gather, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->rhs, top.rhs);
+  return hipGetLastError();
+}
+
+hipError_t pg_scatter_from_mg(pg_handle* h)
+{
+  MgLevel& top = h->lev.back();
+  const PgGrid& G = h->G;
+  hipLaunchKernelGGL(k_scatter, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, top.pi, h->pi);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return pg_fill(h, h->pi, 1, PG_HALO_SCALAR);
+}
+
+static int read_ctl(pg_handle* h)
+{
+  PG_CHECK(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(PgCtl), hipMemcpyDeviceToHost, h->stream));
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
+
+// iterate_MG / full_MG (multigrid.c:1172-1270) on the device hierarchy.
+int pg_mg_replica_solve(pg_handle* h, int* iters)
+{
+  const int nlev = (int)h->lev.size();
+  const MgLevel* dlev = reinterpret_cast<const MgLevel*>(h->partials + h->npartials);
+  const int wx = h->cfg.useWallEW, wy = h->cfg.useWallNS;
+  auto iterate = [&](int n, int* it) -> int {
+    double mr = h->cfg.pi_tol + 1;
+    *it = 0;
+    while (mr > h->cfg.pi_tol && *it < (int)h->cfg.maxiters) {
+      hipLaunchKernelGGL(k_vcycle_replica, dim3(1), dim3(1024), 0, h->stream, dlev, n, h->cfg.omega_MG, wx, wy, h->ctl);
+      PG_CHECK(h, hipGetLastError());
+      int rc = read_ctl(h);
+      if (rc) return rc;
+      mr = h->ctl_host->maxres;
+      (*it)++;
+    }
+    return PG_OK;
+  };
+  PG_CHECK(h, pg_gather_to_mg(h));
+  int rc;
+  if (!h->cfg.use_fullMG) {
+    rc = iterate(nlev - 1, iters);
+  } else {
+    hipLaunchKernelGGL(k_fmg_restrict_rhs, dim3(1), dim3(1024), 0, h->stream, dlev, nlev - 1);
+    hipLaunchKernelGGL(k_vcycle_replica, dim3(1), dim3(1024), 0, h->stream, dlev, 0, h->cfg.omega_MG, wx, wy, h->ctl);
+    rc = PG_OK;
+    *iters = 0;
+    for (int n = 1; n < nlev && rc == PG_OK; n++) {
+      hipLaunchKernelGGL(k_fmg_interp, dim3(1), dim3(1024), 0, h->stream, dlev, n, wx, wy);
+      rc = iterate(n, iters);
+    }
+  }
+  if (rc) return rc;
+  PG_CHECK(h, pg_scatter_from_mg(h));
+  return PG_OK;
+}
+
+hipError_t pg_gpu_mg_vcycle(pg_handle* h)
+{
+  const int nu1 = h->cfg.mg_nu1 ? h->cfg.mg_nu1 : 2, nu2 = h->cfg.mg_nu2 ? h->cfg.mg_nu2 : 2;
+  return vcycle_t<double>(h, nu1, nu2);
+}
+
+hipError_t pg_gpu_mg_vcycle32(pg_handle* h)
+{
+  const int nu1 = h->cfg.mg_nu1 ? h->cfg.mg_nu1 : 2, nu2 = h->cfg.mg_nu2 ? h->cfg.mg_nu2 : 2;
+  return vcycle_t<float>(h, nu1, nu2);
+}
+
+hipError_t pg_gpu_mg_sweep(pg_handle* h)
+{
+  const MgLevel& l = h->lev.back();
+  hipLaunchKernelGGL(k_rb<double>, level_grid(l), dim3(BJ, BI), 0, h->stream, l, 0);
+  hipLaunchKernelGGL(k_rb<double>, level_grid(l), dim3(BJ, BI), 0, h->stream, l, 1);
+  return hipGetLastError();
+}
+
+// GPU multigrid solve on the gathered finest level: V-cycles (FP64, or FP32
+// corrections of an FP64 defect) until ||L pi - rhs|| <= tol ||rhs||.
+int pg_gpu_mg_solve(pg_handle* h, int mixed, int* iters, double* rel)
+{
+  const MgLevel& top = h->lev.back();
+  const dim3 gg = level_grid(top);
+  const int nb = gg.x * gg.y;
+  double* sums = &h->ctl->sumsq;
+  PG_CHECK(h, pg_gather_to_mg(h));
+  // ||rhs||^2
+  hipLaunchKernelGGL(k_sumsq_rhs, dim3(256), dim3(256), 0, h->stream, top, h->partials);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, h->stream, 256, h->partials, &h->ctl->rhs_sumsq);
+  PG_CHECK(h, hipGetLastError());
+  int it = 0;
+  double r = 0;
+  const int maxit = (int)h->cfg.maxiters;
+  for (;;) {
+    if (mixed) hipLaunchKernelGGL(k_defect32, gg, dim3(BJ, BI), 0, h->stream, top, h->partials);
+    else hipLaunchKernelGGL(k_resnorm, gg, dim3(BJ, BI), 0, h->stream, top, h->partials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, h->stream, nb, h->partials, sums);
+    PG_CHECK(h, hipGetLastError());
+    int rc = read_ctl(h);
+    if (rc) return rc;
+    const double rr = h->ctl_host->rhs_sumsq;
+    r = (rr > 0) ? sqrt(h->ctl_host->sumsq / rr) : 0.0;
+    if (r <= h->cfg.pi_tol || it >= maxit) break;
+    if (mixed) {
+      PG_CHECK(h, pg_gpu_mg_vcycle32(h));
+      hipLaunchKernelGGL(k_apply32, gg, dim3(BJ, BI), 0, h->stream, top);
+    } else {
+      PG_CHECK(h, pg_gpu_mg_vcycle(h));
+    }
+    it++;
+  }
+  *iters = it;
+  *rel = r;
+  PG_CHECK(h, pg_scatter_from_mg(h));
+  return PG_OK;
+}

@@ -1,0 +1,480 @@
+// pg_point.h -- per-point stencil math of the PAWSIM timestep.
+//
+// Every function evaluates one value of one reference work array at one
+// ghosted array position (i, j) of layer k, directly from the ghost-filled
+// prognostic arrays u, v, h (reference layout: element (i,j) at i*sy + j,
+// ghost width g = 3).  The reference materialises ~30 such arrays per layer
+// (work.h:93-193) and refreshes their halos with one of four kinds
+// (field.c:121-241); here the halo semantics are reproduced by position
+// mapping instead of storage:
+//
+//  * scalar-kind arrays (h_west, h_south, eta, pp, KE, MM): a ghost
+//    position reads the mirrored owned position on a wall direction
+//    (field.c:188-193, 230-236) and the owned position one period away on
+//    a periodic direction (field.c:121-150);
+//  * "internal" arrays (zeta, h_q, q, the PV coefficients, h*u, h*v;
+//    work.c:184-185, 1283-1285, 1394-1401) keep the values computed from
+//    the ghost inputs on a wall direction (the formula is evaluated in
+//    place) and hold periodic copies on a periodic direction;
+//  * arrays that are never halo-refreshed (DD_T, DD_S, UU4, VV4, DD4_*,
+//    A2_*, A4sqrt_*; work.c:1563-1664) are zero outside their computed
+//    range [1, s-1), which the functions reproduce.
+//
+// Association of every FP64 expression follows the reference source line
+// cited, and the code must be compiled without FMA contraction.
+//
+// The functions are __host__ __device__ so that the same math can be
+// exercised on the CPU by the test-suite.
+#pragma once
+
+#include <stddef.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define PG_HD __host__ __device__ __forceinline__
+#else
+#define PG_HD static inline
+#endif
+
+#define PG_MAXL 16
+
+enum { PG_MOM_AL81 = 0, PG_MOM_HK83 = 1, PG_MOM_S75E = 2, PG_MOM_TW81 = 3 };
+
+// Geometry and physics parameters, passed to kernels by value.
+struct PgGrid {
+  int nx, ny, g, sx, sy;   // owned sizes, ghost width, ghosted sizes
+  int wallx, wally;        // 1 = solid walls, 0 = periodic
+  int L;                   // layers
+  double dx, dy;
+};
+
+struct PgPhys {
+  int mom;                 // momentum scheme
+  int useA2, useA4, useWind, useLin, useQuad;
+  double h0, A2, A4c, lin, quad;
+  double gg[PG_MAXL], geff[PG_MAXL];
+};
+
+// Read-only inputs of the tendency: ghost-filled state and static fields.
+struct PgIn {
+  const double* u;     // [L][sx*sy]
+  const double* v;
+  const double* h;
+  const double* hhb;   // [sx*sy], scalar-filled
+  const double* omz;   // [sx*sy], q-kind filled
+  const double* taux;  // [sx*sy] or NULL
+  const double* tauy;
+};
+
+PG_HD size_t pg_at(const PgGrid& G, int i, int j) { return (size_t)i * (size_t)G.sy + (size_t)j; }
+PG_HD size_t pg_lay(const PgGrid& G, int k) { return (size_t)k * (size_t)G.sx * (size_t)G.sy; }
+
+// Periodic position map: a ghost position of a halo-refreshed array holds
+// the copy of the owned position one period away (field.c:121-150).
+PG_HD int pg_wrap(int i, int g, int n)
+{
+  if (i < g) return i + n;
+  if (i >= g + n) return i - n;
+  return i;
+}
+
+// Scalar-kind map: mirror about the wall face on walls (field.c:188-193,
+// 230-236), periodic copy otherwise.
+PG_HD int pg_smap_x(const PgGrid& G, int i)
+{
+  if (G.wallx) {
+    if (i < G.g) return 2 * G.g - 1 - i;
+    if (i >= G.g + G.nx) return 2 * (G.g + G.nx) - 1 - i;
+    return i;
+  }
+  return pg_wrap(i, G.g, G.nx);
+}
+PG_HD int pg_smap_y(const PgGrid& G, int j)
+{
+  if (G.wally) {
+    if (j < G.g) return 2 * G.g - 1 - j;
+    if (j >= G.g + G.ny) return 2 * (G.g + G.ny) - 1 - j;
+    return j;
+  }
+  return pg_wrap(j, G.g, G.ny);
+}
+
+// Internal-kind map (field.c:416-426): the serial exchange copies periodic y
+// ghosts on owned columns only, then periodic x ghosts over full columns;
+// every other position keeps the value computed from the ghost inputs.
+PG_HD void pg_imap(const PgGrid& G, int& i, int& j)
+{
+  if (!G.wallx) i = pg_wrap(i, G.g, G.nx);
+  if (!G.wally && i >= G.g && i < G.g + G.nx) j = pg_wrap(j, G.g, G.ny);
+}
+
+// Computed range [1, s-1) of the never-refreshed work arrays.
+PG_HD bool pg_in_core(const PgGrid& G, int i, int j) { return i >= 1 && i < G.sx - 1 && j >= 1 && j < G.sy - 1; }
+
+#define PG_U(k, i, j) (I.u[pg_lay(G, k) + pg_at(G, i, j)])
+#define PG_V(k, i, j) (I.v[pg_lay(G, k) + pg_at(G, i, j)])
+#define PG_H(k, i, j) (I.h[pg_lay(G, k) + pg_at(G, i, j)])
+
+// h_west / h_south, AL81 centred average (work.c:588-600), scalar kind.
+PG_HD double pg_hW(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  i = pg_smap_x(G, i);
+  j = pg_smap_y(G, j);
+  return 0.5 * (PG_H(k, i, j) + PG_H(k, i - 1, j));
+}
+PG_HD double pg_hS(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  i = pg_smap_x(G, i);
+  j = pg_smap_y(G, j);
+  return 0.5 * (PG_H(k, i, j) + PG_H(k, i, j - 1));
+}
+
+// Montgomery potential (work.c:1173-1203 eta, 1431-1443 pp, 1499-1504 MM),
+// scalar kind.  Rigid lid: pp_0 = 0 (work.c:1437).
+PG_HD double pg_MM(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  i = pg_smap_x(G, i);
+  j = pg_smap_y(G, j);
+  double eta[PG_MAXL + 1];
+  const size_t p = pg_at(G, i, j);
+  eta[G.L] = I.hhb[p];
+  for (int m = G.L - 1; m >= 1; m--) eta[m] = eta[m + 1] + I.h[pg_lay(G, m) + p];
+  double pp = 0;
+  for (int m = 1; m <= k; m++) pp = pp + P.gg[m] * eta[m];
+  double mm = pp;
+  if (P.h0 != 0) {
+    const double hk = I.h[pg_lay(G, k) + p];
+    mm -= P.geff[k] * (P.h0 * P.h0 * P.h0 * P.h0) / (hk * hk * hk) / 3;
+  }
+  return mm;
+}
+
+// Kinetic energy in the Bernoulli function (work.c:1446-1497), scalar kind.
+PG_HD double pg_KE(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  i = pg_smap_x(G, i);
+  j = pg_smap_y(G, j);
+#define SQ(x) ((x) * (x))
+  if (P.mom == PG_MOM_TW81) {
+    return 0.25 * ((2.0 / 3.0) * SQ(PG_U(k, i, j)) + (1.0 / 3.0) * SQ(0.5 * (PG_U(k, i - 1, j) + PG_U(k, i + 1, j)))
+                   + (2.0 / 3.0) * SQ(PG_U(k, i + 1, j)) + (1.0 / 3.0) * SQ(0.5 * (PG_U(k, i, j) + PG_U(k, i + 2, j)))
+                   + (2.0 / 3.0) * SQ(PG_V(k, i, j)) + (1.0 / 3.0) * SQ(0.5 * (PG_V(k, i, j - 1) + PG_V(k, i, j + 1)))
+                   + (2.0 / 3.0) * SQ(PG_V(k, i, j + 1)) + (1.0 / 3.0) * SQ(0.5 * (PG_V(k, i, j) + PG_V(k, i, j + 2))));
+  }
+  if (P.mom == PG_MOM_HK83) {
+    return (SQ(PG_U(k, i + 1, j)) + SQ(PG_U(k, i, j))
+            + 0.25 * (SQ(PG_U(k, i + 1, j + 1)) + SQ(PG_U(k, i, j + 1)) + SQ(PG_U(k, i + 1, j - 1)) + SQ(PG_U(k, i, j - 1)))
+            + SQ(PG_V(k, i, j + 1)) + SQ(PG_V(k, i, j))
+            + 0.25 * (SQ(PG_V(k, i + 1, j + 1)) + SQ(PG_V(k, i + 1, j)) + SQ(PG_V(k, i - 1, j + 1)) + SQ(PG_V(k, i - 1, j))))
+           / 6;
+  }
+  return (SQ(PG_U(k, i + 1, j)) + SQ(PG_U(k, i, j))) / 4 + (SQ(PG_V(k, i, j + 1)) + SQ(PG_V(k, i, j))) / 4;
+#undef SQ
+}
+
+// q-point layer thickness (work.c:1261-1264), internal kind.
+PG_HD double pg_hq(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  pg_imap(G, i, j);
+  return 0.25 * (PG_H(k, i, j) + PG_H(k, i - 1, j - 1) + PG_H(k, i - 1, j) + PG_H(k, i, j - 1));
+}
+
+// Potential vorticity with Coriolis, q = (2 Omega_z + zeta) / h_q
+// (work.c:1258-1270), internal kind.
+PG_HD double pg_qq(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  pg_imap(G, i, j);
+  const double zeta = (PG_U(k, i, j - 1) - PG_U(k, i, j)) / G.dy + (PG_V(k, i, j) - PG_V(k, i - 1, j)) / G.dx;
+  return (2 * I.omz[pg_at(G, i, j)] + zeta) / pg_hq(G, I, k, i, j);
+}
+
+// Mass fluxes (work.c:176-182), internal kind; face thickness is scalar kind.
+PG_HD double pg_huu(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  pg_imap(G, i, j);
+  return PG_U(k, i, j) * pg_hW(G, I, k, i, j);
+}
+PG_HD double pg_hvv(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  pg_imap(G, i, j);
+  return PG_V(k, i, j) * pg_hS(G, I, k, i, j);
+}
+
+// PV interpolation coefficients (work.c:1294-1404), internal kind: the
+// array value at (i, j) is the formula at the mapped position, reading q at
+// its neighbours (each through q's own map).  `c` selects alpha, beta,
+// gamma, delta, epsilon, phi, lambda, mu (0..7).
+PG_HD double pg_pvcoef(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int c, int i, int j)
+{
+  pg_imap(G, i, j);
+#define QQ(a, b) pg_qq(G, I, k, i + (a), j + (b))
+  if (P.mom == PG_MOM_TW81) {
+    switch (c) {
+      case 0: return (2 * QQ(1, 1) + 3 * QQ(0, 1) + 2 * QQ(0, 0) + QQ(1, 0) - QQ(0, 2) - QQ(-1, 1)) / 24;
+      case 1: return (3 * QQ(0, 1) + 2 * QQ(-1, 1) + QQ(-1, 0) + 2 * QQ(0, 0) - QQ(1, 1) - QQ(0, 2)) / 24;
+      case 2: return (2 * QQ(0, 1) + QQ(-1, 1) + 2 * QQ(-1, 0) + 3 * QQ(0, 0) - QQ(0, -1) - QQ(1, 0)) / 24;
+      case 3: return (QQ(1, 1) + 2 * QQ(0, 1) + 3 * QQ(0, 0) + 2 * QQ(1, 0) - QQ(-1, 0) - QQ(0, -1)) / 24;
+      case 4: return (QQ(1, 1) + QQ(0, 1) - QQ(0, 0) - QQ(1, 0)) / 24;
+      case 5: return (-QQ(1, 1) + QQ(0, 1) + QQ(0, 0) - QQ(1, 0)) / 24;
+      case 6: return (QQ(1, 0) - QQ(-1, 0)) / 24;
+      default: return (QQ(0, -1) - QQ(0, 1)) / 24;
+    }
+  }
+  switch (c) {  // AL81 / HK83
+    case 0: return (2 * QQ(1, 1) + QQ(0, 1) + 2 * QQ(0, 0) + QQ(1, 0)) / 24;
+    case 1: return (QQ(0, 1) + 2 * QQ(-1, 1) + QQ(-1, 0) + 2 * QQ(0, 0)) / 24;
+    case 2: return (2 * QQ(0, 1) + QQ(-1, 1) + 2 * QQ(-1, 0) + QQ(0, 0)) / 24;
+    case 3: return (QQ(1, 1) + 2 * QQ(0, 1) + QQ(0, 0) + 2 * QQ(1, 0)) / 24;
+    case 4: return (QQ(1, 1) + QQ(0, 1) - QQ(0, 0) - QQ(1, 0)) / 24;
+    case 5: return (-QQ(1, 1) + QQ(0, 1) + QQ(0, 0) - QQ(1, 0)) / 24;
+    default: return 0;
+  }
+#undef QQ
+}
+
+// Tension and shear strain (work.c:1584-1590), zero outside [1, s-1).
+PG_HD double pg_DDT(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0;
+  return (PG_U(k, i + 1, j) - PG_U(k, i, j)) / G.dx - (PG_V(k, i, j + 1) - PG_V(k, i, j)) / G.dy;
+}
+PG_HD double pg_DDS(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0;
+  return (PG_U(k, i, j) - PG_U(k, i, j - 1)) / G.dy + (PG_V(k, i, j) - PG_V(k, i - 1, j)) / G.dx;
+}
+
+// Constant viscosity coefficient arrays (work.c:1614-1631): set on the core range only.
+PG_HD double pg_coef(const PgGrid& G, double c, int i, int j) { return pg_in_core(G, i, j) ? c : 0.0; }
+
+// Biharmonic intermediate fields (work.c:1633-1661), zero outside [1, s-1).
+PG_HD double pg_UU4(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0;
+  const double c = P.A4c;
+  return ((pg_coef(G, c, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
+           - pg_coef(G, c, i - 1, j) * PG_H(k, i - 1, j) * pg_DDT(G, I, k, i - 1, j)) / G.dx
+          + (pg_coef(G, c, i, j + 1) * pg_hq(G, I, k, i, j + 1) * pg_DDS(G, I, k, i, j + 1)
+             - pg_coef(G, c, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / G.dy)
+         / pg_hW(G, I, k, i, j);
+}
+PG_HD double pg_VV4(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0;
+  const double c = P.A4c;
+  return ((pg_coef(G, c, i + 1, j) * pg_hq(G, I, k, i + 1, j) * pg_DDS(G, I, k, i + 1, j)
+           - pg_coef(G, c, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / G.dx
+          - (pg_coef(G, c, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
+             - pg_coef(G, c, i, j - 1) * PG_H(k, i, j - 1) * pg_DDT(G, I, k, i, j - 1)) / G.dy)
+         / pg_hS(G, I, k, i, j);
+}
+PG_HD double pg_DD4T(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0;
+  return (pg_UU4(G, P, I, k, i + 1, j) - pg_UU4(G, P, I, k, i, j)) / G.dx
+         - (pg_VV4(G, P, I, k, i, j + 1) - pg_VV4(G, P, I, k, i, j)) / G.dy;
+}
+PG_HD double pg_DD4S(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0;
+  return (pg_UU4(G, P, I, k, i, j) - pg_UU4(G, P, I, k, i, j - 1)) / G.dy
+         + (pg_VV4(G, P, I, k, i, j) - pg_VV4(G, P, I, k, i - 1, j)) / G.dx;
+}
+
+// Bottom speed for quadratic drag (forcing.c:181-223 with hbbl <= 0): the
+// bottom-layer velocity, evaluated on [0, s-1) without halo refresh.
+PG_HD double pg_uabs_bot(const PgGrid& G, const PgIn& I, int i, int j)
+{
+  const int kb = G.L - 1;
+  const double u0 = PG_U(kb, i, j), u1 = PG_U(kb, i + 1, j);
+  const double v0 = PG_V(kb, i, j), v1 = PG_V(kb, i, j + 1);
+  return sqrt(0.5 * (u0 * u0 + u1 * u1) + 0.5 * (v0 * v0 + v1 * v1));
+}
+
+// Tendencies of layer k at the owned point (i, j) (ghosted coordinates):
+// tendency.c:49-75 stage results, i.e. work.c:1667-1855 momentum,
+// forcing.c:230-393 forcing, work.c:1880-1890 thickness.
+PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j,
+                             double* du, double* dv, double* dh)
+{
+  const int gi = i - G.g, gj = j - G.g;
+  const double dx = G.dx, dy = G.dy;
+  // PV coefficients at the positions the fluxes read.  Away from the domain
+  // edge no position is remapped, so they are formed from q on the 4x4
+  // neighbourhood (i-1..i+2, j-1..j+2); near the edge each coefficient is
+  // evaluated at its own mapped position.
+  const bool tw = P.mom == PG_MOM_TW81, al = (P.mom == PG_MOM_AL81 || P.mom == PG_MOM_HK83);
+  const bool inner = i - 2 >= G.g && i + 3 < G.g + G.nx && j - 2 >= G.g && j + 3 < G.g + G.ny;
+  // u-flux coefficients: alpha, beta, gamma, delta, eps at (i,j); eps at (i-1,j);
+  // lambda at (i,j+1) and (i,j).  v-flux: phi at (i,j-1) and (i,j); gamma at
+  // (i+1,j); delta at (i,j); alpha at (i,j-1); beta at (i+1,j-1); mu at (i+1,j), (i,j).
+  double a00 = 0, b00 = 0, c00 = 0, d00 = 0, e00 = 0, em0 = 0, l0p = 0, l00 = 0;
+  double p0m = 0, p00 = 0, cp0 = 0, a0m = 0, bpm = 0, mp0 = 0, m00 = 0;
+  if ((tw || al) && inner) {
+    double q[4][4];
+#define Q(a, b) q[(a) + 1][(b) + 1]
+    Q(-1, 0) = pg_qq(G, I, k, i - 1, j);
+    Q(-1, 1) = pg_qq(G, I, k, i - 1, j + 1);
+    Q(0, -1) = pg_qq(G, I, k, i, j - 1);
+    Q(0, 0) = pg_qq(G, I, k, i, j);
+    Q(0, 1) = pg_qq(G, I, k, i, j + 1);
+    Q(1, -1) = pg_qq(G, I, k, i + 1, j - 1);
+    Q(1, 0) = pg_qq(G, I, k, i + 1, j);
+    Q(1, 1) = pg_qq(G, I, k, i + 1, j + 1);
+    if (tw) {
+      Q(0, 2) = pg_qq(G, I, k, i, j + 2);
+      Q(2, 0) = pg_qq(G, I, k, i + 2, j);
+      // TW81 (work.c:1352-1390)
+      a00 = (2 * Q(1, 1) + 3 * Q(0, 1) + 2 * Q(0, 0) + Q(1, 0) - Q(0, 2) - Q(-1, 1)) / 24;
+      b00 = (3 * Q(0, 1) + 2 * Q(-1, 1) + Q(-1, 0) + 2 * Q(0, 0) - Q(1, 1) - Q(0, 2)) / 24;
+      c00 = (2 * Q(0, 1) + Q(-1, 1) + 2 * Q(-1, 0) + 3 * Q(0, 0) - Q(0, -1) - Q(1, 0)) / 24;
+      d00 = (Q(1, 1) + 2 * Q(0, 1) + 3 * Q(0, 0) + 2 * Q(1, 0) - Q(-1, 0) - Q(0, -1)) / 24;
+      l0p = (Q(1, 1) - Q(-1, 1)) / 24;
+      l00 = (Q(1, 0) - Q(-1, 0)) / 24;
+      cp0 = (2 * Q(1, 1) + Q(0, 1) + 2 * Q(0, 0) + 3 * Q(1, 0) - Q(1, -1) - Q(2, 0)) / 24;
+      a0m = (2 * Q(1, 0) + 3 * Q(0, 0) + 2 * Q(0, -1) + Q(1, -1) - Q(0, 1) - Q(-1, 0)) / 24;
+      bpm = (3 * Q(1, 0) + 2 * Q(0, 0) + Q(0, -1) + 2 * Q(1, -1) - Q(2, 0) - Q(1, 1)) / 24;
+      mp0 = (Q(1, -1) - Q(1, 1)) / 24;
+      m00 = (Q(0, -1) - Q(0, 1)) / 24;
+    } else {
+      // AL81 / HK83 (work.c:1324-1350)
+      a00 = (2 * Q(1, 1) + Q(0, 1) + 2 * Q(0, 0) + Q(1, 0)) / 24;
+      b00 = (Q(0, 1) + 2 * Q(-1, 1) + Q(-1, 0) + 2 * Q(0, 0)) / 24;
+      c00 = (2 * Q(0, 1) + Q(-1, 1) + 2 * Q(-1, 0) + Q(0, 0)) / 24;
+      d00 = (Q(1, 1) + 2 * Q(0, 1) + Q(0, 0) + 2 * Q(1, 0)) / 24;
+      cp0 = (2 * Q(1, 1) + Q(0, 1) + 2 * Q(0, 0) + Q(1, 0)) / 24;
+      a0m = (2 * Q(1, 0) + Q(0, 0) + 2 * Q(0, -1) + Q(1, -1)) / 24;
+      bpm = (Q(1, 0) + 2 * Q(0, 0) + Q(0, -1) + 2 * Q(1, -1)) / 24;
+    }
+    e00 = (Q(1, 1) + Q(0, 1) - Q(0, 0) - Q(1, 0)) / 24;
+    em0 = (Q(0, 1) + Q(-1, 1) - Q(-1, 0) - Q(0, 0)) / 24;
+    p0m = (-Q(1, 0) + Q(0, 0) + Q(0, -1) - Q(1, -1)) / 24;
+    p00 = (-Q(1, 1) + Q(0, 1) + Q(0, 0) - Q(1, 0)) / 24;
+#undef Q
+  } else if (tw || al) {
+    a00 = pg_pvcoef(G, P, I, k, 0, i, j);
+    b00 = pg_pvcoef(G, P, I, k, 1, i, j);
+    c00 = pg_pvcoef(G, P, I, k, 2, i, j);
+    d00 = pg_pvcoef(G, P, I, k, 3, i, j);
+    e00 = pg_pvcoef(G, P, I, k, 4, i, j);
+    em0 = pg_pvcoef(G, P, I, k, 4, i - 1, j);
+    p0m = pg_pvcoef(G, P, I, k, 5, i, j - 1);
+    p00 = pg_pvcoef(G, P, I, k, 5, i, j);
+    cp0 = pg_pvcoef(G, P, I, k, 2, i + 1, j);
+    a0m = pg_pvcoef(G, P, I, k, 0, i, j - 1);
+    bpm = pg_pvcoef(G, P, I, k, 1, i + 1, j - 1);
+    if (tw) {
+      l0p = pg_pvcoef(G, P, I, k, 6, i, j + 1);
+      l00 = pg_pvcoef(G, P, I, k, 6, i, j);
+      mp0 = pg_pvcoef(G, P, I, k, 7, i + 1, j);
+      m00 = pg_pvcoef(G, P, I, k, 7, i, j);
+    }
+  }
+  // Mass fluxes on the needed faces.
+  const double huu_00 = pg_huu(G, I, k, i, j), huu_p0 = pg_huu(G, I, k, i + 1, j);
+  const double huu_m0 = pg_huu(G, I, k, i - 1, j), huu_0m = pg_huu(G, I, k, i, j - 1);
+  const double huu_pm = pg_huu(G, I, k, i + 1, j - 1);
+  const double hvv_00 = pg_hvv(G, I, k, i, j), hvv_0p = pg_hvv(G, I, k, i, j + 1);
+  const double hvv_m0 = pg_hvv(G, I, k, i - 1, j), hvv_mp = pg_hvv(G, I, k, i - 1, j + 1);
+  const double hvv_0m = pg_hvv(G, I, k, i, j - 1);
+  const double hwest = pg_hW(G, I, k, i, j), hsouth = pg_hS(G, I, k, i, j);
+  const double MM00 = pg_MM(G, P, I, k, i, j), KE00 = pg_KE(G, P, I, k, i, j);
+
+  // ---- u momentum (work.c:1681-1752) ----
+  double ru = 0.0;
+  if (!(G.wallx && gi == 0)) {
+    const double gM = -(MM00 - pg_MM(G, P, I, k, i - 1, j)) / dx;
+    const double gK = -(KE00 - pg_KE(G, P, I, k, i - 1, j)) / dx;
+    double qf = 0;
+    if (tw) {
+      qf += a00 * hvv_0p + b00 * hvv_mp + c00 * hvv_m0 + d00 * hvv_00 - e00 * huu_p0 + em0 * huu_m0
+            - l0p * huu_00 + l00 * huu_0m;
+    } else if (al) {
+      qf += a00 * hvv_0p + b00 * hvv_mp + c00 * hvv_m0 + d00 * hvv_00 - e00 * huu_p0 + em0 * huu_m0;
+    } else {
+      qf += 0.25 * pg_qq(G, I, k, i, j) * (hvv_00 + hvv_m0) + 0.25 * pg_qq(G, I, k, i, j + 1) * (hvv_0p + hvv_mp);
+    }
+    double a2 = 0, a4 = 0;
+    const double buoy = 0;
+    if (P.useA2) {
+      a2 = ((pg_coef(G, P.A2, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
+             - pg_coef(G, P.A2, i - 1, j) * PG_H(k, i - 1, j) * pg_DDT(G, I, k, i - 1, j)) / dx
+            + (pg_coef(G, P.A2, i, j + 1) * pg_hq(G, I, k, i, j + 1) * pg_DDS(G, I, k, i, j + 1)
+               - pg_coef(G, P.A2, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / dy);
+    }
+    if (P.useA4) {
+      a4 = -((pg_coef(G, P.A4c, i, j) * PG_H(k, i, j) * pg_DD4T(G, P, I, k, i, j)
+              - pg_coef(G, P.A4c, i - 1, j) * PG_H(k, i - 1, j) * pg_DD4T(G, P, I, k, i - 1, j)) / dx
+             + (pg_coef(G, P.A4c, i, j + 1) * pg_hq(G, I, k, i, j + 1) * pg_DD4S(G, P, I, k, i, j + 1)
+                - pg_coef(G, P.A4c, i, j) * pg_hq(G, I, k, i, j) * pg_DD4S(G, P, I, k, i, j)) / dy);
+    }
+    ru = gM + gK + qf + ((hwest != 0) ? (a2 + a4 + buoy) / hwest : 0);
+    // forcing (forcing.c:244-358): wind on the top layer, drag on the bottom layer
+    const double fs = (k == 0) ? 1.0 : 0.0, fb = (k == G.L - 1) ? 1.0 : 0.0;
+    if (P.useWind) {
+      const size_t p = pg_at(G, i, j);
+      const double tx = 1.0 * I.taux[p] + 0.0 * I.taux[p];
+      const double r = tx * fs;
+      if (hwest != 0) ru += r / hwest;
+    }
+    if (P.useLin && hwest != 0) {
+      const double r = -P.lin * PG_U(k, i, j) * fb;
+      ru += r / hwest;
+    }
+    if (P.useQuad && hwest != 0) {
+      const double ua = 0.5 * (pg_uabs_bot(G, I, i, j) + pg_uabs_bot(G, I, i - 1, j));
+      ru -= P.quad * ua * PG_U(k, i, j) * fb / hwest;
+    }
+  }
+
+  // ---- v momentum (work.c:1758-1836) ----
+  double rv = 0.0;
+  if (!(G.wally && gj == 0)) {
+    const double gM = -(MM00 - pg_MM(G, P, I, k, i, j - 1)) / dy;
+    const double gK = -(KE00 - pg_KE(G, P, I, k, i, j - 1)) / dy;
+    double qf = 0;
+    if (tw) {
+      qf += p0m * hvv_0m - p00 * hvv_0p - cp0 * huu_p0 - d00 * huu_00 - a0m * huu_0m - bpm * huu_pm - mp0 * hvv_00
+            + m00 * hvv_m0;
+    } else if (al) {
+      qf += p0m * hvv_0m - p00 * hvv_0p - cp0 * huu_p0 - d00 * huu_00 - a0m * huu_0m - bpm * huu_pm;
+    } else {
+      qf -= 0.25 * pg_qq(G, I, k, i, j) * (huu_00 + huu_0m) + 0.25 * pg_qq(G, I, k, i + 1, j) * (huu_p0 + huu_pm);
+    }
+    double a2 = 0, a4 = 0;
+    const double buoy = 0;
+    if (P.useA2) {
+      a2 = ((pg_coef(G, P.A2, i + 1, j) * pg_hq(G, I, k, i + 1, j) * pg_DDS(G, I, k, i + 1, j)
+             - pg_coef(G, P.A2, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / dx
+            - (pg_coef(G, P.A2, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
+               - pg_coef(G, P.A2, i, j - 1) * PG_H(k, i, j - 1) * pg_DDT(G, I, k, i, j - 1)) / dy);
+    }
+    if (P.useA4) {
+      a4 = -((pg_coef(G, P.A4c, i + 1, j) * pg_hq(G, I, k, i + 1, j) * pg_DD4S(G, P, I, k, i + 1, j)
+              - pg_coef(G, P.A4c, i, j) * pg_hq(G, I, k, i, j) * pg_DD4S(G, P, I, k, i, j)) / dx
+             - (pg_coef(G, P.A4c, i, j) * PG_H(k, i, j) * pg_DD4T(G, P, I, k, i, j)
+                - pg_coef(G, P.A4c, i, j - 1) * PG_H(k, i, j - 1) * pg_DD4T(G, P, I, k, i, j - 1)) / dy);
+    }
+    rv = gM + gK + qf + ((hsouth != 0) ? (a2 + a4 + buoy) / hsouth : 0);
+    const double fs = (k == 0) ? 1.0 : 0.0, fb = (k == G.L - 1) ? 1.0 : 0.0;
+    if (P.useWind) {
+      const size_t p = pg_at(G, i, j);
+      const double ty = 1.0 * I.tauy[p] + 0.0 * I.tauy[p];
+      const double r = ty * fs;
+      if (hsouth != 0) rv += r / hsouth;
+    }
+    if (P.useLin && hsouth != 0) {
+      const double r = -P.lin * PG_V(k, i, j) * fb;
+      rv += r / hsouth;
+    }
+    if (P.useQuad && hsouth != 0) {
+      const double ua = 0.5 * (pg_uabs_bot(G, I, i, j) + pg_uabs_bot(G, I, i, j - 1));
+      rv -= P.quad * ua * PG_V(k, i, j) * fb / hsouth;
+    }
+  }
+  *du = ru;
+  *dv = rv;
+  // ---- thickness (work.c:1880-1890); the -(0 - 0) diapycnal term is the
+  // identity under round-to-nearest and is omitted ----
+  *dh = (huu_00 - huu_p0) / dx + (hvv_00 - hvv_0p) / dy;
+}

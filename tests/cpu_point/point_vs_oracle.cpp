@@ -1,0 +1,96 @@
+// Host check of the device stencil math (pawsim_amd/csrc/pg_point.h)
+// against the CPU oracle: run the oracle for a few steps, then evaluate
+// pg_tendency_point at every owned point of every layer on the oracle's
+// ghost-filled state and compare bit for bit with the oracle's tendency.
+//
+//   point_vs_oracle <input_file> <nsteps>
+// prints "points=<n> mismatches=<m> maxabs=<d>" and exits 0 if m == 0.
+extern "C" {
+#include "oracle.h"
+}
+#include "pg_point.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+int main(int argc, char** argv)
+{
+  if (argc < 3) return 2;
+  or_model m;
+  if (!or_model_init(&m, argv[1], stderr)) return 2;
+  const unsigned nsteps = (unsigned)atoi(argv[2]);
+  or_prepare_work(&m);
+  or_forcing_update(&m);
+  or_tendency_core(&m);
+  or_pressure_correct(&m);
+  for (unsigned n = 1; n <= nsteps; n++) {
+    or_step(&m, n);
+    or_pressure_correct(&m);
+  }
+  or_tendency_eval(&m, 0);
+
+  PgGrid G;
+  G.nx = (int)m.cfg.Nx;
+  G.ny = (int)m.cfg.Ny;
+  G.g = (int)m.g;
+  G.sx = G.nx + 2 * G.g;
+  G.sy = G.ny + 2 * G.g;
+  G.wallx = m.cfg.wallEW;
+  G.wally = m.cfg.wallNS;
+  G.L = (int)m.L;
+  G.dx = m.dx;
+  G.dy = m.dy;
+  PgPhys P;
+  memset(&P, 0, sizeof(P));
+  P.mom = (int)m.cfg.momScheme;
+  P.useA2 = m.cfg.A2 > 0;
+  P.useA4 = m.cfg.A4 > 0;
+  P.useWind = m.cfg.useWind;
+  P.useLin = m.cfg.linDrag > 0;
+  P.useQuad = m.cfg.quadDrag > 0;
+  P.h0 = m.cfg.h0;
+  P.A2 = m.cfg.A2;
+  P.A4c = sqrt(m.cfg.A4);
+  P.lin = m.cfg.linDrag;
+  P.quad = m.cfg.quadDrag;
+  for (int k = 0; k < G.L; k++) {
+    P.gg[k] = m.gg[k];
+    P.geff[k] = m.geff[k];
+  }
+  const size_t fs = (size_t)G.sx * G.sy;
+  double* u = (double*)malloc(G.L * fs * sizeof(double));
+  double* v = (double*)malloc(G.L * fs * sizeof(double));
+  double* h = (double*)malloc(G.L * fs * sizeof(double));
+  for (int k = 0; k < G.L; k++) {
+    memcpy(u + k * fs, m.u[k].d, fs * sizeof(double));
+    memcpy(v + k * fs, m.v[k].d, fs * sizeof(double));
+    memcpy(h + k * fs, m.h[k].d, fs * sizeof(double));
+  }
+  PgIn I{u, v, h, m.hhb.d, m.omz.d, m.taux.d, m.tauy.d};
+  long pts = 0, bad = 0;
+  double maxabs = 0;
+  for (int k = 0; k < G.L; k++)
+    for (int i = G.g; i < G.g + G.nx; i++)
+      for (int j = G.g; j < G.g + G.ny; j++) {
+        double du, dv, dh;
+        pg_tendency_point(G, P, I, k, i, j, &du, &dv, &dh);
+        const double ref[3] = {OF(&m.tu[k], i, j), OF(&m.tv[k], i, j), OF(&m.th[k], i, j)};
+        const double got[3] = {du, dv, dh};
+        for (int q = 0; q < 3; q++) {
+          pts++;
+          if (memcmp(&ref[q], &got[q], sizeof(double)) != 0 && !(ref[q] == 0 && got[q] == 0)) {
+            if (bad < 5) fprintf(stderr, "field %d k=%d i=%d j=%d ref=%.17g got=%.17g\n", q, k, i, j, ref[q], got[q]);
+            bad++;
+            maxabs = fmax(maxabs, fabs(ref[q] - got[q]));
+          }
+        }
+      }
+  printf("points=%ld mismatches=%ld maxabs=%g\n", pts, bad, maxabs);
+  free(u);
+  free(v);
+  free(h);
+  or_model_free(&m);
+  return bad == 0 ? 0 : 1;
+}

@@ -1,0 +1,122 @@
+"""Write golden fixtures from the reference serial executable.
+
+Runs oracle/_ref/PAWSIM.exe (built from the reference sources by
+`make -C oracle ref`) on small deterministic inputs and stores, per case, an
+.npz holding the inputs, the parameter file text, the model state at a few
+save indices, selected init_* work-array dumps (writeInitDiagnostics) and the
+per-solve iteration counts and norms printed with pressureTiming.
+
+Only this script runs the reference, and only in the build container; the
+fixtures are data (inputs and the reference's outputs on them).
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+from pawsim_amd import inputs  # noqa: E402
+
+REF = os.path.join(REPO, "oracle", "_ref", "PAWSIM.exe")
+
+INIT_FIELDS = ("dudt", "dvdt", "dhdt", "qq", "KE", "MM", "hwest", "alpha", "mu")
+
+T = dict(pressureTiming=1, writeInitDiagnostics=1)
+
+
+def cases():
+    """Small cases covering the restated path (see the test docstrings)."""
+    return {
+        "box32_sor": inputs.gyre(nx=32, ny=32, nt=10, **T),
+        "box32_mg": inputs.gyre(nx=32, ny=32, nt=10, solver=1, **T),
+        "box24x20_mg_general": inputs.gyre(nx=24, ny=20, nt=5, solver=1, **T),
+        "box16_fullmg": inputs.gyre(nx=16, ny=16, nt=5, solver=1, use_fullMG=1, **T),
+        "chan32x16_mg_drag": inputs.channel(nx=32, ny=16, nt=8, dx=20e3, solver=1, linDragCoeff=1e-3, A2=100.0, **T),
+        "chan32x16_sor_quad": inputs.channel(nx=32, ny=16, nt=5, dx=20e3, solver=0, SOR_rp=1.8, quadDragCoeff=2e-3, **T),
+        "per16_sor": inputs.periodic_box(nx=16, ny=16, nt=10, **T),
+        "per15x17_sor_odd": inputs.periodic_box(nx=15, ny=17, nt=6, SOR_rp=1.5, **T),
+        "per15x18_mg_odd": inputs.periodic_box(nx=15, ny=18, nt=6, solver=1, **T),
+        "box24x32_visc": inputs.gyre(nx=24, ny=32, nt=8, A2=200.0, A4=1e10, quadDragCoeff=2e-3,
+                                     linDragCoeff=1e-4, h0=10.0, **T),
+        "box16_ab2_al81": inputs.gyre(nx=16, ny=16, nt=6, timeSteppingScheme=4, momentumScheme=0, **T),
+        "box16_ab1_hk83": inputs.gyre(nx=16, ny=16, nt=6, timeSteppingScheme=3, momentumScheme=1, **T),
+        "box16_s75e": inputs.gyre(nx=16, ny=16, nt=6, momentumScheme=2, A2=50.0, **T),
+        "wallx16_pery_a4": inputs.periodic_box(nx=16, ny=16, nt=6, A4=1e9, useWallEW=1, **T),
+    }
+
+
+SOLVE_RE = re.compile(r"phase=(\w+) n=(\d+) solver=(\w+) iters=(\d+) walltime=\S+ s residual_max=(\S+) "
+                      r"residual_rms=(\S+) div_max=(\S+) div_rms=(\S+)")
+
+
+def parse_solves(stdout: str) -> np.ndarray:
+    rows = []
+    for m in SOLVE_RE.finditer(stdout):
+        rows.append([int(m.group(2)), int(m.group(4))] + [float(m.group(k)) for k in range(5, 9)])
+    return np.array(rows, dtype=np.float64)
+
+
+def make_case(name: str, setup: inputs.Setup, outdir: str) -> str:
+    work = tempfile.mkdtemp(prefix=f"golden_{name}_")
+    try:
+        infile = setup.write(work)
+        res = os.path.join(work, "out")
+        os.makedirs(res)
+        proc = subprocess.run([REF, infile, res], capture_output=True, text=True, check=True)
+        nt = int(setup.params["Nt"])
+        nx, ny, nl = setup.nx, setup.ny, setup.nlay
+        saves = sorted({0, 1, 2, 3, nt})
+        arrays = {
+            "hhb": setup.hhb, "omegaz": setup.omegaz, "gg": np.asarray(setup.gg), "h": setup.h,
+            "saves": np.array(saves),
+            "solves": parse_solves(proc.stdout),
+        }
+        for key in ("u", "v", "taux", "tauy"):
+            val = getattr(setup, key)
+            if val is not None:
+                arrays[key] = val
+        for n in saves:
+            out = inputs.read_outputs(res, nx, ny, nl, n)
+            for k, v in out.items():
+                arrays[f"{k}_n{n}"] = v
+        for f in INIT_FIELDS:
+            arrays[f"init_{f}"] = np.stack([
+                inputs.read_2d(os.path.join(res, f"init_{f}{k}_n=0.dat"), nx, ny) for k in range(nl)])
+        meta = {"name": name, "params": setup.params, "notes": setup.notes}
+        arrays["meta"] = np.array(json.dumps(meta, default=str))
+        path = os.path.join(outdir, f"{name}.npz")
+        np.savez_compressed(path, **arrays)
+        return path
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def load_setup(npz) -> inputs.Setup:
+    """Rebuild the Setup stored in a fixture."""
+    meta = json.loads(str(npz["meta"]))
+    get = lambda k: npz[k] if k in npz.files else None  # noqa: E731
+    return inputs.Setup(dict(meta["params"]), npz["hhb"], None, npz["omegaz"], npz["gg"], npz["h"],
+                        u=get("u"), v=get("v"), taux=get("taux"), tauy=get("tauy"), name=meta["name"])
+
+
+def main() -> None:
+    if not os.path.exists(REF):
+        sys.exit("build the reference first: make -C oracle ref")
+    for name, setup in cases().items():
+        p = make_case(name, setup, HERE)
+        print(f"{p}: {os.path.getsize(p)} bytes")
+
+
+if __name__ == "__main__":
+    main()
