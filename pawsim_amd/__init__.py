@@ -1,0 +1,236 @@
+"""pawsim_amd: MI355X-native PAWSIM timestep (HIP kernels behind a C ABI).
+
+The product is ``libpawsim_gpu.so`` (C ABI declared in include/pawsim_gpu.h)
+and the ``bin/pawsim_gpu`` host driver.  This module is a thin ctypes binding
+of that ABI for Python callers (tests and benchmarks).  Loading fails loudly
+when the library has not been built; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libpawsim_gpu.so")
+DRIVER_PATH = os.path.join(PKG, "bin", "pawsim_gpu")
+
+SOLVER_SOR = 0
+SOLVER_MG_REPLICA = 1
+SOLVER_GPU_MG = 3
+SOLVER_GPU_MG_MIXED = 4
+
+PG_OK, PG_EARG, PG_EHIP, PG_ENOCONV, PG_ENODEV = 0, -1, -2, -3, -4
+
+# Exported entry points (must match include/pawsim_gpu.h).
+ABI_SYMBOLS = (
+    "pg_create", "pg_destroy", "pg_last_error", "pg_upload_state", "pg_download_state",
+    "pg_upload_pi", "pg_download_pi", "pg_tendency", "pg_step", "pg_pressure_correct",
+    "pg_solve_pressure", "pg_time", "pg_set_stream", "pg_synchronize", "pg_bench_kernel",
+)
+
+
+class PgConfig(ctypes.Structure):
+    _fields_ = [
+        ("Nx", ctypes.c_uint32), ("Ny", ctypes.c_uint32), ("Nlay", ctypes.c_uint32),
+        ("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("dt", ctypes.c_double),
+        ("timeSteppingScheme", ctypes.c_uint32), ("momentumScheme", ctypes.c_uint32),
+        ("thicknessScheme", ctypes.c_uint32), ("useWallEW", ctypes.c_int32), ("useWallNS", ctypes.c_int32),
+        ("h0", ctypes.c_double), ("A2", ctypes.c_double), ("A4", ctypes.c_double),
+        ("linDragCoeff", ctypes.c_double), ("quadDragCoeff", ctypes.c_double),
+        ("pressureSolver", ctypes.c_uint32), ("pi_tol", ctypes.c_double), ("SOR_rp", ctypes.c_double),
+        ("omega_MG", ctypes.c_double), ("maxiters", ctypes.c_uint32), ("use_fullMG", ctypes.c_int32),
+        ("pressureTiming", ctypes.c_int32), ("mg_nu1", ctypes.c_uint32), ("mg_nu2", ctypes.c_uint32),
+    ]
+
+
+class PgStatic(ctypes.Structure):
+    _fields_ = [(n, ctypes.POINTER(ctypes.c_double)) for n in ("hhb", "hhs", "omega_z", "gg", "taux", "tauy")]
+
+
+class PgSolveStats(ctypes.Structure):
+    _fields_ = [
+        ("iters", ctypes.c_uint32), ("solver", ctypes.c_uint32), ("fallback", ctypes.c_uint32),
+        ("residual_max", ctypes.c_double), ("residual_rms", ctypes.c_double),
+        ("divergence_max", ctypes.c_double), ("divergence_rms", ctypes.c_double),
+        ("rel_residual", ctypes.c_double), ("solve_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library() -> ctypes.CDLL:
+    """Load libpawsim_gpu.so; raise if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m pawsim_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    d = ctypes.POINTER(ctypes.c_double)
+    lib.pg_create.argtypes = [P(PgConfig), P(PgStatic), P(ctypes.c_void_p)]
+    lib.pg_destroy.argtypes = [ctypes.c_void_p]
+    lib.pg_destroy.restype = None
+    lib.pg_last_error.argtypes = [ctypes.c_void_p]
+    lib.pg_last_error.restype = ctypes.c_char_p
+    lib.pg_upload_state.argtypes = [ctypes.c_void_p, d, d, d]
+    lib.pg_download_state.argtypes = [ctypes.c_void_p, d, d, d]
+    lib.pg_upload_pi.argtypes = [ctypes.c_void_p, d]
+    lib.pg_download_pi.argtypes = [ctypes.c_void_p, d]
+    lib.pg_tendency.argtypes = [ctypes.c_void_p, d, d, d]
+    lib.pg_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    lib.pg_pressure_correct.argtypes = [ctypes.c_void_p, P(PgSolveStats)]
+    lib.pg_solve_pressure.argtypes = [ctypes.c_void_p, ctypes.c_uint32, d, d, P(PgSolveStats)]
+    lib.pg_time.argtypes = [ctypes.c_void_p]
+    lib.pg_time.restype = ctypes.c_double
+    lib.pg_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.pg_synchronize.argtypes = [ctypes.c_void_p]
+    lib.pg_bench_kernel.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    _lib = lib
+    return lib
+
+
+def _dptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _c64(a) -> Optional[np.ndarray]:
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Model:
+    """One device model (a pg_handle).  Arrays are [Nlay, Nx, Ny] float64."""
+
+    def __init__(self, params: dict, hhb, omegaz, gg, hhs=None, taux=None, tauy=None,
+                 solver: Optional[int] = None):
+        self.lib = load_library()
+        p = dict(params)
+        c = PgConfig()
+        c.Nx, c.Ny, c.Nlay = int(p["Nx"]), int(p["Ny"]), int(p["Nlay"])
+        c.Lx = float(p.get("Lx", 0.0))
+        c.Ly = float(p["Ly"])
+        c.dt = float(p["dt"])
+        c.timeSteppingScheme = int(p.get("timeSteppingScheme", 5))
+        c.momentumScheme = int(p.get("momentumScheme", 3))
+        c.thicknessScheme = int(p.get("thicknessScheme", 0))
+        c.useWallEW = int(p.get("useWallEW", 0))
+        c.useWallNS = int(p.get("useWallNS", 0))
+        c.h0 = float(p.get("h0", 0.0))
+        c.A2 = float(p.get("A2", 0.0))
+        c.A4 = float(p.get("A4", 0.0))
+        c.linDragCoeff = float(p.get("linDragCoeff", 0.0))
+        c.quadDragCoeff = float(p.get("quadDragCoeff", 0.0))
+        if "pressureSolver" in p:
+            c.pressureSolver = int(p["pressureSolver"])
+        else:
+            c.pressureSolver = SOLVER_MG_REPLICA if int(p.get("use_MG", 0)) else SOLVER_SOR
+        if solver is not None:
+            c.pressureSolver = solver
+        c.pi_tol = float(p.get("tol", 1e-10))
+        c.SOR_rp = float(p.get("SOR_rp", 1.0))
+        c.omega_MG = float(p.get("omega_MG", 2.0 / 3.0))
+        c.maxiters = int(p.get("maxiters", 10000))
+        c.use_fullMG = int(p.get("use_fullMG", 0))
+        c.pressureTiming = int(p.get("pressureTiming", 0))
+        c.mg_nu1 = int(p.get("mg_nu1", 0))
+        c.mg_nu2 = int(p.get("mg_nu2", 0))
+        self.cfg = c
+        self.shape = (c.Nlay, c.Nx, c.Ny)
+        self._keep = [_c64(hhb), _c64(hhs), _c64(omegaz), _c64(gg), _c64(taux), _c64(tauy)]
+        st = PgStatic(*[_dptr(a) for a in self._keep])
+        h = ctypes.c_void_p()
+        rc = self.lib.pg_create(ctypes.byref(c), ctypes.byref(st), ctypes.byref(h))
+        if rc != PG_OK:
+            raise RuntimeError(f"pg_create failed with code {rc}")
+        self.h = h
+
+    @classmethod
+    def from_setup(cls, setup, solver: Optional[int] = None, **overrides) -> "Model":
+        params = dict(setup.params)
+        params.update(overrides)
+        m = cls(params, setup.hhb, setup.omegaz, setup.gg, hhs=setup.hhs, taux=setup.taux, tauy=setup.tauy,
+                solver=solver)
+        L, X, Y = m.shape
+        z = np.zeros((L, X, Y))
+        m.upload_state(setup.u if setup.u is not None else z, setup.v if setup.v is not None else z, setup.h)
+        return m
+
+    def _check(self, rc: int, what: str, allow_noconv: bool = False) -> int:
+        if rc == PG_OK or (allow_noconv and rc == PG_ENOCONV):
+            return rc
+        msg = self.lib.pg_last_error(self.h).decode()
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+    def upload_state(self, u, v, h) -> None:
+        u, v, h = _c64(u), _c64(v), _c64(h)
+        self._check(self.lib.pg_upload_state(self.h, _dptr(u), _dptr(v), _dptr(h)), "pg_upload_state")
+
+    def state(self):
+        u, v, h = (np.empty(self.shape) for _ in range(3))
+        self._check(self.lib.pg_download_state(self.h, _dptr(u), _dptr(v), _dptr(h)), "pg_download_state")
+        return u, v, h
+
+    def pi(self) -> np.ndarray:
+        p = np.empty(self.shape[1:])
+        self._check(self.lib.pg_download_pi(self.h, _dptr(p)), "pg_download_pi")
+        return p
+
+    def upload_pi(self, p) -> None:
+        p = _c64(p)
+        self._check(self.lib.pg_upload_pi(self.h, _dptr(p)), "pg_upload_pi")
+
+    def tendency(self):
+        du, dv, dh = (np.empty(self.shape) for _ in range(3))
+        self._check(self.lib.pg_tendency(self.h, _dptr(du), _dptr(dv), _dptr(dh)), "pg_tendency")
+        return du, dv, dh
+
+    def step(self, n: int) -> None:
+        self._check(self.lib.pg_step(self.h, n), "pg_step")
+
+    def pressure_correct(self) -> dict:
+        st = PgSolveStats()
+        self._check(self.lib.pg_pressure_correct(self.h, ctypes.byref(st)), "pg_pressure_correct", allow_noconv=True)
+        return st.as_dict()
+
+    def solve_pressure(self, solver: int, pi, rhs):
+        pi = np.array(pi, dtype=np.float64, copy=True)
+        rhs = _c64(rhs)
+        st = PgSolveStats()
+        self._check(self.lib.pg_solve_pressure(self.h, solver, _dptr(pi), _dptr(rhs), ctypes.byref(st)),
+                    "pg_solve_pressure", allow_noconv=True)
+        return pi, st.as_dict()
+
+    def set_stream(self, stream_handle: int) -> None:
+        self._check(self.lib.pg_set_stream(self.h, ctypes.c_void_p(stream_handle)), "pg_set_stream")
+
+    def synchronize(self) -> None:
+        self._check(self.lib.pg_synchronize(self.h), "pg_synchronize")
+
+    def bench_kernel(self, which: int, reps: int) -> None:
+        self._check(self.lib.pg_bench_kernel(self.h, which, reps), "pg_bench_kernel")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.pg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

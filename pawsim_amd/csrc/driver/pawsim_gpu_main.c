@@ -1,0 +1,394 @@
+/*
+ * pawsim_gpu_main.c -- host driver: `pawsim_gpu <input_file> <output_dir>`.
+ *
+ * Reads the same key/value parameter file and raw float64 inputs as the
+ * reference executable (config.c:129-314, state.c:957-1100), runs the same
+ * sequence as pawsim.c main (355-528) -- preliminary rigid-lid correction,
+ * n = 0 output, then step / correction / time-scheduled output -- and
+ * writes U/V/H/W/P files in the reference layout (diagnostics.c:874-908,
+ * io.c:17-129).  All model arithmetic happens on the GPU behind the C ABI
+ * of libpawsim_gpu.so; this file only parses, sequences and writes.
+ *
+ * Extensions: pressureSolver 3 (GPU multigrid, FP64) and 4 (GPU multigrid,
+ * FP32 V-cycles in FP64 iterative refinement); tol is then the relative
+ * residual ||rhs - L pi|| / ||rhs||.  mg_nu1 / mg_nu2 set the smoothing counts.
+ */
+#include "pawsim_gpu.h"
+
+#include <math.h>
+#include <stdbool.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+typedef struct drv_config {
+  pg_config pg;
+  unsigned Nt, tauNrecs;
+  double tmin, tmax, savefrequency, hsml, hbbl, A2smag, A4smag, linDragSurf, quadDragSurf;
+  bool useRL, writeInit, useMG, solverGiven;
+  char unsupported[128];
+  char dir[512];
+  char hInit[256], uInit[256], vInit[256], hbFile[256], hsFile[256], omFile[256], gFile[256];
+  char tauxFile[256], tauyFile[256];
+} drv_config;
+
+static double now_s(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static void note_unsupported(drv_config *c, const char *key)
+{
+  if (!c->unsupported[0]) snprintf(c->unsupported, sizeof(c->unsupported), "%s", key);
+}
+
+/* Defaults of config.c:42-122 for the keys the timestep reads. */
+static void defaults(drv_config *c)
+{
+  memset(c, 0, sizeof(*c));
+  c->pg.timeSteppingScheme = 5;
+  c->pg.momentumScheme = 3;
+  c->pg.thicknessScheme = 0;
+  c->pg.pi_tol = 1e-10;
+  c->pg.SOR_rp = 1.0;
+  c->pg.omega_MG = 2.0 / 3.0;
+  c->pg.maxiters = 10000;
+  c->tmin = -1;
+  c->tauNrecs = 1;
+  c->useRL = true;
+  strcpy(c->dir, ".");
+}
+
+static void copy_name(char *dst, const char *src)
+{
+  snprintf(dst, 256, "%s", src);
+}
+
+/* Whitespace key/value pairs; unknown keys are ignored (config.c:172-282). */
+static bool read_config(const char *fname, drv_config *c)
+{
+  char key[256], val[256];
+  const char *slash;
+  FILE *f = fopen(fname, "r");
+  if (!f) {
+    fprintf(stderr, "ERROR: Could not open input parameter file %s\n", fname);
+    return false;
+  }
+  slash = strrchr(fname, '/');
+  if (slash) snprintf(c->dir, sizeof(c->dir), "%.*s", (int)(slash - fname), fname);
+  while (fscanf(f, "%255s", key) == 1) {
+    if (fscanf(f, "%255s", val) != 1) {
+      fprintf(stderr, "ERROR: Missing value for parameter %s\n", key);
+      fclose(f);
+      return false;
+    }
+    unsigned long uv = strtoul(val, NULL, 10);
+    double dv = strtod(val, NULL);
+    int iv = atoi(val);
+    if (!strcmp(key, "Nlay")) c->pg.Nlay = (uint32_t)uv;
+    else if (!strcmp(key, "Nx")) c->pg.Nx = (uint32_t)uv;
+    else if (!strcmp(key, "Ny")) c->pg.Ny = (uint32_t)uv;
+    else if (!strcmp(key, "Nt")) c->Nt = (unsigned)uv;
+    else if (!strcmp(key, "timeSteppingScheme")) c->pg.timeSteppingScheme = (uint32_t)uv;
+    else if (!strcmp(key, "momentumScheme")) c->pg.momentumScheme = (uint32_t)uv;
+    else if (!strcmp(key, "thicknessScheme")) c->pg.thicknessScheme = (uint32_t)uv;
+    else if (!strcmp(key, "maxiters")) c->pg.maxiters = (uint32_t)uv;
+    else if (!strcmp(key, "tauNrecs")) c->tauNrecs = (unsigned)uv;
+    else if (!strcmp(key, "mg_nu1")) c->pg.mg_nu1 = (uint32_t)uv;
+    else if (!strcmp(key, "mg_nu2")) c->pg.mg_nu2 = (uint32_t)uv;
+    else if (!strcmp(key, "Lx")) c->pg.Lx = dv;
+    else if (!strcmp(key, "Ly")) c->pg.Ly = dv;
+    else if (!strcmp(key, "dt")) c->pg.dt = dv;
+    else if (!strcmp(key, "tmin")) c->tmin = dv;
+    else if (!strcmp(key, "tmax")) c->tmax = dv;
+    else if (!strcmp(key, "savefrequency")) c->savefrequency = dv;
+    else if (!strcmp(key, "h0")) c->pg.h0 = dv;
+    else if (!strcmp(key, "hsml")) c->hsml = dv;
+    else if (!strcmp(key, "hbbl")) c->hbbl = dv;
+    else if (!strcmp(key, "A2")) c->pg.A2 = dv;
+    else if (!strcmp(key, "A4")) c->pg.A4 = dv;
+    else if (!strcmp(key, "A2smag")) c->A2smag = dv;
+    else if (!strcmp(key, "A4smag")) c->A4smag = dv;
+    else if (!strcmp(key, "linDragCoeff")) c->pg.linDragCoeff = dv;
+    else if (!strcmp(key, "quadDragCoeff")) c->pg.quadDragCoeff = dv;
+    else if (!strcmp(key, "linDragSurf")) c->linDragSurf = dv;
+    else if (!strcmp(key, "quadDragSurf")) c->quadDragSurf = dv;
+    else if (!strcmp(key, "tol")) c->pg.pi_tol = dv;
+    else if (!strcmp(key, "SOR_rp") || !strcmp(key, "SOR_rp_min")) c->pg.SOR_rp = dv;
+    else if (!strcmp(key, "omega_MG")) c->pg.omega_MG = dv;
+    else if (!strcmp(key, "useWallEW")) c->pg.useWallEW = iv != 0;
+    else if (!strcmp(key, "useWallNS")) c->pg.useWallNS = iv != 0;
+    else if (!strcmp(key, "useRL")) c->useRL = iv != 0;
+    else if (!strcmp(key, "use_fullMG")) c->pg.use_fullMG = iv != 0;
+    else if (!strcmp(key, "pressureTiming")) c->pg.pressureTiming = iv != 0;
+    else if (!strcmp(key, "writeInitDiagnostics")) c->writeInit = iv != 0;
+    else if (!strcmp(key, "use_MG")) c->useMG = iv != 0;
+    else if (!strcmp(key, "pressureSolver")) { c->pg.pressureSolver = (uint32_t)uv; c->solverGiven = true; }
+    else if (!strcmp(key, "hInitFile")) copy_name(c->hInit, val);
+    else if (!strcmp(key, "uInitFile")) copy_name(c->uInit, val);
+    else if (!strcmp(key, "vInitFile")) copy_name(c->vInit, val);
+    else if (!strcmp(key, "hbFile")) copy_name(c->hbFile, val);
+    else if (!strcmp(key, "hsFile")) copy_name(c->hsFile, val);
+    else if (!strcmp(key, "OmegazFile")) copy_name(c->omFile, val);
+    else if (!strcmp(key, "gFile")) copy_name(c->gFile, val);
+    else if (!strcmp(key, "tauxFile")) copy_name(c->tauxFile, val);
+    else if (!strcmp(key, "tauyFile")) copy_name(c->tauyFile, val);
+    else if (!strcmp(key, "useTracer") || !strcmp(key, "useBuoyancy") || !strcmp(key, "restart")
+             || !strcmp(key, "windFeedback") || !strcmp(key, "useRandomForcing") || !strcmp(key, "useTrad")) {
+      if (iv != 0) note_unsupported(c, key);
+    } else if (!strcmp(key, "FbaroXFile") || !strcmp(key, "FbaroYFile") || !strcmp(key, "wDiaFile")
+               || !strcmp(key, "uTimeFile") || !strcmp(key, "vTimeFile") || !strcmp(key, "hTimeFile")
+               || !strcmp(key, "eTimeFile") || !strcmp(key, "diaViscUFile") || !strcmp(key, "diaViscVFile")
+               || !strcmp(key, "bFluxFile") || !strcmp(key, "diaDiffFile")) {
+      note_unsupported(c, key);
+    } else if (!strcmp(key, "convDiff") || !strcmp(key, "savefreqAvg") || !strcmp(key, "savefreqEZ")
+               || !strcmp(key, "savefreqUMom") || !strcmp(key, "savefreqVMom") || !strcmp(key, "savefreqThic")
+               || !strcmp(key, "savefreqEnergy") || !strcmp(key, "savefreqTracer")) {
+      if (dv != 0) note_unsupported(c, key);
+    }
+  }
+  fclose(f);
+  if (!c->solverGiven) c->pg.pressureSolver = c->useMG ? PG_SOLVER_MG_REPLICA : PG_SOLVER_SOR;  /* config.c:289-297 */
+  if (c->hsml > 0) note_unsupported(c, "hsml");
+  if (c->hbbl > 0) note_unsupported(c, "hbbl");
+  if (c->A2smag > 0 || c->A4smag > 0) note_unsupported(c, "Smagorinsky viscosity");
+  if (c->linDragSurf > 0 || c->quadDragSurf > 0) note_unsupported(c, "surface drag");
+  if (c->tauNrecs != 1) note_unsupported(c, "tauNrecs");
+  if (!c->useRL) note_unsupported(c, "useRL 0");
+  return true;
+}
+
+static void resolve(const drv_config *c, const char *name, char *out, size_t n)
+{
+  if (name[0] == '/') snprintf(out, n, "%s", name);
+  else snprintf(out, n, "%s/%s", c->dir, name);
+}
+
+/* Raw float64 reader: m x n (x fastest), optionally `layers` fastest
+ * (defs.c:191-211, 291-309); output a[(layer*m + i)*n + j]. */
+static double *read_raw(const drv_config *c, const char *name, unsigned m, unsigned n, unsigned layers)
+{
+  char path[1024];
+  size_t cnt = (size_t)m * n * layers;
+  double *buf = malloc(cnt * sizeof(double)), *a = malloc(cnt * sizeof(double));
+  FILE *f;
+  resolve(c, name, path, sizeof(path));
+  f = fopen(path, "rb");
+  if (!f || fread(buf, sizeof(double), cnt, f) != cnt) {
+    fprintf(stderr, "ERROR: Could not read %zu values from %s\n", cnt, path);
+    if (f) fclose(f);
+    free(buf);
+    free(a);
+    return NULL;
+  }
+  fclose(f);
+  for (unsigned j = 0; j < n; j++)
+    for (unsigned i = 0; i < m; i++)
+      for (unsigned k = 0; k < layers; k++) a[((size_t)k * m + i) * n + j] = buf[((size_t)j * m + i) * layers + k];
+  free(buf);
+  return a;
+}
+
+/* Write an Nx x Ny field (element (i,j) at a[i*Ny+j]) with x fastest. */
+static bool write_field(const char *dir, const char *name, const double *a, unsigned Nx, unsigned Ny)
+{
+  char path[1024];
+  FILE *f;
+  snprintf(path, sizeof(path), "%s/%s", dir, name);
+  f = fopen(path, "wb");
+  if (!f) return false;
+  for (unsigned j = 0; j < Ny; j++)
+    for (unsigned i = 0; i < Nx; i++) fwrite(&a[(size_t)i * Ny + j], sizeof(double), 1, f);
+  fclose(f);
+  return true;
+}
+
+typedef struct model {
+  drv_config c;
+  pg_handle *h;
+  double *u, *v, *hh, *pi, *zero, t;
+  unsigned n_saves, solves, iters_total;
+  double solve_ms;
+} model;
+
+/* pawsim_diagnostics_write_model_state (diagnostics.c:874-908). */
+static bool write_state(model *M, const char *dir, unsigned n)
+{
+  const unsigned Nx = M->c.pg.Nx, Ny = M->c.pg.Ny, L = M->c.pg.Nlay;
+  const size_t N = (size_t)Nx * Ny;
+  char name[256];
+  if (pg_download_state(M->h, M->u, M->v, M->hh) || pg_download_pi(M->h, M->pi)) return false;
+  for (unsigned k = 0; k < L; k++) {
+    snprintf(name, sizeof(name), "U%u_n=%u.dat", k, n);
+    if (!write_field(dir, name, M->u + k * N, Nx, Ny)) return false;
+    snprintf(name, sizeof(name), "V%u_n=%u.dat", k, n);
+    if (!write_field(dir, name, M->v + k * N, Nx, Ny)) return false;
+    snprintf(name, sizeof(name), "H%u_n=%u.dat", k, n);
+    if (!write_field(dir, name, M->hh + k * N, Nx, Ny)) return false;
+    snprintf(name, sizeof(name), "W%u_n=%u.dat", k, n);  /* diapycnal velocity: zero on this path */
+    if (!write_field(dir, name, M->zero, Nx, Ny)) return false;
+  }
+  snprintf(name, sizeof(name), "W%u_n=%u.dat", L, n);
+  if (!write_field(dir, name, M->zero, Nx, Ny)) return false;
+  snprintf(name, sizeof(name), "P_n=%u.dat", n);
+  return write_field(dir, name, M->pi, Nx, Ny);
+}
+
+static const char *solver_name(uint32_t s)
+{
+  switch (s) {
+    case PG_SOLVER_SOR: return "SOR";
+    case PG_SOLVER_MG_REPLICA: return "MG_GATHERED";
+    case PG_SOLVER_GPU_MG: return "GPU_MG";
+    case PG_SOLVER_GPU_MG_MIXED: return "GPU_MG_MIXED";
+    default: return "UNKNOWN";
+  }
+}
+
+/* pressureCorrectTimed (pawsim.c:76-113). */
+static int correct(model *M, const char *phase, unsigned n)
+{
+  pg_solve_stats st;
+  double t0 = now_s(), el;
+  int rc = pg_pressure_correct(M->h, &st);
+  if (rc && rc != PG_ENOCONV) {
+    fprintf(stderr, "ERROR: pressure correction failed: %s\n", pg_last_error(M->h));
+    return -1;
+  }
+  el = now_s() - t0;
+  M->solves++;
+  M->iters_total += st.iters;
+  M->solve_ms += st.solve_ms;
+  if (M->c.pg.pressureTiming)
+    printf("PAWSIM pressure solve: phase=%s n=%u solver=%s iters=%u walltime=%0.6f s residual_max=%0.6e "
+           "residual_rms=%0.6e div_max=%0.6e div_rms=%0.6e rel_residual=%0.6e solve_ms=%0.4f\n",
+           phase, n, solver_name(st.solver), st.iters, el, st.residual_max, st.residual_rms, st.divergence_max,
+           st.divergence_rms, st.rel_residual, st.solve_ms);
+  return (int)st.iters;
+}
+
+int main(int argc, char **argv)
+{
+  model M;
+  pg_static st;
+  unsigned k, tcount;
+  int it;
+  double *hhb, *hhs = NULL, *om, *gg, *taux = NULL, *tauy = NULL, t_all, t_loop = 0;
+  drv_config *c = &M.c;
+  memset(&M, 0, sizeof(M));
+  if (argc < 3) {
+    fprintf(stderr, "Usage: pawsim_gpu <input_file> <output_dir>\n");
+    return 1;
+  }
+  defaults(c);
+  if (!read_config(argv[1], c)) return 1;
+  if (c->unsupported[0]) {
+    fprintf(stderr, "ERROR: '%s' is outside the GPU timestep path\n", c->unsupported);
+    return 1;
+  }
+  if (c->pg.Nlay == 0 || c->pg.Nx == 0 || c->pg.Ny == 0 || c->pg.Ly <= 0) {
+    fprintf(stderr, "ERROR: Nlay, Nx, Ny and Ly must all be specified and positive\n");
+    return 1;
+  }
+  /* setGridDimensions / setTimeParameters (pawsim.c:126-196) */
+  if (c->pg.Lx <= 0) c->pg.Lx = c->pg.Ly * c->pg.Nx / c->pg.Ny;
+  tcount = (c->Nt > 0) + (c->tmax > 0) + (c->pg.dt > 0);
+  if (tcount != 2) {
+    fprintf(stderr, "Must specify exactly two of {Nt, tmax, dt}\n");
+    return 1;
+  }
+  if (c->tmin < 0) c->tmin = 0;
+  if (c->Nt == 0) c->Nt = (unsigned)ceil((c->tmax - c->tmin) / c->pg.dt);
+  if (c->pg.dt == 0) c->pg.dt = (c->tmax - c->tmin) / c->Nt;
+  if (c->savefrequency == 0) c->savefrequency = c->pg.dt;
+  M.t = c->tmin;
+
+  const unsigned Nx = c->pg.Nx, Ny = c->pg.Ny, L = c->pg.Nlay;
+  const size_t N = (size_t)Nx * Ny;
+  hhb = c->hbFile[0] ? read_raw(c, c->hbFile, Nx, Ny, 1) : calloc(N, sizeof(double));
+  if (c->hsFile[0]) hhs = read_raw(c, c->hsFile, Nx, Ny, 1);
+  om = c->omFile[0] ? read_raw(c, c->omFile, Nx + 1, Ny + 1, 1) : calloc((Nx + 1) * (size_t)(Ny + 1), sizeof(double));
+  gg = malloc(L * sizeof(double));
+  for (k = 0; k < L; k++) gg[k] = 9.81;  /* state.c:986 */
+  if (c->gFile[0]) {
+    char path[1024];
+    FILE *f;
+    resolve(c, c->gFile, path, sizeof(path));
+    f = fopen(path, "rb");
+    if (!f || fread(gg, sizeof(double), L, f) != L) {
+      fprintf(stderr, "ERROR: Could not read %s\n", path);
+      return 1;
+    }
+    fclose(f);
+  }
+  if (c->tauxFile[0] || c->tauyFile[0]) {  /* useWind (config.c:298) */
+    taux = c->tauxFile[0] ? read_raw(c, c->tauxFile, Nx, Ny, 1) : calloc(N, sizeof(double));
+    tauy = c->tauyFile[0] ? read_raw(c, c->tauyFile, Nx, Ny, 1) : calloc(N, sizeof(double));
+  }
+  M.u = c->uInit[0] ? read_raw(c, c->uInit, Nx, Ny, L) : calloc(L * N, sizeof(double));
+  M.v = c->vInit[0] ? read_raw(c, c->vInit, Nx, Ny, L) : calloc(L * N, sizeof(double));
+  M.hh = c->hInit[0] ? read_raw(c, c->hInit, Nx, Ny, L) : calloc(L * N, sizeof(double));
+  M.pi = calloc(N, sizeof(double));
+  M.zero = calloc(N, sizeof(double));
+  if (!hhb || !om || !M.u || !M.v || !M.hh || (c->hsFile[0] && !hhs)) return 1;
+  st.hhb = hhb;
+  st.hhs = hhs;
+  st.omega_z = om;
+  st.gg = gg;
+  st.taux = taux;
+  st.tauy = tauy;
+  if (pg_create(&c->pg, &st, &M.h) != PG_OK) {
+    fprintf(stderr, "ERROR: Could not initialize the GPU model: %s\n", M.h ? pg_last_error(M.h) : "");
+    return 1;
+  }
+  if (pg_upload_state(M.h, M.u, M.v, M.hh) != PG_OK) {
+    fprintf(stderr, "ERROR: state upload failed: %s\n", pg_last_error(M.h));
+    return 1;
+  }
+  printf("PAWSIM parsed %u x %u x %u AWSIM input\n", Nx, Ny, L);
+  t_all = now_s();
+  /* preliminary correction (pawsim.c:282-295) */
+  it = correct(&M, "prelim", 0);
+  if (it < 0) return 1;
+  printf("PAWSIM pressure iterations: %d\n", it);
+  if (!write_state(&M, argv[2], 0)) {
+    fprintf(stderr, "Unable to write model initial state\n");
+    return 1;
+  }
+  M.n_saves = 1;
+  for (unsigned n = 1; n <= c->Nt; n++) {
+    double ts = now_s();
+    if (pg_step(M.h, n) != PG_OK) {
+      fprintf(stderr, "ERROR: PAWSIM time step failed at n=%u: %s\n", n, pg_last_error(M.h));
+      return 1;
+    }
+    M.t += c->pg.dt;
+    it = correct(&M, "step", n);
+    if (it < 0) return 1;
+    pg_synchronize(M.h);
+    t_loop += now_s() - ts;
+    if ((unsigned)it == c->pg.maxiters) fprintf(stderr, "WARNING: PAWSIM pressure solve reached maxiters at n=%u\n", n);
+    /* writeScheduledModelState (pawsim.c:318-350) */
+    if (c->savefrequency > 0) {
+      const double eps = 1e-9 * fabs(c->pg.dt);
+      for (;;) {
+        double tsave = M.n_saves * c->savefrequency;
+        if (M.t + eps < tsave) break;
+        if (!write_state(&M, argv[2], M.n_saves)) {
+          fprintf(stderr, "Unable to write model state n=%u\n", M.n_saves);
+          return 1;
+        }
+        M.n_saves++;
+      }
+    }
+  }
+  printf("GPU timing: steps=%u loop_seconds=%0.6f solve_ms_total=%0.3f total_seconds=%0.6f mean_iters=%0.2f\n",
+         c->Nt, t_loop, M.solve_ms, now_s() - t_all, M.solves ? (double)M.iters_total / M.solves : 0.0);
+  pg_destroy(M.h);
+  free(hhb); free(hhs); free(om); free(gg); free(taux); free(tauy);
+  free(M.u); free(M.v); free(M.hh); free(M.pi); free(M.zero);
+  return 0;
+}

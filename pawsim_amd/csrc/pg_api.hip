@@ -114,10 +114,27 @@ int run_solver(pg_handle* h, uint32_t solver, pg_solve_stats* st)
 
 extern "C" {
 
+static int create_body(const pg_config* cfg, const pg_static* st, pg_handle** out);
+
 int pg_create(const pg_config* cfg, const pg_static* st, pg_handle** out)
 {
   if (!out) return PG_EARG;
   *out = nullptr;
+  pg_handle* h = nullptr;
+  const int rc = create_body(cfg, st, &h);
+  if (rc != PG_OK) {
+    if (h) {
+      fprintf(stderr, "pg_create: %s\n", h->err.c_str());
+      pg_destroy(h);
+    }
+    return rc;
+  }
+  *out = h;
+  return PG_OK;
+}
+
+static int create_body(const pg_config* cfg, const pg_static* st, pg_handle** out)
+{
   if (!cfg || !st || !st->hhb || !st->omega_z || !st->gg) return PG_EARG;
   std::string why;
   if (validate(cfg, why)) {
@@ -223,17 +240,17 @@ int pg_create(const pg_config* cfg, const pg_static* st, pg_handle** out)
 void pg_destroy(pg_handle* h)
 {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   pg_mg_free(h);
   double* bufs[] = {h->u, h->v, h->h, h->fu[0], h->fu[1], h->fu[2], h->fv[0], h->fv[1], h->fv[2], h->fh[0], h->fh[1],
                     h->fh[2], h->hhb, h->hhs, h->omz, h->taux, h->tauy, h->Hc, h->Hw, h->Hs, h->pi, h->rhs,
                     h->sor_c, h->partials};
-  for (double* p : bufs) hipFree(p);
-  hipFree(h->ctl);
-  if (h->ctl_host) hipHostFree(h->ctl_host);
-  if (h->ev0) hipEventDestroy(h->ev0);
-  if (h->ev1) hipEventDestroy(h->ev1);
-  if (h->own_stream) hipStreamDestroy(h->own_stream);
+  for (double* p : bufs) (void)hipFree(p);
+  (void)hipFree(h->ctl);
+  if (h->ctl_host) (void)hipHostFree(h->ctl_host);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
 }
 

@@ -554,9 +554,9 @@ void pg_mg_free(pg_handle* h)
 {
   for (MgLevel& l : h->lev) {
     double* d[] = {l.tmp, l.Hc, l.Ow, l.Os, l.Osum, l.rOsum, l.wmm, l.wmp, l.wpm, l.wpp, l.pi, l.rhs};
-    for (double* p : d) hipFree(p);
+    for (double* p : d) (void)hipFree(p);
     float* f[] = {l.pi32, l.tmp32, l.rhs32, l.Ow32, l.Os32, l.rOsum32};
-    for (float* p : f) hipFree(p);
+    for (float* p : f) (void)hipFree(p);
   }
   h->lev.clear();
   h->mg_built = false;

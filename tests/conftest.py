@@ -1,0 +1,57 @@
+import glob
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "tests", "golden"))
+
+ORACLE_EXE = os.path.join(REPO, "oracle", "_build", "pawsim_oracle")
+ORACLE_LIB = os.path.join(REPO, "oracle", "_build", "liboracle.so")
+GOLDEN_DIR = os.path.join(REPO, "tests", "golden")
+GOLDEN = sorted(glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+
+
+def golden_ids():
+    return [os.path.splitext(os.path.basename(p))[0] for p in GOLDEN]
+
+
+def load_golden(name):
+    from make_golden import load_setup
+
+    z = np.load(os.path.join(GOLDEN_DIR, name + ".npz"), allow_pickle=False)
+    return z, load_setup(z)
+
+
+def run_exe(exe, infile, outdir, timeout=600):
+    os.makedirs(outdir, exist_ok=True)
+    r = subprocess.run([exe, infile, outdir], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, f"{exe} failed: {r.stderr[-2000:]}"
+    return r.stdout
+
+
+def parse_iters(stdout):
+    return [int(l.split("iters=")[1].split()[0]) for l in stdout.splitlines() if "iters=" in l and "phase=" in l]
+
+
+def same_bits(a, b):
+    """Bitwise equality, treating +0 and -0 as equal."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return a.shape == b.shape and bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+@pytest.fixture(scope="session")
+def oracle_exe():
+    if not os.path.exists(ORACLE_EXE):
+        subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True)
+    return ORACLE_EXE
