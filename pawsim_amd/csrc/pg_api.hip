@@ -44,6 +44,19 @@ int download_owned(pg_handle* h, double* dst, const double* src, std::vector<dou
   return PG_OK;
 }
 
+// pawsim_state_exchange_prognostic_halos (state.c:929-950).  Every refresh
+// of the reference is repeated, also for fields whose owned cells did not
+// change: on a periodic-x / wall-y domain the corner ghosts are copied from
+// the previous refresh (field.c:137-150 before 158-198), so their contents
+// depend on the refresh history.
+hipError_t fill_prognostic(pg_handle* h)
+{
+  hipError_t e = pg_fill(h, h->u, h->G.L, PG_HALO_U);
+  if (e == hipSuccess) e = pg_fill(h, h->v, h->G.L, PG_HALO_V);
+  if (e == hipSuccess) e = pg_fill(h, h->h, h->G.L, PG_HALO_SCALAR);
+  return e;
+}
+
 int validate(const pg_config* c, std::string& why)
 {
   if (c->Nx < 4 || c->Ny < 4) return why = "Nx and Ny must be at least 4", PG_EARG;
@@ -348,9 +361,7 @@ int pg_step(pg_handle* h, uint32_t n)
     h->slot[2] = t1;
   }
   h->t += h->cfg.dt;
-  PG_CHECK(h, pg_fill(h, h->u, h->G.L, PG_HALO_U));
-  PG_CHECK(h, pg_fill(h, h->v, h->G.L, PG_HALO_V));
-  PG_CHECK(h, pg_fill(h, h->h, h->G.L, PG_HALO_SCALAR));
+  PG_CHECK(h, fill_prognostic(h));
   return PG_OK;
 }
 
@@ -363,7 +374,7 @@ int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
   const uint32_t solver = h->cfg.pressureSolver;
   // thickness constraint, then halo refresh (pressure.c:525-561)
   PG_CHECK(h, pg_launch_thickness(h));
-  PG_CHECK(h, pg_fill(h, h->h, h->G.L, PG_HALO_SCALAR));
+  PG_CHECK(h, fill_prognostic(h));
   // RHS with mean removal (pressure.c:570-606)
   PG_CHECK(h, pg_launch_rhs(h));
   PG_CHECK(h, pg_launch_serial_mean(h, h->rhs));
@@ -373,8 +384,7 @@ int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
   if (rc && rc != PG_ENOCONV) return rc;
   if (h->cfg.pressureTiming) PG_CHECK(h, pg_launch_norms(h, 0));
   PG_CHECK(h, pg_launch_correct(h));
-  PG_CHECK(h, pg_fill(h, h->u, h->G.L, PG_HALO_U));
-  PG_CHECK(h, pg_fill(h, h->v, h->G.L, PG_HALO_V));
+  PG_CHECK(h, fill_prognostic(h));
   if (h->cfg.pressureTiming) {
     PG_CHECK(h, pg_launch_norms(h, 1));
     PG_CHECK(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(PgCtl), hipMemcpyDeviceToHost, h->stream));

@@ -30,3 +30,31 @@ def test_gpu_matches_reference_golden(name, tmp_path):
             ref = z[f"{key}_n{n}"]
             assert same_bits(arr, ref), f"{key} at save {n}: max|diff|={np.max(np.abs(arr - ref)):.3e}"
     assert parse_iters(stdout) == [int(x) for x in z["solves"][:, 1]]
+
+
+TRAJECTORY_CASES = {
+    "gyre128_sor": dict(kind="gyre", nx=128, ny=128, nt=12, solver=0),
+    "gyre64_mg": dict(kind="gyre", nx=64, ny=64, nt=12, solver=1, A2=50.0, quadDragCoeff=2e-3),
+    "chan64x32_mg": dict(kind="channel", nx=64, ny=32, nt=12, dx=20e3, solver=1, linDragCoeff=1e-3),
+    "chan48x24_sor_quad": dict(kind="channel", nx=48, ny=24, nt=10, dx=20e3, solver=0, SOR_rp=1.8, quadDragCoeff=2e-3),
+    "per40x24_mg": dict(kind="periodic", nx=40, ny=24, nt=10, solver=1),
+}
+
+
+@pytest.mark.parametrize("name", sorted(TRAJECTORY_CASES))
+def test_gpu_matches_oracle_every_step(name, tmp_path, oracle_exe):
+    """Run GPU and oracle on the same input; every saved step must be bitwise equal."""
+    spec = dict(TRAJECTORY_CASES[name])
+    kind = spec.pop("kind")
+    setup = getattr(inputs, kind)(name=name, pressureTiming=1, **spec)
+    infile = setup.write(str(tmp_path / "in"))
+    print(f"[gpu trajectory] {name}", flush=True)
+    gout = run_exe(pawsim_amd.DRIVER_PATH, infile, str(tmp_path / "gpu"), timeout=300)
+    oout = run_exe(oracle_exe, infile, str(tmp_path / "ora"), timeout=300)
+    for n in range(int(setup.params["Nt"]) + 1):
+        g = inputs.read_outputs(str(tmp_path / "gpu"), setup.nx, setup.ny, setup.nlay, n)
+        o = inputs.read_outputs(str(tmp_path / "ora"), setup.nx, setup.ny, setup.nlay, n)
+        for key in g:
+            d = np.max(np.abs(g[key] - o[key]))
+            assert same_bits(g[key], o[key]), f"first mismatch: {key} at step {n}, max|diff|={d:.3e}"
+    assert parse_iters(gout) == parse_iters(oout)
