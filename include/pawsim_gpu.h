@@ -115,6 +115,13 @@ int pg_step(pg_handle *h, uint32_t n);
  * pressure.c:2254-2300). */
 int pg_pressure_correct(pg_handle *h, pg_solve_stats *stats);
 
+/* Refresh the prognostic halos once more, as the reference's rebuild of its
+ * work arrays after the preliminary correction does (pawsim.c:296-301 ->
+ * pawsim_work_load_state, work.c:538-559).  The device arrays serve as those
+ * work arrays; on a periodic-x / wall-y domain the extra refresh moves the
+ * corner ghost values exactly as the reload does. */
+int pg_refresh_halos(pg_handle *h);
+
 /* Standalone pressure solve of L pi = rhs on host arrays [Nx][Ny] with the
  * handle's operator; pi is the initial guess and the result
  * (solve_MG, multigrid.h:24; solve_pressure_sor, pressure.c:743). */

@@ -28,7 +28,7 @@ PG_OK, PG_EARG, PG_EHIP, PG_ENOCONV, PG_ENODEV = 0, -1, -2, -3, -4
 ABI_SYMBOLS = (
     "pg_create", "pg_destroy", "pg_last_error", "pg_upload_state", "pg_download_state",
     "pg_upload_pi", "pg_download_pi", "pg_tendency", "pg_step", "pg_pressure_correct",
-    "pg_solve_pressure", "pg_time", "pg_set_stream", "pg_synchronize", "pg_bench_kernel",
+    "pg_solve_pressure", "pg_refresh_halos", "pg_time", "pg_set_stream", "pg_synchronize", "pg_bench_kernel",
 )
 
 
@@ -88,6 +88,7 @@ def load_library() -> ctypes.CDLL:
     lib.pg_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
     lib.pg_pressure_correct.argtypes = [ctypes.c_void_p, P(PgSolveStats)]
     lib.pg_solve_pressure.argtypes = [ctypes.c_void_p, ctypes.c_uint32, d, d, P(PgSolveStats)]
+    lib.pg_refresh_halos.argtypes = [ctypes.c_void_p]
     lib.pg_time.argtypes = [ctypes.c_void_p]
     lib.pg_time.restype = ctypes.c_double
     lib.pg_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -208,6 +209,9 @@ class Model:
         self._check(self.lib.pg_solve_pressure(self.h, solver, _dptr(pi), _dptr(rhs), ctypes.byref(st)),
                     "pg_solve_pressure", allow_noconv=True)
         return pi, st.as_dict()
+
+    def refresh_halos(self) -> None:
+        self._check(self.lib.pg_refresh_halos(self.h), "pg_refresh_halos")
 
     def set_stream(self, stream_handle: int) -> None:
         self._check(self.lib.pg_set_stream(self.h, ctypes.c_void_p(stream_handle)), "pg_set_stream")

@@ -354,6 +354,7 @@ int main(int argc, char **argv)
   it = correct(&M, "prelim", 0);
   if (it < 0) return 1;
   printf("PAWSIM pressure iterations: %d\n", it);
+  pg_refresh_halos(M.h);  /* work-array rebuild after the preliminary correction */
   if (!write_state(&M, argv[2], 0)) {
     fprintf(stderr, "Unable to write model initial state\n");
     return 1;

@@ -387,6 +387,8 @@ int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
   PG_CHECK(h, fill_prognostic(h));
   if (h->cfg.pressureTiming) {
     PG_CHECK(h, pg_launch_norms(h, 1));
+    // the divergence diagnostic reloads the work arrays (pressure.c:703)
+    PG_CHECK(h, fill_prognostic(h));
     PG_CHECK(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(PgCtl), hipMemcpyDeviceToHost, h->stream));
     PG_CHECK(h, hipStreamSynchronize(h->stream));
     const double N = (double)h->G.nx * (double)h->G.ny;
@@ -422,6 +424,13 @@ int pg_solve_pressure(pg_handle* h, uint32_t solver, double* pi, const double* r
   st->residual_rms = std::sqrt(h->ctl_host->norm_ss[0] / ((double)h->G.nx * (double)h->G.ny));
   int rc2 = download_owned(h, pi, h->pi, stage);
   return rc2 ? rc2 : rc;
+}
+
+int pg_refresh_halos(pg_handle* h)
+{
+  if (!h) return PG_EARG;
+  PG_CHECK(h, fill_prognostic(h));
+  return PG_OK;
 }
 
 double pg_time(const pg_handle* h) { return h ? h->t : 0.0; }

@@ -37,7 +37,7 @@ TRAJECTORY_CASES = {
     "gyre64_mg": dict(kind="gyre", nx=64, ny=64, nt=12, solver=1, A2=50.0, quadDragCoeff=2e-3),
     "chan64x32_mg": dict(kind="channel", nx=64, ny=32, nt=12, dx=20e3, solver=1, linDragCoeff=1e-3),
     "chan48x24_sor_quad": dict(kind="channel", nx=48, ny=24, nt=10, dx=20e3, solver=0, SOR_rp=1.8, quadDragCoeff=2e-3),
-    "per40x24_mg": dict(kind="periodic", nx=40, ny=24, nt=10, solver=1),
+    "per40x24_mg": dict(kind="periodic_box", nx=40, ny=24, nt=10, solver=1),
 }
 
 
