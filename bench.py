@@ -1,0 +1,206 @@
+"""Benchmark of the PAWSIM timestep on one MI355X.
+
+Workload: the 4-layer zonal-jets domain, 1024 x 1024 doubly bounded, dx = 5 km
+(random-phase smooth topography, rms 200 m), AB3 dt = 300 s, rigid-lid
+pressure solved each step to a relative residual of 1e-8.  One "step" is one
+pg_step (tendency + AB3 update) plus one pg_pressure_correct (thickness fix,
+RHS, solve, velocity projection) with all fields resident in HBM.
+
+Prints one JSON line with steps/s, the per-solve milliseconds of the FP64
+GPU multigrid and of the mixed-precision multigrid, the roofline of the
+dominant kernel measured with events on the library's stream, and the CPU
+oracle timed on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "steps/s at 1024²×4 layers; pressure-solve ms to 1e-8 resid, learned vs plain MG"
+HBM_PEAK = 8.0e12  # B/s
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=40)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--nx", type=int, default=1024)
+    p.add_argument("--ny", type=int, default=1024)
+    p.add_argument("--nlay", type=int, default=4)
+    p.add_argument("--solver", type=int, default=4, help="3 FP64 GPU MG, 4 mixed-precision GPU MG")
+    p.add_argument("--tol", type=float, default=1e-8)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--cpu-steps", type=int, default=6)
+    return p.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(the multi-GPU decomposition is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import pawsim_amd
+    from pawsim_amd import inputs
+
+    setup = inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=a.steps + a.warmup, solver=a.solver, tol=a.tol,
+                        A2=10.0, maxiters=200)
+    m = pawsim_amd.Model.from_setup(setup)
+    # A dedicated (non-default) stream shared by the library and torch's
+    # events, so event timings bracket exactly the library's kernels.
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    m.set_stream(stream.cuda_stream)
+    # preliminary correction and work-array rebuild (pawsim.c:282-301)
+    m.pressure_correct()
+    m.refresh_halos()
+    n = 0
+    solve_ms, iters = [], []
+    for _ in range(a.warmup):
+        n += 1
+        m.step(n)
+        m.pressure_correct()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        n += 1
+        m.step(n)
+        st = m.pressure_correct()
+        solve_ms.append(st["solve_ms"])
+        iters.append(st["iters"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / a.steps
+    value = a.steps / elapsed
+
+    N = a.nx * a.ny
+    L = a.nlay
+    w = 8
+
+    # Per-solve milliseconds of both GPU multigrids on the same right-hand
+    # side: the current state's RHS is re-solved from pi = 0 by each solver.
+    _, _, _ = m.state()
+    rhs_solve = {}
+    pi0 = np.zeros((a.nx, a.ny))
+    rng_rhs = inputs.normal(11, N).reshape(a.nx, a.ny) * 1e-9
+    rng_rhs -= rng_rhs.mean()
+    for name, solver in (("gpu_mg_fp64", pawsim_amd.SOLVER_GPU_MG), ("gpu_mg_mixed", pawsim_amd.SOLVER_GPU_MG_MIXED)):
+        ms, its = [], []
+        for _ in range(3):
+            _, st = m.solve_pressure(solver, pi0, rng_rhs)
+            ms.append(st["solve_ms"])
+            its.append(st["iters"])
+        rhs_solve[name] = {"ms": float(np.median(ms)), "iters": int(np.median(its)),
+                           "rel_residual_target": a.tol}
+
+    # Roofline of the dominant kernels, timed with events on the library's stream.
+    def time_kernel(which, reps=20):
+        m.bench_kernel(which, 2)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        m.bench_kernel(which, reps)
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3  # seconds per launch
+
+    levels = []
+    x, y = a.nx, a.ny
+    while True:
+        levels.append(x * y)
+        if x == 1 and y == 1:
+            break
+        x, y = (x + 1) // 2 if x > 1 else 1, (y + 1) // 2 if y > 1 else 1
+    sum_nl = float(sum(levels))
+    t_tend = time_kernel(0)
+    t_sweep = time_kernel(5)
+    t_vc64 = time_kernel(4, reps=10)
+    t_vc32 = time_kernel(6, reps=10)
+    # Algorithmic bytes: tendency reads u,v,h and the four static fields and
+    # writes three tendencies per layer-point: (6L+4) N w.  One red-black
+    # sweep reads pi, rhs, Hc and writes pi: 4 N w.  One V(2,2) cycle:
+    # [(4(nu1+nu2)+5.5) sum_l N_l / N + 3] N w (w = 4 in FP32).
+    b_tend = (6 * L + 4) * N * w
+    b_sweep = 4 * N * w
+    b_vc = lambda ww: ((4 * 4 + 5.5) * sum_nl / N + 3) * N * ww  # noqa: E731
+    kernels = {
+        "tendency": (b_tend, t_tend),
+        "rbgs_sweep_fp64": (b_sweep, t_sweep),
+        "vcycle_fp64": (b_vc(8), t_vc64),
+        "vcycle_fp32": (b_vc(4), t_vc32),
+    }
+    # dominant per step: the solve's V-cycles (mixed) vs the tendency
+    dom = "vcycle_fp32" if a.solver == pawsim_amd.SOLVER_GPU_MG_MIXED else "vcycle_fp64"
+    per_step_cost = {"tendency": t_tend, dom: kernels[dom][1] * float(np.mean(iters))}
+    dominant = max(per_step_cost, key=per_step_cost.get)
+    B, t = kernels[dominant]
+    roofline = {"bound": "hbm", "achieved": B / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": B / t / HBM_PEAK, "traffic": None, "kernel": dominant,
+                "per_kernel": {k: {"algorithmic_bytes": b, "ms": tt * 1e3, "GB/s": b / tt / 1e9,
+                                   "frac": b / tt / HBM_PEAK} for k, (b, tt) in kernels.items()}}
+    m.close()
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(a)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"zonal jets {a.nx}x{a.ny}x{a.nlay}, walls, dx 5 km, AB3 dt 300 s, "
+                               f"rigid lid, GPU multigrid solver {a.solver} to rel. residual {a.tol:g}",
+                   "solver": a.solver, "tol": a.tol},
+        "pressure_solve": {"in_run_ms_mean": float(np.mean(solve_ms)), "in_run_iters_mean": float(np.mean(iters)),
+                           "same_rhs_from_zero": rhs_solve, "learned": None},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(a):
+    """The CPU oracle (restatement of the serial reference, gathered MG,
+    tol 1e-10 in its own max-norm criterion) on the same 1024^2 x 4 input for
+    a few steps; 1 core."""
+    from pawsim_amd import inputs
+
+    exe = os.path.join(REPO, "oracle", "_build", "pawsim_oracle")
+    setup = inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=a.cpu_steps, solver=1, tol=1e-10, A2=10.0,
+                        savefrequency=1e12)
+    with tempfile.TemporaryDirectory() as d:
+        infile = setup.write(d)
+        out = os.path.join(d, "out")
+        os.makedirs(out)
+        r = subprocess.run([exe, infile, out], capture_output=True, text=True, timeout=900)
+    mt = re.search(r"ORACLE timing: steps=(\d+) step_seconds=(\S+) solve_seconds=(\S+)", r.stdout)
+    if r.returncode != 0 or not mt:
+        return {"value": None, "unit": "steps/s", "cores": 1, "kind": "port", "sample": "oracle failed"}
+    steps, secs, solve = int(mt.group(1)), float(mt.group(2)), float(mt.group(3))
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except Exception:
+        pass
+    return {"value": steps / secs, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps of the same {a.nx}x{a.ny}x{a.nlay} jets input (gathered multigrid.c V(1,1), "
+                      f"tol 1e-10 max-norm), {secs:.2f} s, solve {solve:.2f} s; 1 of {os.cpu_count()} cores "
+                      f"({cpu_model})"}
+
+
+if __name__ == "__main__":
+    main()

@@ -60,6 +60,8 @@ typedef struct pg_config {
   int32_t use_fullMG;
   int32_t pressureTiming;      /* compute residual/divergence norms per solve */
   uint32_t mg_nu1, mg_nu2;     /* GPU MG smoothing counts (0 -> 2, 2) */
+  int32_t tendency_kernel;     /* 0: LDS-tiled interior + edge ring where it applies, 1: per-point kernel;
+                                  both give bitwise identical tendencies */
 } pg_config;
 
 /* Static inputs (state.c:1012-1047).  NULL hhs means a flat lid at 0;

@@ -72,6 +72,7 @@ int validate(const pg_config* c, std::string& why)
   const uint32_t s = c->pressureSolver;
   if (s != PG_SOLVER_SOR && s != PG_SOLVER_MG_REPLICA && s != PG_SOLVER_GPU_MG && s != PG_SOLVER_GPU_MG_MIXED)
     return why = "unknown pressureSolver", PG_EARG;
+  if (c->tendency_kernel != 0 && c->tendency_kernel != 1) return why = "tendency_kernel must be 0 or 1", PG_EARG;
   return PG_OK;
 }
 
@@ -163,6 +164,7 @@ static int create_body(const pg_config* cfg, const pg_static* st, pg_handle** ou
   if (!h) return PG_EARG;
   h->cfg = *cfg;
   pg_config& c = h->cfg;
+  h->fast_tendency = c.tendency_kernel == 0;
   if (c.Lx <= 0) c.Lx = c.Ly * c.Nx / c.Ny;  // pawsim.c:129-132
   PgGrid& G = h->G;
   G.nx = (int)c.Nx;
@@ -377,7 +379,10 @@ int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
   PG_CHECK(h, fill_prognostic(h));
   // RHS with mean removal (pressure.c:570-606)
   PG_CHECK(h, pg_launch_rhs(h));
-  PG_CHECK(h, pg_launch_serial_mean(h, h->rhs));
+  // The bit-replica solvers need the serial-order mean of pressure.c:55-72;
+  // the GPU multigrid is not bitwise and takes a tree-reduced mean.
+  if (solver == PG_SOLVER_SOR || solver == PG_SOLVER_MG_REPLICA) PG_CHECK(h, pg_launch_serial_mean(h, h->rhs));
+  else PG_CHECK(h, pg_launch_parallel_mean(h, h->rhs));
   PG_CHECK(h, pg_launch_sub_mean(h, h->rhs));
   PG_CHECK(h, pg_fill(h, h->rhs, 1, PG_HALO_SCALAR));
   int rc = run_solver(h, solver, st);

@@ -55,6 +55,7 @@ struct pg_handle {
   PgCtl *ctl_host = nullptr;           // pinned mirror
   std::vector<MgLevel> lev;            // [0] = coarsest
   bool mg_built = false;
+  bool fast_tendency = true;           // inner LDS kernel + edge ring (bitwise equal to the per-point kernel)
   double t = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
@@ -73,10 +74,15 @@ struct pg_handle {
 hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind);
 hipError_t pg_fill1(pg_handle* h, double* f);
 hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh);
+
+// ---- fast tendency (pg_tendency.hip) ----
+bool pg_tendency_fast_ok(const pg_handle* h);
+hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double* dh);
 hipError_t pg_launch_ab(pg_handle* h, int order);
 hipError_t pg_launch_thickness(pg_handle* h);
 hipError_t pg_launch_rhs(pg_handle* h);
 hipError_t pg_launch_serial_mean(pg_handle* h, double* f);
+hipError_t pg_launch_parallel_mean(pg_handle* h, double* f);
 hipError_t pg_launch_sub_mean(pg_handle* h, double* f);
 hipError_t pg_launch_correct(pg_handle* h);
 hipError_t pg_launch_norms(pg_handle* h, int which);

@@ -22,7 +22,7 @@ constexpr int TI = 4;   // rows of i per workgroup
 // phase is independent per column (y phases) or per row (x phases), and a
 // thread performs its column/row in the reference's loop order.
 // ---------------------------------------------------------------------------
-__global__ void k_fill(double* base, size_t stride, PgGrid G, int kind)
+__global__ __launch_bounds__(1024) void k_fill(double* base, size_t stride, PgGrid G, int kind)
 {
   double* f = base + (size_t)blockIdx.x * stride;
   const int g = G.g, nx = G.nx, ny = G.ny, sx = G.sx, sy = G.sy;
@@ -196,18 +196,51 @@ __global__ __launch_bounds__(TJ* TI) void k_rhs(PgGrid G, const double* u, const
   rhs[pg_at(G, i + G.g, j + G.g)] = 0 + divergence(G, u, v, h, i, j, dxdt, dydt);
 }
 
-// Serial-order mean of the owned cells (pressure.c:55-72): one thread, the
-// same i-outer j-inner accumulation as the reference, so the rounding of
-// the sum is reproduced exactly.
-__global__ void k_serial_mean(PgGrid G, const double* f, PgCtl* ctl)
+// Serial-order mean of the owned cells (pressure.c:55-72): the same
+// i-outer j-inner chain of additions as the reference, so the rounding of
+// the sum is reproduced exactly.  One wavefront: the 64 lanes load 64
+// consecutive values of a column, then every lane adds the broadcast values
+// in order (all lanes carry the same sum), so the loads are coalesced and
+// only the dependent add chain is serial.
+__global__ __launch_bounds__(64) void k_serial_mean(PgGrid G, const double* f, PgCtl* ctl)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (blockIdx.x != 0) return;
+  const int lane = threadIdx.x;
   double s = 0;
   for (int i = 0; i < G.nx; i++) {
     const double* col = f + pg_at(G, i + G.g, G.g);
-    for (int j = 0; j < G.ny; j++) s += col[j];
+    for (int j0 = 0; j0 < G.ny; j0 += 64) {
+      const double x = (j0 + lane < G.ny) ? col[j0 + lane] : 0.0;
+      const int cnt = (G.ny - j0 < 64) ? G.ny - j0 : 64;
+      for (int q = 0; q < cnt; q++) s += __shfl(x, q, 64);
+    }
   }
-  ctl->mean = s / ((double)G.nx * (double)G.ny);
+  if (lane == 0) ctl->mean = s / ((double)G.nx * (double)G.ny);
+}
+
+// Parallel mean of the owned cells (tree reduction; used where bitwise
+// agreement with the serial sum is not required, i.e. the GPU multigrid).
+__global__ __launch_bounds__(256) void k_sum_partials_owned(PgGrid G, const double* f, double* part)
+{
+  double s = 0;
+  const size_t n = (size_t)G.nx * G.ny;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / G.ny), j = (int)(idx % G.ny);
+    s += f[pg_at(G, i + G.g, j + G.g)];
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void k_mean_final(PgGrid G, int nb, const double* part, PgCtl* ctl)
+{
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (threadIdx.x == 0) ctl->mean = s / ((double)G.nx * (double)G.ny);
 }
 
 __global__ __launch_bounds__(TJ* TI) void k_sub_mean(PgGrid G, double* f, const PgCtl* ctl)
@@ -479,12 +512,13 @@ dim3 owned_grid(const PgGrid& G, int z) { return dim3((G.ny + TJ - 1) / TJ, (G.n
 // ---------------------------------------------------------------------------
 hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind)
 {
-  hipLaunchKernelGGL(k_fill, dim3(nfields), dim3(256), 0, h->stream, base, h->fsz, h->G, kind);
+  hipLaunchKernelGGL(k_fill, dim3(nfields), dim3(1024), 0, h->stream, base, h->fsz, h->G, kind);
   return hipGetLastError();
 }
 
 hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh)
 {
+  if (h->fast_tendency && pg_tendency_fast_ok(h)) return pg_launch_tendency_fast(h, du, dv, dh);
   PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
   hipLaunchKernelGGL(k_tendency, owned_grid(h->G, h->G.L), dim3(TJ, TI), 0, h->stream, h->G, h->P, I, du, dv, dh);
   return hipGetLastError();
@@ -531,6 +565,13 @@ hipError_t pg_launch_rhs(pg_handle* h)
 hipError_t pg_launch_serial_mean(pg_handle* h, double* f)
 {
   hipLaunchKernelGGL(k_serial_mean, dim3(1), dim3(64), 0, h->stream, h->G, f, h->ctl);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_parallel_mean(pg_handle* h, double* f)
+{
+  hipLaunchKernelGGL(k_sum_partials_owned, dim3(256), dim3(256), 0, h->stream, h->G, f, h->partials);
+  hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(64), 0, h->stream, h->G, 256, h->partials, h->ctl);
   return hipGetLastError();
 }
 

@@ -391,9 +391,42 @@ __global__ __launch_bounds__(BJ* BI) void k_prolong_correct(MgLevel f, MgLevel c
   LvPtr<T>::pi(f)[(size_t)i * f.Ny + j] -= e;
 }
 
-// Coarse tail in one workgroup: V(nu1,nu2) from level `top` with
-// lexicographic GS (wavefront schedule; race free for every size and
-// boundary type) and the reference's transfers.
+// Red-black GS sweep of one level inside the workgroup (colours (i+j)&1
+// in the order given).  Race free unless a periodic direction has odd
+// length (the two cells across the seam then share a colour).
+template <typename T>
+__device__ void wg_relax_rb(const MgLevel& l, T* pi, const T* rhs, int first, int tid, int nt)
+{
+  const int X = l.Nx, Y = l.Ny;
+  const T* Os = LvPtr<T>::Os(l);
+  const T* Ow = LvPtr<T>::Ow(l);
+  const T* rO = LvPtr<T>::rOsum(l);
+  for (int cc = 0; cc < 2; cc++) {
+    const int color = first ^ cc;
+    for (int idx = tid; idx < X * Y; idx += nt) {
+      const int i = idx / Y, j = idx % Y;
+      if (((i + j) & 1) != color) continue;
+      const T r = rO[idx];
+      if (!(r == r) || r == (T)INFINITY) continue;
+      const int im = im1(i, X), ip = ip1(i, X), jm = im1(j, Y), jp = ip1(j, Y);
+      pi[idx] = r * (Os[(size_t)i * Y + jp] * pi[(size_t)i * Y + jp] + Os[idx] * pi[(size_t)i * Y + jm]
+                     + Ow[(size_t)ip * Y + j] * pi[(size_t)ip * Y + j] + Ow[idx] * pi[(size_t)im * Y + j] - rhs[idx]);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__device__ void wg_smooth(const MgLevel& l, int wallEW, int wallNS, int first, int tid, int nt)
+{
+  const bool odd_periodic = (!wallEW && (l.Nx & 1) && l.Nx > 1) || (!wallNS && (l.Ny & 1) && l.Ny > 1);
+  if (odd_periodic) wg_relax_lex<T>(l, LvPtr<T>::pi(l), LvPtr<T>::rhs(l), (T)1, tid, nt);
+  else wg_relax_rb<T>(l, LvPtr<T>::pi(l), LvPtr<T>::rhs(l), first, tid, nt);
+}
+
+// Coarse tail in one workgroup: V(nu1,nu2) from level `top` with red-black
+// GS (lexicographic wavefront GS on odd periodic levels) and the
+// reference's transfers.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_vcycle_coarse(const MgLevel* lv, int top, int nu1, int nu2, int wallEW, int wallNS)
 {
@@ -401,7 +434,7 @@ __global__ __launch_bounds__(1024) void k_vcycle_coarse(const MgLevel* lv, int t
   for (int n = top; n >= 1; n--) {
     const MgLevel& l = lv[n];
     const MgLevel& c = lv[n - 1];
-    for (int s = 0; s < nu1; s++) wg_relax_lex<T>(l, LvPtr<T>::pi(l), LvPtr<T>::rhs(l), (T)1, tid, nt);
+    for (int s = 0; s < nu1; s++) wg_smooth<T>(l, wallEW, wallNS, 0, tid, nt);
     wg_residual<T>(l, LvPtr<T>::pi(l), LvPtr<T>::rhs(l), LvPtr<T>::tmp(l), tid, nt);
     __syncthreads();
     wg_restrict<T>(l, LvPtr<T>::tmp(l), c, LvPtr<T>::rhs(c), tid, nt);
@@ -416,7 +449,7 @@ __global__ __launch_bounds__(1024) void k_vcycle_coarse(const MgLevel* lv, int t
     for (int idx = tid; idx < l.Nx * l.Ny; idx += nt)
       LvPtr<T>::pi(l)[idx] -= interp_point<T>(l, c, LvPtr<T>::pi(c), idx / l.Ny, idx % l.Ny, wallEW, wallNS);
     __syncthreads();
-    for (int s = 0; s < nu2; s++) wg_relax_lex<T>(l, LvPtr<T>::pi(l), LvPtr<T>::rhs(l), (T)1, tid, nt);
+    for (int s = 0; s < nu2; s++) wg_smooth<T>(l, wallEW, wallNS, 1, tid, nt);
   }
 }
 

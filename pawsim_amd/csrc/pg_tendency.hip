@@ -1,0 +1,306 @@
+// pg_tendency.hip -- fast tendency kernels (gfx950).
+//
+// Split of the owned points of every layer:
+//  * inner points (at least two cells from every domain edge): every work
+//    array value they read sits at an owned position, and every input at
+//    most two cells away is owned too, so no halo rule is involved.  A
+//    workgroup stages u, v, h of an (TI+4) x (TJ+4) region into LDS with
+//    coalesced row loads, builds q, h*u, h*v, KE, the strains and h_q once
+//    per position in LDS, and combines them with plain index arithmetic;
+//  * the edge ring (the remaining two/three cells along each edge) goes
+//    through pg_tendency_point, which applies the per-array halo rules.
+// Expressions and association are those of pg_tendency_point (and of the
+// reference lines cited there), so both kernels produce bitwise identical
+// values to the per-point kernel.  Biharmonic viscosity is served by the
+// per-point kernel only.
+#include "pg_internal.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int TI = 8, TJ = 64, NTH = 256;
+constexpr int SH = TI + 4, SW = TJ + 4;   // staged u, v, h: origin (i0-2, j0-2)
+constexpr int QH = TI + 3, QW = TJ + 3;   // q: origin (i0-1, j0-1)
+constexpr int UH = TI + 2, UW = TJ + 1;   // h*u: origin (i0-1, j0-1)
+constexpr int VH = TI + 1, VW = TJ + 2;   // h*v: origin (i0-1, j0-1)
+constexpr int KH = TI + 1, KW = TJ + 1;   // KE, MM, DD_T, |u_bot|: origin (i0-1, j0-1)
+constexpr int DH = TI + 1, DW = TJ + 1;   // DD_S, h_q: origin (i0, j0)
+
+__device__ __forceinline__ bool is_inner(const PgGrid& G, int i, int j)
+{
+  return i - 2 >= G.g && i + 3 < G.g + G.nx && j - 2 >= G.g && j + 3 < G.g + G.ny;
+}
+
+__global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn I, double* du, double* dv, double* dh)
+{
+  __shared__ double su[SH * SW], sv[SH * SW], sh[SH * SW];
+  __shared__ double sq[QH * QW], suu[UH * UW], svv[VH * VW], ske[KH * KW];
+  __shared__ double sdt[KH * KW], sds[DH * DW], shq[DH * DW], sub[KH * KW];
+  extern __shared__ double smm[];  // [L][KH * KW]
+  const int tid = threadIdx.x;
+  const int i0 = G.g + blockIdx.y * TI, j0 = G.g + blockIdx.x * TJ;
+  const bool tw = P.mom == PG_MOM_TW81, al = (P.mom == PG_MOM_AL81 || P.mom == PG_MOM_HK83);
+  const double dx = G.dx, dy = G.dy;
+
+  // Montgomery potential of every layer (work.c:1173-1203, 1431-1443,
+  // 1499-1504) and the bottom speed (forcing.c:181-223) on the KE region.
+  for (int idx = tid; idx < KH * KW; idx += NTH) {
+    const int i = i0 - 1 + idx / KW, j = j0 - 1 + idx % KW;
+    if (i >= G.sx - 1 || j >= G.sy - 1) continue;
+    const size_t p = pg_at(G, i, j);
+    // interface heights bottom-up, parked in the MM slots of layers 1..L-1
+    double eta = I.hhb[p];
+    for (int m = G.L - 1; m >= 1; m--) {
+      eta = eta + I.h[pg_lay(G, m) + p];
+      smm[m * (KH * KW) + idx] = eta;
+    }
+    double pp = 0;
+    for (int k = 0; k < G.L; k++) {
+      if (k >= 1) pp = pp + P.gg[k] * smm[k * (KH * KW) + idx];
+      double mm = pp;
+      if (P.h0 != 0) {
+        const double hk = I.h[pg_lay(G, k) + p];
+        mm -= P.geff[k] * (P.h0 * P.h0 * P.h0 * P.h0) / (hk * hk * hk) / 3;
+      }
+      smm[k * (KH * KW) + idx] = mm;
+    }
+    if (P.useQuad) sub[idx] = pg_uabs_bot(G, I, i, j);
+  }
+
+  for (int k = 0; k < G.L; k++) {
+    const size_t lay = pg_lay(G, k);
+    for (int idx = tid; idx < SH * SW; idx += NTH) {
+      const int i = i0 - 2 + idx / SW, j = j0 - 2 + idx % SW;
+      if (i < G.sx && j < G.sy) {
+        const size_t p = lay + pg_at(G, i, j);
+        su[idx] = I.u[p];
+        sv[idx] = I.v[p];
+        sh[idx] = I.h[p];
+      }
+    }
+    __syncthreads();
+#define U(a, b) su[((a) + 2) * SW + (b) + 2]
+#define V(a, b) sv[((a) + 2) * SW + (b) + 2]
+#define H(a, b) sh[((a) + 2) * SW + (b) + 2]
+    // q (work.c:1258-1270) on (-1..TI+1) x (-1..TJ+1)
+    for (int idx = tid; idx < QH * QW; idx += NTH) {
+      const int a = idx / QW - 1, b = idx % QW - 1;
+      const int i = i0 + a, j = j0 + b;
+      if (i >= G.sx - 2 || j >= G.sy - 2) continue;
+      const double zeta = (U(a, b - 1) - U(a, b)) / dy + (V(a, b) - V(a - 1, b)) / dx;
+      const double hq = 0.25 * (H(a, b) + H(a - 1, b - 1) + H(a - 1, b) + H(a, b - 1));
+      sq[idx] = (2 * I.omz[pg_at(G, i, j)] + zeta) / hq;
+    }
+    // h*u (work.c:176-182) on (-1..TI) x (-1..TJ-1), h*v on (-1..TI-1) x (-1..TJ)
+    for (int idx = tid; idx < UH * UW; idx += NTH) {
+      const int a = idx / UW - 1, b = idx % UW - 1;
+      suu[idx] = U(a, b) * (0.5 * (H(a, b) + H(a - 1, b)));
+    }
+    for (int idx = tid; idx < VH * VW; idx += NTH) {
+      const int a = idx / VW - 1, b = idx % VW - 1;
+      svv[idx] = V(a, b) * (0.5 * (H(a, b) + H(a, b - 1)));
+    }
+    // KE (work.c:1446-1497) and DD_T (work.c:1584-1587) on (-1..TI-1) x (-1..TJ-1)
+#define SQ(x) ((x) * (x))
+    for (int idx = tid; idx < KH * KW; idx += NTH) {
+      const int a = idx / KW - 1, b = idx % KW - 1;
+      if (tw)
+        ske[idx] = 0.25 * ((2.0 / 3.0) * SQ(U(a, b)) + (1.0 / 3.0) * SQ(0.5 * (U(a - 1, b) + U(a + 1, b)))
+                           + (2.0 / 3.0) * SQ(U(a + 1, b)) + (1.0 / 3.0) * SQ(0.5 * (U(a, b) + U(a + 2, b)))
+                           + (2.0 / 3.0) * SQ(V(a, b)) + (1.0 / 3.0) * SQ(0.5 * (V(a, b - 1) + V(a, b + 1)))
+                           + (2.0 / 3.0) * SQ(V(a, b + 1)) + (1.0 / 3.0) * SQ(0.5 * (V(a, b) + V(a, b + 2))));
+      else if (P.mom == PG_MOM_HK83)
+        ske[idx] = (SQ(U(a + 1, b)) + SQ(U(a, b))
+                    + 0.25 * (SQ(U(a + 1, b + 1)) + SQ(U(a, b + 1)) + SQ(U(a + 1, b - 1)) + SQ(U(a, b - 1)))
+                    + SQ(V(a, b + 1)) + SQ(V(a, b))
+                    + 0.25 * (SQ(V(a + 1, b + 1)) + SQ(V(a + 1, b)) + SQ(V(a - 1, b + 1)) + SQ(V(a - 1, b))))
+                   / 6;
+      else
+        ske[idx] = (SQ(U(a + 1, b)) + SQ(U(a, b))) / 4 + (SQ(V(a, b + 1)) + SQ(V(a, b))) / 4;
+      if (P.useA2) sdt[idx] = (U(a + 1, b) - U(a, b)) / dx - (V(a, b + 1) - V(a, b)) / dy;
+    }
+#undef SQ
+    // DD_S (work.c:1588-1589) and h_q on (0..TI) x (0..TJ)
+    if (P.useA2)
+      for (int idx = tid; idx < DH * DW; idx += NTH) {
+        const int a = idx / DW, b = idx % DW;
+        sds[idx] = (U(a, b) - U(a, b - 1)) / dy + (V(a, b) - V(a - 1, b)) / dx;
+        shq[idx] = 0.25 * (H(a, b) + H(a - 1, b - 1) + H(a - 1, b) + H(a, b - 1));
+      }
+    __syncthreads();
+#define Q(a, b) sq[((li) + (a) + 1) * QW + (lj) + (b) + 1]
+#define HU(a, b) suu[((li) + (a) + 1) * UW + (lj) + (b) + 1]
+#define HV(a, b) svv[((li) + (a) + 1) * VW + (lj) + (b) + 1]
+#define KE(a, b) ske[((li) + (a) + 1) * KW + (lj) + (b) + 1]
+#define MM(a, b) smm[k * (KH * KW) + ((li) + (a) + 1) * KW + (lj) + (b) + 1]
+#define DT(a, b) sdt[((li) + (a) + 1) * KW + (lj) + (b) + 1]
+#define UB(a, b) sub[((li) + (a) + 1) * KW + (lj) + (b) + 1]
+#define DS(a, b) sds[((li) + (a)) * DW + (lj) + (b)]
+#define HQ(a, b) shq[((li) + (a)) * DW + (lj) + (b)]
+    for (int idx = tid; idx < TI * TJ; idx += NTH) {
+      const int li = idx / TJ, lj = idx % TJ;
+      const int i = i0 + li, j = j0 + lj;
+      if (i >= G.g + G.nx || j >= G.g + G.ny || !is_inner(G, i, j)) continue;
+      // PV coefficients (work.c:1324-1390)
+      double a00 = 0, b00 = 0, c00 = 0, d00 = 0, l0p = 0, l00 = 0, cp0 = 0, a0m = 0, bpm = 0, mp0 = 0, m00 = 0;
+      double e00 = 0, em0 = 0, p0m = 0, p00 = 0;
+      if (tw) {
+        a00 = (2 * Q(1, 1) + 3 * Q(0, 1) + 2 * Q(0, 0) + Q(1, 0) - Q(0, 2) - Q(-1, 1)) / 24;
+        b00 = (3 * Q(0, 1) + 2 * Q(-1, 1) + Q(-1, 0) + 2 * Q(0, 0) - Q(1, 1) - Q(0, 2)) / 24;
+        c00 = (2 * Q(0, 1) + Q(-1, 1) + 2 * Q(-1, 0) + 3 * Q(0, 0) - Q(0, -1) - Q(1, 0)) / 24;
+        d00 = (Q(1, 1) + 2 * Q(0, 1) + 3 * Q(0, 0) + 2 * Q(1, 0) - Q(-1, 0) - Q(0, -1)) / 24;
+        l0p = (Q(1, 1) - Q(-1, 1)) / 24;
+        l00 = (Q(1, 0) - Q(-1, 0)) / 24;
+        cp0 = (2 * Q(1, 1) + Q(0, 1) + 2 * Q(0, 0) + 3 * Q(1, 0) - Q(1, -1) - Q(2, 0)) / 24;
+        a0m = (2 * Q(1, 0) + 3 * Q(0, 0) + 2 * Q(0, -1) + Q(1, -1) - Q(0, 1) - Q(-1, 0)) / 24;
+        bpm = (3 * Q(1, 0) + 2 * Q(0, 0) + Q(0, -1) + 2 * Q(1, -1) - Q(2, 0) - Q(1, 1)) / 24;
+        mp0 = (Q(1, -1) - Q(1, 1)) / 24;
+        m00 = (Q(0, -1) - Q(0, 1)) / 24;
+      } else if (al) {
+        a00 = (2 * Q(1, 1) + Q(0, 1) + 2 * Q(0, 0) + Q(1, 0)) / 24;
+        b00 = (Q(0, 1) + 2 * Q(-1, 1) + Q(-1, 0) + 2 * Q(0, 0)) / 24;
+        c00 = (2 * Q(0, 1) + Q(-1, 1) + 2 * Q(-1, 0) + Q(0, 0)) / 24;
+        d00 = (Q(1, 1) + 2 * Q(0, 1) + Q(0, 0) + 2 * Q(1, 0)) / 24;
+        cp0 = (2 * Q(1, 1) + Q(0, 1) + 2 * Q(0, 0) + Q(1, 0)) / 24;
+        a0m = (2 * Q(1, 0) + Q(0, 0) + 2 * Q(0, -1) + Q(1, -1)) / 24;
+        bpm = (Q(1, 0) + 2 * Q(0, 0) + Q(0, -1) + 2 * Q(1, -1)) / 24;
+      }
+      if (tw || al) {
+        e00 = (Q(1, 1) + Q(0, 1) - Q(0, 0) - Q(1, 0)) / 24;
+        em0 = (Q(0, 1) + Q(-1, 1) - Q(-1, 0) - Q(0, 0)) / 24;
+        p0m = (-Q(1, 0) + Q(0, 0) + Q(0, -1) - Q(1, -1)) / 24;
+        p00 = (-Q(1, 1) + Q(0, 1) + Q(0, 0) - Q(1, 0)) / 24;
+      }
+      const double hwest = 0.5 * (H(li, lj) + H(li - 1, lj));
+      const double hsouth = 0.5 * (H(li, lj) + H(li, lj - 1));
+      const double fs = (k == 0) ? 1.0 : 0.0, fb = (k == G.L - 1) ? 1.0 : 0.0;
+      const size_t q = pg_at(G, i, j);
+      double ru, rv;
+      {  // u momentum (work.c:1681-1752), forcing (forcing.c:244-358)
+        const double gM = -(MM(0, 0) - MM(-1, 0)) / dx;
+        const double gK = -(KE(0, 0) - KE(-1, 0)) / dx;
+        double qf = 0;
+        if (tw)
+          qf += a00 * HV(0, 1) + b00 * HV(-1, 1) + c00 * HV(-1, 0) + d00 * HV(0, 0) - e00 * HU(1, 0) + em0 * HU(-1, 0)
+                - l0p * HU(0, 0) + l00 * HU(0, -1);
+        else if (al)
+          qf += a00 * HV(0, 1) + b00 * HV(-1, 1) + c00 * HV(-1, 0) + d00 * HV(0, 0) - e00 * HU(1, 0) + em0 * HU(-1, 0);
+        else
+          qf += 0.25 * Q(0, 0) * (HV(0, 0) + HV(-1, 0)) + 0.25 * Q(0, 1) * (HV(0, 1) + HV(-1, 1));
+        double a2 = 0;
+        const double a4 = 0, buoy = 0;
+        if (P.useA2)
+          a2 = ((P.A2 * H(li, lj) * DT(0, 0) - P.A2 * H(li - 1, lj) * DT(-1, 0)) / dx
+                + (P.A2 * HQ(0, 1) * DS(0, 1) - P.A2 * HQ(0, 0) * DS(0, 0)) / dy);
+        ru = gM + gK + qf + ((hwest != 0) ? (a2 + a4 + buoy) / hwest : 0);
+        if (P.useWind) {
+          const double tx = 1.0 * I.taux[q] + 0.0 * I.taux[q];
+          const double r = tx * fs;
+          if (hwest != 0) ru += r / hwest;
+        }
+        if (P.useLin && hwest != 0) {
+          const double r = -P.lin * U(li, lj) * fb;
+          ru += r / hwest;
+        }
+        if (P.useQuad && hwest != 0) {
+          const double ua = 0.5 * (UB(0, 0) + UB(-1, 0));
+          ru -= P.quad * ua * U(li, lj) * fb / hwest;
+        }
+      }
+      {  // v momentum (work.c:1758-1836)
+        const double gM = -(MM(0, 0) - MM(0, -1)) / dy;
+        const double gK = -(KE(0, 0) - KE(0, -1)) / dy;
+        double qf = 0;
+        if (tw)
+          qf += p0m * HV(0, -1) - p00 * HV(0, 1) - cp0 * HU(1, 0) - d00 * HU(0, 0) - a0m * HU(0, -1) - bpm * HU(1, -1)
+                - mp0 * HV(0, 0) + m00 * HV(-1, 0);
+        else if (al)
+          qf += p0m * HV(0, -1) - p00 * HV(0, 1) - cp0 * HU(1, 0) - d00 * HU(0, 0) - a0m * HU(0, -1) - bpm * HU(1, -1);
+        else
+          qf -= 0.25 * Q(0, 0) * (HU(0, 0) + HU(0, -1)) + 0.25 * Q(1, 0) * (HU(1, 0) + HU(1, -1));
+        double a2 = 0;
+        const double a4 = 0, buoy = 0;
+        if (P.useA2)
+          a2 = ((P.A2 * HQ(1, 0) * DS(1, 0) - P.A2 * HQ(0, 0) * DS(0, 0)) / dx
+                - (P.A2 * H(li, lj) * DT(0, 0) - P.A2 * H(li, lj - 1) * DT(0, -1)) / dy);
+        rv = gM + gK + qf + ((hsouth != 0) ? (a2 + a4 + buoy) / hsouth : 0);
+        if (P.useWind) {
+          const double ty = 1.0 * I.tauy[q] + 0.0 * I.tauy[q];
+          const double r = ty * fs;
+          if (hsouth != 0) rv += r / hsouth;
+        }
+        if (P.useLin && hsouth != 0) {
+          const double r = -P.lin * V(li, lj) * fb;
+          rv += r / hsouth;
+        }
+        if (P.useQuad && hsouth != 0) {
+          const double ua = 0.5 * (UB(0, 0) + UB(0, -1));
+          rv -= P.quad * ua * V(li, lj) * fb / hsouth;
+        }
+      }
+      const size_t p = lay + q;
+      du[p] = ru;
+      dv[p] = rv;
+      dh[p] = (HU(0, 0) - HU(1, 0)) / dx + (HV(0, 0) - HV(0, 1)) / dy;
+    }
+#undef Q
+#undef HU
+#undef HV
+#undef KE
+#undef MM
+#undef DT
+#undef UB
+#undef DS
+#undef HQ
+#undef U
+#undef V
+#undef H
+    __syncthreads();
+  }
+}
+
+// Edge ring: owned points that are not inner, through the per-point path.
+__global__ __launch_bounds__(256) void k_tendency_edge(PgGrid G, PgPhys P, PgIn I, double* du, double* dv, double* dh)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  const int nx = G.nx, ny = G.ny, g = G.g;
+  const int band = 5 * ny, total = band + (nx - 5) * 5;
+  if (e >= total) return;
+  int i, j;
+  if (e < band) {
+    const int r = e / ny;
+    i = (r < 2) ? g + r : g + nx - 3 + (r - 2);
+    j = g + e % ny;
+  } else {
+    const int f = e - band, c = f % 5;
+    i = g + 2 + f / 5;
+    j = (c < 2) ? g + c : g + ny - 3 + (c - 2);
+  }
+  double a, b, c;
+  pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
+  const size_t p = pg_lay(G, k) + pg_at(G, i, j);
+  du[p] = a;
+  dv[p] = b;
+  dh[p] = c;
+}
+
+}  // namespace
+
+// True if the fast pair of kernels serves this configuration.
+bool pg_tendency_fast_ok(const pg_handle* h)
+{
+  return !h->P.useA4 && h->G.nx >= 8 && h->G.ny >= 8 && h->G.L <= 8;
+}
+
+hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double* dh)
+{
+  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
+  const PgGrid& G = h->G;
+  const size_t dyn = (size_t)G.L * KH * KW * sizeof(double);
+  dim3 grid((G.ny + TJ - 1) / TJ, (G.nx + TI - 1) / TI);
+  hipLaunchKernelGGL(k_tendency_inner, grid, dim3(NTH), dyn, h->stream, G, h->P, I, du, dv, dh);
+  const int total = 5 * G.ny + (G.nx - 5) * 5;
+  hipLaunchKernelGGL(k_tendency_edge, dim3((total + 255) / 256, G.L), dim3(256), 0, h->stream, G, h->P, I, du, dv, dh);
+  return hipGetLastError();
+}

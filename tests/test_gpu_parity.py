@@ -38,6 +38,17 @@ TRAJECTORY_CASES = {
     "chan64x32_mg": dict(kind="channel", nx=64, ny=32, nt=12, dx=20e3, solver=1, linDragCoeff=1e-3),
     "chan48x24_sor_quad": dict(kind="channel", nx=48, ny=24, nt=10, dx=20e3, solver=0, SOR_rp=1.8, quadDragCoeff=2e-3),
     "per40x24_mg": dict(kind="periodic_box", nx=40, ny=24, nt=10, solver=1),
+    # tiled interior kernel across several tiles, every momentum scheme
+    "jets136x72_tw81_a2": dict(kind="jets", nx=136, ny=72, nlay=3, dx=20e3, nt=6, solver=1, A2=200.0,
+                               H_rest=(500.0, 1500.0, 2000.0),
+                               quadDragCoeff=2e-3, momentumScheme=3),
+    "jets80x136_hk83": dict(kind="jets", nx=80, ny=136, nlay=2, dx=20e3, nt=6, solver=1, momentumScheme=1,
+                            H_rest=(1000.0, 3000.0),
+                            h0=50.0, linDragCoeff=1e-3),
+    "chan72x40_s75e": dict(kind="channel", nx=72, ny=40, nt=6, dx=20e3, solver=1, momentumScheme=2, A2=100.0),
+    # per-point kernel everywhere
+    "jets96x64_pointkernel": dict(kind="jets", nx=96, ny=64, nlay=3, dx=20e3, nt=6, solver=1, A2=200.0,
+                                  H_rest=(500.0, 1500.0, 2000.0), quadDragCoeff=2e-3, tendency_kernel=1),
 }
 
 
