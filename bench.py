@@ -8,8 +8,12 @@ RHS, solve, velocity projection) with all fields resident in HBM.
 
 Prints one JSON line with steps/s, the per-solve milliseconds of the FP64
 GPU multigrid and of the mixed-precision multigrid, the roofline of the
-dominant kernel measured with events on the library's stream, and the CPU
-oracle timed on a bounded sample on this host.
+dominant kernel measured with events on the library's stream, and a CPU
+baseline (the reference serial build when it has been built, else the CPU
+restatement) timed on a bounded sample on this host.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu]
+    python bench.py --pmc-probe     # short run for hardware-counter passes
 """
 from __future__ import annotations
 
@@ -27,6 +31,8 @@ sys.path.insert(0, REPO)
 
 METRIC = "steps/s at 1024²×4 layers; pressure-solve ms to 1e-8 resid, learned vs plain MG"
 HBM_PEAK = 8.0e12  # B/s
+DOMINANT = "k_tendency_inner"
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_tendency_inner_jets1024x4_traffic.md")
 
 
 def parse():
@@ -39,9 +45,57 @@ def parse():
     p.add_argument("--nlay", type=int, default=4)
     p.add_argument("--solver", type=int, default=4, help="3 FP64 GPU MG, 4 mixed-precision GPU MG")
     p.add_argument("--tol", type=float, default=1e-8)
-    p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    p.add_argument("--cpu-steps", type=int, default=6)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-steps", type=int, default=12)
+    p.add_argument("--pmc-probe", action="store_true",
+                   help="only launch the dominant kernel and a device copy of known size (for rocprofv3 --pmc)")
     return p.parse_args()
+
+
+def make_model(a):
+    import pawsim_amd
+    from pawsim_amd import inputs
+
+    setup = inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=a.steps + a.warmup, solver=a.solver, tol=a.tol,
+                        A2=10.0, maxiters=200)
+    return setup, pawsim_amd.Model.from_setup(setup)
+
+
+def tendency_inner_bytes(a):
+    """Algorithmic bytes of one k_tendency_inner launch: per interior point
+    (the owned points at least two cells from every edge) and layer it reads
+    u, v, h and writes three tendencies; per point it reads hhb, Omega_z,
+    tau_x, tau_y once: (6L + 4) * 8 bytes per interior point."""
+    n_inner = (a.nx - 5) * (a.ny - 5)
+    return (6 * a.nlay + 4) * n_inner * 8
+
+
+def pmc_probe(a):
+    """Five launches of the dominant kernel and five device copies of 256 MiB."""
+    import torch
+
+    _, m = make_model(a)
+    m.bench_kernel(9, 5)
+    m.synchronize()
+    src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    m.close()
+    print(json.dumps({"probe": DOMINANT, "launches": 5, "copy_bytes": 256 << 20, "copies": 5,
+                      "algorithmic_bytes": tendency_inner_bytes(a)}))
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def main():
@@ -50,15 +104,16 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
               "(the multi-GPU decomposition is a design note only)", file=sys.stderr)
         sys.exit(2)
+    if a.pmc_probe:
+        pmc_probe(a)
+        return
     import numpy as np
     import torch
 
     import pawsim_amd
     from pawsim_amd import inputs
 
-    setup = inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=a.steps + a.warmup, solver=a.solver, tol=a.tol,
-                        A2=10.0, maxiters=200)
-    m = pawsim_amd.Model.from_setup(setup)
+    setup, m = make_model(a)
     # A dedicated (non-default) stream shared by the library and torch's
     # events, so event timings bracket exactly the library's kernels.
     stream = torch.cuda.Stream()
@@ -86,13 +141,26 @@ def main():
     ms_per_step = 1e3 * elapsed / a.steps
     value = a.steps / elapsed
 
+    # The same step with the state handed over through host buffers (upload
+    # before, download after): the rate a caller that keeps u, v, h on the
+    # host would see.
+    u, v, h = m.state()
+    t0 = time.perf_counter()
+    pcie_steps = 3
+    for _ in range(pcie_steps):
+        n += 1
+        m.upload_state(u, v, h)
+        m.step(n)
+        m.pressure_correct()
+        u, v, h = m.state()
+    pcie_rate = pcie_steps / (time.perf_counter() - t0)
+
     N = a.nx * a.ny
     L = a.nlay
     w = 8
 
-    # Per-solve milliseconds of both GPU multigrids on the same right-hand
-    # side: the current state's RHS is re-solved from pi = 0 by each solver.
-    _, _, _ = m.state()
+    # Per-solve milliseconds of both GPU multigrids on one right-hand side,
+    # each from pi = 0.
     rhs_solve = {}
     pi0 = np.zeros((a.nx, a.ny))
     rng_rhs = inputs.normal(11, N).reshape(a.nx, a.ny) * 1e-9
@@ -106,7 +174,7 @@ def main():
         rhs_solve[name] = {"ms": float(np.median(ms)), "iters": int(np.median(its)),
                            "rel_residual_target": a.tol}
 
-    # Roofline of the dominant kernels, timed with events on the library's stream.
+    # Kernels timed with events on the library's stream.
     def time_kernel(which, reps=20):
         m.bench_kernel(which, 2)
         e0 = torch.cuda.Event(enable_timing=True)
@@ -125,30 +193,33 @@ def main():
             break
         x, y = (x + 1) // 2 if x > 1 else 1, (y + 1) // 2 if y > 1 else 1
     sum_nl = float(sum(levels))
+    t_inner = time_kernel(9)
     t_tend = time_kernel(0)
     t_sweep = time_kernel(5)
     t_vc64 = time_kernel(4, reps=10)
     t_vc32 = time_kernel(6, reps=10)
-    # Algorithmic bytes: tendency reads u,v,h and the four static fields and
-    # writes three tendencies per layer-point: (6L+4) N w.  One red-black
-    # sweep reads pi, rhs, Hc and writes pi: 4 N w.  One V(2,2) cycle:
-    # [(4(nu1+nu2)+5.5) sum_l N_l / N + 3] N w (w = 4 in FP32).
+    t_vc64_split = time_kernel(7, reps=10)
+    t_vc32_split = time_kernel(8, reps=10)
+    # Algorithmic bytes: the tendency reads u, v, h and the four static
+    # fields and writes three tendencies per layer-point: (6L+4) N w.  One
+    # red-black sweep reads pi, rhs and the operator and writes pi: 4 N w.
+    # One V(2,2) cycle: [(4(nu1+nu2)+5.5) sum_l N_l / N + 3] N w (w = 4 in FP32).
     b_tend = (6 * L + 4) * N * w
     b_sweep = 4 * N * w
     b_vc = lambda ww: ((4 * 4 + 5.5) * sum_nl / N + 3) * N * ww  # noqa: E731
     kernels = {
-        "tendency": (b_tend, t_tend),
+        DOMINANT: (tendency_inner_bytes(a), t_inner),
+        "tendency_inner_plus_edge": (b_tend, t_tend),
         "rbgs_sweep_fp64": (b_sweep, t_sweep),
-        "vcycle_fp64": (b_vc(8), t_vc64),
-        "vcycle_fp32": (b_vc(4), t_vc32),
+        "vcycle_fp64_fused": (b_vc(8), t_vc64),
+        "vcycle_fp32_fused": (b_vc(4), t_vc32),
+        "vcycle_fp64_per_colour": (b_vc(8), t_vc64_split),
+        "vcycle_fp32_per_colour": (b_vc(4), t_vc32_split),
     }
-    # dominant per step: the solve's V-cycles (mixed) vs the tendency
-    dom = "vcycle_fp32" if a.solver == pawsim_amd.SOLVER_GPU_MG_MIXED else "vcycle_fp64"
-    per_step_cost = {"tendency": t_tend, dom: kernels[dom][1] * float(np.mean(iters))}
-    dominant = max(per_step_cost, key=per_step_cost.get)
-    B, t = kernels[dominant]
+    B, t = kernels[DOMINANT]
     roofline = {"bound": "hbm", "achieved": B / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": B / t / HBM_PEAK, "traffic": None, "kernel": dominant,
+                "frac": B / t / HBM_PEAK, "traffic": read_traffic(), "kernel": DOMINANT,
+                "kernel_ms": t * 1e3,
                 "per_kernel": {k: {"algorithmic_bytes": b, "ms": tt * 1e3, "GB/s": b / tt / 1e9,
                                    "frac": b / tt / HBM_PEAK} for k, (b, tt) in kernels.items()}}
     m.close()
@@ -166,40 +237,52 @@ def main():
                    "solver": a.solver, "tol": a.tol},
         "pressure_solve": {"in_run_ms_mean": float(np.mean(solve_ms)), "in_run_iters_mean": float(np.mean(iters)),
                            "same_rhs_from_zero": rhs_solve, "learned": None},
+        "pcie_inclusive_steps_per_s": pcie_rate,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
 
+def _run_timed(exe, setup, d, tag):
+    infile = setup.write(os.path.join(d, tag))
+    out = os.path.join(d, tag + "_out")
+    os.makedirs(out)
+    t0 = time.perf_counter()
+    r = subprocess.run([exe, infile, out], capture_output=True, text=True, timeout=900)
+    return r.returncode, time.perf_counter() - t0
+
+
 def cpu_baseline(a):
-    """The CPU oracle (restatement of the serial reference, gathered MG,
-    tol 1e-10 in its own max-norm criterion) on the same 1024^2 x 4 input for
-    a few steps; 1 core."""
+    """Whole steps (tendency, AB3, gathered multigrid.c V(1,1) to its own
+    max-norm tolerance 1e-10, correction) of the same jets input on one core:
+    the reference serial binary when it has been built next to the oracle,
+    else the CPU restatement.  Wall time of a k-step run minus a 1-step run,
+    so input reading and set-up cancel."""
     from pawsim_amd import inputs
 
-    exe = os.path.join(REPO, "oracle", "_build", "pawsim_oracle")
-    setup = inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=a.cpu_steps, solver=1, tol=1e-10, A2=10.0,
-                        savefrequency=1e12)
+    ref = os.path.join(REPO, "oracle", "_ref", "PAWSIM.exe")
+    port = os.path.join(REPO, "oracle", "_build", "pawsim_oracle")
+    exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
+    k = max(2, a.cpu_steps)
+    mk = lambda nt: inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=nt, solver=1, tol=1e-10, A2=10.0,  # noqa: E731
+                                savefrequency=1e12)
     with tempfile.TemporaryDirectory() as d:
-        infile = setup.write(d)
-        out = os.path.join(d, "out")
-        os.makedirs(out)
-        r = subprocess.run([exe, infile, out], capture_output=True, text=True, timeout=900)
-    mt = re.search(r"ORACLE timing: steps=(\d+) step_seconds=(\S+) solve_seconds=(\S+)", r.stdout)
-    if r.returncode != 0 or not mt:
-        return {"value": None, "unit": "steps/s", "cores": 1, "kind": "port", "sample": "oracle failed"}
-    steps, secs, solve = int(mt.group(1)), float(mt.group(2)), float(mt.group(3))
+        rc1, t1 = _run_timed(exe, mk(1), d, "one")
+        rck, tk = _run_timed(exe, mk(k), d, "many")
+    if rc1 != 0 or rck != 0 or tk <= t1:
+        return {"value": None, "unit": "steps/s", "cores": 1, "kind": kind, "sample": f"{exe} failed"}
     cpu_model = ""
     try:
         with open("/proc/cpuinfo") as f:
             cpu_model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
     except Exception:
         pass
-    return {"value": steps / secs, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps of the same {a.nx}x{a.ny}x{a.nlay} jets input (gathered multigrid.c V(1,1), "
-                      f"tol 1e-10 max-norm), {secs:.2f} s, solve {solve:.2f} s; 1 of {os.cpu_count()} cores "
-                      f"({cpu_model})"}
+    rate = (k - 1) / (tk - t1)
+    return {"value": rate, "unit": "steps/s", "cores": 1, "kind": kind,
+            "sample": f"{k - 1} steps of the same {a.nx}x{a.ny}x{a.nlay} jets input with the gathered multigrid.c "
+                      f"solver (tol 1e-10 max-norm): ({tk:.1f} s for {k} steps) - ({t1:.1f} s for 1 step); "
+                      f"1 of {os.cpu_count()} cores ({cpu_model})"}
 
 
 if __name__ == "__main__":

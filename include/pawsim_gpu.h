@@ -136,8 +136,10 @@ int pg_synchronize(pg_handle *h);
 
 /* Benchmark hooks: launch one hot kernel `reps` times on the handle's stream
  * without synchronising.  which: 0 tendency, 1 AB update, 2 RHS build,
- * 3 velocity correction, 4 one GPU-MG V-cycle, 5 one RB-GS sweep on the
- * finest level, 6 one FP32 V-cycle. */
+ * 3 velocity correction, 4 one GPU-MG V-cycle (FP64), 5 one RB-GS sweep on
+ * the finest level, 6 one FP32 V-cycle, 7 / 8 one FP64 / FP32 V-cycle with
+ * one launch per colour and transfer, 9 the tiled interior tendency kernel
+ * alone.  No reference counterpart. */
 int pg_bench_kernel(pg_handle *h, int which, int reps);
 
 #ifdef __cplusplus

@@ -108,6 +108,7 @@ int run_solver(pg_handle* h, uint32_t solver, pg_solve_stats* st)
   PG_CHECK(h, hipEventRecord(h->ev0, h->stream));
   if (solver == PG_SOLVER_SOR) rc = pg_sor_solve(h, &iters);
   else if (solver == PG_SOLVER_MG_REPLICA) rc = pg_mg_replica_solve(h, &iters);
+  else if (pg_fused_ok(h)) rc = pg_fused_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel);
   else rc = pg_gpu_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel);
   if (rc) return rc;
   PG_CHECK(h, hipEventRecord(h->ev1, h->stream));
@@ -458,8 +459,8 @@ int pg_synchronize(pg_handle* h)
 int pg_bench_kernel(pg_handle* h, int which, int reps)
 {
   if (!h || reps < 0) return PG_EARG;
-  if ((which == 4 || which == 5) && ensure_solver(h, PG_SOLVER_GPU_MG)) return PG_EHIP;
-  if (which == 6 && ensure_solver(h, PG_SOLVER_GPU_MG_MIXED)) return PG_EHIP;
+  if ((which == 4 || which == 5 || which == 7) && ensure_solver(h, PG_SOLVER_GPU_MG)) return PG_EHIP;
+  if ((which == 6 || which == 8) && ensure_solver(h, PG_SOLVER_GPU_MG_MIXED)) return PG_EHIP;
   const int s = h->slot[0];
   for (int r = 0; r < reps; r++) {
     hipError_t e;
@@ -468,9 +469,15 @@ int pg_bench_kernel(pg_handle* h, int which, int reps)
       case 1: e = pg_launch_ab(h, 3); break;
       case 2: e = pg_launch_rhs(h); break;
       case 3: e = pg_launch_correct(h); break;
-      case 4: e = pg_gpu_mg_vcycle(h); break;
+      case 4: e = pg_fused_ok(h) ? pg_fused_vcycle64(h) : pg_gpu_mg_vcycle(h); break;
       case 5: e = pg_gpu_mg_sweep(h); break;
-      case 6: e = pg_gpu_mg_vcycle32(h); break;
+      case 6: e = pg_fused_ok(h) ? pg_fused_vcycle32(h) : pg_gpu_mg_vcycle32(h); break;
+      case 7: e = pg_gpu_mg_vcycle(h); break;
+      case 8: e = pg_gpu_mg_vcycle32(h); break;
+      case 9:
+        if (!pg_tendency_fast_ok(h)) return fail(h, PG_EARG, "the tiled tendency kernel does not serve this configuration");
+        e = pg_launch_tendency_inner(h, h->fu[s], h->fv[s], h->fh[s]);
+        break;
       default: return fail(h, PG_EARG, "unknown kernel id");
     }
     PG_CHECK(h, e);

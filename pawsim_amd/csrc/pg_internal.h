@@ -55,6 +55,8 @@ struct pg_handle {
   PgCtl *ctl_host = nullptr;           // pinned mirror
   std::vector<MgLevel> lev;            // [0] = coarsest
   bool mg_built = false;
+  int mg_pred = 0;                     // V-cycles the previous GPU-MG solve took
+  long mg_tail_cells = 1024;           // largest level the LDS coarse tail starts from
   bool fast_tendency = true;           // inner LDS kernel + edge ring (bitwise equal to the per-point kernel)
   double t = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -78,6 +80,7 @@ hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh);
 // ---- fast tendency (pg_tendency.hip) ----
 bool pg_tendency_fast_ok(const pg_handle* h);
 hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double* dh);
+hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double* dh);
 hipError_t pg_launch_ab(pg_handle* h, int order);
 hipError_t pg_launch_thickness(pg_handle* h);
 hipError_t pg_launch_rhs(pg_handle* h);
@@ -100,3 +103,10 @@ hipError_t pg_gpu_mg_vcycle32(pg_handle* h);
 hipError_t pg_gpu_mg_sweep(pg_handle* h);
 hipError_t pg_gather_to_mg(pg_handle* h);
 hipError_t pg_scatter_from_mg(pg_handle* h);
+hipError_t pg_launch_coarse_global(pg_handle* h, int top, int nu1, int nu2, bool fp32);
+
+// ---- fused GPU multigrid (pg_mgfast.hip) ----
+bool pg_fused_ok(const pg_handle* h);
+int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel);
+hipError_t pg_fused_vcycle64(pg_handle* h);
+hipError_t pg_fused_vcycle32(pg_handle* h);

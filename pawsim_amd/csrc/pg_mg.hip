@@ -19,6 +19,7 @@
 //    mixed variant runs the V-cycles in FP32 on the defect of an FP64
 //    outer iteration (iterative refinement).
 #include "pg_internal.h"
+#include "pg_mg_common.h"
 
 #include <cmath>
 
@@ -27,60 +28,9 @@ namespace {
 constexpr int BJ = 64, BI = 4;
 constexpr int COARSE_CELLS = 4096;  // levels at or below this size run in one workgroup
 
-__host__ __device__ inline int coarsen(int N) { return N > 1 ? (N + 1) / 2 : 1; }
-__device__ __forceinline__ int im1(int i, int N) { return (i + N - 1) % N; }
-__device__ __forceinline__ int ip1(int i, int N) { return (i + N + 1) % N; }
-
-template <typename T> struct LvPtr;
-template <> struct LvPtr<double> {
-  __device__ static double* pi(const MgLevel& l) { return l.pi; }
-  __device__ static double* tmp(const MgLevel& l) { return l.tmp; }
-  __device__ static double* rhs(const MgLevel& l) { return l.rhs; }
-  __device__ static const double* Ow(const MgLevel& l) { return l.Ow; }
-  __device__ static const double* Os(const MgLevel& l) { return l.Os; }
-  __device__ static const double* rOsum(const MgLevel& l) { return l.rOsum; }
-};
-template <> struct LvPtr<float> {
-  __device__ static float* pi(const MgLevel& l) { return l.pi32; }
-  __device__ static float* tmp(const MgLevel& l) { return l.tmp32; }
-  __device__ static float* rhs(const MgLevel& l) { return l.rhs32; }
-  __device__ static const float* Ow(const MgLevel& l) { return l.Ow32; }
-  __device__ static const float* Os(const MgLevel& l) { return l.Os32; }
-  __device__ static const float* rOsum(const MgLevel& l) { return l.rOsum32; }
-};
-
 // ------------------------------------------------------------------------
 // hierarchy construction (multigrid.c:196-446)
 // ------------------------------------------------------------------------
-
-// restrict_power2_MG / restrict_general_MG (multigrid.c:651-767), one coarse
-// cell per thread; `f` fine, `c` coarse arrays.
-template <typename T>
-__device__ void restrict_cell(const MgLevel& F, const T* f, const MgLevel& C, T* c, int ic, int jc)
-{
-  if (F.pow2) {
-    c[(size_t)ic * C.Ny + jc] = (T)0.25 * (f[(size_t)(2 * ic) * F.Ny + 2 * jc] + f[(size_t)(2 * ic + 1) * F.Ny + 2 * jc]
-                                           + f[(size_t)(2 * ic) * F.Ny + 2 * jc + 1] + f[(size_t)(2 * ic + 1) * F.Ny + 2 * jc + 1]);
-    return;
-  }
-  const double x0 = ((double)ic) * F.Nx / C.Nx, x1 = ((double)(ic + 1)) * F.Nx / C.Nx;
-  const double y0 = ((double)jc) * F.Ny / C.Ny, y1 = ((double)(jc + 1)) * F.Ny / C.Ny;
-  T acc = 0;
-  double wsum = 0;
-  for (unsigned ifn = (unsigned)floor(x0); ifn < (unsigned)ceil(x1); ifn++) {
-    const double wx = fmin(x1, (double)(ifn + 1)) - fmax(x0, (double)ifn);
-    if (wx <= 0) continue;
-    for (unsigned jfn = (unsigned)floor(y0); jfn < (unsigned)ceil(y1); jfn++) {
-      const double wy = fmin(y1, (double)(jfn + 1)) - fmax(y0, (double)jfn);
-      if (wy <= 0) continue;
-      const double w = wx * wy;
-      acc += (T)w * f[(size_t)ifn * F.Ny + jfn];
-      wsum += w;
-    }
-  }
-  if (wsum > 0) acc /= (T)wsum;
-  c[(size_t)ic * C.Ny + jc] = acc;
-}
 
 __global__ void k_restrict_hc(const MgLevel* lv, int m)
 {
@@ -225,17 +175,6 @@ __device__ void wg_restrict(const MgLevel& F, const T* f, const MgLevel& C, T* c
   for (int idx = tid; idx < C.Nx * C.Ny; idx += nt) restrict_cell<T>(F, f, C, c, idx / C.Ny, idx % C.Ny);
 }
 
-__device__ __forceinline__ unsigned interp_index(int idx, unsigned N, bool wall)
-{
-  if (N <= 1) return 0;
-  if (wall) {
-    if (idx < 0) return 0;
-    if (idx >= (int)N) return N - 1;
-    return (unsigned)idx;
-  }
-  return (unsigned)((idx + (int)N) % (int)N);
-}
-
 // interpolate_power2_MG / interpolate_general_MG (multigrid.c:827-932) at one fine cell.
 template <typename T>
 __device__ __forceinline__ T interp_point(const MgLevel& F, const MgLevel& C, const T* c, int i, int j, int wallEW, int wallNS)
@@ -250,22 +189,7 @@ __device__ __forceinline__ T interp_point(const MgLevel& F, const MgLevel& C, co
     return (T)F.wmm[p] * c[(size_t)im * C.Ny + jm] + (T)F.wpm[p] * c[(size_t)ip * C.Ny + jm]
            + (T)F.wmp[p] * c[(size_t)im * C.Ny + jp] + (T)F.wpp[p] * c[(size_t)ip * C.Ny + jp];
   }
-  const double x = (((double)i) + 0.5) * C.Nx / F.Nx - 0.5;
-  double tx;
-  int imr;
-  if (wallEW && x <= 0) { imr = 0; tx = 0; }
-  else if (wallEW && x >= (unsigned)(C.Nx - 1)) { imr = C.Nx - 1; tx = 0; }
-  else { imr = (int)floor(x); tx = x - imr; }
-  const unsigned im = interp_index(imr, C.Nx, wallEW), ip = interp_index(imr + 1, C.Nx, wallEW);
-  const double y = (((double)j) + 0.5) * C.Ny / F.Ny - 0.5;
-  double ty;
-  int jmr;
-  if (wallNS && y <= 0) { jmr = 0; ty = 0; }
-  else if (wallNS && y >= (unsigned)(C.Ny - 1)) { jmr = C.Ny - 1; ty = 0; }
-  else { jmr = (int)floor(y); ty = y - jmr; }
-  const unsigned jm = interp_index(jmr, C.Ny, wallNS), jp = interp_index(jmr + 1, C.Ny, wallNS);
-  return (T)((1 - tx) * (1 - ty)) * c[(size_t)im * C.Ny + jm] + (T)(tx * (1 - ty)) * c[(size_t)ip * C.Ny + jm]
-         + (T)((1 - tx) * ty) * c[(size_t)im * C.Ny + jp] + (T)(tx * ty) * c[(size_t)ip * C.Ny + jp];
+  return interp_general<T>(F, C, c, i, j, wallEW, wallNS);
 }
 
 // max_residual (multigrid.c:1123-1156).
@@ -602,8 +526,8 @@ int pg_mg_build(pg_handle* h)
   const bool mixed = h->cfg.pressureSolver == PG_SOLVER_GPU_MG_MIXED;
   int nlev = 1, x = Nx, y = Ny;
   while (x > 1 || y > 1) {
-    x = coarsen(x);
-    y = coarsen(y);
+    x = pg_coarsen(x);
+    y = pg_coarsen(y);
     nlev++;
   }
   h->lev.assign(nlev, MgLevel{});
@@ -615,8 +539,8 @@ int pg_mg_build(pg_handle* h)
     l.Ny = y;
     l.dx = h->cfg.Lx / x;
     l.dy = h->cfg.Ly / y;
-    x = coarsen(x);
-    y = coarsen(y);
+    x = pg_coarsen(x);
+    y = pg_coarsen(y);
   }
   for (int m = nlev - 1; m > 0; m--)
     h->lev[m].pow2 = (h->lev[m].Nx == 2 * h->lev[m - 1].Nx) && (h->lev[m].Ny == 2 * h->lev[m - 1].Ny);
@@ -710,6 +634,16 @@ int pg_mg_replica_solve(pg_handle* h, int* iters)
   if (rc) return rc;
   PG_CHECK(h, pg_scatter_from_mg(h));
   return PG_OK;
+}
+
+hipError_t pg_launch_coarse_global(pg_handle* h, int top, int nu1, int nu2, bool fp32)
+{
+  const MgLevel* dlev = reinterpret_cast<const MgLevel*>(h->partials + h->npartials);
+  if (fp32)
+    hipLaunchKernelGGL(k_vcycle_coarse<float>, dim3(1), dim3(1024), 0, h->stream, dlev, top, nu1, nu2, h->G.wallx, h->G.wally);
+  else
+    hipLaunchKernelGGL(k_vcycle_coarse<double>, dim3(1), dim3(1024), 0, h->stream, dlev, top, nu1, nu2, h->G.wallx, h->G.wally);
+  return hipGetLastError();
 }
 
 hipError_t pg_gpu_mg_vcycle(pg_handle* h)

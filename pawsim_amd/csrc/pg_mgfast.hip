@@ -1,0 +1,663 @@
+// pg_mgfast.hip -- fused red-black multigrid V-cycle (GPU multigrid solvers).
+//
+// Same operator, transfers and stopping test as the per-colour GPU
+// multigrid, with far fewer launches:
+//  * k_down: one launch per fine level runs nu1 red-black sweeps, the
+//    residual and the 2:1 restriction.  A workgroup stages its tile with a
+//    halo of 2*nu1 + 1 cells (modular indices: the tile is a window of the
+//    torus; on walls the zero face coefficients decouple the wrapped cells)
+//    in LDS, and every colour pass shrinks the valid region by one cell,
+//    so no inter-workgroup exchange is needed.  The smoothed iterate goes
+//    to the level's tmp array (neighbouring workgroups still read the
+//    unsmoothed pi in their halos).
+//  * k_up: one launch per fine level prolongs the coarse correction, applies
+//    it to tmp and runs nu2 sweeps, writing pi.  On the finest FP64 level it
+//    can also form ||L pi - rhs||^2 (halo one cell wider).
+//  * k_tail: every level from a small one (<= 32 x 32 cells by default) down
+//    to 1 x 1 in one workgroup with all arrays resident in LDS.
+//  * norms are block partials summed in a fixed order by a one-block kernel
+//    (deterministic, no device-wide atomics), so a cycle needs no host round
+//    trip; the host reads the residual only when a stop is possible.
+#include "pg_internal.h"
+#include "pg_mg_common.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int FJ = 64;    // tile width along j (one wavefront)
+constexpr int HMAX = 5;   // halo for nu <= 2: 2 nu + 1
+constexpr int NT = 512;   // threads per fused-level workgroup
+constexpr int TAIL_THREADS = 1024;
+constexpr size_t TAIL_LDS_MAX = 150 * 1024;
+
+template <typename T> struct Tile;
+template <> struct Tile<float> { static constexpr int TI = 32; };
+template <> struct Tile<double> { static constexpr int TI = 16; };
+
+__device__ __forceinline__ int wrapm(int i, int N)
+{
+  i %= N;
+  return i < 0 ? i + N : i;
+}
+
+// Bilinear weights of the 2:1 prolongation at fine cell (i, j) with the wall
+// and corner overrides of multigrid.c:377-442 (same values as the stored
+// weight tables; all are exact in FP32).  Order: wmm, wmp, wpm, wpp.
+template <typename T>
+__device__ __forceinline__ void pow2_weights(int i, int j, int X, int Y, int wallEW, int wallNS, T w[4])
+{
+  const T ai = (i % 2 == 1) ? (T)0.75 : (T)0.25, bi = (i % 2 == 1) ? (T)0.25 : (T)0.75;
+  const T aj = (j % 2 == 1) ? (T)0.75 : (T)0.25, bj = (j % 2 == 1) ? (T)0.25 : (T)0.75;
+  T wmm = ai * aj, wmp = ai * bj, wpm = bi * aj, wpp = bi * bj;
+  if (wallNS) {
+    if (j == 0) { wmm = 0; wpm = 0; wmp = ai; wpp = bi; }
+    if (j == Y - 1) { wmp = 0; wpp = 0; wmm = ai; wpm = bi; }
+  }
+  if (wallEW) {
+    if (i == 0) { wmm = 0; wmp = 0; wpm = aj; wpp = bj; }
+    if (i == X - 1) { wpm = 0; wpp = 0; wmm = aj; wmp = bj; }
+  }
+  if (wallNS && wallEW) {
+    if (i == 0 && j == 0) { wmm = 0; wmp = 0; wpm = 0; wpp = 1; }
+    if (i == 0 && j == Y - 1) { wmm = 0; wmp = 0; wpm = 1; wpp = 0; }
+    if (i == X - 1 && j == 0) { wmm = 0; wmp = 1; wpm = 0; wpp = 0; }
+    if (i == X - 1 && j == Y - 1) { wmm = 1; wmp = 0; wpm = 0; wpp = 0; }
+  }
+  w[0] = wmm;
+  w[1] = wmp;
+  w[2] = wpm;
+  w[3] = wpp;
+}
+
+// 2:1 bilinear interpolation of the coarse array c (CY columns) at fine
+// cell (i, j) of an X x Y level (interpolate_power2_MG, multigrid.c:827-862).
+template <typename T>
+__device__ __forceinline__ T interp2(const T* c, int CY, int i, int j, int X, int Y, int wallEW, int wallNS)
+{
+  const unsigned hx = X / 2, hy = Y / 2;
+  const unsigned ip = (((i + 1) - (i + 1) % 2) / 2 + hx) % hx;
+  const unsigned im = (ip - 1 + hx) % hx;
+  const unsigned jp = (((j + 1) - (j + 1) % 2) / 2 + hy) % hy;
+  const unsigned jm = (jp - 1 + hy) % hy;
+  T w[4];
+  pow2_weights<T>(i, j, X, Y, wallEW, wallNS, w);
+  return (T)w[0] * c[(size_t)im * CY + jm] + (T)w[2] * c[(size_t)ip * CY + jm] + (T)w[1] * c[(size_t)im * CY + jp]
+         + (T)w[3] * c[(size_t)ip * CY + jp];
+}
+
+// One colour pass over LDS rows [alo, ahi] x columns [blo, bhi] of a tile
+// with pitch P; `par` is the parity of (global i + global j) at LDS (0, 0)
+// plus the colour, so cell (a, b) is updated when (par + a + b) is even.
+template <typename T, int NTH>
+__device__ __forceinline__ void rb_pass(T* spi, const T* srhs, const T* sOw, const T* sOs, const T* srO, int P, int alo,
+                                        int ahi, int blo, int bhi, int par, int tid)
+{
+  const int rows = ahi - alo + 1, half = (bhi - blo + 2) / 2;
+  for (int idx = tid; idx < rows * half; idx += NTH) {
+    const int a = alo + idx / half;
+    const int b = blo + ((par + a + blo) & 1) + 2 * (idx % half);
+    if (b > bhi) continue;
+    const int s = a * P + b;
+    const T r = srO[s];
+    if (!(r == r) || r == (T)INFINITY) continue;
+    spi[s] = r * (sOs[s + 1] * spi[s + 1] + sOs[s] * spi[s - 1] + sOw[s + P] * spi[s + P] + sOw[s] * spi[s - P] - srhs[s]);
+  }
+}
+
+// L pi - rhs at LDS cell s (residual_MG, multigrid.c:599-627); Osum is
+// rebuilt from the face coefficients exactly as the level construction
+// forms it (Ow + Ow_east + Os + Os_north).
+template <typename T>
+__device__ __forceinline__ T lds_residual(const T* spi, const T* srhs, const T* sOw, const T* sOs, int P, int s)
+{
+  const T osum = sOw[s] + sOw[s + P] + sOs[s] + sOs[s + 1];
+  return sOs[s + 1] * spi[s + 1] + sOs[s] * spi[s - 1] + sOw[s + P] * spi[s + P] + sOw[s] * spi[s - P] - osum * spi[s]
+         - srhs[s];
+}
+
+// Block partial of a grid-wide sum: part[block] = sum of v over the block.
+// k_finish_sum adds the partials in index order afterwards (fixed order, so
+// the result does not depend on scheduling; no cross-XCD atomics).
+__device__ void block_partial(double v, double* part)
+{
+  __shared__ double red[16];
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nth = blockDim.x * blockDim.y;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) {
+    double s = red[0];
+    for (int q = 1; q < nth / 64; q++) s += red[q];
+    part[blockIdx.x + blockIdx.y * gridDim.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_finish_sum(const double* part, int nb, double* out)
+{
+  __shared__ double red[4];
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += part[b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+}
+
+// ---------------------------------------------------------------------------
+// fused fine-level kernels
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct TileLds {
+  static constexpr int TI = Tile<T>::TI;
+  static constexpr int P = FJ + 2 * HMAX + 1;     // pitch
+  static constexpr int RMAX = TI + 2 * HMAX + 1;  // rows
+  T pi[RMAX * P], rhs[RMAX * P], Ow[RMAX * P], Os[RMAX * P], rO[RMAX * P];
+};
+
+// Stage the tile with halo H: coefficients over (R+1) x (C+1) cells, pi and
+// rhs over R x C, R = TI + 2H, C = FJ + 2H.  pi is the level's `src` array
+// minus the prolonged coarse correction when `coarse` is given, or zero.
+template <typename T>
+__device__ void stage_tile(TileLds<T>& S, const MgLevel& f, const T* src, const T* coarse, int CY, int H, int i0, int j0,
+                           int wallEW, int wallNS, int tid)
+{
+  constexpr int P = TileLds<T>::P;
+  constexpr int U = 8;  // cells per thread whose loads are in flight together
+  const int R = TileLds<T>::TI + 2 * H, C = FJ + 2 * H;
+  const int X = f.Nx, Y = f.Ny;
+  const T* gOw = LvPtr<T>::Ow(f);
+  const T* gOs = LvPtr<T>::Os(f);
+  const T* grO = LvPtr<T>::rOsum(f);
+  const T* grhs = LvPtr<T>::rhs(f);
+  const int total = (R + 1) * (C + 1);
+  for (int base = tid; base < total; base += U * NT) {
+    T ow[U], os[U], pv[U], rh[U], ro[U];
+    int sl[U];
+    bool in[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int idx = base + u * NT;
+      sl[u] = -1;
+      in[u] = false;
+      if (idx < total) {
+        const int a = idx / (C + 1), b = idx - (idx / (C + 1)) * (C + 1);
+        const int gi = wrapm(i0 - H + a, X), gj = wrapm(j0 - H + b, Y);
+        const size_t p = (size_t)gi * Y + gj;
+        sl[u] = a * P + b;
+        ow[u] = gOw[p];
+        os[u] = gOs[p];
+        in[u] = a < R && b < C;
+        if (in[u]) {
+          T v = src ? src[p] : (T)0;
+          if (coarse) v -= interp2<T>(coarse, CY, gi, gj, X, Y, wallEW, wallNS);
+          pv[u] = v;
+          rh[u] = grhs[p];
+          ro[u] = grO[p];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (sl[u] < 0) continue;
+      S.Ow[sl[u]] = ow[u];
+      S.Os[sl[u]] = os[u];
+      if (in[u]) {
+        S.pi[sl[u]] = pv[u];
+        S.rhs[sl[u]] = rh[u];
+        S.rO[sl[u]] = ro[u];
+      }
+    }
+  }
+}
+
+// nu1 sweeps (colour 0 first) + residual + 2:1 restriction.  Reads f.pi
+// (or zero when zero_init), writes f.tmp (smoothed), c.rhs and c.pi = 0.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_down(MgLevel f, MgLevel c, int nu, int zero_init)
+{
+  constexpr int TI = Tile<T>::TI, P = TileLds<T>::P;
+  __shared__ TileLds<T> S;
+  const int tid = threadIdx.x;
+  const int H = 2 * nu + 1, R = TI + 2 * H, C = FJ + 2 * H;
+  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * FJ;
+  stage_tile<T>(S, f, zero_init ? nullptr : LvPtr<T>::pi(f), nullptr, 0, H, i0, j0, 0, 0, tid);
+  __syncthreads();
+  const int base = (i0 + j0) & 1;  // parity of LDS (0, 0): 2H is even
+  for (int pass = 0; pass < 2 * nu; pass++) {
+    const int color = pass & 1;
+    rb_pass<T, NT>(S.pi, S.rhs, S.Ow, S.Os, S.rO, P, pass + 1, R - 2 - pass, pass + 1, C - 2 - pass, (base + color) & 1,
+                   tid);
+    __syncthreads();
+  }
+  constexpr int CI = TI / 2, CJ = FJ / 2;
+  const int ci0 = i0 / 2, cj0 = j0 / 2;
+  T* crhs = LvPtr<T>::rhs(c);
+  T* cpi = LvPtr<T>::pi(c);
+  for (int idx = tid; idx < CI * CJ; idx += NT) {
+    const int ic = idx / CJ, jc = idx % CJ;
+    if (ci0 + ic >= c.Nx || cj0 + jc >= c.Ny) continue;
+    const int s = (H + 2 * ic) * P + H + 2 * jc;
+    const T r00 = lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s);
+    const T r10 = lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s + P);
+    const T r01 = lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s + 1);
+    const T r11 = lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s + P + 1);
+    const size_t q = (size_t)(ci0 + ic) * c.Ny + cj0 + jc;
+    crhs[q] = (T)0.25 * (r00 + r10 + r01 + r11);
+    cpi[q] = 0;
+  }
+  T* out = LvPtr<T>::tmp(f);
+  for (int idx = tid; idx < TI * FJ; idx += NT) {
+    const int gi = i0 + idx / FJ, gj = j0 + idx % FJ;
+    if (gi < f.Nx && gj < f.Ny) out[(size_t)gi * f.Ny + gj] = S.pi[(H + idx / FJ) * P + H + idx % FJ];
+  }
+}
+
+// Prolongation + correction of f.tmp by c.pi, nu2 sweeps (colour 1 first),
+// result into f.pi.  With `norm` (FP64 only) also ||L pi - rhs||^2.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wallEW, int wallNS, int norm, double* part)
+{
+  constexpr int TI = Tile<T>::TI, P = TileLds<T>::P;
+  __shared__ TileLds<T> S;
+  const int tid = threadIdx.x;
+  const int H = 2 * nu + (norm ? 1 : 0), R = TI + 2 * H, C = FJ + 2 * H;
+  const int i0 = blockIdx.y * TI, j0 = blockIdx.x * FJ;
+  stage_tile<T>(S, f, LvPtr<T>::tmp(f), LvPtr<T>::pi(c), c.Ny, H, i0, j0, wallEW, wallNS, tid);
+  __syncthreads();
+  const int base = (i0 + j0) & 1;
+  for (int pass = 0; pass < 2 * nu; pass++) {
+    const int color = 1 ^ (pass & 1);
+    rb_pass<T, NT>(S.pi, S.rhs, S.Ow, S.Os, S.rO, P, pass + 1, R - 2 - pass, pass + 1, C - 2 - pass, (base + color) & 1,
+                   tid);
+    __syncthreads();
+  }
+  T* out = LvPtr<T>::pi(f);
+  double ss = 0;
+  for (int idx = tid; idx < TI * FJ; idx += NT) {
+    const int gi = i0 + idx / FJ, gj = j0 + idx % FJ;
+    if (gi >= f.Nx || gj >= f.Ny) continue;
+    const int s = (H + idx / FJ) * P + H + idx % FJ;
+    out[(size_t)gi * f.Ny + gj] = S.pi[s];
+    if (norm) {
+      const double r = (double)lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s);
+      ss += r * r;
+    }
+  }
+  if (norm) block_partial(ss, part);
+}
+
+// ---------------------------------------------------------------------------
+// coarse tail in LDS
+// ---------------------------------------------------------------------------
+
+// Level layout inside the tail's LDS: six X*Y arrays from `off`.
+struct TailLevel {
+  int X, Y, off;
+};
+
+// Typed views formed from the LDS base inside the kernel, so the compiler
+// keeps LDS addressing.
+template <typename T>
+struct TV {
+  int X, Y;
+  T *pi, *rhs, *tmp, *Ow, *Os, *rO;
+};
+
+template <typename T>
+__device__ __forceinline__ TV<T> tview(T* mem, const TailLevel& L)
+{
+  const int N = L.X * L.Y;
+  T* b = mem + L.off;
+  return TV<T>{L.X, L.Y, b, b + N, b + 2 * N, b + 3 * N, b + 4 * N, b + 5 * N};
+}
+
+__device__ __forceinline__ int dec(int i, int N) { return i == 0 ? N - 1 : i - 1; }
+__device__ __forceinline__ int inc(int i, int N) { return i == N - 1 ? 0 : i + 1; }
+
+// f(i, j, p) for every cell of an X x Y level: one cell per thread when the
+// level fits 32 x 32, else a strided loop.
+template <typename F>
+__device__ __forceinline__ void tail_cells(int X, int Y, int tid, F&& f)
+{
+  if (X <= 32 && Y <= 32) {
+    const int i = tid >> 5, j = tid & 31;
+    if (i < X && j < Y) f(i, j, i * Y + j);
+  } else {
+    for (int idx = tid; idx < X * Y; idx += TAIL_THREADS) {
+      const int i = idx / Y;
+      f(i, idx - i * Y, idx);
+    }
+  }
+}
+
+template <typename T>
+__device__ void tail_rb(const TV<T>& L, int first, int tid)
+{
+  const int X = L.X, Y = L.Y;
+  for (int cc = 0; cc < 2; cc++) {
+    const int color = first ^ cc;
+    tail_cells(X, Y, tid, [&](int i, int j, int p) {
+      if (((i + j) & 1) != color) return;
+      const T r = L.rO[p];
+      if (!(r == r) || r == (T)INFINITY) return;
+      const int im = dec(i, X), ip = inc(i, X), jm = dec(j, Y), jp = inc(j, Y);
+      L.pi[p] = r * (L.Os[i * Y + jp] * L.pi[i * Y + jp] + L.Os[p] * L.pi[i * Y + jm] + L.Ow[ip * Y + j] * L.pi[ip * Y + j]
+                     + L.Ow[p] * L.pi[im * Y + j] - L.rhs[p]);
+    });
+    __syncthreads();
+  }
+}
+
+// Lexicographic GS (omega = 1) by anti-diagonal wavefronts, for levels whose
+// periodic length is odd (red-black colouring is not consistent there).
+template <typename T>
+__device__ void tail_lex(const TV<T>& L, int tid)
+{
+  const int X = L.X, Y = L.Y;
+  for (int d = 0; d <= X + Y - 2; d++) {
+    const int ilo = d - (Y - 1) > 0 ? d - (Y - 1) : 0;
+    const int ihi = d < X - 1 ? d : X - 1;
+    for (int i = ilo + tid; i <= ihi; i += TAIL_THREADS) {
+      const int j = d - i;
+      const int im = dec(i, X), ip = inc(i, X), jm = dec(j, Y), jp = inc(j, Y);
+      const int p = i * Y + j;
+      L.pi[p] = (T)1 * L.rO[p]
+                    * (L.Os[i * Y + jp] * L.pi[i * Y + jp] + L.Os[p] * L.pi[i * Y + jm] + L.Ow[ip * Y + j] * L.pi[ip * Y + j]
+                       + L.Ow[p] * L.pi[im * Y + j] - L.rhs[p])
+                + (1 - (T)1) * L.pi[p];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__device__ void tail_smooth(const TV<T>& L, int wallEW, int wallNS, int first, int tid)
+{
+  const bool odd_periodic = (!wallEW && (L.X & 1) && L.X > 1) || (!wallNS && (L.Y & 1) && L.Y > 1);
+  if (odd_periodic) tail_lex<T>(L, tid);
+  else tail_rb<T>(L, first, tid);
+}
+
+// V(nu1, nu2) over levels m .. 0 with every array in LDS.  pi of level m is
+// loaded (or zero) and written back; the transfers are the reference's
+// (2:1 or general).
+template <typename T>
+__global__ __launch_bounds__(TAIL_THREADS) void k_tail(const MgLevel* lv, int m, int nu1, int nu2, int wallEW, int wallNS,
+                                                       int zero_init)
+{
+  extern __shared__ double tail_smem[];
+  T* mem = reinterpret_cast<T*>(tail_smem);
+  __shared__ TailLevel LL[32];
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    int off = 0;
+    for (int n = 0; n <= m; n++) {
+      LL[n] = TailLevel{lv[n].Nx, lv[n].Ny, off};
+      off += 6 * lv[n].Nx * lv[n].Ny;
+    }
+  }
+  __syncthreads();
+  for (int n = 0; n <= m; n++) {
+    const TV<T> L = tview(mem, LL[n]);
+    const T* gOw = LvPtr<T>::Ow(lv[n]);
+    const T* gOs = LvPtr<T>::Os(lv[n]);
+    const T* grO = LvPtr<T>::rOsum(lv[n]);
+    const T* gpi = LvPtr<T>::pi(lv[n]);
+    const T* grhs = LvPtr<T>::rhs(lv[n]);
+    for (int idx = tid; idx < L.X * L.Y; idx += TAIL_THREADS) {
+      L.Ow[idx] = gOw[idx];
+      L.Os[idx] = gOs[idx];
+      L.rO[idx] = grO[idx];
+      if (n == m) {
+        L.pi[idx] = zero_init ? (T)0 : gpi[idx];
+        L.rhs[idx] = grhs[idx];
+      }
+    }
+  }
+  __syncthreads();
+  for (int n = m; n >= 1; n--) {
+    const TV<T> F = tview(mem, LL[n]);
+    const TV<T> Cl = tview(mem, LL[n - 1]);
+    for (int s = 0; s < nu1; s++) tail_smooth<T>(F, wallEW, wallNS, 0, tid);
+    const int X = F.X, Y = F.Y;
+    tail_cells(X, Y, tid, [&](int i, int j, int p) {
+      const int im = dec(i, X), ip = inc(i, X), jm = dec(j, Y), jp = inc(j, Y);
+      const T osum = F.Ow[p] + F.Ow[ip * Y + j] + F.Os[p] + F.Os[i * Y + jp];
+      F.tmp[p] = F.Os[i * Y + jp] * F.pi[i * Y + jp] + F.Os[p] * F.pi[i * Y + jm] + F.Ow[ip * Y + j] * F.pi[ip * Y + j]
+                 + F.Ow[p] * F.pi[im * Y + j] - osum * F.pi[p] - F.rhs[p];
+    });
+    __syncthreads();
+    const MgLevel& Fg = lv[n];
+    const MgLevel& Cg = lv[n - 1];
+    tail_cells(Cl.X, Cl.Y, tid, [&](int i, int j, int p) {
+      restrict_cell<T>(Fg, F.tmp, Cg, Cl.rhs, i, j);
+      Cl.pi[p] = 0;
+    });
+    __syncthreads();
+  }
+  if (tid == 0 && LL[0].X * LL[0].Y == 1) tview(mem, LL[0]).pi[0] = 0;
+  __syncthreads();
+  for (int n = 1; n <= m; n++) {
+    const TV<T> F = tview(mem, LL[n]);
+    const TV<T> Cl = tview(mem, LL[n - 1]);
+    const bool p2 = lv[n].pow2;
+    const MgLevel& Fg = lv[n];
+    const MgLevel& Cg = lv[n - 1];
+    tail_cells(F.X, F.Y, tid, [&](int i, int j, int p) {
+      const T e = p2 ? interp2<T>(Cl.pi, Cl.Y, i, j, F.X, F.Y, wallEW, wallNS)
+                     : interp_general<T>(Fg, Cg, Cl.pi, i, j, wallEW, wallNS);
+      F.pi[p] -= e;
+    });
+    __syncthreads();
+    for (int s = 0; s < nu2; s++) tail_smooth<T>(F, wallEW, wallNS, 1, tid);
+  }
+  const TV<T> Lm = tview(mem, LL[m]);
+  T* gpi = LvPtr<T>::pi(lv[m]);
+  for (int idx = tid; idx < Lm.X * Lm.Y; idx += TAIL_THREADS) gpi[idx] = Lm.pi[idx];
+}
+
+// ---------------------------------------------------------------------------
+// finest-level FP64 kernels
+// ---------------------------------------------------------------------------
+
+constexpr int DJ = 64, DI = 4, DROWS = 4;  // each block covers DI * DROWS rows
+
+// Residual of the finest FP64 level, optionally after applying an FP32
+// correction: v = pcur - e (the iterate after refinement), r = L v - rhs.
+// Writes v to pout when e is given, r to rhs32 when given, and
+// sum r^2 to ctl->sumsq.
+__global__ __launch_bounds__(DJ* DI) void k_defect(MgLevel l, const double* pcur, const float* e, double* pout, float* rhs32,
+                                                    double* part)
+{
+  const int j = blockIdx.x * DJ + threadIdx.x;
+  const int X = l.Nx, Y = l.Ny;
+  double ss = 0;
+  for (int q = 0; q < DROWS; q++) {
+    const int i = (blockIdx.y * DROWS + q) * DI + threadIdx.y;
+    if (i >= X || j >= Y) continue;
+    const int im = im1(i, X), ip = ip1(i, X), jm = im1(j, Y), jp = ip1(j, Y);
+    const size_t p = (size_t)i * Y + j, pE = (size_t)ip * Y + j, pW = (size_t)im * Y + j, pN = (size_t)i * Y + jp,
+                 pS = (size_t)i * Y + jm;
+    double v = pcur[p], vE = pcur[pE], vW = pcur[pW], vN = pcur[pN], vS = pcur[pS];
+    if (e) {
+      v -= (double)e[p];
+      vE -= (double)e[pE];
+      vW -= (double)e[pW];
+      vN -= (double)e[pN];
+      vS -= (double)e[pS];
+      pout[p] = v;
+    }
+    const double r = l.Os[pN] * vN + l.Os[p] * vS + l.Ow[pE] * vE + l.Ow[p] * vW - l.Osum[p] * v - l.rhs[p];
+    if (rhs32) rhs32[p] = (float)r;
+    ss += r * r;
+  }
+  block_partial(ss, part);
+}
+
+__global__ __launch_bounds__(256) void k_sumsq(const double* x, size_t n, double* part)
+{
+  double ss = 0;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) ss += x[p] * x[p];
+  block_partial(ss, part);
+}
+
+__global__ void k_zero32(float* x, size_t n)
+{
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) x[p] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+bool odd_periodic(const pg_handle* h, const MgLevel& l)
+{
+  return (!h->G.wallx && (l.Nx & 1) && l.Nx > 1) || (!h->G.wally && (l.Ny & 1) && l.Ny > 1);
+}
+
+template <typename T>
+size_t tail_bytes(const pg_handle* h, int m)
+{
+  size_t cells = 0;
+  for (int n = 0; n <= m; n++) cells += (size_t)h->lev[n].Nx * h->lev[n].Ny;
+  return cells * 6 * sizeof(T);
+}
+
+}  // namespace
+
+// Level from which the coarse tail runs, and whether it fits in LDS.
+template <typename T>
+static int tail_level(const pg_handle* h, bool* in_lds)
+{
+  const int top = (int)h->lev.size() - 1;
+  int m = top;  // levels above m are fused: 2:1 transfer to the next level, consistent colouring
+  int mf = top;
+  while (mf > 0 && h->lev[mf].pow2 && !odd_periodic(h, h->lev[mf])) mf--;
+  m = mf;
+  // go further up while the level is small and the tail still fits
+  while (m < top && (long)h->lev[m + 1].Nx * h->lev[m + 1].Ny <= h->mg_tail_cells && tail_bytes<T>(h, m + 1) <= TAIL_LDS_MAX)
+    m++;
+  *in_lds = tail_bytes<T>(h, m) <= TAIL_LDS_MAX;
+  return m;
+}
+
+template <typename T>
+static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top, int norm_top, bool* norm_done)
+{
+  const int top = (int)h->lev.size() - 1;
+  bool lds = false;
+  const int mt = tail_level<T>(h, &lds);
+  const int wx = h->G.wallx, wy = h->G.wally;
+  constexpr int TI = Tile<T>::TI;
+  for (int n = top; n > mt; n--) {
+    const MgLevel& l = h->lev[n];
+    hipLaunchKernelGGL(k_down<T>, dim3((l.Ny + FJ - 1) / FJ, (l.Nx + TI - 1) / TI), dim3(NT), 0, h->stream, l, h->lev[n - 1],
+                       nu1, (n == top) ? zero_init_top : 0);
+  }
+  const MgLevel* dlev = reinterpret_cast<const MgLevel*>(h->partials + h->npartials);
+  const int zi = (mt == top) ? zero_init_top : 0;
+  if (lds) {
+    const size_t bytes = tail_bytes<T>(h, mt);
+    static bool attr_set[2] = {false, false};
+    if (!attr_set[sizeof(T) == 8]) {
+      hipError_t e = hipFuncSetAttribute((const void*)k_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)TAIL_LDS_MAX);
+      if (e != hipSuccess) return e;
+      attr_set[sizeof(T) == 8] = true;
+    }
+    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(TAIL_THREADS), bytes, h->stream, dlev, mt, nu1, nu2, wx, wy, zi);
+  } else {
+    if (zi) {
+      if constexpr (sizeof(T) == 4) {
+        const MgLevel& l = h->lev[mt];
+        hipLaunchKernelGGL(k_zero32, dim3(256), dim3(256), 0, h->stream, l.pi32, (size_t)l.Nx * l.Ny);
+      }
+    }
+    hipError_t e = pg_launch_coarse_global(h, mt, nu1, nu2, sizeof(T) == 4);
+    if (e != hipSuccess) return e;
+  }
+  for (int n = mt + 1; n <= top; n++) {
+    const MgLevel& l = h->lev[n];
+    const int norm = (n == top) ? norm_top : 0;
+    const dim3 grid((l.Ny + FJ - 1) / FJ, (l.Nx + TI - 1) / TI);
+    hipLaunchKernelGGL(k_up<T>, grid, dim3(NT), 0, h->stream, l, h->lev[n - 1], nu2, wx, wy, norm, h->partials);
+    if (norm)
+      hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, (int)(grid.x * grid.y), &h->ctl->sumsq);
+  }
+  if (norm_done) *norm_done = norm_top && mt < top;
+  return hipGetLastError();
+}
+
+hipError_t pg_fused_vcycle64(pg_handle* h)
+{
+  const int nu1 = h->cfg.mg_nu1 ? h->cfg.mg_nu1 : 2, nu2 = h->cfg.mg_nu2 ? h->cfg.mg_nu2 : 2;
+  return fused_vcycle<double>(h, nu1, nu2, 0, 0, nullptr);
+}
+
+hipError_t pg_fused_vcycle32(pg_handle* h)
+{
+  const int nu1 = h->cfg.mg_nu1 ? h->cfg.mg_nu1 : 2, nu2 = h->cfg.mg_nu2 ? h->cfg.mg_nu2 : 2;
+  return fused_vcycle<float>(h, nu1, nu2, 0, 0, nullptr);
+}
+
+bool pg_fused_ok(const pg_handle* h)
+{
+  const uint32_t nu1 = h->cfg.mg_nu1 ? h->cfg.mg_nu1 : 2, nu2 = h->cfg.mg_nu2 ? h->cfg.mg_nu2 : 2;
+  return nu1 <= 2 && nu2 <= 2;
+}
+
+// GPU multigrid solve on the gathered finest level: V-cycles (FP64, or FP32
+// corrections of an FP64 defect) until ||L pi - rhs|| <= tol ||rhs||.  The
+// host reads the residual only from the cycle before the previous solve's
+// count on (the count rarely changes between steps), then after each cycle.
+int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel)
+{
+  MgLevel& top = h->lev.back();
+  const int nu1 = h->cfg.mg_nu1 ? h->cfg.mg_nu1 : 2, nu2 = h->cfg.mg_nu2 ? h->cfg.mg_nu2 : 2;
+  const size_t N = (size_t)top.Nx * top.Ny;
+  const dim3 dg((top.Ny + DJ - 1) / DJ, (top.Nx + DI * DROWS - 1) / (DI * DROWS));
+  const int nbd = (int)(dg.x * dg.y);
+  auto defect = [&](const double* pcur, const float* e, double* pout, float* r32) {
+    hipLaunchKernelGGL(k_defect, dg, dim3(DJ, DI), 0, h->stream, top, pcur, e, pout, r32, h->partials);
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, nbd, &h->ctl->sumsq);
+  };
+  PG_CHECK(h, pg_gather_to_mg(h));
+  hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, h->stream, top.rhs, N, h->partials);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, &h->ctl->rhs_sumsq);
+  defect(top.pi, nullptr, nullptr, mixed ? top.rhs32 : nullptr);
+  PG_CHECK(h, hipGetLastError());
+  double* cur = top.pi;
+  double* alt = top.tmp;
+  const int maxit = (int)h->cfg.maxiters;
+  const int first_check = h->mg_pred > 1 ? h->mg_pred - 1 : 0;
+  int it = 0;
+  double r = 0;
+  for (;;) {
+    if (it >= first_check || it >= maxit) {
+      PG_CHECK(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(PgCtl), hipMemcpyDeviceToHost, h->stream));
+      PG_CHECK(h, hipStreamSynchronize(h->stream));
+      const double rr = h->ctl_host->rhs_sumsq;
+      r = (rr > 0) ? std::sqrt(h->ctl_host->sumsq / rr) : 0.0;
+      if (r <= h->cfg.pi_tol || it >= maxit) break;
+    }
+    if (mixed) {
+      PG_CHECK(h, fused_vcycle<float>(h, nu1, nu2, 1, 0, nullptr));
+      defect(cur, top.pi32, alt, top.rhs32);
+      std::swap(cur, alt);
+    } else {
+      bool norm_done = false;
+      PG_CHECK(h, fused_vcycle<double>(h, nu1, nu2, 0, 1, &norm_done));
+      if (!norm_done) defect(top.pi, nullptr, nullptr, nullptr);
+    }
+    PG_CHECK(h, hipGetLastError());
+    it++;
+  }
+  if (cur != top.pi) PG_CHECK(h, hipMemcpyAsync(top.pi, cur, N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  h->mg_pred = it;
+  *iters = it;
+  *rel = r;
+  PG_CHECK(h, pg_scatter_from_mg(h));
+  return PG_OK;
+}

@@ -293,13 +293,22 @@ bool pg_tendency_fast_ok(const pg_handle* h)
   return !h->P.useA4 && h->G.nx >= 8 && h->G.ny >= 8 && h->G.L <= 8;
 }
 
-hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double* dh)
+hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double* dh)
 {
   PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
   const PgGrid& G = h->G;
   const size_t dyn = (size_t)G.L * KH * KW * sizeof(double);
   dim3 grid((G.ny + TJ - 1) / TJ, (G.nx + TI - 1) / TI);
   hipLaunchKernelGGL(k_tendency_inner, grid, dim3(NTH), dyn, h->stream, G, h->P, I, du, dv, dh);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double* dh)
+{
+  hipError_t e = pg_launch_tendency_inner(h, du, dv, dh);
+  if (e != hipSuccess) return e;
+  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
+  const PgGrid& G = h->G;
   const int total = 5 * G.ny + (G.nx - 5) * 5;
   hipLaunchKernelGGL(k_tendency_edge, dim3((total + 255) / 256, G.L), dim3(256), 0, h->stream, G, h->P, I, du, dv, dh);
   return hipGetLastError();
