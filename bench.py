@@ -2,9 +2,14 @@
 
 Workload: the 4-layer zonal-jets domain, 1024 x 1024 doubly bounded, dx = 5 km
 (random-phase smooth topography, rms 200 m), AB3 dt = 300 s, rigid-lid
-pressure solved each step to a relative residual of 1e-8.  One "step" is one
-pg_step (tendency + AB3 update) plus one pg_pressure_correct (thickness fix,
-RHS, solve, velocity projection) with all fields resident in HBM.
+pressure solved each step to a relative residual of 1e-8, in the
+mixed-precision form: FP32 stencils and V-cycle smoothing, FP64 state, AB
+update, outer residual, coarse correction of the refinement and norms.
+One "step" is one pg_step (tendency + AB3 update) plus one
+pg_pressure_correct (thickness fix, RHS, solve, velocity projection) with
+all fields resident in HBM.  The same steps also run on the FP64 path (the
+reference's arithmetic with the FP64 GPU multigrid); the per-field error of
+the mixed run against it is reported after W + K steps.
 
 Prints one JSON line with steps/s, the per-solve milliseconds of the FP64
 GPU multigrid and of the mixed-precision multigrid, the roofline of the
@@ -31,14 +36,14 @@ sys.path.insert(0, REPO)
 
 METRIC = "steps/s at 1024²×4 layers; pressure-solve ms to 1e-8 resid, learned vs plain MG"
 HBM_PEAK = 8.0e12  # B/s
-DOMINANT = "k_tendency_inner"
-TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_tendency_inner_jets1024x4_traffic.md")
+DOMINANT = "k_tendency_inner32"
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_tendency_inner32_jets1024x4_traffic.md")
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=40)
+    p.add_argument("--steps", type=int, default=495)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--nx", type=int, default=1024)
     p.add_argument("--ny", type=int, default=1024)
@@ -52,30 +57,34 @@ def parse():
     return p.parse_args()
 
 
-def make_model(a):
+def make_model(a, mixed=1, solver=None, tol=None):
     import pawsim_amd
     from pawsim_amd import inputs
 
-    setup = inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=a.steps + a.warmup, solver=a.solver, tol=a.tol,
-                        A2=10.0, maxiters=200)
+    setup = inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=a.steps + a.warmup,
+                        solver=a.solver if solver is None else solver, tol=a.tol if tol is None else tol,
+                        A2=10.0, maxiters=200, mixed_precision=mixed)
     return setup, pawsim_amd.Model.from_setup(setup)
 
 
-def tendency_inner_bytes(a):
-    """Algorithmic bytes of one k_tendency_inner launch: per interior point
+def interior_points(a):
+    return (a.nx - 5) * (a.ny - 5)
+
+
+def tendency_inner_bytes(a, w_out):
+    """Algorithmic bytes of one interior tendency launch: per interior point
     (the owned points at least two cells from every edge) and layer it reads
-    u, v, h and writes three tendencies; per point it reads hhb, Omega_z,
-    tau_x, tau_y once: (6L + 4) * 8 bytes per interior point."""
-    n_inner = (a.nx - 5) * (a.ny - 5)
-    return (6 * a.nlay + 4) * n_inner * 8
+    u, v, h (FP64) and writes three tendencies (w_out bytes each); per point
+    it reads hhb, Omega_z, tau_x, tau_y once (FP64)."""
+    return (3 * a.nlay * 8 + 3 * a.nlay * w_out + 4 * 8) * interior_points(a)
 
 
 def pmc_probe(a):
     """Five launches of the dominant kernel and five device copies of 256 MiB."""
     import torch
 
-    _, m = make_model(a)
-    m.bench_kernel(9, 5)
+    _, m = make_model(a, mixed=1)
+    m.bench_kernel(10, 5)
     m.synchronize()
     src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
     dst = torch.empty_like(src)
@@ -84,7 +93,7 @@ def pmc_probe(a):
     torch.cuda.synchronize()
     m.close()
     print(json.dumps({"probe": DOMINANT, "launches": 5, "copy_bytes": 256 << 20, "copies": 5,
-                      "algorithmic_bytes": tendency_inner_bytes(a)}))
+                      "algorithmic_bytes": tendency_inner_bytes(a, 4)}))
 
 
 def read_traffic():
@@ -96,6 +105,34 @@ def read_traffic():
         return float(m.group(1)) if m else None
     except OSError:
         return None
+
+
+def run_steps(a, m, stream):
+    """Preliminary correction, W untimed and K timed steps; returns the timing
+    and per-solve statistics of the timed steps."""
+    import numpy as np
+    import torch
+
+    m.set_stream(stream.cuda_stream)
+    m.pressure_correct()  # preliminary correction and work-array rebuild (pawsim.c:282-301)
+    m.refresh_halos()
+    n = 0
+    for _ in range(a.warmup):
+        n += 1
+        m.step(n)
+        m.pressure_correct()
+    solve_ms, iters = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        n += 1
+        m.step(n)
+        st = m.pressure_correct()
+        solve_ms.append(st["solve_ms"])
+        iters.append(st["iters"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    return {"n": n, "elapsed": elapsed, "solve_ms": float(np.mean(solve_ms)), "iters": float(np.mean(iters))}
 
 
 def main():
@@ -113,38 +150,31 @@ def main():
     import pawsim_amd
     from pawsim_amd import inputs
 
-    setup, m = make_model(a)
     # A dedicated (non-default) stream shared by the library and torch's
     # events, so event timings bracket exactly the library's kernels.
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
-    m.set_stream(stream.cuda_stream)
-    # preliminary correction and work-array rebuild (pawsim.c:282-301)
-    m.pressure_correct()
-    m.refresh_halos()
-    n = 0
-    solve_ms, iters = [], []
-    for _ in range(a.warmup):
-        n += 1
-        m.step(n)
-        m.pressure_correct()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        n += 1
-        m.step(n)
-        st = m.pressure_correct()
-        solve_ms.append(st["solve_ms"])
-        iters.append(st["iters"])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / a.steps
-    value = a.steps / elapsed
+
+    # FP64 path (the reference's arithmetic, FP64 GPU multigrid): the error
+    # reference for the mixed-precision run and the plain-MG solve time.
+    _, m64 = make_model(a, mixed=0, solver=pawsim_amd.SOLVER_GPU_MG)
+    r64 = run_steps(a, m64, stream)
+    s64 = m64.state()
+    m64.close()
+
+    # Headline: mixed precision (FP32 stencils, FP32 V-cycles in FP64 refinement).
+    setup, m = make_model(a, mixed=1)
+    r32 = run_steps(a, m, stream)
+    n = r32["n"]
+    s32 = m.state()
+    ms_per_step = 1e3 * r32["elapsed"] / a.steps
+    value = a.steps / r32["elapsed"]
+    err = {k: float(np.max(np.abs(x - y)) / np.max(np.abs(y))) for k, x, y in zip(("u", "v", "h"), s32, s64)}
 
     # The same step with the state handed over through host buffers (upload
     # before, download after): the rate a caller that keeps u, v, h on the
     # host would see.
-    u, v, h = m.state()
+    u, v, h = s32
     t0 = time.perf_counter()
     pcie_steps = 3
     for _ in range(pcie_steps):
@@ -157,7 +187,6 @@ def main():
 
     N = a.nx * a.ny
     L = a.nlay
-    w = 8
 
     # Per-solve milliseconds of both GPU multigrids on one right-hand side,
     # each from pi = 0.
@@ -175,12 +204,12 @@ def main():
                            "rel_residual_target": a.tol}
 
     # Kernels timed with events on the library's stream.
-    def time_kernel(which, reps=20):
-        m.bench_kernel(which, 2)
+    def time_kernel(mdl, which, reps=20):
+        mdl.bench_kernel(which, 2)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        m.bench_kernel(which, reps)
+        mdl.bench_kernel(which, reps)
         e1.record(stream)
         e1.synchronize()
         return e0.elapsed_time(e1) / reps * 1e-3  # seconds per launch
@@ -193,24 +222,29 @@ def main():
             break
         x, y = (x + 1) // 2 if x > 1 else 1, (y + 1) // 2 if y > 1 else 1
     sum_nl = float(sum(levels))
-    t_inner = time_kernel(9)
-    t_tend = time_kernel(0)
-    t_sweep = time_kernel(5)
-    t_vc64 = time_kernel(4, reps=10)
-    t_vc32 = time_kernel(6, reps=10)
-    t_vc64_split = time_kernel(7, reps=10)
-    t_vc32_split = time_kernel(8, reps=10)
+    t_inner32 = time_kernel(m, 10)
+    t_tend32 = time_kernel(m, 0)
+    t_vc32 = time_kernel(m, 6, reps=10)
+    t_vc32_split = time_kernel(m, 8, reps=10)
+    _, m64 = make_model(a, mixed=0, solver=pawsim_amd.SOLVER_GPU_MG)
+    m64.set_stream(stream.cuda_stream)
+    t_inner64 = time_kernel(m64, 9)
+    t_tend64 = time_kernel(m64, 0)
+    t_sweep = time_kernel(m64, 5)
+    t_vc64 = time_kernel(m64, 4, reps=10)
+    t_vc64_split = time_kernel(m64, 7, reps=10)
+    m64.close()
     # Algorithmic bytes: the tendency reads u, v, h and the four static
-    # fields and writes three tendencies per layer-point: (6L+4) N w.  One
-    # red-black sweep reads pi, rhs and the operator and writes pi: 4 N w.
-    # One V(2,2) cycle: [(4(nu1+nu2)+5.5) sum_l N_l / N + 3] N w (w = 4 in FP32).
-    b_tend = (6 * L + 4) * N * w
-    b_sweep = 4 * N * w
+    # fields and writes three tendencies per layer-point.  One red-black
+    # sweep reads pi, rhs and the operator and writes pi: 4 N w.  One V(2,2)
+    # cycle: [(4(nu1+nu2)+5.5) sum_l N_l / N + 3] N w (w = 4 in FP32).
     b_vc = lambda ww: ((4 * 4 + 5.5) * sum_nl / N + 3) * N * ww  # noqa: E731
     kernels = {
-        DOMINANT: (tendency_inner_bytes(a), t_inner),
-        "tendency_inner_plus_edge": (b_tend, t_tend),
-        "rbgs_sweep_fp64": (b_sweep, t_sweep),
+        DOMINANT: (tendency_inner_bytes(a, 4), t_inner32),
+        "tendency_fp32_inner_plus_edge": ((3 * L * 8 + 3 * L * 4 + 32) * N, t_tend32),
+        "k_tendency_inner_fp64": (tendency_inner_bytes(a, 8), t_inner64),
+        "tendency_fp64_inner_plus_edge": ((6 * L + 4) * N * 8, t_tend64),
+        "rbgs_sweep_fp64": (4 * N * 8, t_sweep),
         "vcycle_fp64_fused": (b_vc(8), t_vc64),
         "vcycle_fp32_fused": (b_vc(4), t_vc32),
         "vcycle_fp64_per_colour": (b_vc(8), t_vc64_split),
@@ -231,11 +265,17 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"zonal jets {a.nx}x{a.ny}x{a.nlay}, walls, dx 5 km, AB3 dt 300 s, "
-                               f"rigid lid, GPU multigrid solver {a.solver} to rel. residual {a.tol:g}",
+        "vs_baseline": None, "dtype": "f32 stencils+smoothing / f64 state, refinement, norms", "data": "synthetic",
+        "config": {"workload": f"zonal jets {a.nx}x{a.ny}x{a.nlay}, walls, dx 5 km, AB3 dt 300 s, rigid lid; "
+                               f"FP32 stencils and V-cycles, FP64 outer residual, coarse solve and norms, "
+                               f"iterative refinement to rel. residual {a.tol:g}",
                    "solver": a.solver, "tol": a.tol},
-        "pressure_solve": {"in_run_ms_mean": float(np.mean(solve_ms)), "in_run_iters_mean": float(np.mean(iters)),
+        "error_vs_fp64": {"after_steps": a.warmup + a.steps, "max_abs_diff_over_max_abs": err},
+        "fp64_path": {"steps_per_s": a.steps / r64["elapsed"], "ms_per_step": 1e3 * r64["elapsed"] / a.steps,
+                      "solver": "FP64 GPU multigrid (plain MG)", "solve_ms_mean": r64["solve_ms"],
+                      "iters_mean": r64["iters"]},
+        "pressure_solve": {"in_run_ms_mean": r32["solve_ms"], "in_run_iters_mean": r32["iters"],
+                           "plain_fp64_mg_in_run_ms_mean": r64["solve_ms"],
                            "same_rhs_from_zero": rhs_solve, "learned": None},
         "pcie_inclusive_steps_per_s": pcie_rate,
         "roofline": roofline,

@@ -62,6 +62,9 @@ typedef struct pg_config {
   uint32_t mg_nu1, mg_nu2;     /* GPU MG smoothing counts (0 -> 2, 2) */
   int32_t tendency_kernel;     /* 0: LDS-tiled interior + edge ring where it applies, 1: per-point kernel;
                                   both give bitwise identical tendencies */
+  int32_t mixed_precision;     /* 0: FP64 throughout (bitwise with the reference);
+                                  1: FP32 interior stencils and tendency history, FP64 state, AB update,
+                                  pressure correction, norms (requires A4 = 0, Nlay <= 8, Nx, Ny >= 8) */
 } pg_config;
 
 /* Static inputs (state.c:1012-1047).  NULL hhs means a flat lid at 0;
@@ -139,7 +142,8 @@ int pg_synchronize(pg_handle *h);
  * 3 velocity correction, 4 one GPU-MG V-cycle (FP64), 5 one RB-GS sweep on
  * the finest level, 6 one FP32 V-cycle, 7 / 8 one FP64 / FP32 V-cycle with
  * one launch per colour and transfer, 9 the tiled interior tendency kernel
- * alone.  No reference counterpart. */
+ * alone, 10 the FP32 interior tendency kernel alone (mixed_precision
+ * only).  No reference counterpart. */
 int pg_bench_kernel(pg_handle *h, int which, int reps);
 
 #ifdef __cplusplus

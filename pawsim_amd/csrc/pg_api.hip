@@ -73,6 +73,9 @@ int validate(const pg_config* c, std::string& why)
   if (s != PG_SOLVER_SOR && s != PG_SOLVER_MG_REPLICA && s != PG_SOLVER_GPU_MG && s != PG_SOLVER_GPU_MG_MIXED)
     return why = "unknown pressureSolver", PG_EARG;
   if (c->tendency_kernel != 0 && c->tendency_kernel != 1) return why = "tendency_kernel must be 0 or 1", PG_EARG;
+  if (c->mixed_precision != 0 && c->mixed_precision != 1) return why = "mixed_precision must be 0 or 1", PG_EARG;
+  if (c->mixed_precision && (c->A4 > 0 || c->Nlay > 8 || c->Nx < 8 || c->Ny < 8 || c->tendency_kernel != 0))
+    return why = "mixed_precision needs A4 = 0, Nlay <= 8, Nx and Ny >= 8 and the tiled tendency kernel", PG_EARG;
   return PG_OK;
 }
 
@@ -166,6 +169,7 @@ static int create_body(const pg_config* cfg, const pg_static* st, pg_handle** ou
   h->cfg = *cfg;
   pg_config& c = h->cfg;
   h->fast_tendency = c.tendency_kernel == 0;
+  h->mixed = c.mixed_precision == 1;
   if (c.Lx <= 0) c.Lx = c.Ly * c.Nx / c.Ny;  // pawsim.c:129-132
   PgGrid& G = h->G;
   G.nx = (int)c.Nx;
@@ -211,6 +215,14 @@ static int create_body(const pg_config* cfg, const pg_static* st, pg_handle** ou
   for (double** p : layered) {
     PG_CHECK(h, hipMalloc(p, lf));
     PG_CHECK(h, hipMemsetAsync(*p, 0, lf, h->stream));
+  }
+  if (h->mixed) {
+    float** layered32[] = {&h->fu32[0], &h->fu32[1], &h->fu32[2], &h->fv32[0], &h->fv32[1], &h->fv32[2],
+                           &h->fh32[0], &h->fh32[1], &h->fh32[2]};
+    for (float** p : layered32) {
+      PG_CHECK(h, hipMalloc(p, lf / 2));
+      PG_CHECK(h, hipMemsetAsync(*p, 0, lf / 2, h->stream));
+    }
   }
   double** single[] = {&h->hhb, &h->hhs, &h->omz, &h->taux, &h->tauy, &h->Hc, &h->Hw, &h->Hs, &h->pi, &h->rhs};
   for (double** p : single) {
@@ -262,6 +274,11 @@ void pg_destroy(pg_handle* h)
                     h->fh[2], h->hhb, h->hhs, h->omz, h->taux, h->tauy, h->Hc, h->Hw, h->Hs, h->pi, h->rhs,
                     h->sor_c, h->partials};
   for (double* p : bufs) (void)hipFree(p);
+  for (int q = 0; q < 3; q++) {
+    (void)hipFree(h->fu32[q]);
+    (void)hipFree(h->fv32[q]);
+    (void)hipFree(h->fh32[q]);
+  }
   (void)hipFree(h->ctl);
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -326,7 +343,8 @@ int pg_tendency(pg_handle* h, double* du, double* dv, double* dh)
 {
   if (!h) return PG_EARG;
   const int s = h->slot[0];
-  PG_CHECK(h, pg_launch_tendency(h, h->fu[s], h->fv[s], h->fh[s]));
+  PG_CHECK(h, pg_launch_tendency_slot(h, s));
+  if (h->mixed) PG_CHECK(h, pg_widen_slot(h, s));
   std::vector<double> stage;
   const size_t n = (size_t)h->G.nx * h->G.ny;
   for (int k = 0; k < h->G.L; k++) {
@@ -343,7 +361,7 @@ int pg_step(pg_handle* h, uint32_t n)
 {
   if (!h || n == 0) return PG_EARG;
   const int s = h->slot[0];
-  PG_CHECK(h, pg_launch_tendency(h, h->fu[s], h->fv[s], h->fh[s]));
+  PG_CHECK(h, pg_launch_tendency_slot(h, s));
   const uint32_t scheme = h->cfg.timeSteppingScheme;
   bool shift = true;
   int order;
@@ -465,7 +483,7 @@ int pg_bench_kernel(pg_handle* h, int which, int reps)
   for (int r = 0; r < reps; r++) {
     hipError_t e;
     switch (which) {
-      case 0: e = pg_launch_tendency(h, h->fu[s], h->fv[s], h->fh[s]); break;
+      case 0: e = pg_launch_tendency_slot(h, s); break;
       case 1: e = pg_launch_ab(h, 3); break;
       case 2: e = pg_launch_rhs(h); break;
       case 3: e = pg_launch_correct(h); break;
@@ -477,6 +495,10 @@ int pg_bench_kernel(pg_handle* h, int which, int reps)
       case 9:
         if (!pg_tendency_fast_ok(h)) return fail(h, PG_EARG, "the tiled tendency kernel does not serve this configuration");
         e = pg_launch_tendency_inner(h, h->fu[s], h->fv[s], h->fh[s]);
+        break;
+      case 10:
+        if (!h->mixed) return fail(h, PG_EARG, "kernel 10 needs mixed_precision = 1");
+        e = pg_launch_tendency_inner32(h, h->fu32[s], h->fv32[s], h->fh32[s]);
         break;
       default: return fail(h, PG_EARG, "unknown kernel id");
     }

@@ -44,6 +44,8 @@ struct pg_handle {
   size_t fsz = 0;                      // sx * sy
   double *u = nullptr, *v = nullptr, *h = nullptr;   // [L][fsz] state, halos filled
   double *fu[3] = {}, *fv[3] = {}, *fh[3] = {};      // AB history buffers
+  bool mixed = false;                  // FP32 stencils and tendency history
+  float *fu32[3] = {}, *fv32[3] = {}, *fh32[3] = {}; // AB history buffers in mixed precision
   int slot[3] = {0, 1, 2};             // buffers holding tendency, tendency_1, tendency_2
   double *hhb = nullptr, *hhs = nullptr, *omz = nullptr, *taux = nullptr, *tauy = nullptr;
   double *Hc = nullptr, *Hw = nullptr, *Hs = nullptr;  // ghosted static geometry
@@ -76,11 +78,15 @@ struct pg_handle {
 hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind);
 hipError_t pg_fill1(pg_handle* h, double* f);
 hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh);
+hipError_t pg_launch_tendency_slot(pg_handle* h, int s);  // into history slot s, per precision mode
+hipError_t pg_widen_slot(pg_handle* h, int s);            // FP32 history slot -> FP64 slot
 
 // ---- fast tendency (pg_tendency.hip) ----
 bool pg_tendency_fast_ok(const pg_handle* h);
 hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double* dh);
 hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double* dh);
+hipError_t pg_launch_tendency32(pg_handle* h, float* du, float* dv, float* dh);
+hipError_t pg_launch_tendency_inner32(pg_handle* h, float* du, float* dv, float* dh);
 hipError_t pg_launch_ab(pg_handle* h, int order);
 hipError_t pg_launch_thickness(pg_handle* h);
 hipError_t pg_launch_rhs(pg_handle* h);

@@ -131,6 +131,36 @@ __global__ __launch_bounds__(TJ* TI) void k_ab(PgGrid G, AbArgs A)
   A.x[fld][p] = x;
 }
 
+// Mixed precision: FP64 state, FP32 tendency history, FP64 arithmetic.
+struct Ab32Args {
+  double* x[3];
+  const float* f0[3];
+  const float* f1[3];
+  const float* f2[3];
+  double c0, c1, c2;
+  int order;
+};
+
+__global__ __launch_bounds__(TJ* TI) void k_ab32(PgGrid G, Ab32Args A)
+{
+  const int j = G.g + blockIdx.x * TJ + threadIdx.x;
+  const int i = G.g + blockIdx.y * TI + threadIdx.y;
+  const int k = blockIdx.z % G.L, fld = blockIdx.z / G.L;
+  if (i >= G.g + G.nx || j >= G.g + G.ny) return;
+  const size_t p = pg_lay(G, k) + pg_at(G, i, j);
+  double x = A.x[fld][p];
+  if (A.order == 1) x += A.c0 * (double)A.f0[fld][p];
+  else if (A.order == 2) x += A.c0 * (double)A.f0[fld][p] - A.c1 * (double)A.f1[fld][p];
+  else x += A.c0 * (double)A.f0[fld][p] - A.c1 * (double)A.f1[fld][p] + A.c2 * (double)A.f2[fld][p];
+  A.x[fld][p] = x;
+}
+
+__global__ void k_widen(const float* src, double* dst, size_t n)
+{
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
+    dst[p] = (double)src[p];
+}
+
 // ---------------------------------------------------------------------------
 // Rigid-lid thickness fix (pressure.c:525-561): rescale every layer so that
 // the column sum equals Hc.
@@ -516,6 +546,21 @@ hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind)
   return hipGetLastError();
 }
 
+hipError_t pg_launch_tendency_slot(pg_handle* h, int s)
+{
+  if (h->mixed) return pg_launch_tendency32(h, h->fu32[s], h->fv32[s], h->fh32[s]);
+  return pg_launch_tendency(h, h->fu[s], h->fv[s], h->fh[s]);
+}
+
+hipError_t pg_widen_slot(pg_handle* h, int s)
+{
+  const size_t n = (size_t)h->G.L * h->fsz;
+  float* src[3] = {h->fu32[s], h->fv32[s], h->fh32[s]};
+  double* dst[3] = {h->fu[s], h->fv[s], h->fh[s]};
+  for (int f = 0; f < 3; f++) hipLaunchKernelGGL(k_widen, dim3(1024), dim3(256), 0, h->stream, src[f], dst[f], n);
+  return hipGetLastError();
+}
+
 hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh)
 {
   if (h->fast_tendency && pg_tendency_fast_ok(h)) return pg_launch_tendency_fast(h, du, dv, dh);
@@ -527,6 +572,27 @@ hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh)
 hipError_t pg_launch_ab(pg_handle* h, int order)
 {
   const double dt = h->cfg.dt;
+  if (h->mixed) {
+    Ab32Args A;
+    float** fs[3] = {h->fu32, h->fv32, h->fh32};
+    double* xs[3] = {h->u, h->v, h->h};
+    for (int f = 0; f < 3; f++) {
+      A.x[f] = xs[f];
+      A.f0[f] = fs[f][h->slot[0]];
+      A.f1[f] = fs[f][h->slot[1]];
+      A.f2[f] = fs[f][h->slot[2]];
+    }
+    A.order = order;
+    if (order == 1) {
+      A.c0 = dt; A.c1 = 0; A.c2 = 0;
+    } else if (order == 2) {
+      A.c0 = 3 * dt / 2; A.c1 = dt / 2; A.c2 = 0;
+    } else {
+      A.c0 = 23 * dt / 12; A.c1 = 4 * dt / 3; A.c2 = 5 * dt / 12;
+    }
+    hipLaunchKernelGGL(k_ab32, owned_grid(h->G, 3 * h->G.L), dim3(TJ, TI), 0, h->stream, h->G, A);
+    return hipGetLastError();
+  }
   AbArgs A;
   double* xs[3] = {h->u, h->v, h->h};
   double** fs[3] = {h->fu, h->fv, h->fh};

@@ -260,8 +260,10 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
   }
 }
 
-// Edge ring: owned points that are not inner, through the per-point path.
-__global__ __launch_bounds__(256) void k_tendency_edge(PgGrid G, PgPhys P, PgIn I, double* du, double* dv, double* dh)
+// Edge ring: owned points that are not inner, through the per-point path
+// (FP64 arithmetic; stored as T).
+template <typename T>
+__global__ __launch_bounds__(256) void k_tendency_edge(PgGrid G, PgPhys P, PgIn I, T* du, T* dv, T* dh)
 {
   const int e = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   const int nx = G.nx, ny = G.ny, g = G.g;
@@ -280,9 +282,220 @@ __global__ __launch_bounds__(256) void k_tendency_edge(PgGrid G, PgPhys P, PgIn 
   double a, b, c;
   pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
   const size_t p = pg_lay(G, k) + pg_at(G, i, j);
-  du[p] = a;
-  dv[p] = b;
-  dh[p] = c;
+  du[p] = (T)a;
+  dv[p] = (T)b;
+  dh[p] = (T)c;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision interior kernel: the same stencils in FP32 (divisions by
+// the grid spacings and constants become multiplications by reciprocals;
+// 1/h by the hardware reciprocal), except the Montgomery potential and its
+// gradient, which stay FP64 (a difference of large, nearly equal values).
+// Tendencies are stored in FP32.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, PgIn I, float* du, float* dv, float* dh)
+{
+#pragma clang fp contract(fast)
+  __shared__ float su[SH * SW], sv[SH * SW], sh[SH * SW];
+  __shared__ float sq[QH * QW], suu[UH * UW], svv[VH * VW], ske[KH * KW];
+  __shared__ float sdt[KH * KW], sds[DH * DW], shq[DH * DW], sub[KH * KW];
+  extern __shared__ double smm[];  // [L][KH * KW], FP64
+  const int tid = threadIdx.x;
+  const int i0 = G.g + blockIdx.y * TI, j0 = G.g + blockIdx.x * TJ;
+  const bool tw = P.mom == PG_MOM_TW81, al = (P.mom == PG_MOM_AL81 || P.mom == PG_MOM_HK83);
+  const double rdx_d = 1.0 / G.dx, rdy_d = 1.0 / G.dy;
+  const float rdx = (float)rdx_d, rdy = (float)rdy_d, r24 = 1.0f / 24.0f;
+  const float A2 = (float)P.A2, lin = (float)P.lin, quad = (float)P.quad;
+
+  for (int idx = tid; idx < KH * KW; idx += NTH) {
+    const int i = i0 - 1 + idx / KW, j = j0 - 1 + idx % KW;
+    if (i >= G.sx - 1 || j >= G.sy - 1) continue;
+    const size_t p = pg_at(G, i, j);
+    double eta = I.hhb[p];
+    for (int m = G.L - 1; m >= 1; m--) {
+      eta = eta + I.h[pg_lay(G, m) + p];
+      smm[m * (KH * KW) + idx] = eta;
+    }
+    double pp = 0;
+    for (int k = 0; k < G.L; k++) {
+      if (k >= 1) pp = pp + P.gg[k] * smm[k * (KH * KW) + idx];
+      double mm = pp;
+      if (P.h0 != 0) {
+        const double hk = I.h[pg_lay(G, k) + p];
+        mm -= P.geff[k] * (P.h0 * P.h0 * P.h0 * P.h0) / (hk * hk * hk) / 3;
+      }
+      smm[k * (KH * KW) + idx] = mm;
+    }
+    if (P.useQuad) sub[idx] = (float)pg_uabs_bot(G, I, i, j);
+  }
+
+  for (int k = 0; k < G.L; k++) {
+    const size_t lay = pg_lay(G, k);
+    for (int idx = tid; idx < SH * SW; idx += NTH) {
+      const int i = i0 - 2 + idx / SW, j = j0 - 2 + idx % SW;
+      if (i < G.sx && j < G.sy) {
+        const size_t p = lay + pg_at(G, i, j);
+        su[idx] = (float)I.u[p];
+        sv[idx] = (float)I.v[p];
+        sh[idx] = (float)I.h[p];
+      }
+    }
+    __syncthreads();
+#define U(a, b) su[((a) + 2) * SW + (b) + 2]
+#define V(a, b) sv[((a) + 2) * SW + (b) + 2]
+#define H(a, b) sh[((a) + 2) * SW + (b) + 2]
+    for (int idx = tid; idx < QH * QW; idx += NTH) {
+      const int a = idx / QW - 1, b = idx % QW - 1;
+      const int i = i0 + a, j = j0 + b;
+      if (i >= G.sx - 2 || j >= G.sy - 2) continue;
+      const float zeta = (U(a, b - 1) - U(a, b)) * rdy + (V(a, b) - V(a - 1, b)) * rdx;
+      const float hq = 0.25f * (H(a, b) + H(a - 1, b - 1) + H(a - 1, b) + H(a, b - 1));
+      sq[idx] = (2.0f * (float)I.omz[pg_at(G, i, j)] + zeta) * rcpf(hq);
+    }
+    for (int idx = tid; idx < UH * UW; idx += NTH) {
+      const int a = idx / UW - 1, b = idx % UW - 1;
+      suu[idx] = U(a, b) * (0.5f * (H(a, b) + H(a - 1, b)));
+    }
+    for (int idx = tid; idx < VH * VW; idx += NTH) {
+      const int a = idx / VW - 1, b = idx % VW - 1;
+      svv[idx] = V(a, b) * (0.5f * (H(a, b) + H(a, b - 1)));
+    }
+#define SQ(x) ((x) * (x))
+    for (int idx = tid; idx < KH * KW; idx += NTH) {
+      const int a = idx / KW - 1, b = idx % KW - 1;
+      if (tw)
+        ske[idx] = 0.25f * ((2.0f / 3.0f) * SQ(U(a, b)) + (1.0f / 3.0f) * SQ(0.5f * (U(a - 1, b) + U(a + 1, b)))
+                            + (2.0f / 3.0f) * SQ(U(a + 1, b)) + (1.0f / 3.0f) * SQ(0.5f * (U(a, b) + U(a + 2, b)))
+                            + (2.0f / 3.0f) * SQ(V(a, b)) + (1.0f / 3.0f) * SQ(0.5f * (V(a, b - 1) + V(a, b + 1)))
+                            + (2.0f / 3.0f) * SQ(V(a, b + 1)) + (1.0f / 3.0f) * SQ(0.5f * (V(a, b) + V(a, b + 2))));
+      else if (P.mom == PG_MOM_HK83)
+        ske[idx] = (SQ(U(a + 1, b)) + SQ(U(a, b))
+                    + 0.25f * (SQ(U(a + 1, b + 1)) + SQ(U(a, b + 1)) + SQ(U(a + 1, b - 1)) + SQ(U(a, b - 1)))
+                    + SQ(V(a, b + 1)) + SQ(V(a, b))
+                    + 0.25f * (SQ(V(a + 1, b + 1)) + SQ(V(a + 1, b)) + SQ(V(a - 1, b + 1)) + SQ(V(a - 1, b))))
+                   * (1.0f / 6.0f);
+      else
+        ske[idx] = (SQ(U(a + 1, b)) + SQ(U(a, b))) * 0.25f + (SQ(V(a, b + 1)) + SQ(V(a, b))) * 0.25f;
+      if (P.useA2) sdt[idx] = (U(a + 1, b) - U(a, b)) * rdx - (V(a, b + 1) - V(a, b)) * rdy;
+    }
+#undef SQ
+    if (P.useA2)
+      for (int idx = tid; idx < DH * DW; idx += NTH) {
+        const int a = idx / DW, b = idx % DW;
+        sds[idx] = (U(a, b) - U(a, b - 1)) * rdy + (V(a, b) - V(a - 1, b)) * rdx;
+        shq[idx] = 0.25f * (H(a, b) + H(a - 1, b - 1) + H(a - 1, b) + H(a, b - 1));
+      }
+    __syncthreads();
+#define Q(a, b) sq[((li) + (a) + 1) * QW + (lj) + (b) + 1]
+#define HU(a, b) suu[((li) + (a) + 1) * UW + (lj) + (b) + 1]
+#define HV(a, b) svv[((li) + (a) + 1) * VW + (lj) + (b) + 1]
+#define KE(a, b) ske[((li) + (a) + 1) * KW + (lj) + (b) + 1]
+#define MM(a, b) smm[k * (KH * KW) + ((li) + (a) + 1) * KW + (lj) + (b) + 1]
+#define DT(a, b) sdt[((li) + (a) + 1) * KW + (lj) + (b) + 1]
+#define UB(a, b) sub[((li) + (a) + 1) * KW + (lj) + (b) + 1]
+#define DS(a, b) sds[((li) + (a)) * DW + (lj) + (b)]
+#define HQ(a, b) shq[((li) + (a)) * DW + (lj) + (b)]
+    for (int idx = tid; idx < TI * TJ; idx += NTH) {
+      const int li = idx / TJ, lj = idx % TJ;
+      const int i = i0 + li, j = j0 + lj;
+      if (i >= G.g + G.nx || j >= G.g + G.ny || !is_inner(G, i, j)) continue;
+      float a00 = 0, b00 = 0, c00 = 0, d00 = 0, l0p = 0, l00 = 0, cp0 = 0, a0m = 0, bpm = 0, mp0 = 0, m00 = 0;
+      float e00 = 0, em0 = 0, p0m = 0, p00 = 0;
+      if (tw) {
+        a00 = (2 * Q(1, 1) + 3 * Q(0, 1) + 2 * Q(0, 0) + Q(1, 0) - Q(0, 2) - Q(-1, 1)) * r24;
+        b00 = (3 * Q(0, 1) + 2 * Q(-1, 1) + Q(-1, 0) + 2 * Q(0, 0) - Q(1, 1) - Q(0, 2)) * r24;
+        c00 = (2 * Q(0, 1) + Q(-1, 1) + 2 * Q(-1, 0) + 3 * Q(0, 0) - Q(0, -1) - Q(1, 0)) * r24;
+        d00 = (Q(1, 1) + 2 * Q(0, 1) + 3 * Q(0, 0) + 2 * Q(1, 0) - Q(-1, 0) - Q(0, -1)) * r24;
+        l0p = (Q(1, 1) - Q(-1, 1)) * r24;
+        l00 = (Q(1, 0) - Q(-1, 0)) * r24;
+        cp0 = (2 * Q(1, 1) + Q(0, 1) + 2 * Q(0, 0) + 3 * Q(1, 0) - Q(1, -1) - Q(2, 0)) * r24;
+        a0m = (2 * Q(1, 0) + 3 * Q(0, 0) + 2 * Q(0, -1) + Q(1, -1) - Q(0, 1) - Q(-1, 0)) * r24;
+        bpm = (3 * Q(1, 0) + 2 * Q(0, 0) + Q(0, -1) + 2 * Q(1, -1) - Q(2, 0) - Q(1, 1)) * r24;
+        mp0 = (Q(1, -1) - Q(1, 1)) * r24;
+        m00 = (Q(0, -1) - Q(0, 1)) * r24;
+      } else if (al) {
+        a00 = (2 * Q(1, 1) + Q(0, 1) + 2 * Q(0, 0) + Q(1, 0)) * r24;
+        b00 = (Q(0, 1) + 2 * Q(-1, 1) + Q(-1, 0) + 2 * Q(0, 0)) * r24;
+        c00 = (2 * Q(0, 1) + Q(-1, 1) + 2 * Q(-1, 0) + Q(0, 0)) * r24;
+        d00 = (Q(1, 1) + 2 * Q(0, 1) + Q(0, 0) + 2 * Q(1, 0)) * r24;
+        cp0 = (2 * Q(1, 1) + Q(0, 1) + 2 * Q(0, 0) + Q(1, 0)) * r24;
+        a0m = (2 * Q(1, 0) + Q(0, 0) + 2 * Q(0, -1) + Q(1, -1)) * r24;
+        bpm = (Q(1, 0) + 2 * Q(0, 0) + Q(0, -1) + 2 * Q(1, -1)) * r24;
+      }
+      if (tw || al) {
+        e00 = (Q(1, 1) + Q(0, 1) - Q(0, 0) - Q(1, 0)) * r24;
+        em0 = (Q(0, 1) + Q(-1, 1) - Q(-1, 0) - Q(0, 0)) * r24;
+        p0m = (-Q(1, 0) + Q(0, 0) + Q(0, -1) - Q(1, -1)) * r24;
+        p00 = (-Q(1, 1) + Q(0, 1) + Q(0, 0) - Q(1, 0)) * r24;
+      }
+      const float hwest = 0.5f * (H(li, lj) + H(li - 1, lj));
+      const float hsouth = 0.5f * (H(li, lj) + H(li, lj - 1));
+      const float rhw = (hwest != 0) ? rcpf(hwest) : 0.0f, rhs_ = (hsouth != 0) ? rcpf(hsouth) : 0.0f;
+      const float fs = (k == 0) ? 1.0f : 0.0f, fb = (k == G.L - 1) ? 1.0f : 0.0f;
+      const size_t q = pg_at(G, i, j);
+      float ru, rv;
+      {
+        const float gM = (float)(-(MM(0, 0) - MM(-1, 0)) * rdx_d);
+        const float gK = -(KE(0, 0) - KE(-1, 0)) * rdx;
+        float qf;
+        if (tw)
+          qf = a00 * HV(0, 1) + b00 * HV(-1, 1) + c00 * HV(-1, 0) + d00 * HV(0, 0) - e00 * HU(1, 0) + em0 * HU(-1, 0)
+               - l0p * HU(0, 0) + l00 * HU(0, -1);
+        else if (al)
+          qf = a00 * HV(0, 1) + b00 * HV(-1, 1) + c00 * HV(-1, 0) + d00 * HV(0, 0) - e00 * HU(1, 0) + em0 * HU(-1, 0);
+        else
+          qf = 0.25f * Q(0, 0) * (HV(0, 0) + HV(-1, 0)) + 0.25f * Q(0, 1) * (HV(0, 1) + HV(-1, 1));
+        float a2 = 0;
+        if (P.useA2)
+          a2 = (A2 * H(li, lj) * DT(0, 0) - A2 * H(li - 1, lj) * DT(-1, 0)) * rdx
+               + (A2 * HQ(0, 1) * DS(0, 1) - A2 * HQ(0, 0) * DS(0, 0)) * rdy;
+        ru = gM + gK + qf + a2 * rhw;
+        if (P.useWind) ru += (float)I.taux[q] * fs * rhw;
+        if (P.useLin) ru += -lin * U(li, lj) * fb * rhw;
+        if (P.useQuad) ru -= quad * (0.5f * (UB(0, 0) + UB(-1, 0))) * U(li, lj) * fb * rhw;
+      }
+      {
+        const float gM = (float)(-(MM(0, 0) - MM(0, -1)) * rdy_d);
+        const float gK = -(KE(0, 0) - KE(0, -1)) * rdy;
+        float qf;
+        if (tw)
+          qf = p0m * HV(0, -1) - p00 * HV(0, 1) - cp0 * HU(1, 0) - d00 * HU(0, 0) - a0m * HU(0, -1) - bpm * HU(1, -1)
+               - mp0 * HV(0, 0) + m00 * HV(-1, 0);
+        else if (al)
+          qf = p0m * HV(0, -1) - p00 * HV(0, 1) - cp0 * HU(1, 0) - d00 * HU(0, 0) - a0m * HU(0, -1) - bpm * HU(1, -1);
+        else
+          qf = -(0.25f * Q(0, 0) * (HU(0, 0) + HU(0, -1)) + 0.25f * Q(1, 0) * (HU(1, 0) + HU(1, -1)));
+        float a2 = 0;
+        if (P.useA2)
+          a2 = (A2 * HQ(1, 0) * DS(1, 0) - A2 * HQ(0, 0) * DS(0, 0)) * rdx
+               - (A2 * H(li, lj) * DT(0, 0) - A2 * H(li, lj - 1) * DT(0, -1)) * rdy;
+        rv = gM + gK + qf + a2 * rhs_;
+        if (P.useWind) rv += (float)I.tauy[q] * fs * rhs_;
+        if (P.useLin) rv += -lin * V(li, lj) * fb * rhs_;
+        if (P.useQuad) rv -= quad * (0.5f * (UB(0, 0) + UB(0, -1))) * V(li, lj) * fb * rhs_;
+      }
+      const size_t p = lay + q;
+      du[p] = ru;
+      dv[p] = rv;
+      dh[p] = (HU(0, 0) - HU(1, 0)) * rdx + (HV(0, 0) - HV(0, 1)) * rdy;
+    }
+#undef Q
+#undef HU
+#undef HV
+#undef KE
+#undef MM
+#undef DT
+#undef UB
+#undef DS
+#undef HQ
+#undef U
+#undef V
+#undef H
+    __syncthreads();
+  }
 }
 
 }  // namespace
@@ -310,6 +523,30 @@ hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double*
   PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
   const PgGrid& G = h->G;
   const int total = 5 * G.ny + (G.nx - 5) * 5;
-  hipLaunchKernelGGL(k_tendency_edge, dim3((total + 255) / 256, G.L), dim3(256), 0, h->stream, G, h->P, I, du, dv, dh);
+  hipLaunchKernelGGL(k_tendency_edge<double>, dim3((total + 255) / 256, G.L), dim3(256), 0, h->stream, G, h->P, I, du, dv,
+                     dh);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_tendency_inner32(pg_handle* h, float* du, float* dv, float* dh)
+{
+  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
+  const PgGrid& G = h->G;
+  const size_t dyn = (size_t)G.L * KH * KW * sizeof(double);
+  dim3 grid((G.ny + TJ - 1) / TJ, (G.nx + TI - 1) / TI);
+  hipLaunchKernelGGL(k_tendency_inner32, grid, dim3(NTH), dyn, h->stream, G, h->P, I, du, dv, dh);
+  return hipGetLastError();
+}
+
+// Mixed precision: FP32 interior kernel + FP64 edge ring, FP32 tendencies.
+hipError_t pg_launch_tendency32(pg_handle* h, float* du, float* dv, float* dh)
+{
+  hipError_t e = pg_launch_tendency_inner32(h, du, dv, dh);
+  if (e != hipSuccess) return e;
+  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
+  const PgGrid& G = h->G;
+  const int total = 5 * G.ny + (G.nx - 5) * 5;
+  hipLaunchKernelGGL(k_tendency_edge<float>, dim3((total + 255) / 256, G.L), dim3(256), 0, h->stream, G, h->P, I, du, dv,
+                     dh);
   return hipGetLastError();
 }

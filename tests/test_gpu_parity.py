@@ -69,3 +69,31 @@ def test_gpu_matches_oracle_every_step(name, tmp_path, oracle_exe):
             d = np.max(np.abs(g[key] - o[key]))
             assert same_bits(g[key], o[key]), f"first mismatch: {key} at step {n}, max|diff|={d:.3e}"
     assert parse_iters(gout) == parse_iters(oout)
+
+
+MIXED_CASES = {
+    # closed basin with topography, TW81 + A2 + quadratic drag
+    "jets128x4": dict(kind="jets", nx=128, ny=128, nlay=4, dx=20e3, nt=60, A2=50.0, quadDragCoeff=2e-3),
+    # periodic channel over a ridge, AL81 + linear drag
+    "chan96x64": dict(kind="channel", nx=96, ny=64, nt=60, dx=20e3, momentumScheme=0, linDragCoeff=1e-3),
+}
+# max|mixed - fp64| / max|fp64| per field after 60 steps
+MIXED_TOL = {"U": 2e-4, "V": 2e-4, "H": 1e-6}
+
+
+@pytest.mark.parametrize("name", sorted(MIXED_CASES))
+def test_mixed_precision_tracks_fp64(name, tmp_path):
+    """FP32 stencils + FP32 V-cycles inside FP64 refinement vs the FP64 path
+    (FP64 GPU multigrid), both solved to a relative residual of 1e-12."""
+    spec = dict(MIXED_CASES[name])
+    kind = spec.pop("kind")
+    res = {}
+    for mode, extra in (("fp64", dict(solver=3)), ("mixed", dict(solver=4, mixed_precision=1))):
+        setup = getattr(inputs, kind)(name=f"{name}_{mode}", tol=1e-12, maxiters=400, **extra, **spec)
+        run_exe(pawsim_amd.DRIVER_PATH, setup.write(str(tmp_path / f"in_{mode}")), str(tmp_path / mode), timeout=300)
+        res[mode] = inputs.read_outputs(str(tmp_path / mode), setup.nx, setup.ny, setup.nlay, int(setup.params["Nt"]))
+    for key in ("U", "V", "H"):
+        a, b = res["mixed"][key], res["fp64"][key]
+        rel = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        print(f"[mixed precision] {name} {key} rel={rel:.3e}", flush=True)
+        assert rel <= MIXED_TOL[key], (key, rel)
