@@ -146,6 +146,28 @@ int pg_synchronize(pg_handle *h);
  * only).  No reference counterpart. */
 int pg_bench_kernel(pg_handle *h, int which, int reps);
 
+/* ---- Learned pressure preconditioner network (no reference counterpart) ----
+ * A 4-level convolutional encoder-decoder (3x3 convolutions, 16-32-64-64
+ * channels, BF16 weights and activations, FP32 accumulation on MFMA) that
+ * maps [normalised defect, normalised iterate, depth mask] to a pressure
+ * correction.  Standalone handle for training and checking; images are
+ * H x W with H % 8 == 0 and W % 128 == 0, batch B.
+ * x: [B][H][W][2] FP32 (defect, iterate); mask: [H][W]; y, t: [B][H][W].
+ * Parameters: pg_net_param_count() FP32 values, per layer w[Cout][9][Cin]
+ * then b[Cout], layers (Cin, Cout) = (4,16) (16,32) (32,64) (64,64) (64,32)
+ * (32,16) (16,1); input channel 3 is zero. */
+typedef struct pg_net_handle pg_net_handle;
+int pg_net_param_count(void);
+int pg_net_create(int H, int W, int B, pg_net_handle **out);
+void pg_net_destroy(pg_net_handle *h);
+int pg_net_set_params(pg_net_handle *h, const float *params);   /* also resets the Adam moments */
+int pg_net_get_params(pg_net_handle *h, float *params);
+int pg_net_forward(pg_net_handle *h, const float *x, const float *mask, float *y);
+/* mean squared error against t and its gradient for every parameter */
+int pg_net_loss_grad(pg_net_handle *h, const float *x, const float *mask, const float *t, float *loss, float *grad);
+/* one Adam step (beta 0.9, 0.999, eps 1e-8) with the last gradient */
+int pg_net_adam_step(pg_net_handle *h, float lr);
+
 #ifdef __cplusplus
 }
 #endif

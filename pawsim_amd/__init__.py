@@ -29,6 +29,8 @@ ABI_SYMBOLS = (
     "pg_create", "pg_destroy", "pg_last_error", "pg_upload_state", "pg_download_state",
     "pg_upload_pi", "pg_download_pi", "pg_tendency", "pg_step", "pg_pressure_correct",
     "pg_solve_pressure", "pg_refresh_halos", "pg_time", "pg_set_stream", "pg_synchronize", "pg_bench_kernel",
+    "pg_net_param_count", "pg_net_create", "pg_net_destroy", "pg_net_set_params", "pg_net_get_params",
+    "pg_net_forward", "pg_net_loss_grad", "pg_net_adam_step",
 )
 
 
@@ -95,6 +97,16 @@ def load_library() -> ctypes.CDLL:
     lib.pg_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     lib.pg_synchronize.argtypes = [ctypes.c_void_p]
     lib.pg_bench_kernel.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    f = ctypes.POINTER(ctypes.c_float)
+    lib.pg_net_param_count.argtypes = []
+    lib.pg_net_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(ctypes.c_void_p)]
+    lib.pg_net_destroy.argtypes = [ctypes.c_void_p]
+    lib.pg_net_destroy.restype = None
+    lib.pg_net_set_params.argtypes = [ctypes.c_void_p, f]
+    lib.pg_net_get_params.argtypes = [ctypes.c_void_p, f]
+    lib.pg_net_forward.argtypes = [ctypes.c_void_p, f, f, f]
+    lib.pg_net_loss_grad.argtypes = [ctypes.c_void_p, f, f, f, f, f]
+    lib.pg_net_adam_step.argtypes = [ctypes.c_void_p, ctypes.c_float]
     _lib = lib
     return lib
 
@@ -241,3 +253,69 @@ class Model:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Net:
+    """The learned-preconditioner network on the GPU (standalone handle).
+
+    x: (B, H, W, 2) float32 [defect, iterate]; mask: (H, W); y, t: (B, H, W).
+    """
+
+    def __init__(self, H: int, W: int, B: int = 1):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self._lib.pg_net_create(H, W, B, ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"pg_net_create failed with {rc}")
+        self._h = h
+        self.shape = (B, H, W)
+        self.nparam = self._lib.pg_net_param_count()
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with {rc}")
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        self._ck(self._lib.pg_net_set_params(self._h, _fptr(p)), "pg_net_set_params")
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.nparam, np.float32)
+        self._ck(self._lib.pg_net_get_params(self._h, _fptr(p)), "pg_net_get_params")
+        return p
+
+    def forward(self, x, mask) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        mask = np.ascontiguousarray(mask, dtype=np.float32)
+        y = np.zeros(self.shape, np.float32)
+        self._ck(self._lib.pg_net_forward(self._h, _fptr(x), _fptr(mask), _fptr(y)), "pg_net_forward")
+        return y
+
+    def loss_grad(self, x, mask, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        mask = np.ascontiguousarray(mask, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.nparam, np.float32)
+        self._ck(self._lib.pg_net_loss_grad(self._h, _fptr(x), _fptr(mask), _fptr(t), _fptr(loss), _fptr(g)),
+                 "pg_net_loss_grad")
+        return float(loss[0]), g
+
+    def adam_step(self, lr: float) -> None:
+        self._ck(self._lib.pg_net_adam_step(self._h, ctypes.c_float(lr)), "pg_net_adam_step")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.pg_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

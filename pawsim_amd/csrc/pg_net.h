@@ -1,0 +1,37 @@
+// pg_net.h -- device state of the learned pressure preconditioner network.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+typedef unsigned short bf16_t;
+
+#define PG_NET_LAYERS 7
+
+// One network instance: H x W images (H % 8 == 0, W % 128 == 0), batch B.
+struct pg_net {
+  int H = 0, W = 0, B = 0;
+  hipStream_t stream = nullptr;
+  int nparam = 0;
+  int woff[PG_NET_LAYERS] = {}, boff[PG_NET_LAYERS] = {};
+  float *params = nullptr, *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  int adam_t = 0;
+  bf16_t *wf[PG_NET_LAYERS] = {}, *wt[PG_NET_LAYERS] = {};  // forward / rotated-transposed BF16 weights
+  bf16_t* x0 = nullptr;                                      // input (B, H, W, 4)
+  float* mask = nullptr;                                     // depth mask (H, W)
+  bf16_t *a[6] = {}, *p[3] = {}, *s[3] = {}, *ds[3] = {};    // activations and skip gradients
+  bf16_t *da = nullptr, *dz = nullptr, *dp = nullptr, *dy4 = nullptr;
+  float *y = nullptr, *t = nullptr;                          // output / target (B, H, W)
+  float *part = nullptr, *part2 = nullptr, *loss = nullptr;
+  size_t npart = 0;
+};
+
+int pg_net_param_count_impl();
+int pg_net_alloc(pg_net* N, int H, int W, int B, hipStream_t stream);
+void pg_net_free(pg_net* N);
+hipError_t pg_net_refresh_weights(pg_net* N);
+hipError_t pg_net_pack_input(pg_net* N, const float* c0, const float* c1);
+hipError_t pg_net_forward_dev(pg_net* N);
+hipError_t pg_net_backward_dev(pg_net* N);
+hipError_t pg_net_adam_dev(pg_net* N, float lr);

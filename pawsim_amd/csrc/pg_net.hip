@@ -1,0 +1,622 @@
+// pg_net.hip -- convolutional encoder-decoder used as a learned pressure
+// preconditioner: BF16 weights and activations, FP32 accumulation on MFMA
+// (v_mfma_f32_16x16x16_bf16), forward and backward kernels and Adam.
+//
+// Tensors are NHWC: element (b, y, x, c) at ((b*H + y)*W + x)*C + c, with
+// y the first grid index (i) and x the second (j, contiguous in the pressure
+// arrays).  Network (every convolution 3x3, zero padding, level l has
+// (H >> l) x (W >> l) pixels):
+//   a0 = relu(conv0(x0))  4->16      l0      s2 = a2 + up(a3)
+//   a1 = relu(conv1(pool a0)) 16->32 l1      a4 = relu(conv4(s2)) 64->32 l2
+//   a2 = relu(conv2(pool a1)) 32->64 l2      s1 = a1 + up(a4)
+//   a3 = relu(conv3(pool a2)) 64->64 l3      a5 = relu(conv5(s1)) 32->16 l1
+//                                            s0 = a0 + up(a5)
+//                                            y  = conv6(s0)       16->1  l0
+// pool = 2x2 mean, up = nearest.  x0 channels: defect, iterate, depth mask,
+// zero.  Parameters (FP32 master copy): per layer w[Cout][9][Cin] (tap =
+// 3*(dy+1) + (dx+1)) then b[Cout].
+#include "pg_internal.h"
+#include "pg_net.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(bf16_t x) { return __uint_as_float((unsigned)x << 16); }
+__device__ __forceinline__ bf16_t f2bf(float f)
+{
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (bf16_t)(u >> 16);
+}
+
+// ---------------------------------------------------------------------------
+// convolution (implicit GEMM): one wave computes 16 consecutive pixels of a
+// row (A rows) for NCT x 16 output channels (B columns); K runs over
+// (tap, input channel) with 4 consecutive channels per lane.
+//   out = act(bias + sum_{tap,ci} w[co][tap][ci] * in[y+dy][x+dx][ci])
+// in: Cin channels (Cin % 4 == 0); w: [NCT*16][9][Cin] bf16 (zero rows
+// above Cout); output bf16 (Cout channels) or FP32 (out_f, Cout == 1).
+// ---------------------------------------------------------------------------
+template <int NCT>
+__global__ __launch_bounds__(256) void k_conv(const bf16_t* __restrict__ in, int B, int H, int W, int Cin,
+                                              const bf16_t* __restrict__ w, const float* __restrict__ bias, int Cout,
+                                              int relu, bf16_t* __restrict__ out, float* __restrict__ out_f)
+{
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int segs = W / 16;
+  if (wave >= B * H * segs) return;  // whole waves only
+  const int b = wave / (H * segs), rem = wave - b * (H * segs);
+  const int y = rem / segs, x0 = (rem - (rem / segs) * segs) * 16;
+  const int m = lane & 15, kg = lane >> 4;
+  v4f acc[NCT];
+  for (int t = 0; t < NCT; t++) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  const int K = 9 * Cin;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const int k = k0 + 4 * kg;
+    v4s a = {0, 0, 0, 0};
+    v4s bv[NCT];
+    for (int t = 0; t < NCT; t++) bv[t] = v4s{0, 0, 0, 0};
+    if (k < K) {
+      const int tap = k / Cin, ci = k - (k / Cin) * Cin;
+      const int yy = y + tap / 3 - 1, xx = x0 + m + tap % 3 - 1;
+      if (yy >= 0 && yy < H && xx >= 0 && xx < W) a = *(const v4s*)(in + (((size_t)b * H + yy) * W + xx) * Cin + ci);
+      for (int t = 0; t < NCT; t++) bv[t] = *(const v4s*)(w + (size_t)(t * 16 + m) * K + k);
+    }
+    for (int t = 0; t < NCT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[t], acc[t], 0, 0, 0);
+  }
+  // lane holds D[pixel 4*kg + r][channel m] of each tile
+  for (int t = 0; t < NCT; t++) {
+    const int co = t * 16 + m;
+    if (co >= Cout) continue;
+    const float bb = bias ? bias[co] : 0.0f;
+    for (int r = 0; r < 4; r++) {
+      float v = acc[t][r] + bb;
+      if (relu) v = v > 0.0f ? v : 0.0f;
+      const size_t px = ((size_t)b * H + y) * W + x0 + 4 * kg + r;
+      if (out_f) out_f[px * Cout + co] = v;
+      else out[px * Cout + co] = f2bf(v);
+    }
+  }
+}
+
+// Weight gradient partials: part[chunk][co][n] = sum over the chunk's pixels
+// q of dZ[q][co] * X[q shifted by tap(n)][ci(n)], n = tap*Cin + ci.  One
+// wave per (n tile, co tile, chunk); A = dZ (co x pixels), B = X (pixels x n).
+__global__ __launch_bounds__(64) void k_conv_wgrad(const bf16_t* __restrict__ X, int Cin, const bf16_t* __restrict__ dZ,
+                                                   int Cz, int Cout, int B, int H, int W, int chunk, int NP, int MP,
+                                                   float* __restrict__ part)
+{
+  const int lane = threadIdx.x;
+  const int nt = blockIdx.x, mt = blockIdx.y, ch = blockIdx.z;
+  const int npx = B * H * W;
+  const int q0 = ch * chunk, q1 = min(npx, q0 + chunk);
+  const int mm = lane & 15, kg = lane >> 4;
+  const int co = mt * 16 + mm;
+  const int n = nt * 16 + mm;
+  const bool nvalid = n < 9 * Cin;
+  const int tap = nvalid ? n / Cin : 0, ci = nvalid ? n - (n / Cin) * Cin : 0;
+  const int ty = tap / 3 - 1, tx = tap % 3 - 1;
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+  for (int q = q0; q < q1; q += 16) {
+    v4s a = {0, 0, 0, 0}, bv = {0, 0, 0, 0};
+    for (int r = 0; r < 4; r++) {
+      const int qq = q + 4 * kg + r;
+      if (qq >= q1) continue;
+      if (co < Cout) a[r] = (short)dZ[(size_t)qq * Cz + co];
+      if (nvalid) {
+        const int xq = qq % W, yq = (qq / W) % H, bq = qq / (W * H);
+        const int yy = yq + ty, xx = xq + tx;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) bv[r] = (short)X[(((size_t)bq * H + yy) * W + xx) * Cin + ci];
+      }
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+  }
+  for (int r = 0; r < 4; r++) part[((size_t)ch * MP + mt * 16 + 4 * kg + r) * NP + nt * 16 + mm] = acc[r];
+}
+
+// g[co][n] = sum over chunks (fixed order)
+__global__ void k_wgrad_reduce(const float* __restrict__ part, int nch, int MP, int NP, int Cout, int K9, float* __restrict__ g)
+{
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= Cout * K9) return;
+  const int co = idx / K9, n = idx - (idx / K9) * K9;
+  float s = 0.f;
+  for (int c = 0; c < nch; c++) s += part[((size_t)c * MP + co) * NP + n];
+  g[idx] = s;
+}
+
+// bias gradient partials: part[chunk][co] = sum_q dZ[q][co]; then reduced
+__global__ void k_bgrad(const bf16_t* __restrict__ dZ, int Cz, int Cout, int npx, int chunk, float* __restrict__ part)
+{
+  const int co = threadIdx.x;  // blockDim.x = 64 >= Cout
+  const int q0 = blockIdx.x * chunk, q1 = min(npx, q0 + chunk);
+  if (co >= Cout) return;
+  float s = 0.f;
+  for (int q = q0; q < q1; q++) s += bf2f(dZ[(size_t)q * Cz + co]);
+  part[blockIdx.x * 64 + co] = s;
+}
+
+__global__ void k_bgrad_reduce(const float* __restrict__ part, int nch, int Cout, float* __restrict__ g)
+{
+  const int co = threadIdx.x;
+  if (co >= Cout) return;
+  float s = 0.f;
+  for (int c = 0; c < nch; c++) s += part[c * 64 + co];
+  g[co] = s;
+}
+
+// out (B, H/2, W/2, C) = scale * sum of 2x2 blocks of in (B, H, W, C)
+__global__ void k_pool(const bf16_t* __restrict__ in, int B, int H, int W, int C, float scale, bf16_t* __restrict__ out)
+{
+  const size_t n = (size_t)B * (H / 2) * (W / 2) * C;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int c = idx % C;
+    size_t r = idx / C;
+    const int x = r % (W / 2);
+    r /= (W / 2);
+    const int y = r % (H / 2);
+    const int b = r / (H / 2);
+    const size_t p = (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
+    const float s = bf2f(in[p]) + bf2f(in[p + C]) + bf2f(in[p + (size_t)W * C]) + bf2f(in[p + (size_t)W * C + C]);
+    out[idx] = f2bf(scale * s);
+  }
+}
+
+// out (B, H, W, C) = A + scale * nearest-upsampled S (B, H/2, W/2, C)
+__global__ void k_add_up(const bf16_t* __restrict__ A, const bf16_t* __restrict__ S, int B, int H, int W, int C, float scale,
+                         bf16_t* __restrict__ out)
+{
+  const size_t n = (size_t)B * H * W * C;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const int c = idx % C;
+    size_t r = idx / C;
+    const int x = r % W;
+    r /= W;
+    const int y = r % H;
+    const int b = r / H;
+    const size_t q = (((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c;
+    out[idx] = f2bf(bf2f(A[idx]) + scale * bf2f(S[q]));
+  }
+}
+
+// dz = da where a > 0, else 0
+__global__ void k_relu_mask(const bf16_t* __restrict__ da, const bf16_t* __restrict__ a, size_t n, bf16_t* __restrict__ dz)
+{
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x)
+    dz[idx] = bf2f(a[idx]) > 0.0f ? da[idx] : (bf16_t)0;
+}
+
+// dy (4 channels, channel 0 used) = 2 (y - t) / n; per-block partial sums of (y - t)^2
+__global__ __launch_bounds__(256) void k_mse_grad(const float* __restrict__ y, const float* __restrict__ t, size_t n,
+                                                  bf16_t* __restrict__ dy4, float* __restrict__ part)
+{
+  __shared__ float red[4];
+  float s = 0.f;
+  const float g = 2.0f / (float)n;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    const float d = y[idx] - t[idx];
+    s += d * d;
+    dy4[4 * idx] = f2bf(g * d);
+    dy4[4 * idx + 1] = 0;
+    dy4[4 * idx + 2] = 0;
+    dy4[4 * idx + 3] = 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void k_sum_f(const float* __restrict__ part, int n, float scale, float* __restrict__ out)
+{
+  if (threadIdx.x != 0) return;
+  float s = 0.f;
+  for (int i = 0; i < n; i++) s += part[i];
+  *out = s * scale;
+}
+
+// x0 (B, H, W, 4) from FP32 planes (B, H, W) each: channels c0, c1, mask, 0
+__global__ void k_pack_input(const float* __restrict__ c0, const float* __restrict__ c1, const float* __restrict__ mask,
+                             size_t plane, int B, bf16_t* __restrict__ x0)
+{
+  const size_t n = (size_t)B * plane;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < n; idx += (size_t)gridDim.x * blockDim.x) {
+    x0[4 * idx] = f2bf(c0[idx]);
+    x0[4 * idx + 1] = f2bf(c1[idx]);
+    x0[4 * idx + 2] = f2bf(mask[idx % plane]);
+    x0[4 * idx + 3] = 0;
+  }
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v, int n,
+                       float lr, float b1, float b2, float eps, float c1, float c2)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// BF16 copies of one layer: wf[co][tap][ci] (co < CoutP, zero above Cout)
+// and the rotated transpose wt[ci][tap][co] = w[co][8 - tap][ci]
+// (ci < CinT, co < CoutT, zero outside) used by the data gradient.
+__global__ void k_refresh_w(const float* __restrict__ w, int Cin, int Cout, int CoutP, int CinT, int CoutT,
+                            bf16_t* __restrict__ wf, bf16_t* __restrict__ wt)
+{
+  const int nf = CoutP * 9 * Cin, nt = CinT * 9 * CoutT;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nf + nt; idx += gridDim.x * blockDim.x) {
+    if (idx < nf) {
+      const int co = idx / (9 * Cin);
+      wf[idx] = co < Cout ? f2bf(w[idx]) : (bf16_t)0;
+    } else {
+      const int j = idx - nf;
+      const int ci = j / (9 * CoutT), rem = j - ci * (9 * CoutT);
+      const int tap = rem / CoutT, co = rem - tap * CoutT;
+      wt[j] = (ci < Cin && co < Cout) ? f2bf(w[((size_t)co * 9 + (8 - tap)) * Cin + ci]) : (bf16_t)0;
+    }
+  }
+}
+
+int ceil_to(int x, int m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static const int kCin[PG_NET_LAYERS] = {4, 16, 32, 64, 64, 32, 16};
+static const int kCout[PG_NET_LAYERS] = {16, 32, 64, 64, 32, 16, 1};
+static const int kLevel[PG_NET_LAYERS] = {0, 1, 2, 3, 2, 1, 0};
+
+int pg_net_param_count_impl()
+{
+  int n = 0;
+  for (int l = 0; l < PG_NET_LAYERS; l++) n += kCout[l] * 9 * kCin[l] + kCout[l];
+  return n;
+}
+
+static hipError_t launch_conv(pg_net* N, const bf16_t* in, int lvl, int Cin, const bf16_t* w, const float* bias, int Cout,
+                              int relu, bf16_t* out, float* out_f)
+{
+  const int H = N->H >> lvl, W = N->W >> lvl;
+  const int waves = N->B * H * (W / 16);
+  const dim3 grid((waves + 3) / 4);
+  const int nct = (Cout + 15) / 16;
+  switch (nct) {
+    case 1: hipLaunchKernelGGL(k_conv<1>, grid, dim3(256), 0, N->stream, in, N->B, H, W, Cin, w, bias, Cout, relu, out, out_f); break;
+    case 2: hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, N->stream, in, N->B, H, W, Cin, w, bias, Cout, relu, out, out_f); break;
+    case 4: hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, N->stream, in, N->B, H, W, Cin, w, bias, Cout, relu, out, out_f); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+static hipError_t launch_wgrad(pg_net* N, int l, const bf16_t* X, const bf16_t* dZ, int Cz)
+{
+  const int lvl = kLevel[l], Cin = kCin[l], Cout = kCout[l];
+  const int H = N->H >> lvl, W = N->W >> lvl, npx = N->B * H * W;
+  const int K9 = 9 * Cin, NP = ceil_to(K9, 16), MP = ceil_to(Cout, 16);
+  const int tiles = (NP / 16) * (MP / 16);
+  int nch = 2048 / tiles;
+  if (nch < 1) nch = 1;
+  int chunk = ceil_to((npx + nch - 1) / nch, 16);
+  nch = (npx + chunk - 1) / chunk;
+  if ((size_t)nch * MP * NP > N->npart) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_conv_wgrad, dim3(NP / 16, MP / 16, nch), dim3(64), 0, N->stream, X, Cin, dZ, Cz, Cout, N->B, H, W,
+                     chunk, NP, MP, N->part);
+  float* g = N->grad + N->woff[l];
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3((Cout * K9 + 255) / 256), dim3(256), 0, N->stream, N->part, nch, MP, NP, Cout, K9, g);
+  // bias
+  int bch = 1024;
+  const int bchunk = (npx + bch - 1) / bch;
+  bch = (npx + bchunk - 1) / bchunk;
+  hipLaunchKernelGGL(k_bgrad, dim3(bch), dim3(64), 0, N->stream, dZ, Cz, Cout, npx, bchunk, N->part2);
+  hipLaunchKernelGGL(k_bgrad_reduce, dim3(1), dim3(64), 0, N->stream, N->part2, bch, Cout, N->grad + N->boff[l]);
+  return hipGetLastError();
+}
+
+static dim3 ew_grid(size_t n) { return dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)); }
+
+hipError_t pg_net_refresh_weights(pg_net* N)
+{
+  for (int l = 0; l < PG_NET_LAYERS; l++) {
+    const int Cin = kCin[l], Cout = kCout[l];
+    hipLaunchKernelGGL(k_refresh_w, dim3(64), dim3(256), 0, N->stream, N->params + N->woff[l], Cin, Cout, ceil_to(Cout, 16),
+                       ceil_to(Cin, 16), ceil_to(Cout, 4), N->wf[l], N->wt[l]);
+  }
+  return hipGetLastError();
+}
+
+hipError_t pg_net_forward_dev(pg_net* N)
+{
+  const int B = N->B, H = N->H, W = N->W;
+  hipError_t e;
+#define CK(x)                      \
+  if ((e = (x)) != hipSuccess) return e;
+  const float* P = N->params;
+  CK(launch_conv(N, N->x0, 0, 4, N->wf[0], P + N->boff[0], 16, 1, N->a[0], nullptr));
+  hipLaunchKernelGGL(k_pool, ew_grid((size_t)B * H * W * 16 / 4), dim3(256), 0, N->stream, N->a[0], B, H, W, 16, 0.25f, N->p[0]);
+  CK(launch_conv(N, N->p[0], 1, 16, N->wf[1], P + N->boff[1], 32, 1, N->a[1], nullptr));
+  hipLaunchKernelGGL(k_pool, ew_grid((size_t)B * H * W * 32 / 16), dim3(256), 0, N->stream, N->a[1], B, H / 2, W / 2, 32, 0.25f,
+                     N->p[1]);
+  CK(launch_conv(N, N->p[1], 2, 32, N->wf[2], P + N->boff[2], 64, 1, N->a[2], nullptr));
+  hipLaunchKernelGGL(k_pool, ew_grid((size_t)B * H * W * 64 / 64), dim3(256), 0, N->stream, N->a[2], B, H / 4, W / 4, 64, 0.25f,
+                     N->p[2]);
+  CK(launch_conv(N, N->p[2], 3, 64, N->wf[3], P + N->boff[3], 64, 1, N->a[3], nullptr));
+  hipLaunchKernelGGL(k_add_up, ew_grid((size_t)B * H * W * 64 / 16), dim3(256), 0, N->stream, N->a[2], N->a[3], B, H / 4, W / 4,
+                     64, 1.0f, N->s[2]);
+  CK(launch_conv(N, N->s[2], 2, 64, N->wf[4], P + N->boff[4], 32, 1, N->a[4], nullptr));
+  hipLaunchKernelGGL(k_add_up, ew_grid((size_t)B * H * W * 32 / 4), dim3(256), 0, N->stream, N->a[1], N->a[4], B, H / 2, W / 2, 32,
+                     1.0f, N->s[1]);
+  CK(launch_conv(N, N->s[1], 1, 32, N->wf[5], P + N->boff[5], 16, 1, N->a[5], nullptr));
+  hipLaunchKernelGGL(k_add_up, ew_grid((size_t)B * H * W * 16), dim3(256), 0, N->stream, N->a[0], N->a[5], B, H, W, 16, 1.0f,
+                     N->s[0]);
+  CK(launch_conv(N, N->s[0], 0, 16, N->wf[6], P + N->boff[6], 1, 0, nullptr, N->y));
+  return hipGetLastError();
+}
+
+// Loss (mean squared error against N->t) and gradients of every parameter.
+hipError_t pg_net_backward_dev(pg_net* N)
+{
+  const int B = N->B, H = N->H, W = N->W;
+  const size_t n0 = (size_t)B * H * W;
+  hipError_t e;
+  hipLaunchKernelGGL(k_mse_grad, dim3(256), dim3(256), 0, N->stream, N->y, N->t, n0, N->dy4, N->part2 + 65536);
+  hipLaunchKernelGGL(k_sum_f, dim3(1), dim3(64), 0, N->stream, N->part2 + 65536, 256, 1.0f / (float)n0, N->loss);
+  // layer 6 (no relu): X = s0, dZ = dy4 (4 channels)
+  CK(launch_wgrad(N, 6, N->s[0], N->dy4, 4));
+  CK(launch_conv(N, N->dy4, 0, 4, N->wt[6], nullptr, 16, 0, N->ds[0], nullptr));  // ds0
+  // a5 <- up(ds0)^T
+  hipLaunchKernelGGL(k_pool, ew_grid(n0 * 16 / 4), dim3(256), 0, N->stream, N->ds[0], B, H, W, 16, 1.0f, N->da);
+  hipLaunchKernelGGL(k_relu_mask, ew_grid(n0 * 16 / 4), dim3(256), 0, N->stream, N->da, N->a[5], n0 * 16 / 4, N->dz);
+  CK(launch_wgrad(N, 5, N->s[1], N->dz, 16));
+  CK(launch_conv(N, N->dz, 1, 16, N->wt[5], nullptr, 32, 0, N->ds[1], nullptr));  // ds1
+  // a4 <- up(ds1)^T
+  hipLaunchKernelGGL(k_pool, ew_grid(n0 * 32 / 16), dim3(256), 0, N->stream, N->ds[1], B, H / 2, W / 2, 32, 1.0f, N->da);
+  hipLaunchKernelGGL(k_relu_mask, ew_grid(n0 * 32 / 16), dim3(256), 0, N->stream, N->da, N->a[4], n0 * 32 / 16, N->dz);
+  CK(launch_wgrad(N, 4, N->s[2], N->dz, 32));
+  CK(launch_conv(N, N->dz, 2, 32, N->wt[4], nullptr, 64, 0, N->ds[2], nullptr));  // ds2
+  // a3 <- up(ds2)^T
+  hipLaunchKernelGGL(k_pool, ew_grid(n0 * 64 / 64), dim3(256), 0, N->stream, N->ds[2], B, H / 4, W / 4, 64, 1.0f, N->da);
+  hipLaunchKernelGGL(k_relu_mask, ew_grid(n0 * 64 / 64), dim3(256), 0, N->stream, N->da, N->a[3], n0 * 64 / 64, N->dz);
+  CK(launch_wgrad(N, 3, N->p[2], N->dz, 64));
+  CK(launch_conv(N, N->dz, 3, 64, N->wt[3], nullptr, 64, 0, N->dp, nullptr));  // dp2
+  // a2: ds2 + pool^T(dp2)
+  hipLaunchKernelGGL(k_add_up, ew_grid(n0 * 64 / 16), dim3(256), 0, N->stream, N->ds[2], N->dp, B, H / 4, W / 4, 64, 0.25f, N->da);
+  hipLaunchKernelGGL(k_relu_mask, ew_grid(n0 * 64 / 16), dim3(256), 0, N->stream, N->da, N->a[2], n0 * 64 / 16, N->dz);
+  CK(launch_wgrad(N, 2, N->p[1], N->dz, 64));
+  CK(launch_conv(N, N->dz, 2, 64, N->wt[2], nullptr, 32, 0, N->dp, nullptr));  // dp1
+  // a1: ds1 + pool^T(dp1)
+  hipLaunchKernelGGL(k_add_up, ew_grid(n0 * 32 / 4), dim3(256), 0, N->stream, N->ds[1], N->dp, B, H / 2, W / 2, 32, 0.25f, N->da);
+  hipLaunchKernelGGL(k_relu_mask, ew_grid(n0 * 32 / 4), dim3(256), 0, N->stream, N->da, N->a[1], n0 * 32 / 4, N->dz);
+  CK(launch_wgrad(N, 1, N->p[0], N->dz, 32));
+  CK(launch_conv(N, N->dz, 1, 32, N->wt[1], nullptr, 16, 0, N->dp, nullptr));  // dp0
+  // a0: ds0 + pool^T(dp0)
+  hipLaunchKernelGGL(k_add_up, ew_grid(n0 * 16), dim3(256), 0, N->stream, N->ds[0], N->dp, B, H, W, 16, 0.25f, N->da);
+  hipLaunchKernelGGL(k_relu_mask, ew_grid(n0 * 16), dim3(256), 0, N->stream, N->da, N->a[0], n0 * 16, N->dz);
+  CK(launch_wgrad(N, 0, N->x0, N->dz, 16));
+#undef CK
+  return hipGetLastError();
+}
+
+hipError_t pg_net_adam_dev(pg_net* N, float lr)
+{
+  N->adam_t++;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.0f / (1.0f - powf(b1, (float)N->adam_t)), c2 = 1.0f / (1.0f - powf(b2, (float)N->adam_t));
+  hipLaunchKernelGGL(k_adam, dim3((N->nparam + 255) / 256), dim3(256), 0, N->stream, N->params, N->grad, N->adam_m, N->adam_v,
+                     N->nparam, lr, b1, b2, eps, c1, c2);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return pg_net_refresh_weights(N);
+}
+
+hipError_t pg_net_pack_input(pg_net* N, const float* c0, const float* c1)
+{
+  const size_t plane = (size_t)N->H * N->W;
+  hipLaunchKernelGGL(k_pack_input, ew_grid(N->B * plane), dim3(256), 0, N->stream, c0, c1, N->mask, plane, N->B, N->x0);
+  return hipGetLastError();
+}
+
+int pg_net_alloc(pg_net* N, int H, int W, int B, hipStream_t stream)
+{
+  if (H < 8 || W < 128 || H % 8 || W % 128 || B < 1) return PG_EARG;
+  N->H = H;
+  N->W = W;
+  N->B = B;
+  N->stream = stream;
+  N->nparam = pg_net_param_count_impl();
+  int off = 0;
+  for (int l = 0; l < PG_NET_LAYERS; l++) {
+    N->woff[l] = off;
+    off += kCout[l] * 9 * kCin[l];
+    N->boff[l] = off;
+    off += kCout[l];
+  }
+  auto alloc = [&](void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) return false;
+    return hipMemsetAsync(*p, 0, bytes, stream) == hipSuccess;
+  };
+  const size_t n0 = (size_t)B * H * W;
+  bool ok = true;
+  ok &= alloc((void**)&N->params, N->nparam * 4);
+  ok &= alloc((void**)&N->grad, N->nparam * 4);
+  ok &= alloc((void**)&N->adam_m, N->nparam * 4);
+  ok &= alloc((void**)&N->adam_v, N->nparam * 4);
+  for (int l = 0; l < PG_NET_LAYERS; l++) {
+    ok &= alloc((void**)&N->wf[l], (size_t)ceil_to(kCout[l], 16) * 9 * kCin[l] * 2);
+    ok &= alloc((void**)&N->wt[l], (size_t)ceil_to(kCin[l], 16) * 9 * ceil_to(kCout[l], 4) * 2);
+  }
+  ok &= alloc((void**)&N->x0, n0 * 4 * 2);
+  ok &= alloc((void**)&N->mask, (size_t)H * W * 4);
+  const int ach[6] = {16, 32, 64, 64, 32, 16};
+  const int alv[6] = {0, 1, 2, 3, 2, 1};
+  for (int i = 0; i < 6; i++) ok &= alloc((void**)&N->a[i], (n0 >> (2 * alv[i])) * ach[i] * 2);
+  const int pch[3] = {16, 32, 64};
+  for (int i = 0; i < 3; i++) ok &= alloc((void**)&N->p[i], (n0 >> (2 * (i + 1))) * pch[i] * 2);
+  const int sch[3] = {16, 32, 64};
+  for (int i = 0; i < 3; i++) {
+    ok &= alloc((void**)&N->s[i], (n0 >> (2 * i)) * sch[i] * 2);
+    ok &= alloc((void**)&N->ds[i], (n0 >> (2 * i)) * sch[i] * 2);
+  }
+  // scratch gradients: the largest is level 0 x 16 channels
+  ok &= alloc((void**)&N->da, n0 * 16 * 2);
+  ok &= alloc((void**)&N->dz, n0 * 16 * 2);
+  ok &= alloc((void**)&N->dp, n0 * 16 / 4 * 2 + n0 * 32 / 16 * 2 + n0 * 64 / 64 * 2);
+  ok &= alloc((void**)&N->dy4, n0 * 4 * 2);
+  ok &= alloc((void**)&N->y, n0 * 4);
+  ok &= alloc((void**)&N->t, n0 * 4);
+  N->npart = (size_t)4096 * 256;
+  ok &= alloc((void**)&N->part, N->npart * 4);
+  ok &= alloc((void**)&N->part2, (size_t)(65536 + 4096) * 4);
+  ok &= alloc((void**)&N->loss, 64);
+  if (!ok) return PG_EHIP;
+  return PG_OK;
+}
+
+void pg_net_free(pg_net* N)
+{
+  void* ptrs[] = {N->params, N->grad, N->adam_m, N->adam_v, N->x0, N->mask, N->da, N->dz, N->dp, N->dy4, N->y, N->t,
+                  N->part, N->part2, N->loss};
+  for (void* p : ptrs) (void)hipFree(p);
+  for (int l = 0; l < PG_NET_LAYERS; l++) {
+    (void)hipFree(N->wf[l]);
+    (void)hipFree(N->wt[l]);
+  }
+  for (int i = 0; i < 6; i++) (void)hipFree(N->a[i]);
+  for (int i = 0; i < 3; i++) {
+    (void)hipFree(N->p[i]);
+    (void)hipFree(N->s[i]);
+    (void)hipFree(N->ds[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// standalone C ABI (tests and offline use)
+// ---------------------------------------------------------------------------
+
+struct pg_net_handle {
+  pg_net net;
+  hipStream_t stream = nullptr;
+};
+
+extern "C" {
+
+int pg_net_param_count(void) { return pg_net_param_count_impl(); }
+
+int pg_net_create(int H, int W, int B, pg_net_handle** out)
+{
+  if (!out) return PG_EARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PG_ENODEV;
+  pg_net_handle* h = new (std::nothrow) pg_net_handle();
+  if (!h) return PG_EARG;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return PG_EHIP;
+  }
+  const int rc = pg_net_alloc(&h->net, H, W, B, h->stream);
+  if (rc) {
+    pg_net_free(&h->net);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return PG_OK;
+}
+
+void pg_net_destroy(pg_net_handle* h)
+{
+  if (!h) return;
+  (void)hipStreamSynchronize(h->stream);
+  pg_net_free(&h->net);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int pg_net_set_params(pg_net_handle* h, const float* params)
+{
+  if (!h || !params) return PG_EARG;
+  pg_net& N = h->net;
+  if (hipMemcpyAsync(N.params, params, N.nparam * 4, hipMemcpyHostToDevice, N.stream) != hipSuccess) return PG_EHIP;
+  if (hipMemsetAsync(N.adam_m, 0, N.nparam * 4, N.stream) != hipSuccess) return PG_EHIP;
+  if (hipMemsetAsync(N.adam_v, 0, N.nparam * 4, N.stream) != hipSuccess) return PG_EHIP;
+  N.adam_t = 0;
+  if (pg_net_refresh_weights(&N) != hipSuccess) return PG_EHIP;
+  return hipStreamSynchronize(N.stream) == hipSuccess ? PG_OK : PG_EHIP;
+}
+
+int pg_net_get_params(pg_net_handle* h, float* params)
+{
+  if (!h || !params) return PG_EARG;
+  pg_net& N = h->net;
+  if (hipMemcpyAsync(params, N.params, N.nparam * 4, hipMemcpyDeviceToHost, N.stream) != hipSuccess) return PG_EHIP;
+  return hipStreamSynchronize(N.stream) == hipSuccess ? PG_OK : PG_EHIP;
+}
+
+// x: [B][H][W][2] FP32 (channels 0, 1), mask: [H][W]; y: [B][H][W]
+static int upload_xm(pg_net& N, const float* x, const float* mask)
+{
+  const size_t plane = (size_t)N.H * N.W, n = plane * N.B;
+  std::vector<float> c0(n), c1(n);
+  for (size_t i = 0; i < n; i++) {
+    c0[i] = x[2 * i];
+    c1[i] = x[2 * i + 1];
+  }
+  if (hipMemcpyAsync(N.mask, mask, plane * 4, hipMemcpyHostToDevice, N.stream) != hipSuccess) return PG_EHIP;
+  // stage the two planes in y and t, then pack
+  if (hipMemcpyAsync(N.y, c0.data(), n * 4, hipMemcpyHostToDevice, N.stream) != hipSuccess) return PG_EHIP;
+  if (hipMemcpyAsync(N.t, c1.data(), n * 4, hipMemcpyHostToDevice, N.stream) != hipSuccess) return PG_EHIP;
+  if (pg_net_pack_input(&N, N.y, N.t) != hipSuccess) return PG_EHIP;
+  return hipStreamSynchronize(N.stream) == hipSuccess ? PG_OK : PG_EHIP;
+}
+
+int pg_net_forward(pg_net_handle* h, const float* x, const float* mask, float* y)
+{
+  if (!h || !x || !mask || !y) return PG_EARG;
+  pg_net& N = h->net;
+  int rc = upload_xm(N, x, mask);
+  if (rc) return rc;
+  if (pg_net_forward_dev(&N) != hipSuccess) return PG_EHIP;
+  const size_t n = (size_t)N.H * N.W * N.B;
+  if (hipMemcpyAsync(y, N.y, n * 4, hipMemcpyDeviceToHost, N.stream) != hipSuccess) return PG_EHIP;
+  return hipStreamSynchronize(N.stream) == hipSuccess ? PG_OK : PG_EHIP;
+}
+
+int pg_net_loss_grad(pg_net_handle* h, const float* x, const float* mask, const float* t, float* loss, float* grad)
+{
+  if (!h || !x || !mask || !t) return PG_EARG;
+  pg_net& N = h->net;
+  int rc = upload_xm(N, x, mask);
+  if (rc) return rc;
+  if (pg_net_forward_dev(&N) != hipSuccess) return PG_EHIP;
+  const size_t n = (size_t)N.H * N.W * N.B;
+  if (hipMemcpyAsync(N.t, t, n * 4, hipMemcpyHostToDevice, N.stream) != hipSuccess) return PG_EHIP;
+  if (pg_net_backward_dev(&N) != hipSuccess) return PG_EHIP;
+  if (loss && hipMemcpyAsync(loss, N.loss, 4, hipMemcpyDeviceToHost, N.stream) != hipSuccess) return PG_EHIP;
+  if (grad && hipMemcpyAsync(grad, N.grad, N.nparam * 4, hipMemcpyDeviceToHost, N.stream) != hipSuccess) return PG_EHIP;
+  return hipStreamSynchronize(N.stream) == hipSuccess ? PG_OK : PG_EHIP;
+}
+
+int pg_net_adam_step(pg_net_handle* h, float lr)
+{
+  if (!h) return PG_EARG;
+  if (pg_net_adam_dev(&h->net, lr) != hipSuccess) return PG_EHIP;
+  return hipStreamSynchronize(h->net.stream) == hipSuccess ? PG_OK : PG_EHIP;
+}
+
+}  // extern "C"

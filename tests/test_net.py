@@ -1,0 +1,134 @@
+"""Learned-preconditioner network: the GPU kernels (BF16 MFMA, FP32
+accumulation) against the CPU restatement oracle/netref.py, whose backward
+pass is itself pinned by central finite differences in float64.
+
+Tolerances (GPU vs oracle with BF16 rounding emulated at the same storage
+points; the remaining difference is FP32 summation order, which can flip a
+BF16 rounding now and then): output and loss 2e-2 relative to max|ref|,
+parameter gradient 5e-3 relative (l2) per layer.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import netref  # noqa: E402
+
+import pawsim_amd  # noqa: E402
+
+
+def _case(B, H, W, seed):
+    rng = np.random.default_rng(seed)
+    c0 = rng.standard_normal((B, H, W)).astype(np.float32)
+    c1 = rng.standard_normal((B, H, W)).astype(np.float32)
+    mask = rng.uniform(0.5, 1.0, (H, W)).astype(np.float32)
+    t = rng.standard_normal((B, H, W)).astype(np.float32)
+    return c0, c1, mask, t
+
+
+def test_bf16_rounding_nearest_even():
+    one = np.float32(1.0)
+    ulp = np.float32(2.0 ** -7)  # spacing of BF16 at 1
+    vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, 1.0 + 3 * 2 ** -8, -2.5, 1e-3], np.float32)
+    got = netref.bf16(vals)
+    assert got[0] == one
+    assert got[1] == one                       # tie -> even (1.0)
+    assert got[2] == one + ulp                 # above the tie -> up
+    assert got[3] == one + 2 * ulp             # tie at odd -> even (1 + 2 ulp)
+    assert got[4] == np.float32(-2.5)          # exactly representable
+    assert abs(got[5] - 1e-3) <= 1e-3 * 2 ** -8
+
+
+def test_netref_backward_matches_finite_differences():
+    """float64, no rounding: analytic gradient vs central differences."""
+    B, H, W = 2, 8, 8
+    c0, c1, mask, t = _case(B, H, W, 3)
+    x = netref.pack_input(c0, c1, mask).astype(np.float64)
+    p = netref.init_params(5).astype(np.float64)
+    p[netref.param_count() - 1] = 0.3  # final bias
+    net = netref.Net(p, bf16_storage=False)
+    loss, g, _ = net.loss_grad(x, t.astype(np.float64))
+    rng = np.random.default_rng(9)
+    idx = list(rng.choice(p.size, 40, replace=False))
+    # every bias of the first and last layers as well
+    off = 0
+    offs = []
+    for ci, co in zip(netref.CIN, netref.COUT):
+        offs.append((off, off + co * 9 * ci, co))
+        off += co * 9 * ci + co
+    idx += [offs[0][1] + k for k in range(4)] + [offs[6][1]]
+    eps = 1e-6
+    for i in idx:
+        pp = p.copy()
+        pp[i] += eps
+        lp = netref.Net(pp, bf16_storage=False).loss_grad(x, t.astype(np.float64))[0]
+        pp[i] -= 2 * eps
+        lm = netref.Net(pp, bf16_storage=False).loss_grad(x, t.astype(np.float64))[0]
+        fd = (lp - lm) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-9, (i, fd, g[i])
+
+
+def _layer_slices():
+    off, out = 0, []
+    for ci, co in zip(netref.CIN, netref.COUT):
+        n = co * 9 * ci + co
+        out.append(slice(off, off + n))
+        off += n
+    return out
+
+
+@pytest.mark.gpu
+def test_net_forward_matches_oracle():
+    B, H, W = 2, 16, 128
+    c0, c1, mask, _ = _case(B, H, W, 11)
+    p = netref.init_params(7)
+    p[-1] = 0.1
+    net = pawsim_amd.Net(H, W, B)
+    net.set_params(p)
+    x2 = np.stack([c0, c1], axis=-1)
+    y = net.forward(x2, mask)
+    net.close()
+    ref = netref.Net(p).forward(netref.pack_input(c0, c1, mask))
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    print(f"[net] forward rel err {err:.3e}", flush=True)
+    assert err <= 2e-2
+
+
+@pytest.mark.gpu
+def test_net_gradients_match_oracle():
+    B, H, W = 2, 16, 128
+    c0, c1, mask, t = _case(B, H, W, 12)
+    p = netref.init_params(8)
+    net = pawsim_amd.Net(H, W, B)
+    net.set_params(p)
+    loss, g = net.loss_grad(np.stack([c0, c1], axis=-1), mask, t)
+    net.close()
+    rloss, rg, _ = netref.Net(p).loss_grad(netref.pack_input(c0, c1, mask), t)
+    print(f"[net] loss gpu {loss:.6e} ref {rloss:.6e}", flush=True)
+    assert abs(loss - rloss) <= 2e-2 * abs(rloss)
+    for li, sl in enumerate(_layer_slices()):
+        a, b = g[sl], rg[sl]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"[net] layer {li} grad rel l2 err {rel:.3e}", flush=True)
+        assert rel <= 5e-3, (li, rel)
+
+
+@pytest.mark.gpu
+def test_net_adam_fits_a_fixed_batch():
+    """A few Adam steps on one batch lower its loss (training loop sanity)."""
+    B, H, W = 2, 16, 128
+    c0, c1, mask, _ = _case(B, H, W, 13)
+    t = (0.5 * c0 - 0.25 * c1).astype(np.float32)
+    net = pawsim_amd.Net(H, W, B)
+    net.set_params(netref.init_params(9))
+    x2 = np.stack([c0, c1], axis=-1)
+    l0, _ = net.loss_grad(x2, mask, t)
+    for _ in range(60):
+        net.loss_grad(x2, mask, t)
+        net.adam_step(1e-3)
+    l1, _ = net.loss_grad(x2, mask, t)
+    net.close()
+    print(f"[net] adam loss {l0:.4e} -> {l1:.4e}", flush=True)
+    assert l1 < 0.5 * l0
