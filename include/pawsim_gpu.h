@@ -82,13 +82,15 @@ typedef struct pg_static {
 typedef struct pg_solve_stats {
   uint32_t iters;
   uint32_t solver;
-  uint32_t fallback;         /* solver fell back to plain FP64 MG cycles */
+  uint32_t fallback;         /* the learned correction stalled and was dropped (plain V-cycles only) */
   double residual_max;       /* max |rhs - L pi| (pressure.c:652-690) */
   double residual_rms;
   double divergence_max;     /* max |sum_k div(h u)| after correction (pressure.c:693-737) */
   double divergence_rms;
   double rel_residual;       /* ||rhs - L pi||_2 / ||rhs||_2 */
   double solve_ms;           /* device time of the solve alone */
+  uint32_t learned;          /* the learned correction was applied and kept */
+  double learned_ratio;      /* ||defect after|| / ||defect before|| of the learned correction (0 if not tried) */
 } pg_solve_stats;
 
 /* Build a handle: allocate device state, upload statics, build the static
@@ -167,6 +169,23 @@ int pg_net_forward(pg_net_handle *h, const float *x, const float *mask, float *y
 int pg_net_loss_grad(pg_net_handle *h, const float *x, const float *mask, const float *t, float *loss, float *grad);
 /* one Adam step (beta 0.9, 0.999, eps 1e-8) with the last gradient */
 int pg_net_adam_step(pg_net_handle *h, float lr);
+
+/* The network inside a model's GPU-MG solves (pressureSolver 3 or 4;
+ * Nx % 8 == 0, Ny % 128 == 0).  pg_learn_setup allocates a network
+ * (He-initialised from `seed`, batch up to max_batch) and a device store for
+ * `capacity` training samples.  While capture is on, every GPU-MG solve
+ * stores (defect, starting iterate, exact correction), normalised.
+ * pg_learn_train runs Adam on batches drawn from the store.  With use on,
+ * each GPU-MG solve first applies the network's correction with a
+ * minimal-residual step length and keeps it if the defect norm drops below
+ * `stall` times its value, else drops it (pg_solve_stats.fallback = 1). */
+int pg_learn_setup(pg_handle *h, uint32_t capacity, uint32_t max_batch, uint64_t seed);
+int pg_learn_capture(pg_handle *h, int on);
+int pg_learn_samples(const pg_handle *h);
+int pg_learn_train(pg_handle *h, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float *final_loss);
+int pg_learn_use(pg_handle *h, int on, double stall);
+int pg_learn_get_params(pg_handle *h, float *params);
+int pg_learn_set_params(pg_handle *h, const float *params);
 
 #ifdef __cplusplus
 }

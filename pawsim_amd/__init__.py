@@ -31,6 +31,8 @@ ABI_SYMBOLS = (
     "pg_solve_pressure", "pg_refresh_halos", "pg_time", "pg_set_stream", "pg_synchronize", "pg_bench_kernel",
     "pg_net_param_count", "pg_net_create", "pg_net_destroy", "pg_net_set_params", "pg_net_get_params",
     "pg_net_forward", "pg_net_loss_grad", "pg_net_adam_step",
+    "pg_learn_setup", "pg_learn_capture", "pg_learn_samples", "pg_learn_train", "pg_learn_use",
+    "pg_learn_get_params", "pg_learn_set_params",
 )
 
 
@@ -58,7 +60,8 @@ class PgSolveStats(ctypes.Structure):
         ("iters", ctypes.c_uint32), ("solver", ctypes.c_uint32), ("fallback", ctypes.c_uint32),
         ("residual_max", ctypes.c_double), ("residual_rms", ctypes.c_double),
         ("divergence_max", ctypes.c_double), ("divergence_rms", ctypes.c_double),
-        ("rel_residual", ctypes.c_double), ("solve_ms", ctypes.c_double),
+        ("rel_residual", ctypes.c_double), ("solve_ms", ctypes.c_double), ("learned", ctypes.c_uint32),
+        ("learned_ratio", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -107,6 +110,13 @@ def load_library() -> ctypes.CDLL:
     lib.pg_net_forward.argtypes = [ctypes.c_void_p, f, f, f]
     lib.pg_net_loss_grad.argtypes = [ctypes.c_void_p, f, f, f, f, f]
     lib.pg_net_adam_step.argtypes = [ctypes.c_void_p, ctypes.c_float]
+    lib.pg_learn_setup.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
+    lib.pg_learn_capture.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.pg_learn_samples.argtypes = [ctypes.c_void_p]
+    lib.pg_learn_train.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64, f]
+    lib.pg_learn_use.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+    lib.pg_learn_get_params.argtypes = [ctypes.c_void_p, f]
+    lib.pg_learn_set_params.argtypes = [ctypes.c_void_p, f]
     _lib = lib
     return lib
 
@@ -236,6 +246,35 @@ class Model:
 
     def bench_kernel(self, which: int, reps: int) -> None:
         self._check(self.lib.pg_bench_kernel(self.h, which, reps), "pg_bench_kernel")
+
+    # -- learned preconditioner -------------------------------------------
+    def learn_setup(self, capacity: int, max_batch: int = 8, seed: int = 1) -> None:
+        self._check(self.lib.pg_learn_setup(self.h, capacity, max_batch, seed), "pg_learn_setup")
+
+    def learn_capture(self, on: bool) -> None:
+        self._check(self.lib.pg_learn_capture(self.h, int(on)), "pg_learn_capture")
+
+    def learn_samples(self) -> int:
+        return int(self.lib.pg_learn_samples(self.h))
+
+    def learn_train(self, steps: int, batch: int, lr: float = 1e-3, seed: int = 7) -> float:
+        loss = (ctypes.c_float * 1)()
+        self._check(self.lib.pg_learn_train(self.h, steps, batch, ctypes.c_float(lr), seed, loss), "pg_learn_train")
+        return float(loss[0])
+
+    def learn_use(self, on: bool, stall: float = 0.9) -> None:
+        self._check(self.lib.pg_learn_use(self.h, int(on), stall), "pg_learn_use")
+
+    def learn_params(self) -> np.ndarray:
+        p = np.zeros(self.lib.pg_net_param_count(), np.float32)
+        self._check(self.lib.pg_learn_get_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "pg_learn_get_params")
+        return p
+
+    def set_learn_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self.lib.pg_learn_set_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "pg_learn_set_params")
 
     def close(self) -> None:
         if getattr(self, "h", None):

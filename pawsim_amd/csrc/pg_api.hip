@@ -106,12 +106,12 @@ int run_solver(pg_handle* h, uint32_t solver, pg_solve_stats* st)
 {
   int rc = ensure_solver(h, solver);
   if (rc) return rc;
-  int iters = 0;
-  double rel = 0;
+  int iters = 0, learned = 0, fallback = 0;
+  double rel = 0, ratio = 0;
   PG_CHECK(h, hipEventRecord(h->ev0, h->stream));
   if (solver == PG_SOLVER_SOR) rc = pg_sor_solve(h, &iters);
   else if (solver == PG_SOLVER_MG_REPLICA) rc = pg_mg_replica_solve(h, &iters);
-  else if (pg_fused_ok(h)) rc = pg_fused_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel);
+  else if (pg_fused_ok(h)) rc = pg_fused_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel, &learned, &fallback, &ratio);
   else rc = pg_gpu_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel);
   if (rc) return rc;
   PG_CHECK(h, hipEventRecord(h->ev1, h->stream));
@@ -121,14 +121,18 @@ int run_solver(pg_handle* h, uint32_t solver, pg_solve_stats* st)
   if (st) {
     st->iters = (uint32_t)iters;
     st->solver = solver;
-    st->fallback = 0;
+    st->fallback = (uint32_t)fallback;
     st->rel_residual = rel;
     st->solve_ms = ms;
+    st->learned = (uint32_t)learned;
+    st->learned_ratio = ratio;
   }
   return (uint32_t)iters >= h->cfg.maxiters ? PG_ENOCONV : PG_OK;
 }
 
 }  // namespace
+
+int pg_ensure_solver(pg_handle* h, uint32_t solver) { return ensure_solver(h, solver); }
 
 extern "C" {
 
@@ -269,6 +273,7 @@ void pg_destroy(pg_handle* h)
 {
   if (!h) return;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  pg_learn_free(h);
   pg_mg_free(h);
   double* bufs[] = {h->u, h->v, h->h, h->fu[0], h->fu[1], h->fu[2], h->fv[0], h->fv[1], h->fv[2], h->fh[0], h->fh[1],
                     h->fh[2], h->hhb, h->hhs, h->omz, h->taux, h->tauy, h->Hc, h->Hw, h->Hs, h->pi, h->rhs,

@@ -30,6 +30,10 @@ struct PgCtl {
   double mean;           // last serial-order mean
   double norm_max[2];    // residual / divergence max
   double norm_ss[2];     // residual / divergence sum of squares
+  double pisumsq;        // sum of squares of the iterate (learned step, capture)
+  double emean;          // sum of a captured correction (its mean is removed)
+  double alpha_num, alpha_den, alpha;  // minimal-residual length of the learned correction
+  double sumsq0;         // defect norm^2 before the learned correction
   unsigned long long maxbits; // bit pattern of a non-negative double max
   int iters;
   int done;
@@ -57,6 +61,7 @@ struct pg_handle {
   PgCtl *ctl_host = nullptr;           // pinned mirror
   std::vector<MgLevel> lev;            // [0] = coarsest
   bool mg_built = false;
+  struct PgLearn* learn = nullptr;     // learned preconditioner (pg_learn.hip), optional
   int mg_pred = 0;                     // V-cycles the previous GPU-MG solve took
   long mg_tail_cells = 1024;           // largest level the LDS coarse tail starts from
   bool fast_tendency = true;           // inner LDS kernel + edge ring (bitwise equal to the per-point kernel)
@@ -111,8 +116,25 @@ hipError_t pg_gather_to_mg(pg_handle* h);
 hipError_t pg_scatter_from_mg(pg_handle* h);
 hipError_t pg_launch_coarse_global(pg_handle* h, int top, int nu1, int nu2, bool fp32);
 
+// ---- learned preconditioner (pg_learn.hip) ----
+void pg_learn_free(pg_handle* h);
+bool pg_learn_active(const pg_handle* h);
+bool pg_learn_capturing(const pg_handle* h);
+// Learned first correction of a GPU-MG solve: on entry *cur holds the
+// iterate and ctl->sumsq its defect norm^2; rhs32 (may be null) receives the
+// defect of the kept iterate.  Sets *kept / *fallback.
+int pg_learn_correct(pg_handle* h, double** cur, double** alt, float* rhs32, int* kept, int* fallback, double* ratio);
+void pg_learn_begin_capture(pg_handle* h);   // at the start of a solve
+int pg_learn_end_capture(pg_handle* h);      // after convergence (top.pi = solution)
+
 // ---- fused GPU multigrid (pg_mgfast.hip) ----
 bool pg_fused_ok(const pg_handle* h);
-int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel);
+int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* learned, int* fallback, double* ratio);
+int pg_ensure_solver(pg_handle* h, uint32_t solver);  // builds the solver's device data on first use
 hipError_t pg_fused_vcycle64(pg_handle* h);
 hipError_t pg_fused_vcycle32(pg_handle* h);
+// finest-level defect r = L (pcur - s e) - rhs with sum r^2 into *sumsq_out (see k_defect)
+hipError_t pg_launch_defect(pg_handle* h, const double* pcur, const float* e, const double* escale, double* pout,
+                            float* rhs32, double* sumsq_out);
+hipError_t pg_launch_sumsq(pg_handle* h, const double* x, size_t n, double* out);
+hipError_t pg_launch_finish_sum(pg_handle* h, const double* part, int nb, double* out);

@@ -466,12 +466,13 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(const MgLevel* lv, int m,
 constexpr int DJ = 64, DI = 4, DROWS = 4;  // each block covers DI * DROWS rows
 
 // Residual of the finest FP64 level, optionally after applying an FP32
-// correction: v = pcur - e (the iterate after refinement), r = L v - rhs.
-// Writes v to pout when e is given, r to rhs32 when given, and
-// sum r^2 to ctl->sumsq.
-__global__ __launch_bounds__(DJ* DI) void k_defect(MgLevel l, const double* pcur, const float* e, double* pout, float* rhs32,
-                                                    double* part)
+// correction: v = pcur - s e (the iterate after refinement; s = *escale or
+// 1), r = L v - rhs.  Writes v to pout when e is given, r to rhs32 when
+// given, and block partials of sum r^2.
+__global__ __launch_bounds__(DJ* DI) void k_defect(MgLevel l, const double* pcur, const float* e, const double* escale,
+                                                    double* pout, float* rhs32, double* part)
 {
+  const double sc = escale ? *escale : 1.0;
   const int j = blockIdx.x * DJ + threadIdx.x;
   const int X = l.Nx, Y = l.Ny;
   double ss = 0;
@@ -483,11 +484,11 @@ __global__ __launch_bounds__(DJ* DI) void k_defect(MgLevel l, const double* pcur
                  pS = (size_t)i * Y + jm;
     double v = pcur[p], vE = pcur[pE], vW = pcur[pW], vN = pcur[pN], vS = pcur[pS];
     if (e) {
-      v -= (double)e[p];
-      vE -= (double)e[pE];
-      vW -= (double)e[pW];
-      vN -= (double)e[pN];
-      vS -= (double)e[pS];
+      v -= sc * (double)e[p];
+      vE -= sc * (double)e[pE];
+      vW -= sc * (double)e[pW];
+      vN -= sc * (double)e[pN];
+      vS -= sc * (double)e[pS];
       pout[p] = v;
     }
     const double r = l.Os[pN] * vN + l.Os[p] * vS + l.Ow[pE] * vE + l.Ow[p] * vW - l.Osum[p] * v - l.rhs[p];
@@ -590,6 +591,29 @@ static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top
   return hipGetLastError();
 }
 
+hipError_t pg_launch_defect(pg_handle* h, const double* pcur, const float* e, const double* escale, double* pout,
+                            float* rhs32, double* sumsq_out)
+{
+  const MgLevel& top = h->lev.back();
+  const dim3 dg((top.Ny + DJ - 1) / DJ, (top.Nx + DI * DROWS - 1) / (DI * DROWS));
+  hipLaunchKernelGGL(k_defect, dg, dim3(DJ, DI), 0, h->stream, top, pcur, e, escale, pout, rhs32, h->partials);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, (int)(dg.x * dg.y), sumsq_out);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_sumsq(pg_handle* h, const double* x, size_t n, double* out)
+{
+  hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, h->stream, x, n, h->partials);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, out);
+  return hipGetLastError();
+}
+
+hipError_t pg_launch_finish_sum(pg_handle* h, const double* part, int nb, double* out)
+{
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, part, nb, out);
+  return hipGetLastError();
+}
+
 hipError_t pg_fused_vcycle64(pg_handle* h)
 {
   const int nu1 = h->cfg.mg_nu1 ? h->cfg.mg_nu1 : 2, nu2 = h->cfg.mg_nu2 ? h->cfg.mg_nu2 : 2;
@@ -612,24 +636,38 @@ bool pg_fused_ok(const pg_handle* h)
 // corrections of an FP64 defect) until ||L pi - rhs|| <= tol ||rhs||.  The
 // host reads the residual only from the cycle before the previous solve's
 // count on (the count rarely changes between steps), then after each cycle.
-int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel)
+int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* learned, int* fallback, double* ratio)
 {
+  *learned = 0;
+  *fallback = 0;
+  *ratio = 0;
   MgLevel& top = h->lev.back();
   const int nu1 = h->cfg.mg_nu1 ? h->cfg.mg_nu1 : 2, nu2 = h->cfg.mg_nu2 ? h->cfg.mg_nu2 : 2;
   const size_t N = (size_t)top.Nx * top.Ny;
   const dim3 dg((top.Ny + DJ - 1) / DJ, (top.Nx + DI * DROWS - 1) / (DI * DROWS));
   const int nbd = (int)(dg.x * dg.y);
   auto defect = [&](const double* pcur, const float* e, double* pout, float* r32) {
-    hipLaunchKernelGGL(k_defect, dg, dim3(DJ, DI), 0, h->stream, top, pcur, e, pout, r32, h->partials);
-    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, nbd, &h->ctl->sumsq);
+    pg_launch_defect(h, pcur, e, nullptr, pout, r32, &h->ctl->sumsq);
   };
+  (void)nbd;
   PG_CHECK(h, pg_gather_to_mg(h));
   hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, h->stream, top.rhs, N, h->partials);
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, &h->ctl->rhs_sumsq);
   defect(top.pi, nullptr, nullptr, mixed ? top.rhs32 : nullptr);
   PG_CHECK(h, hipGetLastError());
+  const bool capture = pg_learn_capturing(h);
+  if (capture) pg_learn_begin_capture(h);
   double* cur = top.pi;
   double* alt = top.tmp;
+  if (pg_learn_active(h)) {
+    const int rc = pg_learn_correct(h, &cur, &alt, mixed ? top.rhs32 : nullptr, learned, fallback, ratio);
+    if (rc) return rc;
+    if (!mixed && cur != top.pi) {  // the FP64 cycles work on top.pi in place
+      PG_CHECK(h, hipMemcpyAsync(top.pi, cur, N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      cur = top.pi;
+      alt = top.tmp;
+    }
+  }
   const int maxit = (int)h->cfg.maxiters;
   const int first_check = h->mg_pred > 1 ? h->mg_pred - 1 : 0;
   int it = 0;
@@ -655,6 +693,10 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel)
     it++;
   }
   if (cur != top.pi) PG_CHECK(h, hipMemcpyAsync(top.pi, cur, N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  if (capture) {
+    const int rc = pg_learn_end_capture(h);
+    if (rc) return rc;
+  }
   h->mg_pred = it;
   *iters = it;
   *rel = r;

@@ -11,7 +11,8 @@ typedef unsigned short bf16_t;
 
 // One network instance: H x W images (H % 8 == 0, W % 128 == 0), batch B.
 struct pg_net {
-  int H = 0, W = 0, B = 0;
+  int H = 0, W = 0, B = 0;  // B: batch of the next launches (<= Bmax)
+  int Bmax = 0;
   hipStream_t stream = nullptr;
   int nparam = 0;
   int woff[PG_NET_LAYERS] = {}, boff[PG_NET_LAYERS] = {};

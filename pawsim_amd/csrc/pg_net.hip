@@ -436,6 +436,7 @@ int pg_net_alloc(pg_net* N, int H, int W, int B, hipStream_t stream)
   N->H = H;
   N->W = W;
   N->B = B;
+  N->Bmax = B;
   N->stream = stream;
   N->nparam = pg_net_param_count_impl();
   int off = 0;
