@@ -54,6 +54,11 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=12)
     p.add_argument("--pmc-probe", action="store_true",
                    help="only launch the dominant kernel and a device copy of known size (for rocprofv3 --pmc)")
+    p.add_argument("--learn-samples", type=int, default=5000,
+                   help="captured (defect, iterate, correction) samples for the learned preconditioner (0: skip)")
+    p.add_argument("--learn-steps", type=int, default=20000, help="Adam steps")
+    p.add_argument("--learn-batch", type=int, default=8)
+    p.add_argument("--learn-eval", type=int, default=100, help="steps evaluated with and without the network")
     return p.parse_args()
 
 
@@ -133,6 +138,67 @@ def run_steps(a, m, stream):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     return {"n": n, "elapsed": elapsed, "solve_ms": float(np.mean(solve_ms)), "iters": float(np.mean(iters))}
+
+
+def learned_vs_plain(a, stream):
+    """Learned preconditioner on the re-entrant channel (512x256x3, periodic
+    x, walls y, Gaussian ridge, dx 10 km, dt 200 s), mixed-precision GPU
+    multigrid to a relative residual of 1e-8: capture samples from the
+    model's own solves, train on the GPU, then time a window of solves with
+    the network followed by a window without it."""
+    import numpy as np
+
+    import pawsim_amd
+    from pawsim_amd import inputs
+
+    ncap, neval = a.learn_samples, a.learn_eval
+    setup = inputs.channel(nt=ncap + 2 * neval, solver=pawsim_amd.SOLVER_GPU_MG_MIXED, tol=1e-8, maxiters=200,
+                           mixed_precision=1)
+    m = pawsim_amd.Model.from_setup(setup)
+    m.set_stream(stream.cuda_stream)
+    m.learn_setup(capacity=ncap, max_batch=a.learn_batch, seed=3)
+    m.pressure_correct()
+    m.refresh_halos()
+    m.learn_capture(True)
+    n = 0
+    t0 = time.perf_counter()
+    for _ in range(ncap):
+        n += 1
+        m.step(n)
+        m.pressure_correct()
+    m.synchronize()
+    t_cap = time.perf_counter() - t0
+    m.learn_capture(False)
+    loss0 = m.learn_train(1, a.learn_batch, 0.0)
+    t0 = time.perf_counter()
+    loss = m.learn_train(a.learn_steps, a.learn_batch, 1e-3, seed=11)
+    t_train = time.perf_counter() - t0
+
+    def window(on):
+        m.learn_use(on, 0.9)
+        nonlocal n
+        st = []
+        for _ in range(neval):
+            n += 1
+            m.step(n)
+            st.append(m.pressure_correct())
+        return st
+
+    sl = window(True)
+    sp = window(False)
+    m.close()
+    res = lambda st: {"solve_ms_median": float(np.median([s["solve_ms"] for s in st])),  # noqa: E731
+                      "solve_ms_mean": float(np.mean([s["solve_ms"] for s in st])),
+                      "vcycles_mean": float(np.mean([s["iters"] for s in st])),
+                      "max_rel_residual": float(max(s["rel_residual"] for s in st))}
+    out = {"workload": "re-entrant channel 512x256x3, mixed-precision GPU MG to rel. residual 1e-8",
+           "samples": ncap, "adam_steps": a.learn_steps, "batch": a.learn_batch, "lr": 1e-3,
+           "capture_s": t_cap, "train_s": t_train, "loss_initial": loss0, "loss_final": loss,
+           "learned": res(sl), "plain": res(sp),
+           "learned_kept": int(sum(s["learned"] for s in sl)),
+           "learned_defect_reduction_median": float(np.median([s["learned_ratio"] for s in sl]))}
+    out["speedup_solve_ms"] = out["plain"]["solve_ms_median"] / out["learned"]["solve_ms_median"]
+    return out
 
 
 def main():
@@ -258,6 +324,8 @@ def main():
                                    "frac": b / tt / HBM_PEAK} for k, (b, tt) in kernels.items()}}
     m.close()
 
+    learned = learned_vs_plain(a, stream) if a.learn_samples > 0 else None
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(a)
@@ -276,7 +344,7 @@ def main():
                       "iters_mean": r64["iters"]},
         "pressure_solve": {"in_run_ms_mean": r32["solve_ms"], "in_run_iters_mean": r32["iters"],
                            "plain_fp64_mg_in_run_ms_mean": r64["solve_ms"],
-                           "same_rhs_from_zero": rhs_solve, "learned": None},
+                           "same_rhs_from_zero": rhs_solve, "learned": learned},
         "pcie_inclusive_steps_per_s": pcie_rate,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -182,6 +182,36 @@ int pg_net_adam_step(pg_net_handle *h, float lr);
 int pg_learn_setup(pg_handle *h, uint32_t capacity, uint32_t max_batch, uint64_t seed);
 int pg_learn_capture(pg_handle *h, int on);
 int pg_learn_samples(const pg_handle *h);
+/* ---- Solve mode and ensembles (no reference counterpart) ----
+ * pg_set_solve_cycles: cycles > 0 makes every GPU-MG solve run exactly that
+ * many V-cycles with no host synchronisation (the form a HIP graph can
+ * capture); the worst relative residual is tracked on the device.  0 restores
+ * the adaptive stop. */
+int pg_set_solve_cycles(pg_handle *h, uint32_t cycles);
+
+/* Ensemble: independent members (any grid sizes, GPU-MG solver, no
+ * pressureTiming) advanced together.  pg_ensemble_run runs, on its first
+ * call, the preliminary correction, (with PG_ENS_AUTOTUNE) an autotuning of
+ * the multigrid coarse-tail level per grid size, and the AB1/AB2 start-up
+ * steps with adaptive solves; later steps use a fixed V-cycle count
+ * (`cycles`, or the largest start-up count + 1 when 0) and, with
+ * PG_ENS_GRAPH, are replayed from one HIP graph per three steps of all
+ * members, spread over `branches` streams.  Members keep their own handles (state up/download as usual
+ * between runs). */
+typedef struct pg_ensemble pg_ensemble;
+typedef struct pg_ensemble_stats {
+  uint32_t members, steps, cycles, graph_launches;
+  double seconds;            /* wall time of this call's steps */
+  double max_rel_residual;   /* worst ||rhs - L pi|| / ||rhs|| of the fixed-cycle solves so far */
+} pg_ensemble_stats;
+int pg_ensemble_create(pg_handle *const *members, int count, int branches, pg_ensemble **out);
+#define PG_ENS_GRAPH 1      /* replay steps from a captured HIP graph */
+#define PG_ENS_AUTOTUNE 2   /* choose the coarse-tail level per grid size by timing (first call) */
+int pg_ensemble_run(pg_ensemble *e, uint32_t steps, uint32_t cycles, uint32_t flags, pg_ensemble_stats *st);
+void pg_ensemble_destroy(pg_ensemble *e);
+const char *pg_ensemble_last_error(const pg_ensemble *e);
+
+/* final_loss: mean training loss over the last min(steps, 100) Adam steps */
 int pg_learn_train(pg_handle *h, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float *final_loss);
 int pg_learn_use(pg_handle *h, int on, double stall);
 int pg_learn_get_params(pg_handle *h, float *params);

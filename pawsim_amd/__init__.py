@@ -33,6 +33,7 @@ ABI_SYMBOLS = (
     "pg_net_forward", "pg_net_loss_grad", "pg_net_adam_step",
     "pg_learn_setup", "pg_learn_capture", "pg_learn_samples", "pg_learn_train", "pg_learn_use",
     "pg_learn_get_params", "pg_learn_set_params",
+    "pg_set_solve_cycles", "pg_ensemble_create", "pg_ensemble_run", "pg_ensemble_destroy", "pg_ensemble_last_error",
 )
 
 
@@ -53,6 +54,20 @@ class PgConfig(ctypes.Structure):
 
 class PgStatic(ctypes.Structure):
     _fields_ = [(n, ctypes.POINTER(ctypes.c_double)) for n in ("hhb", "hhs", "omega_z", "gg", "taux", "tauy")]
+
+
+class PgEnsembleStats(ctypes.Structure):
+    _fields_ = [
+        ("members", ctypes.c_uint32), ("steps", ctypes.c_uint32), ("cycles", ctypes.c_uint32),
+        ("graph_launches", ctypes.c_uint32), ("seconds", ctypes.c_double), ("max_rel_residual", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+ENS_GRAPH = 1
+ENS_AUTOTUNE = 2
 
 
 class PgSolveStats(ctypes.Structure):
@@ -110,6 +125,14 @@ def load_library() -> ctypes.CDLL:
     lib.pg_net_forward.argtypes = [ctypes.c_void_p, f, f, f]
     lib.pg_net_loss_grad.argtypes = [ctypes.c_void_p, f, f, f, f, f]
     lib.pg_net_adam_step.argtypes = [ctypes.c_void_p, ctypes.c_float]
+    lib.pg_set_solve_cycles.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+    lib.pg_ensemble_create.argtypes = [P(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, P(ctypes.c_void_p)]
+    lib.pg_ensemble_run.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                    P(PgEnsembleStats)]
+    lib.pg_ensemble_destroy.argtypes = [ctypes.c_void_p]
+    lib.pg_ensemble_destroy.restype = None
+    lib.pg_ensemble_last_error.argtypes = [ctypes.c_void_p]
+    lib.pg_ensemble_last_error.restype = ctypes.c_char_p
     lib.pg_learn_setup.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
     lib.pg_learn_capture.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.pg_learn_samples.argtypes = [ctypes.c_void_p]
@@ -247,6 +270,10 @@ class Model:
     def bench_kernel(self, which: int, reps: int) -> None:
         self._check(self.lib.pg_bench_kernel(self.h, which, reps), "pg_bench_kernel")
 
+    def set_solve_cycles(self, cycles: int) -> None:
+        """> 0: every GPU-MG solve runs exactly `cycles` V-cycles, no host reads."""
+        self._check(self.lib.pg_set_solve_cycles(self.h, cycles), "pg_set_solve_cycles")
+
     # -- learned preconditioner -------------------------------------------
     def learn_setup(self, capacity: int, max_batch: int = 8, seed: int = 1) -> None:
         self._check(self.lib.pg_learn_setup(self.h, capacity, max_batch, seed), "pg_learn_setup")
@@ -352,6 +379,40 @@ class Net:
         if getattr(self, "_h", None):
             self._lib.pg_net_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """Independent models advanced together (HIP-graph replay of three steps)."""
+
+    def __init__(self, models, branches: int = 4):
+        self._lib = load_library()
+        self.models = list(models)
+        arr = (ctypes.c_void_p * len(self.models))(*[m.h for m in self.models])
+        e = ctypes.c_void_p()
+        rc = self._lib.pg_ensemble_create(arr, len(self.models), branches, ctypes.byref(e))
+        if rc != 0:
+            raise RuntimeError(f"pg_ensemble_create failed with {rc}")
+        self._e = e
+
+    def run(self, steps: int, cycles: int = 0, graph: bool = True, autotune: bool = True) -> dict:
+        st = PgEnsembleStats()
+        flags = (ENS_GRAPH if graph else 0) | (ENS_AUTOTUNE if autotune else 0)
+        rc = self._lib.pg_ensemble_run(self._e, steps, cycles, flags, ctypes.byref(st))
+        if rc != 0:
+            msg = self._lib.pg_ensemble_last_error(self._e).decode()
+            raise RuntimeError(f"pg_ensemble_run failed with {rc}: {msg}")
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_e", None):
+            self._lib.pg_ensemble_destroy(self._e)
+            self._e = None
 
     def __del__(self):
         try:

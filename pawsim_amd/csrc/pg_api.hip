@@ -108,16 +108,21 @@ int run_solver(pg_handle* h, uint32_t solver, pg_solve_stats* st)
   if (rc) return rc;
   int iters = 0, learned = 0, fallback = 0;
   double rel = 0, ratio = 0;
-  PG_CHECK(h, hipEventRecord(h->ev0, h->stream));
+  const bool fixed = h->fixed_cycles > 0;  // no host synchronisation (graph capture)
+  if (fixed && (solver < PG_SOLVER_GPU_MG || !pg_fused_ok(h) || pg_learn_active(h) || pg_learn_capturing(h)))
+    return fail(h, PG_EARG, "fixed-cycle solves need the fused GPU multigrid without the learned step");
+  if (!fixed) PG_CHECK(h, hipEventRecord(h->ev0, h->stream));
   if (solver == PG_SOLVER_SOR) rc = pg_sor_solve(h, &iters);
   else if (solver == PG_SOLVER_MG_REPLICA) rc = pg_mg_replica_solve(h, &iters);
   else if (pg_fused_ok(h)) rc = pg_fused_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel, &learned, &fallback, &ratio);
   else rc = pg_gpu_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel);
   if (rc) return rc;
-  PG_CHECK(h, hipEventRecord(h->ev1, h->stream));
-  PG_CHECK(h, hipEventSynchronize(h->ev1));
   float ms = 0;
-  PG_CHECK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (!fixed) {
+    PG_CHECK(h, hipEventRecord(h->ev1, h->stream));
+    PG_CHECK(h, hipEventSynchronize(h->ev1));
+    PG_CHECK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  }
   if (st) {
     st->iters = (uint32_t)iters;
     st->solver = solver;
@@ -127,7 +132,7 @@ int run_solver(pg_handle* h, uint32_t solver, pg_solve_stats* st)
     st->learned = (uint32_t)learned;
     st->learned_ratio = ratio;
   }
-  return (uint32_t)iters >= h->cfg.maxiters ? PG_ENOCONV : PG_OK;
+  return (!fixed && (uint32_t)iters >= h->cfg.maxiters) ? PG_ENOCONV : PG_OK;
 }
 
 }  // namespace
@@ -469,6 +474,15 @@ int pg_set_stream(pg_handle* h, void* stream)
   if (!h) return PG_EARG;
   PG_CHECK(h, hipStreamSynchronize(h->stream));
   h->stream = stream ? (hipStream_t)stream : h->own_stream;
+  return PG_OK;
+}
+
+int pg_set_solve_cycles(pg_handle* h, uint32_t cycles)
+{
+  if (!h) return PG_EARG;
+  if (cycles > 0 && (h->cfg.pressureSolver < PG_SOLVER_GPU_MG || !pg_fused_ok(h)))
+    return fail(h, PG_EARG, "fixed-cycle solves need the fused GPU multigrid (pressureSolver 3 or 4, nu <= 2)");
+  h->fixed_cycles = (int)cycles;
   return PG_OK;
 }
 

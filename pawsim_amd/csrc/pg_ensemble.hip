@@ -1,0 +1,250 @@
+// pg_ensemble.hip -- many independent models advanced together, one
+// hipGraph launch per group of AB3 steps (no counterpart in the reference).
+//
+// Members are ordinary handles (any mix of grid sizes).  The runner
+//  1. autotunes, per distinct grid size, the coarse-tail threshold of the
+//     fused multigrid (the tile level at which one workgroup takes over),
+//     timing a V-cycle with events for each candidate;
+//  2. runs the preliminary correction and the two AB3 start-up steps
+//     eagerly with the adaptive solver and takes the largest V-cycle count
+//     seen, plus a margin, as the fixed cycle count;
+//  3. captures three steps of every member (the AB history rotates with
+//     period three, so the captured buffer pointers repeat) into one graph,
+//     members spread over a few streams that fork from and join into the
+//     capturing stream, and replays it;
+//  4. reports the wall time and the worst relative residual of all the
+//     fixed-cycle solves (tracked on the device, read once at the end).
+#include "pg_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <vector>
+
+struct pg_ensemble {
+  std::vector<pg_handle*> m;
+  std::vector<hipStream_t> streams;  // one per branch
+  hipStream_t main = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t n = 0;       // steps taken
+  uint32_t maxit = 0;   // largest adaptive V-cycle count of the start-up steps
+  std::string err;
+};
+
+extern "C" {
+
+int pg_ensemble_create(pg_handle* const* members, int count, int branches, pg_ensemble** out)
+{
+  if (!out || !members || count < 1 || branches < 1) return PG_EARG;
+  *out = nullptr;
+  for (int i = 0; i < count; i++) {
+    pg_handle* h = members[i];
+    if (!h) return PG_EARG;
+    const uint32_t s = h->cfg.pressureSolver;
+    if ((s != PG_SOLVER_GPU_MG && s != PG_SOLVER_GPU_MG_MIXED) || h->cfg.pressureTiming || !pg_fused_ok(h)) return PG_EARG;
+  }
+  pg_ensemble* e = new (std::nothrow) pg_ensemble();
+  if (!e) return PG_EARG;
+  e->m.assign(members, members + count);
+  branches = std::min(branches, count);
+  auto bad = [&]() {
+    pg_ensemble_destroy(e);
+    return PG_EHIP;
+  };
+  if (hipStreamCreateWithFlags(&e->main, hipStreamNonBlocking) != hipSuccess) return bad();
+  if (hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) return bad();
+  e->streams.resize(branches, nullptr);
+  e->join.resize(branches, nullptr);
+  for (int b = 0; b < branches; b++) {
+    if (hipStreamCreateWithFlags(&e->streams[b], hipStreamNonBlocking) != hipSuccess) return bad();
+    if (hipEventCreateWithFlags(&e->join[b], hipEventDisableTiming) != hipSuccess) return bad();
+  }
+  for (int i = 0; i < count; i++)
+    if (pg_set_stream(e->m[i], e->streams[i % branches])) return bad();
+  *out = e;
+  return PG_OK;
+}
+
+void pg_ensemble_destroy(pg_ensemble* e)
+{
+  if (!e) return;
+  for (pg_handle* h : e->m)
+    if (h) (void)pg_set_stream(h, nullptr);
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (hipEvent_t ev : e->join)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  for (hipStream_t s : e->streams)
+    if (s) (void)hipStreamDestroy(s);
+  if (e->main) (void)hipStreamDestroy(e->main);
+  delete e;
+}
+
+const char* pg_ensemble_last_error(const pg_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+}  // extern "C"
+
+namespace {
+
+int efail(pg_ensemble* e, int rc, const std::string& msg)
+{
+  e->err = msg;
+  return rc;
+}
+
+int sync_all(pg_ensemble* e)
+{
+  for (hipStream_t s : e->streams)
+    if (hipStreamSynchronize(s) != hipSuccess) return PG_EHIP;
+  return hipStreamSynchronize(e->main) == hipSuccess ? PG_OK : PG_EHIP;
+}
+
+// Coarse-tail threshold per distinct grid size: time one V-cycle of the
+// first member of that size for each candidate, keep the fastest.
+int autotune(pg_ensemble* e)
+{
+  std::map<std::pair<int, int>, pg_handle*> first;
+  for (pg_handle* h : e->m) first.emplace(std::make_pair(h->G.nx, h->G.ny), h);
+  const long cand[] = {256, 1024, 4096};
+  for (auto& kv : first) {
+    pg_handle* h = kv.second;
+    const bool mixed = h->cfg.pressureSolver == PG_SOLVER_GPU_MG_MIXED;
+    long best = h->mg_tail_cells;
+    float best_ms = 1e30f;
+    for (long c : cand) {
+      h->mg_tail_cells = c;
+      for (int r = 0; r < 2; r++)
+        if ((mixed ? pg_fused_vcycle32(h) : pg_fused_vcycle64(h)) != hipSuccess) return PG_EHIP;
+      if (hipEventRecord(h->ev0, h->stream) != hipSuccess) return PG_EHIP;
+      for (int r = 0; r < 5; r++)
+        if ((mixed ? pg_fused_vcycle32(h) : pg_fused_vcycle64(h)) != hipSuccess) return PG_EHIP;
+      if (hipEventRecord(h->ev1, h->stream) != hipSuccess || hipEventSynchronize(h->ev1) != hipSuccess) return PG_EHIP;
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) return PG_EHIP;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+    for (pg_handle* o : e->m)
+      if (o->G.nx == h->G.nx && o->G.ny == h->G.ny) o->mg_tail_cells = best;
+  }
+  return PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Advance every member by `steps` steps (from its initial state on the first
+// call: preliminary correction, then n = 1, 2, ...).  cycles = 0 picks the
+// fixed V-cycle count from the start-up solves (largest count + 1).
+int pg_ensemble_run(pg_ensemble* e, uint32_t steps, uint32_t cycles, uint32_t flags, pg_ensemble_stats* st)
+{
+  if (!e) return PG_EARG;
+  pg_ensemble_stats local{};
+  if (!st) st = &local;
+  *st = pg_ensemble_stats{};
+  st->members = (uint32_t)e->m.size();
+  int rc;
+  if (e->n == 0) {
+    for (pg_handle* h : e->m) {
+      pg_solve_stats s;
+      rc = pg_pressure_correct(h, &s);
+      if (rc && rc != PG_ENOCONV) return efail(e, rc, std::string("preliminary correction: ") + h->err);
+      rc = pg_refresh_halos(h);
+      if (rc) return efail(e, rc, "halo refresh failed");
+    }
+    if (flags & PG_ENS_AUTOTUNE) {
+      rc = autotune(e);
+      if (rc) return efail(e, rc, "autotuning failed");
+    }
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  uint32_t done = 0;
+  // start-up steps (AB1, AB2) with the adaptive solver
+  while (done < steps && e->n < 2) {
+    e->n++;
+    for (pg_handle* h : e->m) {
+      pg_solve_stats s;
+      if ((rc = pg_step(h, e->n))) return efail(e, rc, h->err);
+      rc = pg_pressure_correct(h, &s);
+      if (rc && rc != PG_ENOCONV) return efail(e, rc, h->err);
+      e->maxit = std::max(e->maxit, s.iters);
+    }
+    done++;
+  }
+  if (done == steps) {
+    if ((rc = sync_all(e))) return efail(e, rc, "synchronisation failed");
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st->steps = done;
+    return PG_OK;
+  }
+  const uint32_t fixed = cycles ? cycles : std::max<uint32_t>(e->maxit + 1, 2);
+  for (pg_handle* h : e->m) h->fixed_cycles = (int)fixed;
+  st->cycles = fixed;
+  auto one_round = [&](uint32_t k) -> int {  // k steps of every member, on the member streams
+    for (uint32_t q = 0; q < k; q++) {
+      e->n++;
+      for (pg_handle* h : e->m) {
+        int r = pg_step(h, e->n);
+        if (r) return r;
+        r = pg_pressure_correct(h, nullptr);
+        if (r) return r;
+      }
+    }
+    return PG_OK;
+  };
+  const int B = (int)e->streams.size();
+  if (flags & PG_ENS_GRAPH) {
+    while (steps - done >= 3) {
+      bool captured_now = false;  // the capture advanced the members' host-side time already
+      if (!e->exec) {
+        captured_now = true;
+        // capture three steps: fork the member streams from `main`, join them back
+        if (hipStreamBeginCapture(e->main, hipStreamCaptureModeThreadLocal) != hipSuccess) return efail(e, PG_EHIP, "capture");
+        if (hipEventRecord(e->fork, e->main) != hipSuccess) return efail(e, PG_EHIP, "capture fork");
+        for (int b = 0; b < B; b++)
+          if (hipStreamWaitEvent(e->streams[b], e->fork, 0) != hipSuccess) return efail(e, PG_EHIP, "capture fork");
+        rc = one_round(3);
+        for (int b = 0; b < B; b++) {
+          if (hipEventRecord(e->join[b], e->streams[b]) != hipSuccess) return efail(e, PG_EHIP, "capture join");
+          if (hipStreamWaitEvent(e->main, e->join[b], 0) != hipSuccess) return efail(e, PG_EHIP, "capture join");
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(e->main, &g);
+        if (rc) return efail(e, rc, "a step failed during capture");
+        if (ce != hipSuccess || !g) return efail(e, PG_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
+        e->graph = g;
+        if (hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0) != hipSuccess) return efail(e, PG_EHIP, "graph instantiate");
+        e->n -= 3;  // the captured steps have not run yet
+      }
+      if (hipGraphLaunch(e->exec, e->main) != hipSuccess) return efail(e, PG_EHIP, "graph launch");
+      if (!captured_now)
+        for (pg_handle* h : e->m) h->t += 3 * h->cfg.dt;
+      e->n += 3;
+      st->graph_launches++;
+      done += 3;
+    }
+  }
+  if (done < steps) {
+    if ((rc = one_round(steps - done))) return efail(e, rc, "eager step failed");
+    done = steps;
+  }
+  if ((rc = sync_all(e))) return efail(e, rc, "synchronisation failed");
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  st->steps = done;
+  double worst = 0;
+  for (pg_handle* h : e->m) {
+    if (hipMemcpy(h->ctl_host, h->ctl, sizeof(PgCtl), hipMemcpyDeviceToHost) != hipSuccess) return efail(e, PG_EHIP, "read");
+    worst = std::max(worst, h->ctl_host->worst_rel2);
+  }
+  st->max_rel_residual = std::sqrt(worst);
+  return PG_OK;
+}
+
+}  // extern "C"

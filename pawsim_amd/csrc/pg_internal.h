@@ -34,6 +34,7 @@ struct PgCtl {
   double emean;          // sum of a captured correction (its mean is removed)
   double alpha_num, alpha_den, alpha;  // minimal-residual length of the learned correction
   double sumsq0;         // defect norm^2 before the learned correction
+  double worst_rel2;     // largest (||r|| / ||rhs||)^2 after a fixed-cycle solve
   unsigned long long maxbits; // bit pattern of a non-negative double max
   int iters;
   int done;
@@ -63,6 +64,7 @@ struct pg_handle {
   bool mg_built = false;
   struct PgLearn* learn = nullptr;     // learned preconditioner (pg_learn.hip), optional
   int mg_pred = 0;                     // V-cycles the previous GPU-MG solve took
+  int fixed_cycles = 0;                // > 0: GPU-MG solves run exactly this many cycles with no host reads
   long mg_tail_cells = 1024;           // largest level the LDS coarse tail starts from
   bool fast_tendency = true;           // inner LDS kernel + edge ring (bitwise equal to the per-point kernel)
   double t = 0;

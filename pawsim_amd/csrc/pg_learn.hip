@@ -139,6 +139,12 @@ __global__ void k_alpha(PgCtl* ctl)
   ctl->sumsq0 = ctl->sumsq;
 }
 
+// running sum of the per-step loss (the last steps of a training run)
+__global__ void k_loss_acc(const float* loss, float* acc, int reset)
+{
+  if (threadIdx.x == 0) *acc = (reset ? 0.0f : *acc) + *loss;
+}
+
 // batch of training samples (device-drawn indices): c0, c1, t [B][N]
 __global__ void k_gather_batch(const float* store, unsigned count, size_t n, unsigned long long seed, unsigned step, int B,
                                float* c0, float* c1, float* t)
@@ -343,16 +349,23 @@ int pg_learn_train(pg_handle* h, uint32_t steps, uint32_t batch, float lr, uint6
   if (batch < 1 || (int)batch > N.Bmax) return fail(h, PG_EARG, "batch exceeds the network's allocated batch");
   N.stream = h->stream;
   N.B = (int)batch;
+  // the reported loss is the mean over the last (up to) 100 steps
+  const uint32_t tail = steps < 100 ? steps : 100;
+  float* acc = N.loss + 1;
   for (uint32_t s = 0; s < steps; s++) {
     hipLaunchKernelGGL(k_gather_batch, ew(batch * L->N), dim3(256), 0, h->stream, L->store, L->count, L->N,
                        (unsigned long long)seed, s, (int)batch, L->c0, L->c1, N.t);
     PG_CHECK(h, pg_net_pack_input(&N, L->c0, L->c1));
     PG_CHECK(h, pg_net_forward_dev(&N));
     PG_CHECK(h, pg_net_backward_dev(&N));
+    if (s + tail >= steps)
+      hipLaunchKernelGGL(k_loss_acc, dim3(1), dim3(64), 0, h->stream, N.loss, acc, (int)(s + tail == steps));
     PG_CHECK(h, pg_net_adam_dev(&N, lr));
   }
-  if (final_loss) PG_CHECK(h, hipMemcpyAsync(final_loss, N.loss, sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  float sum = 0;
+  if (tail) PG_CHECK(h, hipMemcpyAsync(&sum, acc, sizeof(float), hipMemcpyDeviceToHost, h->stream));
   PG_CHECK(h, hipStreamSynchronize(h->stream));
+  if (final_loss) *final_loss = tail ? sum / (float)tail : 0.0f;
   return PG_OK;
 }
 

@@ -505,6 +505,19 @@ __global__ __launch_bounds__(256) void k_sumsq(const double* x, size_t n, double
   block_partial(ss, part);
 }
 
+__global__ void k_copy64(const double* src, double* dst, size_t n)
+{
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) dst[p] = src[p];
+}
+
+// worst relative residual of fixed-cycle solves
+__global__ void k_track_worst(PgCtl* ctl)
+{
+  if (threadIdx.x != 0) return;
+  const double r2 = ctl->rhs_sumsq > 0 ? ctl->sumsq / ctl->rhs_sumsq : 0.0;
+  if (r2 > ctl->worst_rel2) ctl->worst_rel2 = r2;
+}
+
 __global__ void k_zero32(float* x, size_t n)
 {
   for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) x[p] = 0.0f;
@@ -663,7 +676,7 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
     const int rc = pg_learn_correct(h, &cur, &alt, mixed ? top.rhs32 : nullptr, learned, fallback, ratio);
     if (rc) return rc;
     if (!mixed && cur != top.pi) {  // the FP64 cycles work on top.pi in place
-      PG_CHECK(h, hipMemcpyAsync(top.pi, cur, N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+      hipLaunchKernelGGL(k_copy64, dim3(1024), dim3(256), 0, h->stream, cur, top.pi, N);
       cur = top.pi;
       alt = top.tmp;
     }
@@ -672,8 +685,11 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
   const int first_check = h->mg_pred > 1 ? h->mg_pred - 1 : 0;
   int it = 0;
   double r = 0;
+  const int fixed = h->fixed_cycles;
   for (;;) {
-    if (it >= first_check || it >= maxit) {
+    if (fixed > 0) {
+      if (it >= fixed) break;
+    } else if (it >= first_check || it >= maxit) {
       PG_CHECK(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(PgCtl), hipMemcpyDeviceToHost, h->stream));
       PG_CHECK(h, hipStreamSynchronize(h->stream));
       const double rr = h->ctl_host->rhs_sumsq;
@@ -692,12 +708,13 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
     PG_CHECK(h, hipGetLastError());
     it++;
   }
-  if (cur != top.pi) PG_CHECK(h, hipMemcpyAsync(top.pi, cur, N * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  if (cur != top.pi) hipLaunchKernelGGL(k_copy64, dim3(1024), dim3(256), 0, h->stream, cur, top.pi, N);
+  if (fixed > 0) hipLaunchKernelGGL(k_track_worst, dim3(1), dim3(64), 0, h->stream, h->ctl);
   if (capture) {
     const int rc = pg_learn_end_capture(h);
     if (rc) return rc;
   }
-  h->mg_pred = it;
+  if (fixed <= 0) h->mg_pred = it;
   *iters = it;
   *rel = r;
   PG_CHECK(h, pg_scatter_from_mg(h));

@@ -86,70 +86,83 @@ __global__ __launch_bounds__(256) void k_conv(const bf16_t* __restrict__ in, int
   }
 }
 
-// Weight gradient partials: part[chunk][co][n] = sum over the chunk's pixels
-// q of dZ[q][co] * X[q shifted by tap(n)][ci(n)], n = tap*Cin + ci.  One
-// wave per (n tile, co tile, chunk); A = dZ (co x pixels), B = X (pixels x n).
+// Weight and bias gradient partials: part[chunk][co][n] = sum over the
+// chunk's pixels q of dZ[q][co] * X[q shifted by tap(n)][ci(n)] for
+// n = tap*Cin + ci < 9 Cin, and sum of dZ[q][co] for n = 9 Cin (bias).  One
+// wave per (group of NTW n tiles, co tile, chunk): A = dZ (co x pixels) is
+// loaded once per 16 pixels and reused for every n tile of the group.
+constexpr int NTW = 4;
+
 __global__ __launch_bounds__(64) void k_conv_wgrad(const bf16_t* __restrict__ X, int Cin, const bf16_t* __restrict__ dZ,
                                                    int Cz, int Cout, int B, int H, int W, int chunk, int NP, int MP,
                                                    float* __restrict__ part)
 {
   const int lane = threadIdx.x;
-  const int nt = blockIdx.x, mt = blockIdx.y, ch = blockIdx.z;
+  const int ng = blockIdx.x, mt = blockIdx.y, ch = blockIdx.z;
   const int npx = B * H * W;
   const int q0 = ch * chunk, q1 = min(npx, q0 + chunk);
   const int mm = lane & 15, kg = lane >> 4;
   const int co = mt * 16 + mm;
-  const int n = nt * 16 + mm;
-  const bool nvalid = n < 9 * Cin;
-  const int tap = nvalid ? n / Cin : 0, ci = nvalid ? n - (n / Cin) * Cin : 0;
-  const int ty = tap / 3 - 1, tx = tap % 3 - 1;
-  v4f acc = {0.f, 0.f, 0.f, 0.f};
+  const int K9 = 9 * Cin;
+  int nn[NTW], ci[NTW], ty[NTW], tx[NTW];
+  for (int u = 0; u < NTW; u++) {
+    const int n = (ng * NTW + u) * 16 + mm;
+    nn[u] = n;
+    const int tap = n < K9 ? n / Cin : 0;
+    ci[u] = n < K9 ? n - tap * Cin : 0;
+    ty[u] = tap / 3 - 1;
+    tx[u] = tap % 3 - 1;
+  }
+  v4f acc[NTW];
+  for (int u = 0; u < NTW; u++) acc[u] = v4f{0.f, 0.f, 0.f, 0.f};
   for (int q = q0; q < q1; q += 16) {
-    v4s a = {0, 0, 0, 0}, bv = {0, 0, 0, 0};
+    v4s a = {0, 0, 0, 0};
+    int yq[4], xq[4], bq[4];
+    bool ok[4];
     for (int r = 0; r < 4; r++) {
       const int qq = q + 4 * kg + r;
-      if (qq >= q1) continue;
-      if (co < Cout) a[r] = (short)dZ[(size_t)qq * Cz + co];
-      if (nvalid) {
-        const int xq = qq % W, yq = (qq / W) % H, bq = qq / (W * H);
-        const int yy = yq + ty, xx = xq + tx;
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W) bv[r] = (short)X[(((size_t)bq * H + yy) * W + xx) * Cin + ci];
-      }
+      ok[r] = qq < q1;
+      xq[r] = qq % W;
+      yq[r] = (qq / W) % H;
+      bq[r] = qq / (W * H);
+      if (ok[r] && co < Cout) a[r] = (short)dZ[(size_t)qq * Cz + co];
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
+    for (int u = 0; u < NTW; u++) {
+      if ((ng * NTW + u) * 16 >= NP) break;
+      v4s bv = {0, 0, 0, 0};
+      for (int r = 0; r < 4; r++) {
+        if (!ok[r]) continue;
+        if (nn[u] < K9) {
+          const int yy = yq[r] + ty[u], xx = xq[r] + tx[u];
+          if (yy >= 0 && yy < H && xx >= 0 && xx < W) bv[r] = (short)X[(((size_t)bq[r] * H + yy) * W + xx) * Cin + ci[u]];
+        } else if (nn[u] == K9) {
+          bv[r] = (short)0x3F80;  // 1.0 in BF16: bias column
+        }
+      }
+      acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[u], 0, 0, 0);
+    }
   }
-  for (int r = 0; r < 4; r++) part[((size_t)ch * MP + mt * 16 + 4 * kg + r) * NP + nt * 16 + mm] = acc[r];
+  for (int u = 0; u < NTW; u++) {
+    const int n0 = (ng * NTW + u) * 16;
+    if (n0 >= NP) break;
+    for (int r = 0; r < 4; r++) part[((size_t)ch * MP + mt * 16 + 4 * kg + r) * NP + n0 + mm] = acc[u][r];
+  }
 }
 
-// g[co][n] = sum over chunks (fixed order)
-__global__ void k_wgrad_reduce(const float* __restrict__ part, int nch, int MP, int NP, int Cout, int K9, float* __restrict__ g)
+// g[co][n] (n < 9 Cin) and gb[co] (n = 9 Cin) = sum over chunks, one wave
+// per output, lanes strided over chunks, fixed-order shuffle tree.
+__global__ __launch_bounds__(64) void k_wgrad_reduce(const float* __restrict__ part, int nch, int MP, int NP, int Cout, int K9,
+                                                     float* __restrict__ g, float* __restrict__ gb)
 {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= Cout * K9) return;
-  const int co = idx / K9, n = idx - (idx / K9) * K9;
+  const int idx = blockIdx.x;  // co * (K9 + 1) + n
+  const int co = idx / (K9 + 1), n = idx - co * (K9 + 1);
   float s = 0.f;
-  for (int c = 0; c < nch; c++) s += part[((size_t)c * MP + co) * NP + n];
-  g[idx] = s;
-}
-
-// bias gradient partials: part[chunk][co] = sum_q dZ[q][co]; then reduced
-__global__ void k_bgrad(const bf16_t* __restrict__ dZ, int Cz, int Cout, int npx, int chunk, float* __restrict__ part)
-{
-  const int co = threadIdx.x;  // blockDim.x = 64 >= Cout
-  const int q0 = blockIdx.x * chunk, q1 = min(npx, q0 + chunk);
-  if (co >= Cout) return;
-  float s = 0.f;
-  for (int q = q0; q < q1; q++) s += bf2f(dZ[(size_t)q * Cz + co]);
-  part[blockIdx.x * 64 + co] = s;
-}
-
-__global__ void k_bgrad_reduce(const float* __restrict__ part, int nch, int Cout, float* __restrict__ g)
-{
-  const int co = threadIdx.x;
-  if (co >= Cout) return;
-  float s = 0.f;
-  for (int c = 0; c < nch; c++) s += part[c * 64 + co];
-  g[co] = s;
+  for (int c = threadIdx.x; c < nch; c += 64) s += part[((size_t)c * MP + co) * NP + n];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (threadIdx.x == 0) {
+    if (n < K9) g[co * K9 + n] = s;
+    else gb[co] = s;
+  }
 }
 
 // out (B, H/2, W/2, C) = scale * sum of 2x2 blocks of in (B, H, W, C)
@@ -307,23 +320,18 @@ static hipError_t launch_wgrad(pg_net* N, int l, const bf16_t* X, const bf16_t* 
 {
   const int lvl = kLevel[l], Cin = kCin[l], Cout = kCout[l];
   const int H = N->H >> lvl, W = N->W >> lvl, npx = N->B * H * W;
-  const int K9 = 9 * Cin, NP = ceil_to(K9, 16), MP = ceil_to(Cout, 16);
-  const int tiles = (NP / 16) * (MP / 16);
-  int nch = 2048 / tiles;
+  const int K9 = 9 * Cin, NP = ceil_to(K9 + 1, 16), MP = ceil_to(Cout, 16);
+  const int ngroups = (NP / 16 + NTW - 1) / NTW;
+  const int groups = ngroups * (MP / 16);
+  int nch = 1024 / groups;
   if (nch < 1) nch = 1;
-  int chunk = ceil_to((npx + nch - 1) / nch, 16);
+  const int chunk = ceil_to((npx + nch - 1) / nch, 16);
   nch = (npx + chunk - 1) / chunk;
   if ((size_t)nch * MP * NP > N->npart) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_conv_wgrad, dim3(NP / 16, MP / 16, nch), dim3(64), 0, N->stream, X, Cin, dZ, Cz, Cout, N->B, H, W,
+  hipLaunchKernelGGL(k_conv_wgrad, dim3(ngroups, MP / 16, nch), dim3(64), 0, N->stream, X, Cin, dZ, Cz, Cout, N->B, H, W,
                      chunk, NP, MP, N->part);
-  float* g = N->grad + N->woff[l];
-  hipLaunchKernelGGL(k_wgrad_reduce, dim3((Cout * K9 + 255) / 256), dim3(256), 0, N->stream, N->part, nch, MP, NP, Cout, K9, g);
-  // bias
-  int bch = 1024;
-  const int bchunk = (npx + bch - 1) / bch;
-  bch = (npx + bchunk - 1) / bchunk;
-  hipLaunchKernelGGL(k_bgrad, dim3(bch), dim3(64), 0, N->stream, dZ, Cz, Cout, npx, bchunk, N->part2);
-  hipLaunchKernelGGL(k_bgrad_reduce, dim3(1), dim3(64), 0, N->stream, N->part2, bch, Cout, N->grad + N->boff[l]);
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(Cout * (K9 + 1)), dim3(64), 0, N->stream, N->part, nch, MP, NP, Cout, K9,
+                     N->grad + N->woff[l], N->grad + N->boff[l]);
   return hipGetLastError();
 }
 
