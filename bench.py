@@ -40,6 +40,10 @@ DOMINANT = "k_tendency_inner32"
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "k_tendency_inner32_jets1024x4_traffic.md")
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -59,6 +63,10 @@ def parse():
     p.add_argument("--learn-steps", type=int, default=20000, help="Adam steps")
     p.add_argument("--learn-batch", type=int, default=8)
     p.add_argument("--learn-eval", type=int, default=100, help="steps evaluated with and without the network")
+    p.add_argument("--ens-members", type=int, default=96, help="ensemble members (0: skip the ensemble)")
+    p.add_argument("--ens-steps", type=int, default=1000)
+    p.add_argument("--ens-branches", type=int, default=32)
+    p.add_argument("--ens-eager", action="store_true", help="launch the ensemble steps without the HIP graph")
     return p.parse_args()
 
 
@@ -169,10 +177,12 @@ def learned_vs_plain(a, stream):
     m.synchronize()
     t_cap = time.perf_counter() - t0
     m.learn_capture(False)
+    log(f"learned preconditioner: {ncap} samples captured in {t_cap:.1f} s, training {a.learn_steps} steps")
     loss0 = m.learn_train(1, a.learn_batch, 0.0)
     t0 = time.perf_counter()
     loss = m.learn_train(a.learn_steps, a.learn_batch, 1e-3, seed=11)
     t_train = time.perf_counter() - t0
+    log(f"trained in {t_train:.1f} s, loss {loss0:.3e} -> {loss:.3e}")
 
     def window(on):
         m.learn_use(on, 0.9)
@@ -201,6 +211,45 @@ def learned_vs_plain(a, stream):
     return out
 
 
+def ensemble_run(a):
+    """Batched ensemble: 2-layer wind-driven gyres in three grid sizes
+    (192x192, 256x256, 384x256, a third of the members each), seeded N(0, 1 m)
+    interface noise per member and wind amplitude drawn U(0.09, 0.11),
+    mixed-precision GPU multigrid; AB1/AB2 start-up eagerly, then three
+    steps of every member per HIP-graph launch with a fixed V-cycle count
+    (taken from two adaptive AB3 steps)."""
+    import numpy as np
+
+    import pawsim_amd
+    from pawsim_amd import inputs
+
+    sizes = [(192, 192), (256, 256), (384, 256)]
+    tau = 0.09 + 0.02 * inputs.uniform01(4242, a.ens_members)
+    models = []
+    t0 = time.perf_counter()
+    for i in range(a.ens_members):
+        nx, ny = sizes[i * len(sizes) // a.ens_members]
+        s = inputs.gyre(nx=nx, ny=ny, nt=a.ens_steps, solver=pawsim_amd.SOLVER_GPU_MG_MIXED, tol=1e-8, maxiters=200,
+                        seed=1000 + i, tau0=float(tau[i]), name=f"e{i}")
+        models.append(pawsim_amd.Model.from_setup(s))
+    t_create = time.perf_counter() - t0
+    log(f"ensemble: {a.ens_members} members created in {t_create:.1f} s")
+    e = pawsim_amd.Ensemble(models, branches=a.ens_branches)
+    s1 = e.run(4)
+    s2 = e.run(a.ens_steps - 4, graph=not a.ens_eager)
+    e.close()
+    cells = sum(m.shape[1] * m.shape[2] for m in models)
+    for m in models:
+        m.close()
+    return {"workload": f"{a.ens_members} two-layer gyres, thirds at 192x192 / 256x256 / 384x256 "
+                        f"({cells} cells per layer in total), mixed-precision GPU MG",
+            "members": a.ens_members, "steps": a.ens_steps, "fixed_vcycles": s2["cycles"],
+            "graph_launches": s2["graph_launches"], "graph_steps_seconds": s2["seconds"],
+            "member_steps_per_s": a.ens_members * s2["steps"] / s2["seconds"],
+            "startup_seconds": s1["seconds"], "create_seconds": t_create,
+            "max_rel_residual": s2["max_rel_residual"]}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -225,12 +274,14 @@ def main():
     # reference for the mixed-precision run and the plain-MG solve time.
     _, m64 = make_model(a, mixed=0, solver=pawsim_amd.SOLVER_GPU_MG)
     r64 = run_steps(a, m64, stream)
+    log(f"FP64 path: {a.steps / r64['elapsed']:.1f} steps/s")
     s64 = m64.state()
     m64.close()
 
     # Headline: mixed precision (FP32 stencils, FP32 V-cycles in FP64 refinement).
     setup, m = make_model(a, mixed=1)
     r32 = run_steps(a, m, stream)
+    log(f"mixed precision: {a.steps / r32['elapsed']:.1f} steps/s")
     n = r32["n"]
     s32 = m.state()
     ms_per_step = 1e3 * r32["elapsed"] / a.steps
@@ -325,9 +376,11 @@ def main():
     m.close()
 
     learned = learned_vs_plain(a, stream) if a.learn_samples > 0 else None
+    ensemble = ensemble_run(a) if a.ens_members > 0 else None
 
     cpu = None
     if not a.no_cpu:
+        log("CPU baseline")
         cpu = cpu_baseline(a)
 
     out = {
@@ -346,6 +399,7 @@ def main():
                            "plain_fp64_mg_in_run_ms_mean": r64["solve_ms"],
                            "same_rhs_from_zero": rhs_solve, "learned": learned},
         "pcie_inclusive_steps_per_s": pcie_rate,
+        "ensemble": ensemble,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -192,9 +192,9 @@ int pg_set_solve_cycles(pg_handle *h, uint32_t cycles);
 /* Ensemble: independent members (any grid sizes, GPU-MG solver, no
  * pressureTiming) advanced together.  pg_ensemble_run runs, on its first
  * call, the preliminary correction, (with PG_ENS_AUTOTUNE) an autotuning of
- * the multigrid coarse-tail level per grid size, and the AB1/AB2 start-up
- * steps with adaptive solves; later steps use a fixed V-cycle count
- * (`cycles`, or the largest start-up count + 1 when 0) and, with
+ * the multigrid coarse-tail level per grid size, and four steps (AB1, AB2,
+ * two AB3) with adaptive solves; later steps use a fixed V-cycle count
+ * (`cycles`, or the largest count of the two AB3 steps + 1 when 0) and, with
  * PG_ENS_GRAPH, are replayed from one HIP graph per three steps of all
  * members, spread over `branches` streams.  Members keep their own handles (state up/download as usual
  * between runs). */

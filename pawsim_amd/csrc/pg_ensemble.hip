@@ -5,9 +5,9 @@
 //  1. autotunes, per distinct grid size, the coarse-tail threshold of the
 //     fused multigrid (the tile level at which one workgroup takes over),
 //     timing a V-cycle with events for each candidate;
-//  2. runs the preliminary correction and the two AB3 start-up steps
-//     eagerly with the adaptive solver and takes the largest V-cycle count
-//     seen, plus a margin, as the fixed cycle count;
+//  2. runs the preliminary correction, the AB1/AB2 start-up steps and two
+//     AB3 steps eagerly with the adaptive solver and takes the largest
+//     V-cycle count of the AB3 steps, plus one, as the fixed cycle count;
 //  3. captures three steps of every member (the AB history rotates with
 //     period three, so the captured buffer pointers repeat) into one graph,
 //     members spread over a few streams that fork from and join into the
@@ -166,15 +166,16 @@ int pg_ensemble_run(pg_ensemble* e, uint32_t steps, uint32_t cycles, uint32_t fl
   }
   const auto t0 = std::chrono::steady_clock::now();
   uint32_t done = 0;
-  // start-up steps (AB1, AB2) with the adaptive solver
-  while (done < steps && e->n < 2) {
+  // start-up steps (AB1, AB2) and two AB3 steps with the adaptive solver;
+  // the AB3 ones calibrate the fixed V-cycle count
+  while (done < steps && e->n < 4) {
     e->n++;
     for (pg_handle* h : e->m) {
       pg_solve_stats s;
       if ((rc = pg_step(h, e->n))) return efail(e, rc, h->err);
       rc = pg_pressure_correct(h, &s);
       if (rc && rc != PG_ENOCONV) return efail(e, rc, h->err);
-      e->maxit = std::max(e->maxit, s.iters);
+      if (e->n > 2) e->maxit = std::max(e->maxit, s.iters);
     }
     done++;
   }

@@ -31,29 +31,29 @@ def _close(ms):
 
 
 def test_graph_replay_is_bitwise_eager():
-    a = _models(11)
+    a = _models(13)
     ea = pawsim_amd.Ensemble(a, branches=2)
-    ea.run(2, autotune=False)
+    ea.run(4, autotune=False)
     sa = ea.run(9, autotune=False)
     print(f"[ensemble] graph: {sa}", flush=True)
     assert sa["graph_launches"] == 3 and sa["steps"] == 9
     assert sa["max_rel_residual"] <= 1e-8
     ea.close()
-    b = _models(11)
+    b = _models(13)
     eb = pawsim_amd.Ensemble(b, branches=2)
-    eb.run(2, autotune=False)
+    eb.run(4, autotune=False)
     sb = eb.run(9, cycles=sa["cycles"], graph=False, autotune=False)
     assert sb["graph_launches"] == 0
     eb.close()
-    c = _models(11)
+    c = _models(13)
     for m in c:
         m.pressure_correct()
         m.refresh_halos()
-        for n in (1, 2):
+        for n in (1, 2, 3, 4):
             m.step(n)
             m.pressure_correct()
         m.set_solve_cycles(sa["cycles"])
-        for n in range(3, 12):
+        for n in range(5, 14):
             m.step(n)
             m.pressure_correct()
     for i, (x, y, z) in enumerate(zip(a, b, c)):
@@ -64,10 +64,10 @@ def test_graph_replay_is_bitwise_eager():
 
 
 def test_autotuned_ensemble_meets_tolerance():
-    nt = 2 + 30
+    nt = 4 + 30
     a = _models(nt)
     e = pawsim_amd.Ensemble(a, branches=3)
-    e.run(2)
+    e.run(4)
     st = e.run(30)
     print(f"[ensemble] autotuned: {st}", flush=True)
     assert st["max_rel_residual"] <= 1e-8
