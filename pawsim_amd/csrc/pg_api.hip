@@ -51,11 +51,28 @@ int download_owned(pg_handle* h, double* dst, const double* src, std::vector<dou
 // depend on the refresh history.
 hipError_t fill_prognostic(pg_handle* h)
 {
-  hipError_t e = pg_fill(h, h->u, h->G.L, PG_HALO_U);
-  if (e == hipSuccess) e = pg_fill(h, h->v, h->G.L, PG_HALO_V);
-  if (e == hipSuccess) e = pg_fill(h, h->h, h->G.L, PG_HALO_SCALAR);
-  return e;
+  h->halo_dirty = false;
+  return pg_fill_prognostic(h);
 }
+
+// The reference refreshes the prognostic halos after every change of u, v,
+// h.  A refresh depends only on the owned values, except on a periodic-x /
+// wall-y domain, where the wall-y phase skips the x-ghost columns and the
+// corner ghosts keep values from the previous refresh (field.c:137-198); the
+// residual and divergence norms read ghost values too.  Those cases refresh
+// at every reference point; otherwise a refresh is deferred to the next
+// reader of ghost values (the tendency, or the end of the correction) --
+// nothing between the AB update and the velocity correction reads them.
+bool exact_refresh(const pg_handle* h) { return (!h->G.wallx && h->G.wally) || h->cfg.pressureTiming; }
+
+hipError_t refresh_after_change(pg_handle* h)
+{
+  if (exact_refresh(h)) return fill_prognostic(h);
+  h->halo_dirty = true;
+  return hipSuccess;
+}
+
+hipError_t ensure_halos(pg_handle* h) { return h->halo_dirty ? fill_prognostic(h) : hipSuccess; }
 
 int validate(const pg_config* c, std::string& why)
 {
@@ -310,9 +327,7 @@ int pg_upload_state(pg_handle* h, const double* u, const double* v, const double
     if (!rc) rc = upload_owned(h, h->h + pg_lay(h->G, k), hh + k * n, h->G.ny, stage);
     if (rc) return rc;
   }
-  PG_CHECK(h, pg_fill(h, h->u, h->G.L, PG_HALO_U));
-  PG_CHECK(h, pg_fill(h, h->v, h->G.L, PG_HALO_V));
-  PG_CHECK(h, pg_fill(h, h->h, h->G.L, PG_HALO_SCALAR));
+  PG_CHECK(h, fill_prognostic(h));
   PG_CHECK(h, hipStreamSynchronize(h->stream));
   return PG_OK;
 }
@@ -353,6 +368,7 @@ int pg_tendency(pg_handle* h, double* du, double* dv, double* dh)
 {
   if (!h) return PG_EARG;
   const int s = h->slot[0];
+  PG_CHECK(h, ensure_halos(h));
   PG_CHECK(h, pg_launch_tendency_slot(h, s));
   if (h->mixed) PG_CHECK(h, pg_widen_slot(h, s));
   std::vector<double> stage;
@@ -371,6 +387,7 @@ int pg_step(pg_handle* h, uint32_t n)
 {
   if (!h || n == 0) return PG_EARG;
   const int s = h->slot[0];
+  PG_CHECK(h, ensure_halos(h));
   PG_CHECK(h, pg_launch_tendency_slot(h, s));
   const uint32_t scheme = h->cfg.timeSteppingScheme;
   bool shift = true;
@@ -392,7 +409,7 @@ int pg_step(pg_handle* h, uint32_t n)
     h->slot[2] = t1;
   }
   h->t += h->cfg.dt;
-  PG_CHECK(h, fill_prognostic(h));
+  PG_CHECK(h, refresh_after_change(h));
   return PG_OK;
 }
 
@@ -405,7 +422,7 @@ int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
   const uint32_t solver = h->cfg.pressureSolver;
   // thickness constraint, then halo refresh (pressure.c:525-561)
   PG_CHECK(h, pg_launch_thickness(h));
-  PG_CHECK(h, fill_prognostic(h));
+  PG_CHECK(h, refresh_after_change(h));
   // RHS with mean removal (pressure.c:570-606)
   PG_CHECK(h, pg_launch_rhs(h));
   // The bit-replica solvers need the serial-order mean of pressure.c:55-72;
@@ -413,7 +430,8 @@ int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
   if (solver == PG_SOLVER_SOR || solver == PG_SOLVER_MG_REPLICA) PG_CHECK(h, pg_launch_serial_mean(h, h->rhs));
   else PG_CHECK(h, pg_launch_parallel_mean(h, h->rhs));
   PG_CHECK(h, pg_launch_sub_mean(h, h->rhs));
-  PG_CHECK(h, pg_fill(h, h->rhs, 1, PG_HALO_SCALAR));
+  // the GPU multigrid gathers the owned rhs only
+  if (solver == PG_SOLVER_SOR || solver == PG_SOLVER_MG_REPLICA) PG_CHECK(h, pg_fill(h, h->rhs, 1, PG_HALO_SCALAR));
   int rc = run_solver(h, solver, st);
   if (rc && rc != PG_ENOCONV) return rc;
   if (h->cfg.pressureTiming) PG_CHECK(h, pg_launch_norms(h, 0));

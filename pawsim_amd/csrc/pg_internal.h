@@ -52,6 +52,7 @@ struct pg_handle {
   bool mixed = false;                  // FP32 stencils and tendency history
   float *fu32[3] = {}, *fv32[3] = {}, *fh32[3] = {}; // AB history buffers in mixed precision
   int slot[3] = {0, 1, 2};             // buffers holding tendency, tendency_1, tendency_2
+  bool halo_dirty = false;             // u, v, h halos are stale (refilled before the next reader)
   double *hhb = nullptr, *hhs = nullptr, *omz = nullptr, *taux = nullptr, *tauy = nullptr;
   double *Hc = nullptr, *Hw = nullptr, *Hs = nullptr;  // ghosted static geometry
   double *pi = nullptr, *rhs = nullptr;                // ghosted
@@ -84,6 +85,7 @@ struct pg_handle {
 // ---- launchers (pg_kernels.hip) ----
 hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind);
 hipError_t pg_fill1(pg_handle* h, double* f);
+hipError_t pg_fill_prognostic(pg_handle* h);  // u, v, h halos in one launch
 hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh);
 hipError_t pg_launch_tendency_slot(pg_handle* h, int s);  // into history slot s, per precision mode
 hipError_t pg_widen_slot(pg_handle* h, int s);            // FP32 history slot -> FP64 slot

@@ -22,9 +22,19 @@ constexpr int TI = 4;   // rows of i per workgroup
 // phase is independent per column (y phases) or per row (x phases), and a
 // thread performs its column/row in the reference's loop order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_fill(double* base, size_t stride, PgGrid G, int kind)
+// Up to three layered arrays per launch: block b fills layer b % per of
+// array b / per.
+struct FillSet {
+  double* base[3];
+  int kind[3];
+  int per;
+};
+
+__global__ __launch_bounds__(1024) void k_fill(FillSet S, size_t stride, PgGrid G)
 {
-  double* f = base + (size_t)blockIdx.x * stride;
+  const int a = blockIdx.x / S.per;
+  double* f = S.base[a] + (size_t)(blockIdx.x - a * S.per) * stride;
+  const int kind = S.kind[a];
   const int g = G.g, nx = G.nx, ny = G.ny, sx = G.sx, sy = G.sy;
   auto F = [&](int i, int j) -> double& { return f[(size_t)i * sy + j]; };
   if (!G.wally) {  // periodic y over owned columns (field.c:121-134)
@@ -542,7 +552,15 @@ dim3 owned_grid(const PgGrid& G, int z) { return dim3((G.ny + TJ - 1) / TJ, (G.n
 // ---------------------------------------------------------------------------
 hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind)
 {
-  hipLaunchKernelGGL(k_fill, dim3(nfields), dim3(1024), 0, h->stream, base, h->fsz, h->G, kind);
+  FillSet S{{base, nullptr, nullptr}, {kind, 0, 0}, nfields};
+  hipLaunchKernelGGL(k_fill, dim3(nfields), dim3(1024), 0, h->stream, S, h->fsz, h->G);
+  return hipGetLastError();
+}
+
+hipError_t pg_fill_prognostic(pg_handle* h)
+{
+  FillSet S{{h->u, h->v, h->h}, {PG_HALO_U, PG_HALO_V, PG_HALO_SCALAR}, h->G.L};
+  hipLaunchKernelGGL(k_fill, dim3(3 * h->G.L), dim3(1024), 0, h->stream, S, h->fsz, h->G);
   return hipGetLastError();
 }
 
