@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--nlay", type=int, default=4)
     p.add_argument("--solver", type=int, default=4, help="3 FP64 GPU MG, 4 mixed-precision GPU MG")
     p.add_argument("--tol", type=float, default=1e-8)
+    p.add_argument("--mg-nu1", type=int, default=2, help="pre-smoothing sweeps of the GPU multigrid")
+    p.add_argument("--mg-nu2", type=int, default=2, help="post-smoothing sweeps")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-steps", type=int, default=12)
     p.add_argument("--pmc-probe", action="store_true",
@@ -76,7 +78,7 @@ def make_model(a, mixed=1, solver=None, tol=None):
 
     setup = inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=a.steps + a.warmup,
                         solver=a.solver if solver is None else solver, tol=a.tol if tol is None else tol,
-                        A2=10.0, maxiters=200, mixed_precision=mixed)
+                        A2=10.0, maxiters=200, mixed_precision=mixed, mg_nu1=a.mg_nu1, mg_nu2=a.mg_nu2)
     return setup, pawsim_amd.Model.from_setup(setup)
 
 

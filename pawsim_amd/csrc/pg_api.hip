@@ -261,6 +261,8 @@ static int create_body(const pg_config* cfg, const pg_static* st, pg_handle** ou
   PG_CHECK(h, hipMalloc(&h->ctl, sizeof(PgCtl)));
   PG_CHECK(h, hipMemsetAsync(h->ctl, 0, sizeof(PgCtl), h->stream));
   PG_CHECK(h, hipHostMalloc(&h->ctl_host, sizeof(PgCtl), hipHostMallocDefault));
+  PG_CHECK(h, hipHostMalloc((void**)&h->mapped_host, 64, hipHostMallocMapped | hipHostMallocCoherent));
+  PG_CHECK(h, hipHostGetDevicePointer((void**)&h->mapped, h->mapped_host, 0));
 
   // Static inputs with their halo kinds (state.c:1049-1055).
   std::vector<double> stage;
@@ -308,6 +310,7 @@ void pg_destroy(pg_handle* h)
   }
   (void)hipFree(h->ctl);
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
+  if (h->mapped_host) (void)hipHostFree(h->mapped_host);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);

@@ -61,12 +61,16 @@ struct pg_handle {
   int npartials = 0;
   PgCtl *ctl = nullptr;                // device control block
   PgCtl *ctl_host = nullptr;           // pinned mirror
+  double* mapped_host = nullptr;       // coherent host memory the convergence sums are also stored to:
+  double* mapped = nullptr;            //   [0] ||r||^2, [1] ||rhs||^2 (host / device addresses)
   std::vector<MgLevel> lev;            // [0] = coarsest
   bool mg_built = false;
   struct PgLearn* learn = nullptr;     // learned preconditioner (pg_learn.hip), optional
   int mg_pred = 0;                     // V-cycles the previous GPU-MG solve took
   int fixed_cycles = 0;                // > 0: GPU-MG solves run exactly this many cycles with no host reads
   long mg_tail_cells = 1024;           // largest level the LDS coarse tail starts from
+  void* tail_img[2] = {};              // packed LDS image of the coarse-tail coefficients (FP32, FP64)
+  int tail_img_m[2] = {-1, -1};        // tail level the image was built for
   bool fast_tendency = true;           // inner LDS kernel + edge ring (bitwise equal to the per-point kernel)
   double t = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -517,6 +517,11 @@ void pg_mg_free(pg_handle* h)
   }
   h->lev.clear();
   h->mg_built = false;
+  for (int q = 0; q < 2; q++) {
+    (void)hipFree(h->tail_img[q]);
+    h->tail_img[q] = nullptr;
+    h->tail_img_m[q] = -1;
+  }
 }
 
 int pg_mg_build(pg_handle* h)

@@ -134,7 +134,7 @@ __device__ void block_partial(double v, double* part)
   }
 }
 
-__global__ __launch_bounds__(256) void k_finish_sum(const double* part, int nb, double* out)
+__global__ __launch_bounds__(256) void k_finish_sum(const double* part, int nb, double* out, double* out_host)
 {
   __shared__ double red[4];
   double s = 0;
@@ -142,7 +142,11 @@ __global__ __launch_bounds__(256) void k_finish_sum(const double* part, int nb, 
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) {
+    const double t = red[0] + red[1] + red[2] + red[3];
+    *out = t;
+    if (out_host) __hip_atomic_store(out_host, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -256,9 +260,16 @@ __global__ __launch_bounds__(NT) void k_down(MgLevel f, MgLevel c, int nu, int z
 }
 
 // Prolongation + correction of f.tmp by c.pi, nu2 sweeps (colour 1 first),
-// result into f.pi.  With `norm` (FP64 only) also ||L pi - rhs||^2.
+// result into f.pi.  Finest level only, one of:
+//  norm = 1 (FP64 cycles): also block partials of ||L pi - rhs||^2;
+//  norm = 2 (FP32 correction of an FP64 iterate): the FP32 result e is not
+//    stored; instead v = pcur - e becomes the new FP64 iterate (pout) and
+//    its FP64 defect r = L v - rhs goes to f.rhs32 with block partials of
+//    r^2 -- the same expressions as k_defect, fused.
+// Both need e one cell beyond the tile (halo one wider).
 template <typename T>
-__global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wallEW, int wallNS, int norm, double* part)
+__global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wallEW, int wallNS, int norm, double* part,
+                                           const double* pcur, double* pout)
 {
   constexpr int TI = Tile<T>::TI, P = TileLds<T>::P;
   __shared__ TileLds<T> S;
@@ -275,15 +286,29 @@ __global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wal
     __syncthreads();
   }
   T* out = LvPtr<T>::pi(f);
+  const int X = f.Nx, Y = f.Ny;
   double ss = 0;
   for (int idx = tid; idx < TI * FJ; idx += NT) {
     const int gi = i0 + idx / FJ, gj = j0 + idx % FJ;
-    if (gi >= f.Nx || gj >= f.Ny) continue;
+    if (gi >= X || gj >= Y) continue;
     const int s = (H + idx / FJ) * P + H + idx % FJ;
-    out[(size_t)gi * f.Ny + gj] = S.pi[s];
-    if (norm) {
-      const double r = (double)lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s);
+    const size_t p = (size_t)gi * Y + gj;
+    if (norm == 2) {
+      const size_t pE = (size_t)ip1(gi, X) * Y + gj, pW = (size_t)im1(gi, X) * Y + gj;
+      const size_t pN = (size_t)gi * Y + ip1(gj, Y), pS = (size_t)gi * Y + im1(gj, Y);
+      const double v = pcur[p] - 1.0 * (double)S.pi[s], vE = pcur[pE] - 1.0 * (double)S.pi[s + P];
+      const double vW = pcur[pW] - 1.0 * (double)S.pi[s - P], vN = pcur[pN] - 1.0 * (double)S.pi[s + 1];
+      const double vS = pcur[pS] - 1.0 * (double)S.pi[s - 1];
+      pout[p] = v;
+      const double r = f.Os[pN] * vN + f.Os[p] * vS + f.Ow[pE] * vE + f.Ow[p] * vW - f.Osum[p] * v - f.rhs[p];
+      f.rhs32[p] = (float)r;
       ss += r * r;
+    } else {
+      out[p] = S.pi[s];
+      if (norm) {
+        const double r = (double)lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s);
+        ss += r * r;
+      }
     }
   }
   if (norm) block_partial(ss, part);
@@ -317,29 +342,47 @@ __device__ __forceinline__ TV<T> tview(T* mem, const TailLevel& L)
 __device__ __forceinline__ int dec(int i, int N) { return i == 0 ? N - 1 : i - 1; }
 __device__ __forceinline__ int inc(int i, int N) { return i == N - 1 ? 0 : i + 1; }
 
+// Synchronisation scopes of the tail: the whole workgroup, or wavefront 0
+// alone for the smallest levels (the lanes of a wavefront run in lockstep,
+// LDS operations of one wavefront complete in order; the fences keep the
+// compiler from moving LDS accesses across the barrier).
+struct BlockScope {
+  static constexpr int NT = TAIL_THREADS, SIDE = 32;
+  __device__ static void sync() { __syncthreads(); }
+};
+struct WaveScope {
+  static constexpr int NT = 64, SIDE = 8;
+  __device__ static void sync()
+  {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
 // f(i, j, p) for every cell of an X x Y level: one cell per thread when the
-// level fits 32 x 32, else a strided loop.
-template <typename F>
+// level fits SIDE x SIDE, else a strided loop.
+template <typename S, typename F>
 __device__ __forceinline__ void tail_cells(int X, int Y, int tid, F&& f)
 {
-  if (X <= 32 && Y <= 32) {
-    const int i = tid >> 5, j = tid & 31;
+  if (X <= S::SIDE && Y <= S::SIDE) {
+    const int i = tid / S::SIDE, j = tid % S::SIDE;
     if (i < X && j < Y) f(i, j, i * Y + j);
   } else {
-    for (int idx = tid; idx < X * Y; idx += TAIL_THREADS) {
+    for (int idx = tid; idx < X * Y; idx += S::NT) {
       const int i = idx / Y;
       f(i, idx - i * Y, idx);
     }
   }
 }
 
-template <typename T>
+template <typename T, typename S>
 __device__ void tail_rb(const TV<T>& L, int first, int tid)
 {
   const int X = L.X, Y = L.Y;
   for (int cc = 0; cc < 2; cc++) {
     const int color = first ^ cc;
-    tail_cells(X, Y, tid, [&](int i, int j, int p) {
+    tail_cells<S>(X, Y, tid, [&](int i, int j, int p) {
       if (((i + j) & 1) != color) return;
       const T r = L.rO[p];
       if (!(r == r) || r == (T)INFINITY) return;
@@ -347,20 +390,20 @@ __device__ void tail_rb(const TV<T>& L, int first, int tid)
       L.pi[p] = r * (L.Os[i * Y + jp] * L.pi[i * Y + jp] + L.Os[p] * L.pi[i * Y + jm] + L.Ow[ip * Y + j] * L.pi[ip * Y + j]
                      + L.Ow[p] * L.pi[im * Y + j] - L.rhs[p]);
     });
-    __syncthreads();
+    S::sync();
   }
 }
 
 // Lexicographic GS (omega = 1) by anti-diagonal wavefronts, for levels whose
 // periodic length is odd (red-black colouring is not consistent there).
-template <typename T>
+template <typename T, typename S>
 __device__ void tail_lex(const TV<T>& L, int tid)
 {
   const int X = L.X, Y = L.Y;
   for (int d = 0; d <= X + Y - 2; d++) {
     const int ilo = d - (Y - 1) > 0 ? d - (Y - 1) : 0;
     const int ihi = d < X - 1 ? d : X - 1;
-    for (int i = ilo + tid; i <= ihi; i += TAIL_THREADS) {
+    for (int i = ilo + tid; i <= ihi; i += S::NT) {
       const int j = d - i;
       const int im = dec(i, X), ip = inc(i, X), jm = dec(j, Y), jp = inc(j, Y);
       const int p = i * Y + j;
@@ -369,26 +412,88 @@ __device__ void tail_lex(const TV<T>& L, int tid)
                        + L.Ow[p] * L.pi[im * Y + j] - L.rhs[p])
                 + (1 - (T)1) * L.pi[p];
     }
-    __syncthreads();
+    S::sync();
   }
 }
 
-template <typename T>
+template <typename T, typename S>
 __device__ void tail_smooth(const TV<T>& L, int wallEW, int wallNS, int first, int tid)
 {
   const bool odd_periodic = (!wallEW && (L.X & 1) && L.X > 1) || (!wallNS && (L.Y & 1) && L.Y > 1);
-  if (odd_periodic) tail_lex<T>(L, tid);
-  else tail_rb<T>(L, first, tid);
+  if (odd_periodic) tail_lex<T, S>(L, tid);
+  else tail_rb<T, S>(L, first, tid);
+}
+
+// Down leg at level n: smoothing, residual, restriction into level n - 1.
+template <typename T, typename S>
+__device__ void tail_down(const MgLevel* lv, const TailLevel* LL, T* mem, int n, int nu1, int wallEW, int wallNS, int tid)
+{
+  const TV<T> F = tview(mem, LL[n]);
+  const TV<T> Cl = tview(mem, LL[n - 1]);
+  for (int s = 0; s < nu1; s++) tail_smooth<T, S>(F, wallEW, wallNS, 0, tid);
+  const int X = F.X, Y = F.Y;
+  tail_cells<S>(X, Y, tid, [&](int i, int j, int p) {
+    const int im = dec(i, X), ip = inc(i, X), jm = dec(j, Y), jp = inc(j, Y);
+    const T osum = F.Ow[p] + F.Ow[ip * Y + j] + F.Os[p] + F.Os[i * Y + jp];
+    F.tmp[p] = F.Os[i * Y + jp] * F.pi[i * Y + jp] + F.Os[p] * F.pi[i * Y + jm] + F.Ow[ip * Y + j] * F.pi[ip * Y + j]
+               + F.Ow[p] * F.pi[im * Y + j] - osum * F.pi[p] - F.rhs[p];
+  });
+  S::sync();
+  const MgLevel& Fg = lv[n];
+  const MgLevel& Cg = lv[n - 1];
+  tail_cells<S>(Cl.X, Cl.Y, tid, [&](int i, int j, int p) {
+    restrict_cell<T>(Fg, F.tmp, Cg, Cl.rhs, i, j);
+    Cl.pi[p] = 0;
+  });
+  S::sync();
+}
+
+// Up leg at level n: prolongation of level n - 1, correction, smoothing.
+template <typename T, typename S>
+__device__ void tail_up(const MgLevel* lv, const TailLevel* LL, T* mem, int n, int nu2, int wallEW, int wallNS, int tid)
+{
+  const TV<T> F = tview(mem, LL[n]);
+  const TV<T> Cl = tview(mem, LL[n - 1]);
+  const bool p2 = lv[n].pow2;
+  const MgLevel& Fg = lv[n];
+  const MgLevel& Cg = lv[n - 1];
+  tail_cells<S>(F.X, F.Y, tid, [&](int i, int j, int p) {
+    const T e = p2 ? interp2<T>(Cl.pi, Cl.Y, i, j, F.X, F.Y, wallEW, wallNS)
+                   : interp_general<T>(Fg, Cg, Cl.pi, i, j, wallEW, wallNS);
+    F.pi[p] -= e;
+  });
+  S::sync();
+  for (int s = 0; s < nu2; s++) tail_smooth<T, S>(F, wallEW, wallNS, 1, tid);
+}
+
+// Packed image of the tail's LDS for levels 0..m: per level six X*Y arrays
+// (pi, rhs, tmp zero; Ow, Os, 1/Osum from the level).
+template <typename T>
+__global__ void k_tail_image(const MgLevel* lv, int m, T* img)
+{
+  int off = 0;
+  for (int n = 0; n <= m; n++) {
+    const int N = lv[n].Nx * lv[n].Ny;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += gridDim.x * blockDim.x) {
+      img[off + idx] = 0;
+      img[off + N + idx] = 0;
+      img[off + 2 * N + idx] = 0;
+      img[off + 3 * N + idx] = LvPtr<T>::Ow(lv[n])[idx];
+      img[off + 4 * N + idx] = LvPtr<T>::Os(lv[n])[idx];
+      img[off + 5 * N + idx] = LvPtr<T>::rOsum(lv[n])[idx];
+    }
+    off += 6 * N;
+  }
 }
 
 // V(nu1, nu2) over levels m .. 0 with every array in LDS.  pi of level m is
 // loaded (or zero) and written back; the transfers are the reference's
-// (2:1 or general).
+// (2:1 or general).  Levels of at most 8 x 8 cells run in wavefront 0 alone.
 template <typename T>
 __global__ __launch_bounds__(TAIL_THREADS) void k_tail(const MgLevel* lv, int m, int nu1, int nu2, int wallEW, int wallNS,
-                                                       int zero_init)
+                                                       int zero_init, const void* img, size_t bytes)
 {
-  extern __shared__ double tail_smem[];
+  extern __shared__ __align__(16) double tail_smem[];
   T* mem = reinterpret_cast<T*>(tail_smem);
   __shared__ TailLevel LL[32];
   const int tid = threadIdx.x;
@@ -400,60 +505,38 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(const MgLevel* lv, int m,
     }
   }
   __syncthreads();
-  for (int n = 0; n <= m; n++) {
-    const TV<T> L = tview(mem, LL[n]);
-    const T* gOw = LvPtr<T>::Ow(lv[n]);
-    const T* gOs = LvPtr<T>::Os(lv[n]);
-    const T* grO = LvPtr<T>::rOsum(lv[n]);
-    const T* gpi = LvPtr<T>::pi(lv[n]);
-    const T* grhs = LvPtr<T>::rhs(lv[n]);
+  {
+    // the packed image (coefficients of every level, zero iterates) in one
+    // coalesced copy, then the top level's iterate and right-hand side
+    const int n16 = (int)(bytes / 16);
+    const uint4* src = reinterpret_cast<const uint4*>(img);
+    uint4* dst = reinterpret_cast<uint4*>(tail_smem);
+    for (int q = tid; q < n16; q += TAIL_THREADS) dst[q] = src[q];
+    const unsigned* s4 = reinterpret_cast<const unsigned*>(img);
+    unsigned* d4 = reinterpret_cast<unsigned*>(tail_smem);
+    for (int q = 4 * n16 + tid; q < (int)(bytes / 4); q += TAIL_THREADS) d4[q] = s4[q];
+    __syncthreads();
+    const TV<T> L = tview(mem, LL[m]);
+    const T* gpi = LvPtr<T>::pi(lv[m]);
+    const T* grhs = LvPtr<T>::rhs(lv[m]);
     for (int idx = tid; idx < L.X * L.Y; idx += TAIL_THREADS) {
-      L.Ow[idx] = gOw[idx];
-      L.Os[idx] = gOs[idx];
-      L.rO[idx] = grO[idx];
-      if (n == m) {
-        L.pi[idx] = zero_init ? (T)0 : gpi[idx];
-        L.rhs[idx] = grhs[idx];
-      }
+      L.pi[idx] = zero_init ? (T)0 : gpi[idx];
+      L.rhs[idx] = grhs[idx];
     }
   }
   __syncthreads();
-  for (int n = m; n >= 1; n--) {
-    const TV<T> F = tview(mem, LL[n]);
-    const TV<T> Cl = tview(mem, LL[n - 1]);
-    for (int s = 0; s < nu1; s++) tail_smooth<T>(F, wallEW, wallNS, 0, tid);
-    const int X = F.X, Y = F.Y;
-    tail_cells(X, Y, tid, [&](int i, int j, int p) {
-      const int im = dec(i, X), ip = inc(i, X), jm = dec(j, Y), jp = inc(j, Y);
-      const T osum = F.Ow[p] + F.Ow[ip * Y + j] + F.Os[p] + F.Os[i * Y + jp];
-      F.tmp[p] = F.Os[i * Y + jp] * F.pi[i * Y + jp] + F.Os[p] * F.pi[i * Y + jm] + F.Ow[ip * Y + j] * F.pi[ip * Y + j]
-                 + F.Ow[p] * F.pi[im * Y + j] - osum * F.pi[p] - F.rhs[p];
-    });
-    __syncthreads();
-    const MgLevel& Fg = lv[n];
-    const MgLevel& Cg = lv[n - 1];
-    tail_cells(Cl.X, Cl.Y, tid, [&](int i, int j, int p) {
-      restrict_cell<T>(Fg, F.tmp, Cg, Cl.rhs, i, j);
-      Cl.pi[p] = 0;
-    });
-    __syncthreads();
+  // levels 0 .. mw fit one wavefront (at most 8 x 8 cells)
+  int mw = 0;
+  while (mw < m && LL[mw + 1].X <= 8 && LL[mw + 1].Y <= 8) mw++;
+  for (int n = m; n > mw; n--) tail_down<T, BlockScope>(lv, LL, mem, n, nu1, wallEW, wallNS, tid);
+  if (tid < 64) {
+    for (int n = mw; n >= 1; n--) tail_down<T, WaveScope>(lv, LL, mem, n, nu1, wallEW, wallNS, tid);
+    if (tid == 0 && LL[0].X * LL[0].Y == 1) tview(mem, LL[0]).pi[0] = 0;
+    WaveScope::sync();
+    for (int n = 1; n <= mw; n++) tail_up<T, WaveScope>(lv, LL, mem, n, nu2, wallEW, wallNS, tid);
   }
-  if (tid == 0 && LL[0].X * LL[0].Y == 1) tview(mem, LL[0]).pi[0] = 0;
   __syncthreads();
-  for (int n = 1; n <= m; n++) {
-    const TV<T> F = tview(mem, LL[n]);
-    const TV<T> Cl = tview(mem, LL[n - 1]);
-    const bool p2 = lv[n].pow2;
-    const MgLevel& Fg = lv[n];
-    const MgLevel& Cg = lv[n - 1];
-    tail_cells(F.X, F.Y, tid, [&](int i, int j, int p) {
-      const T e = p2 ? interp2<T>(Cl.pi, Cl.Y, i, j, F.X, F.Y, wallEW, wallNS)
-                     : interp_general<T>(Fg, Cg, Cl.pi, i, j, wallEW, wallNS);
-      F.pi[p] -= e;
-    });
-    __syncthreads();
-    for (int s = 0; s < nu2; s++) tail_smooth<T>(F, wallEW, wallNS, 1, tid);
-  }
+  for (int n = mw + 1; n <= m; n++) tail_up<T, BlockScope>(lv, LL, mem, n, nu2, wallEW, wallNS, tid);
   const TV<T> Lm = tview(mem, LL[m]);
   T* gpi = LvPtr<T>::pi(lv[m]);
   for (int idx = tid; idx < Lm.X * Lm.Y; idx += TAIL_THREADS) gpi[idx] = Lm.pi[idx];
@@ -559,7 +642,8 @@ static int tail_level(const pg_handle* h, bool* in_lds)
 }
 
 template <typename T>
-static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top, int norm_top, bool* norm_done)
+static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top, int norm_top, bool* norm_done,
+                               const double* pcur = nullptr, double* pout = nullptr)
 {
   const int top = (int)h->lev.size() - 1;
   bool lds = false;
@@ -575,13 +659,23 @@ static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top
   const int zi = (mt == top) ? zero_init_top : 0;
   if (lds) {
     const size_t bytes = tail_bytes<T>(h, mt);
+    const int q = sizeof(T) == 8;
+    if (h->tail_img_m[q] != mt) {
+      (void)hipFree(h->tail_img[q]);
+      h->tail_img[q] = nullptr;
+      hipError_t e = hipMalloc(&h->tail_img[q], bytes);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_tail_image<T>, dim3(16), dim3(256), 0, h->stream, dlev, mt, (T*)h->tail_img[q]);
+      h->tail_img_m[q] = mt;
+    }
     static bool attr_set[2] = {false, false};
     if (!attr_set[sizeof(T) == 8]) {
       hipError_t e = hipFuncSetAttribute((const void*)k_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)TAIL_LDS_MAX);
       if (e != hipSuccess) return e;
       attr_set[sizeof(T) == 8] = true;
     }
-    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(TAIL_THREADS), bytes, h->stream, dlev, mt, nu1, nu2, wx, wy, zi);
+    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(TAIL_THREADS), bytes, h->stream, dlev, mt, nu1, nu2, wx, wy, zi,
+                       (const void*)h->tail_img[q], bytes);
   } else {
     if (zi) {
       if constexpr (sizeof(T) == 4) {
@@ -596,9 +690,10 @@ static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top
     const MgLevel& l = h->lev[n];
     const int norm = (n == top) ? norm_top : 0;
     const dim3 grid((l.Ny + FJ - 1) / FJ, (l.Nx + TI - 1) / TI);
-    hipLaunchKernelGGL(k_up<T>, grid, dim3(NT), 0, h->stream, l, h->lev[n - 1], nu2, wx, wy, norm, h->partials);
+    hipLaunchKernelGGL(k_up<T>, grid, dim3(NT), 0, h->stream, l, h->lev[n - 1], nu2, wx, wy, norm, h->partials, pcur, pout);
     if (norm)
-      hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, (int)(grid.x * grid.y), &h->ctl->sumsq);
+      hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, (int)(grid.x * grid.y), &h->ctl->sumsq,
+                         h->mapped);
   }
   if (norm_done) *norm_done = norm_top && mt < top;
   return hipGetLastError();
@@ -610,20 +705,22 @@ hipError_t pg_launch_defect(pg_handle* h, const double* pcur, const float* e, co
   const MgLevel& top = h->lev.back();
   const dim3 dg((top.Ny + DJ - 1) / DJ, (top.Nx + DI * DROWS - 1) / (DI * DROWS));
   hipLaunchKernelGGL(k_defect, dg, dim3(DJ, DI), 0, h->stream, top, pcur, e, escale, pout, rhs32, h->partials);
-  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, (int)(dg.x * dg.y), sumsq_out);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, (int)(dg.x * dg.y), sumsq_out,
+                     sumsq_out == &h->ctl->sumsq ? h->mapped : nullptr);
   return hipGetLastError();
 }
 
 hipError_t pg_launch_sumsq(pg_handle* h, const double* x, size_t n, double* out)
 {
   hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, h->stream, x, n, h->partials);
-  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, out);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, out,
+                     out == &h->ctl->rhs_sumsq ? h->mapped + 1 : nullptr);
   return hipGetLastError();
 }
 
 hipError_t pg_launch_finish_sum(pg_handle* h, const double* part, int nb, double* out)
 {
-  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, part, nb, out);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, part, nb, out, (double*)nullptr);
   return hipGetLastError();
 }
 
@@ -665,7 +762,7 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
   (void)nbd;
   PG_CHECK(h, pg_gather_to_mg(h));
   hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, h->stream, top.rhs, N, h->partials);
-  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, &h->ctl->rhs_sumsq);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, &h->ctl->rhs_sumsq, h->mapped + 1);
   defect(top.pi, nullptr, nullptr, mixed ? top.rhs32 : nullptr);
   PG_CHECK(h, hipGetLastError());
   const bool capture = pg_learn_capturing(h);
@@ -690,15 +787,17 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
     if (fixed > 0) {
       if (it >= fixed) break;
     } else if (it >= first_check || it >= maxit) {
-      PG_CHECK(h, hipMemcpyAsync(h->ctl_host, h->ctl, sizeof(PgCtl), hipMemcpyDeviceToHost, h->stream));
+      // the finishing kernels also store both sums in host-mapped memory
       PG_CHECK(h, hipStreamSynchronize(h->stream));
-      const double rr = h->ctl_host->rhs_sumsq;
-      r = (rr > 0) ? std::sqrt(h->ctl_host->sumsq / rr) : 0.0;
+      const double rr = *(volatile double*)(h->mapped_host + 1);
+      const double ss = *(volatile double*)(h->mapped_host);
+      r = (rr > 0) ? std::sqrt(ss / rr) : 0.0;
       if (r <= h->cfg.pi_tol || it >= maxit) break;
     }
     if (mixed) {
-      PG_CHECK(h, fused_vcycle<float>(h, nu1, nu2, 1, 0, nullptr));
-      defect(cur, top.pi32, alt, top.rhs32);
+      bool fused_defect = false;
+      PG_CHECK(h, fused_vcycle<float>(h, nu1, nu2, 1, 2, &fused_defect, cur, alt));
+      if (!fused_defect) defect(cur, top.pi32, alt, top.rhs32);
       std::swap(cur, alt);
     } else {
       bool norm_done = false;
