@@ -95,11 +95,13 @@ def tendency_inner_bytes(a, w_out):
 
 
 def pmc_probe(a):
-    """Five launches of the dominant kernel and five device copies of 256 MiB."""
+    """Five launches of the dominant kernel, five mixed-precision V-cycles and
+    five device copies of 256 MiB."""
     import torch
 
     _, m = make_model(a, mixed=1)
     m.bench_kernel(10, 5)
+    m.bench_kernel(6, 5)
     m.synchronize()
     src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
     dst = torch.empty_like(src)

@@ -134,6 +134,13 @@ int pg_refresh_halos(pg_handle *h);
  * (solve_MG, multigrid.h:24; solve_pressure_sor, pressure.c:743). */
 int pg_solve_pressure(pg_handle *h, uint32_t solver, double *pi, const double *rhs, pg_solve_stats *stats);
 
+/* Total kinetic energy, potential energy and per-layer potential enstrophy
+ * (Z: Nlay values) of the current state in FP64: the values of one
+ * EZdiags.dat line (pawsim_diagnostics_write_EZ_if_due,
+ * diagnostics.c:483-703; AL81 thickness, no active buoyancy).  Tree-ordered
+ * sums: equal to the reference's serial sums up to rounding. */
+int pg_energy(pg_handle *h, double *KE, double *PE, double *Z);
+
 /* Model time (ctx->t) and stream control. */
 double pg_time(const pg_handle *h);
 int pg_set_stream(pg_handle *h, void *hip_stream); /* NULL restores the handle's own stream */

@@ -37,6 +37,7 @@ typedef struct or_config {
   uint Nlay, Nx, Ny, Nt, tsScheme, momScheme, thkScheme, maxiters, solver, tauNrecs;
   real Lx, Ly, dt, tmin, tmax, savefrequency, h0, hsml, hbbl, A2, A4, A2smag, A4smag;
   real linDrag, quadDrag, linDragSurf, quadDragSurf, pi_tol, SOR_rp, omega_MG;
+  real savefreqEZ;            /* energy/enstrophy diagnostics interval (config.c:206) */
   bool wallEW, wallNS, useRL, useFullMG, pressureTiming, writeInit, useWind;
   bool unsupported;           /* some active feature outside the restated path */
   char unsupported_what[128];
@@ -78,10 +79,19 @@ typedef struct or_model {
   /* pressure diagnostics (context.h:52-61) */
   real res_max, res_rms, div_max, div_rms;
   uint solves, iters_total, n_saves;
+  uint n_EZ;                        /* EZ diagnostics schedule (pawsim.c:434-438) */
+  real t_next_EZ;
   double solve_seconds;             /* wall time inside the solver only */
   double correct_seconds;           /* wall time of whole corrections */
   int last_iters;
 } or_model;
+
+/* ---- oracle_diag.c ---- */
+/* Total kinetic and potential energy and per-layer potential enstrophy of
+ * the current state (diagnostics.c:483-703, AL81 thickness, no buoyancy). */
+void or_energy(const or_model *m, double *KE, double *PE, double *Z);
+/* EZdiags.dat line(s) due at the current time (diagnostics.c:493-699). */
+bool or_write_EZ_if_due(or_model *m, const char *outdir);
 
 /* ---- oracle_io.c ---- */
 void or_config_defaults(or_config *c);

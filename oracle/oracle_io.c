@@ -104,7 +104,8 @@ bool or_config_read(const char *fname, or_config *c, FILE *err)
              || !strcmp(key, "bFluxFile") || !strcmp(key, "diaDiffFile")) {
       flag_unsupported(c, key);
     }
-    else if (!strcmp(key, "convDiff") || !strcmp(key, "savefreqAvg") || !strcmp(key, "savefreqEZ")
+    else if (!strcmp(key, "savefreqEZ")) c->savefreqEZ = strtod(val, NULL);
+    else if (!strcmp(key, "convDiff") || !strcmp(key, "savefreqAvg")
              || !strcmp(key, "savefreqUMom") || !strcmp(key, "savefreqVMom") || !strcmp(key, "savefreqThic")
              || !strcmp(key, "savefreqEnergy") || !strcmp(key, "savefreqTracer")) {
       if (strtod(val, NULL) != 0) flag_unsupported(c, key);

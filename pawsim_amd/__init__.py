@@ -28,7 +28,8 @@ PG_OK, PG_EARG, PG_EHIP, PG_ENOCONV, PG_ENODEV = 0, -1, -2, -3, -4
 ABI_SYMBOLS = (
     "pg_create", "pg_destroy", "pg_last_error", "pg_upload_state", "pg_download_state",
     "pg_upload_pi", "pg_download_pi", "pg_tendency", "pg_step", "pg_pressure_correct",
-    "pg_solve_pressure", "pg_refresh_halos", "pg_time", "pg_set_stream", "pg_synchronize", "pg_bench_kernel",
+    "pg_solve_pressure", "pg_refresh_halos", "pg_energy", "pg_time", "pg_set_stream", "pg_synchronize",
+    "pg_bench_kernel",
     "pg_net_param_count", "pg_net_create", "pg_net_destroy", "pg_net_set_params", "pg_net_get_params",
     "pg_net_forward", "pg_net_loss_grad", "pg_net_adam_step",
     "pg_learn_setup", "pg_learn_capture", "pg_learn_samples", "pg_learn_train", "pg_learn_use",
@@ -110,6 +111,7 @@ def load_library() -> ctypes.CDLL:
     lib.pg_pressure_correct.argtypes = [ctypes.c_void_p, P(PgSolveStats)]
     lib.pg_solve_pressure.argtypes = [ctypes.c_void_p, ctypes.c_uint32, d, d, P(PgSolveStats)]
     lib.pg_refresh_halos.argtypes = [ctypes.c_void_p]
+    lib.pg_energy.argtypes = [ctypes.c_void_p, d, d, d]
     lib.pg_time.argtypes = [ctypes.c_void_p]
     lib.pg_time.restype = ctypes.c_double
     lib.pg_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -257,6 +259,14 @@ class Model:
         self._check(self.lib.pg_solve_pressure(self.h, solver, _dptr(pi), _dptr(rhs), ctypes.byref(st)),
                     "pg_solve_pressure", allow_noconv=True)
         return pi, st.as_dict()
+
+    def energy(self):
+        """(KE, PE, Z[Nlay]) of the current state (one EZdiags.dat line)."""
+        ke = (ctypes.c_double * 1)()
+        pe = (ctypes.c_double * 1)()
+        z = np.zeros(self.shape[0])
+        self._check(self.lib.pg_energy(self.h, ke, pe, z.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "pg_energy")
+        return float(ke[0]), float(pe[0]), z
 
     def refresh_halos(self) -> None:
         self._check(self.lib.pg_refresh_halos(self.h), "pg_refresh_halos")

@@ -26,6 +26,7 @@ typedef struct drv_config {
   pg_config pg;
   unsigned Nt, tauNrecs;
   double tmin, tmax, savefrequency, hsml, hbbl, A2smag, A4smag, linDragSurf, quadDragSurf;
+  double savefreqEZ;
   bool useRL, writeInit, useMG, solverGiven;
   char unsupported[128];
   char dir[512];
@@ -146,7 +147,9 @@ static bool read_config(const char *fname, drv_config *c)
                || !strcmp(key, "eTimeFile") || !strcmp(key, "diaViscUFile") || !strcmp(key, "diaViscVFile")
                || !strcmp(key, "bFluxFile") || !strcmp(key, "diaDiffFile")) {
       note_unsupported(c, key);
-    } else if (!strcmp(key, "convDiff") || !strcmp(key, "savefreqAvg") || !strcmp(key, "savefreqEZ")
+    } else if (!strcmp(key, "savefreqEZ")) {
+      c->savefreqEZ = dv;
+    } else if (!strcmp(key, "convDiff") || !strcmp(key, "savefreqAvg")
                || !strcmp(key, "savefreqUMom") || !strcmp(key, "savefreqVMom") || !strcmp(key, "savefreqThic")
                || !strcmp(key, "savefreqEnergy") || !strcmp(key, "savefreqTracer")) {
       if (dv != 0) note_unsupported(c, key);
@@ -214,7 +217,34 @@ typedef struct model {
   double *u, *v, *hh, *pi, *zero, t;
   unsigned n_saves, solves, iters_total;
   double solve_ms;
+  unsigned n_EZ;
+  double t_next_EZ;
 } model;
+
+/* pawsim_diagnostics_write_EZ_if_due (diagnostics.c:483-703): one line per
+ * due time -- the time, KE, PE, KE + PE and the enstrophy of every layer. */
+static bool write_EZ_if_due(model *M, const char *dir)
+{
+  if (M->c.savefreqEZ <= 0) return true;
+  while (M->t >= M->t_next_EZ) {
+    const unsigned L = M->c.pg.Nlay;
+    double KE, PE, Z[16];
+    char path[1024];
+    FILE *f;
+    if (pg_energy(M->h, &KE, &PE, Z) != PG_OK) return false;
+    snprintf(path, sizeof(path), "%s/EZdiags.dat", dir);
+    f = fopen(path, (M->n_EZ > 1) ? "a" : "w");
+    if (!f) return false;
+    fprintf(f, "%le", M->t_next_EZ);
+    fprintf(f, " %.50e %.50e %.50e ", KE, PE, KE + PE);
+    for (unsigned k = 0; k < L; k++) fprintf(f, "%.50e ", Z[k]);
+    fprintf(f, "\n");
+    fclose(f);
+    M->n_EZ++;
+    M->t_next_EZ = M->n_EZ * M->c.savefreqEZ;
+  }
+  return true;
+}
 
 /* pawsim_diagnostics_write_model_state (diagnostics.c:874-908). */
 static bool write_state(model *M, const char *dir, unsigned n)
@@ -362,6 +392,10 @@ int main(int argc, char **argv)
     return 1;
   }
   M.n_saves = 1;
+  if (c->savefreqEZ > 0) { /* pawsim.c:434-438 */
+    M.n_EZ = (unsigned)round(c->tmin / c->savefreqEZ) + 1;
+    M.t_next_EZ = c->tmin + c->savefreqEZ;
+  }
   for (unsigned n = 1; n <= c->Nt; n++) {
     double ts = now_s();
     if (pg_step(M.h, n) != PG_OK) {
@@ -386,6 +420,10 @@ int main(int argc, char **argv)
         }
         M.n_saves++;
       }
+    }
+    if (!write_EZ_if_due(&M, argv[2])) {
+      fprintf(stderr, "ERROR: PAWSIM diagnostic averaging failed at n=%u\n", n);
+      return 1;
     }
   }
   printf("GPU timing: steps=%u loop_seconds=%0.6f solve_ms_total=%0.3f total_seconds=%0.6f mean_iters=%0.2f\n",

@@ -155,6 +155,7 @@ int run_solver(pg_handle* h, uint32_t solver, pg_solve_stats* st)
 }  // namespace
 
 int pg_ensure_solver(pg_handle* h, uint32_t solver) { return ensure_solver(h, solver); }
+hipError_t pg_ensure_halos(pg_handle* h) { return ensure_halos(h); }
 
 extern "C" {
 
@@ -309,6 +310,7 @@ void pg_destroy(pg_handle* h)
     (void)hipFree(h->fh32[q]);
   }
   (void)hipFree(h->ctl);
+  (void)hipFree(h->diag);
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
   if (h->mapped_host) (void)hipHostFree(h->mapped_host);
   if (h->ev0) (void)hipEventDestroy(h->ev0);

@@ -1,0 +1,132 @@
+// pg_diag.hip -- energy and potential-enstrophy diagnostics on the device
+// (pawsim_diagnostics_write_EZ_if_due, diagnostics.c:483-703; AL81
+// thickness scheme, no active buoyancy).
+//
+// Per owned cell and layer (FP64, the reference's expressions):
+//   h_q, zeta: 4-point average and relative vorticity, or the wall-row forms
+//              (2-point h_q, zeta = 0) on the first row / column of a wall;
+//   Z_k  += dx dy / 2 h_q q^2,  q = (2 Omega_z + zeta) / h_q;
+//   KE   += dx dy / 8 [(h + h_E) u_E^2 + (h_W + h) u^2 + (h + h_N) v_N^2 + (h_S + h) v^2];
+//   PE   += dx dy g'_k h (eta_k - h / 2) + dx dy g'_k h0^4 / h^2 / 6,
+//           eta_k = hhb + sum_{m >= k} h_m.
+// The sums are block partials added in a fixed order (deterministic; the
+// order differs from the reference's serial loop, so totals agree to
+// rounding, not bitwise).
+#include "pg_internal.h"
+
+#include <vector>
+
+namespace {
+
+constexpr int EJ = 64, EI = 4;
+
+__global__ __launch_bounds__(EJ* EI) void k_energy(PgGrid G, PgPhys P, const double* u, const double* v, const double* hh,
+                                                   const double* hhb, const double* omz, double* part)
+{
+  const int k = blockIdx.z;
+  const int gj = blockIdx.x * EJ + threadIdx.x, gi = blockIdx.y * EI + threadIdx.y;
+  double ke = 0, pe = 0, z = 0;
+  if (gi < G.nx && gj < G.ny) {
+    const int i = gi + G.g, j = gj + G.g;
+    const size_t lay = pg_lay(G, k);
+    auto H = [&](int a, int b) { return hh[lay + pg_at(G, a, b)]; };
+    auto U = [&](int a, int b) { return u[lay + pg_at(G, a, b)]; };
+    auto V = [&](int a, int b) { return v[lay + pg_at(G, a, b)]; };
+    const double dx = G.dx, dy = G.dy;
+    double h_q, zeta;
+    if (G.wally && gj == 0) {
+      h_q = 0.5 * (H(i - 1, j) + H(i, j));
+      zeta = 0;
+    } else if (G.wallx && gi == 0) {
+      h_q = 0.5 * (H(i, j - 1) + H(i, j));
+      zeta = 0;
+    } else {
+      h_q = 0.25 * (H(i, j) + H(i - 1, j - 1) + H(i - 1, j) + H(i, j - 1));
+      zeta = (U(i, j - 1) - U(i, j)) / dy + (V(i, j) - V(i - 1, j)) / dx;
+    }
+    if (h_q != 0) {
+      const double pv = (2 * omz[pg_at(G, i, j)] + zeta) / h_q;
+      z = dx * dy * 0.5 * h_q * (pv * pv);
+    }
+    ke = 0.125 * dx * dy
+         * ((H(i, j) + H(i + 1, j)) * (U(i + 1, j) * U(i + 1, j)) + (H(i - 1, j) + H(i, j)) * (U(i, j) * U(i, j))
+            + (H(i, j) + H(i, j + 1)) * (V(i, j + 1) * V(i, j + 1)) + (H(i, j - 1) + H(i, j)) * (V(i, j) * V(i, j)));
+    // eta_k bottom-up (work.c:1173-1203)
+    const size_t p = pg_at(G, i, j);
+    double eta = hhb[p];
+    for (int m = G.L - 1; m >= k; m--) eta = eta + hh[pg_lay(G, m) + p];
+    const double gtot = P.geff[k], hk = H(i, j);
+    pe = dx * dy * gtot * hk * (eta - 0.5 * hk);
+    pe = pe + dx * dy * gtot * (P.h0 * P.h0 * P.h0 * P.h0) / (hk * hk) / 6;
+  }
+  __shared__ double red[3][EJ * EI / 64];
+  const int tid = threadIdx.x + threadIdx.y * EJ;
+  for (int o = 32; o > 0; o >>= 1) {
+    ke += __shfl_xor(ke, o, 64);
+    pe += __shfl_xor(pe, o, 64);
+    z += __shfl_xor(z, o, 64);
+  }
+  if ((tid & 63) == 0) {
+    red[0][tid >> 6] = ke;
+    red[1][tid >> 6] = pe;
+    red[2][tid >> 6] = z;
+  }
+  __syncthreads();
+  if (tid < 3) {
+    double s = 0;
+    for (int q = 0; q < EJ * EI / 64; q++) s += red[tid][q];
+    const size_t nb = (size_t)gridDim.x * gridDim.y;
+    part[((size_t)k * nb + blockIdx.x + (size_t)blockIdx.y * gridDim.x) * 3 + tid] = s;
+  }
+}
+
+// out[0] = KE, out[1] = PE, out[2 + k] = Z_k; layer sums first, then over layers
+__global__ __launch_bounds__(64) void k_energy_finish(const double* part, int nb, int L, double* out)
+{
+  double KE = 0, PE = 0;
+  for (int k = 0; k < L; k++) {
+    double s[3] = {0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += 64)
+      for (int c = 0; c < 3; c++) s[c] += part[((size_t)k * nb + b) * 3 + c];
+    for (int o = 32; o > 0; o >>= 1)
+      for (int c = 0; c < 3; c++) s[c] += __shfl_xor(s[c], o, 64);
+    KE += s[0];
+    PE += s[1];
+    if (threadIdx.x == 0) out[2 + k] = s[2];
+  }
+  if (threadIdx.x == 0) {
+    out[0] = KE;
+    out[1] = PE;
+  }
+}
+
+}  // namespace
+
+extern "C" int pg_energy(pg_handle* h, double* KE, double* PE, double* Z)
+{
+  if (!h) return PG_EARG;
+  const PgGrid& G = h->G;
+  const dim3 grid((G.ny + EJ - 1) / EJ, (G.nx + EI - 1) / EI, G.L);
+  const int nb = (int)(grid.x * grid.y);
+  const size_t need = (size_t)nb * G.L * 3 + 2 + G.L;
+  if (h->ndiag < need) {
+    (void)hipFree(h->diag);
+    h->diag = nullptr;
+    h->ndiag = 0;
+    PG_CHECK(h, hipMalloc(&h->diag, need * sizeof(double)));
+    h->ndiag = need;
+  }
+  PG_CHECK(h, pg_ensure_halos(h));
+  double* out = h->diag + (size_t)nb * G.L * 3;
+  hipLaunchKernelGGL(k_energy, grid, dim3(EJ, EI), 0, h->stream, G, h->P, h->u, h->v, h->h, h->hhb, h->omz, h->diag);
+  hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(64), 0, h->stream, h->diag, nb, G.L, out);
+  PG_CHECK(h, hipGetLastError());
+  std::vector<double> res(2 + G.L);
+  PG_CHECK(h, hipMemcpyAsync(res.data(), out, res.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  if (KE) *KE = res[0];
+  if (PE) *PE = res[1];
+  if (Z)
+    for (int k = 0; k < G.L; k++) Z[k] = res[2 + k];
+  return PG_OK;
+}

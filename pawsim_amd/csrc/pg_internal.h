@@ -67,6 +67,8 @@ struct pg_handle {
   bool mg_built = false;
   struct PgLearn* learn = nullptr;     // learned preconditioner (pg_learn.hip), optional
   int mg_pred = 0;                     // V-cycles the previous GPU-MG solve took
+  double* diag = nullptr;              // diagnostics reduction scratch
+  size_t ndiag = 0;
   int fixed_cycles = 0;                // > 0: GPU-MG solves run exactly this many cycles with no host reads
   long mg_tail_cells = 1024;           // largest level the LDS coarse tail starts from
   void* tail_img[2] = {};              // packed LDS image of the coarse-tail coefficients (FP32, FP64)
@@ -90,6 +92,7 @@ struct pg_handle {
 hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind);
 hipError_t pg_fill1(pg_handle* h, double* f);
 hipError_t pg_fill_prognostic(pg_handle* h);  // u, v, h halos in one launch
+hipError_t pg_ensure_halos(pg_handle* h);     // refresh u, v, h halos if a refresh was deferred
 hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh);
 hipError_t pg_launch_tendency_slot(pg_handle* h, int s);  // into history slot s, per precision mode
 hipError_t pg_widen_slot(pg_handle* h, int s);            // FP32 history slot -> FP64 slot

@@ -757,13 +757,13 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
   const dim3 dg((top.Ny + DJ - 1) / DJ, (top.Nx + DI * DROWS - 1) / (DI * DROWS));
   const int nbd = (int)(dg.x * dg.y);
   auto defect = [&](const double* pcur, const float* e, double* pout, float* r32) {
-    pg_launch_defect(h, pcur, e, nullptr, pout, r32, &h->ctl->sumsq);
+    return pg_launch_defect(h, pcur, e, nullptr, pout, r32, &h->ctl->sumsq);
   };
   (void)nbd;
   PG_CHECK(h, pg_gather_to_mg(h));
   hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, h->stream, top.rhs, N, h->partials);
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, &h->ctl->rhs_sumsq, h->mapped + 1);
-  defect(top.pi, nullptr, nullptr, mixed ? top.rhs32 : nullptr);
+  PG_CHECK(h, defect(top.pi, nullptr, nullptr, mixed ? top.rhs32 : nullptr));
   PG_CHECK(h, hipGetLastError());
   const bool capture = pg_learn_capturing(h);
   if (capture) pg_learn_begin_capture(h);
@@ -797,12 +797,12 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
     if (mixed) {
       bool fused_defect = false;
       PG_CHECK(h, fused_vcycle<float>(h, nu1, nu2, 1, 2, &fused_defect, cur, alt));
-      if (!fused_defect) defect(cur, top.pi32, alt, top.rhs32);
+      if (!fused_defect) PG_CHECK(h, defect(cur, top.pi32, alt, top.rhs32));
       std::swap(cur, alt);
     } else {
       bool norm_done = false;
       PG_CHECK(h, fused_vcycle<double>(h, nu1, nu2, 0, 1, &norm_done));
-      if (!norm_done) defect(top.pi, nullptr, nullptr, nullptr);
+      if (!norm_done) PG_CHECK(h, defect(top.pi, nullptr, nullptr, nullptr));
     }
     PG_CHECK(h, hipGetLastError());
     it++;

@@ -43,6 +43,26 @@ def parse_iters(stdout):
     return [int(l.split("iters=")[1].split()[0]) for l in stdout.splitlines() if "iters=" in l and "phase=" in l]
 
 
+def read_ez(path):
+    """EZdiags.dat rows: time, KE, PE, KE + PE, Z per layer."""
+    with open(path) as f:
+        return np.array([[float(x) for x in line.split()] for line in f if line.strip()])
+
+
+def ez_close(got, ref, rtol=1e-12):
+    """Same rows and times; energies within rtol of the largest |value| of their column.
+
+    The device sums the cells as block partials in a fixed order, the
+    reference in one serial loop, so the totals agree to rounding.
+    """
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    if got.shape != ref.shape or not same_bits(got[:, 0], ref[:, 0]):
+        return False
+    scale = np.max(np.abs(ref[:, 1:]), axis=0)
+    return bool(np.all(np.abs(got[:, 1:] - ref[:, 1:]) <= rtol * np.maximum(scale, 1e-300)))
+
+
 def same_bits(a, b):
     """Bitwise equality, treating +0 and -0 as equal."""
     a = np.asarray(a)

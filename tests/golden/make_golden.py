@@ -53,6 +53,9 @@ def cases():
         "box16_ab1_hk83": inputs.gyre(nx=16, ny=16, nt=6, timeSteppingScheme=3, momentumScheme=1, **T),
         "box16_s75e": inputs.gyre(nx=16, ny=16, nt=6, momentumScheme=2, A2=50.0, **T),
         "wallx16_pery_a4": inputs.periodic_box(nx=16, ny=16, nt=6, A4=1e9, useWallEW=1, **T),
+        # energy / enstrophy diagnostics (EZdiags.dat)
+        "box24_ez": inputs.gyre(nx=24, ny=24, nt=8, A2=50.0, h0=20.0, savefreqEZ=600.0, **T),
+        "chan32x16_ez": inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1, savefreqEZ=200.0, **T),
     }
 
 
@@ -93,6 +96,10 @@ def make_case(name: str, setup: inputs.Setup, outdir: str) -> str:
         for f in INIT_FIELDS:
             arrays[f"init_{f}"] = np.stack([
                 inputs.read_2d(os.path.join(res, f"init_{f}{k}_n=0.dat"), nx, ny) for k in range(nl)])
+        ez = os.path.join(res, "EZdiags.dat")
+        if os.path.exists(ez):
+            with open(ez) as f:
+                arrays["ezdiags"] = np.array([[float(x) for x in line.split()] for line in f if line.strip()])
         meta = {"name": name, "params": setup.params, "notes": setup.notes}
         arrays["meta"] = np.array(json.dumps(meta, default=str))
         path = os.path.join(outdir, f"{name}.npz")
@@ -111,9 +118,13 @@ def load_setup(npz) -> inputs.Setup:
 
 
 def main() -> None:
+    """Regenerate every fixture, or only the cases named on the command line."""
     if not os.path.exists(REF):
         sys.exit("build the reference first: make -C oracle ref")
+    only = set(sys.argv[1:])
     for name, setup in cases().items():
+        if only and name not in only:
+            continue
         p = make_case(name, setup, HERE)
         print(f"{p}: {os.path.getsize(p)} bytes")
 

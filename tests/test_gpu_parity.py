@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import pawsim_amd
-from conftest import golden_ids, load_golden, parse_iters, run_exe, same_bits
+from conftest import ez_close, golden_ids, load_golden, parse_iters, read_ez, run_exe, same_bits
 from pawsim_amd import inputs
 
 pytestmark = pytest.mark.gpu
@@ -30,22 +30,25 @@ def test_gpu_matches_reference_golden(name, tmp_path):
             ref = z[f"{key}_n{n}"]
             assert same_bits(arr, ref), f"{key} at save {n}: max|diff|={np.max(np.abs(arr - ref)):.3e}"
     assert parse_iters(stdout) == [int(x) for x in z["solves"][:, 1]]
+    if "ezdiags" in z.files:  # energy / enstrophy: same times, sums to rounding
+        assert ez_close(read_ez(os.path.join(out, "EZdiags.dat")), z["ezdiags"])
 
 
 TRAJECTORY_CASES = {
-    "gyre128_sor": dict(kind="gyre", nx=128, ny=128, nt=12, solver=0),
+    "gyre128_sor": dict(kind="gyre", nx=128, ny=128, nt=12, solver=0, savefreqEZ=900.0),
     "gyre64_mg": dict(kind="gyre", nx=64, ny=64, nt=12, solver=1, A2=50.0, quadDragCoeff=2e-3),
     "chan64x32_mg": dict(kind="channel", nx=64, ny=32, nt=12, dx=20e3, solver=1, linDragCoeff=1e-3),
     "chan48x24_sor_quad": dict(kind="channel", nx=48, ny=24, nt=10, dx=20e3, solver=0, SOR_rp=1.8, quadDragCoeff=2e-3),
-    "per40x24_mg": dict(kind="periodic_box", nx=40, ny=24, nt=10, solver=1),
+    "per40x24_mg": dict(kind="periodic_box", nx=40, ny=24, nt=10, solver=1, savefreqEZ=600.0),
     # tiled interior kernel across several tiles, every momentum scheme
     "jets136x72_tw81_a2": dict(kind="jets", nx=136, ny=72, nlay=3, dx=20e3, nt=6, solver=1, A2=200.0,
                                H_rest=(500.0, 1500.0, 2000.0),
-                               quadDragCoeff=2e-3, momentumScheme=3),
+                               quadDragCoeff=2e-3, momentumScheme=3, savefreqEZ=300.0),
     "jets80x136_hk83": dict(kind="jets", nx=80, ny=136, nlay=2, dx=20e3, nt=6, solver=1, momentumScheme=1,
                             H_rest=(1000.0, 3000.0),
                             h0=50.0, linDragCoeff=1e-3),
-    "chan72x40_s75e": dict(kind="channel", nx=72, ny=40, nt=6, dx=20e3, solver=1, momentumScheme=2, A2=100.0),
+    "chan72x40_s75e": dict(kind="channel", nx=72, ny=40, nt=6, dx=20e3, solver=1, momentumScheme=2, A2=100.0,
+                           savefreqEZ=400.0),
     # per-point kernel everywhere
     "jets96x64_pointkernel": dict(kind="jets", nx=96, ny=64, nlay=3, dx=20e3, nt=6, solver=1, A2=200.0,
                                   H_rest=(500.0, 1500.0, 2000.0), quadDragCoeff=2e-3, tendency_kernel=1),
@@ -69,6 +72,32 @@ def test_gpu_matches_oracle_every_step(name, tmp_path, oracle_exe):
             d = np.max(np.abs(g[key] - o[key]))
             assert same_bits(g[key], o[key]), f"first mismatch: {key} at step {n}, max|diff|={d:.3e}"
     assert parse_iters(gout) == parse_iters(oout)
+    if "savefreqEZ" in spec:
+        ez_g = read_ez(str(tmp_path / "gpu" / "EZdiags.dat"))
+        ez_o = read_ez(str(tmp_path / "ora" / "EZdiags.dat"))
+        assert len(ez_o) >= 2
+        assert ez_close(ez_g, ez_o)
+
+
+def test_model_energy_matches_oracle(tmp_path, oracle_exe):
+    """Model.energy() after each step against the oracle's EZdiags.dat rows."""
+    setup = inputs.jets(nx=72, ny=56, nlay=3, dx=20e3, nt=5, solver=1, A2=200.0, H_rest=(500.0, 1500.0, 2000.0),
+                        savefreqEZ=300.0, name="ez_model")
+    infile = setup.write(str(tmp_path / "in"))
+    run_exe(oracle_exe, infile, str(tmp_path / "ora"), timeout=300)
+    ref = read_ez(str(tmp_path / "ora" / "EZdiags.dat"))
+    assert ref.shape == (5, 4 + setup.nlay)
+    m = pawsim_amd.Model.from_setup(setup)
+    m.pressure_correct()
+    m.refresh_halos()
+    rows = []
+    for n in range(1, 6):
+        m.step(n)
+        m.pressure_correct()
+        ke, pe, z = m.energy()
+        rows.append([n * 300.0, ke, pe, ke + pe] + list(z))
+    m.close()
+    assert ez_close(np.array(rows), ref)
 
 
 MIXED_CASES = {

@@ -8,14 +8,15 @@ The oracle must reproduce every array bit for bit and every iteration count
 exactly.  The cases cover SOR / gathered MG / full MG / general (non 2:1)
 transfers, walls / periodic / mixed boundaries, odd grids, A2 and A4
 viscosity, linear and quadratic drag, h0, AB1/AB2/AB3 and all four
-momentum schemes.
+momentum schemes; two cases also pin the energy / enstrophy diagnostics
+(EZdiags.dat, diagnostics.c:483-703).
 """
 import os
 
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, golden_ids, load_golden, parse_iters, run_exe, same_bits
+from conftest import GOLDEN, golden_ids, load_golden, parse_iters, read_ez, run_exe, same_bits
 from pawsim_amd import inputs
 
 
@@ -33,6 +34,10 @@ def test_oracle_matches_reference(name, oracle_exe, tmp_path):
     for f in ("dudt", "dvdt", "dhdt", "qq", "KE", "MM", "hwest", "alpha", "mu"):
         got = np.stack([inputs.read_2d(os.path.join(out, f"init_{f}{k}_n=0.dat"), nx, ny) for k in range(nl)])
         assert same_bits(got, z[f"init_{f}"]), f"init_{f}"
+    if "ezdiags" in z.files:  # energy / enstrophy diagnostics, bit for bit
+        got = read_ez(os.path.join(out, "EZdiags.dat"))
+        assert got.shape == z["ezdiags"].shape
+        assert same_bits(got, z["ezdiags"])
     solves = z["solves"]
     assert parse_iters(stdout) == [int(x) for x in solves[:, 1]]
     # norms are printed with 7 significant digits by both programs
