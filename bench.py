@@ -56,6 +56,8 @@ def parse():
     p.add_argument("--tol", type=float, default=1e-8)
     p.add_argument("--mg-nu1", type=int, default=2, help="pre-smoothing sweeps of the GPU multigrid")
     p.add_argument("--mg-nu2", type=int, default=2, help="post-smoothing sweeps")
+    p.add_argument("--pi-guess", type=int, default=0,
+                   help="initial guess of each solve: 0 previous pi, 1/2 linear/quadratic extrapolation")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-steps", type=int, default=12)
     p.add_argument("--pmc-probe", action="store_true",
@@ -78,7 +80,8 @@ def make_model(a, mixed=1, solver=None, tol=None):
 
     setup = inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=a.steps + a.warmup,
                         solver=a.solver if solver is None else solver, tol=a.tol if tol is None else tol,
-                        A2=10.0, maxiters=200, mixed_precision=mixed, mg_nu1=a.mg_nu1, mg_nu2=a.mg_nu2)
+                        A2=10.0, maxiters=200, mixed_precision=mixed, mg_nu1=a.mg_nu1, mg_nu2=a.mg_nu2,
+                        pi_guess=a.pi_guess)
     return setup, pawsim_amd.Model.from_setup(setup)
 
 
@@ -394,7 +397,7 @@ def main():
         "config": {"workload": f"zonal jets {a.nx}x{a.ny}x{a.nlay}, walls, dx 5 km, AB3 dt 300 s, rigid lid; "
                                f"FP32 stencils and V-cycles, FP64 outer residual, coarse solve and norms, "
                                f"iterative refinement to rel. residual {a.tol:g}",
-                   "solver": a.solver, "tol": a.tol},
+                   "solver": a.solver, "tol": a.tol, "pi_guess": a.pi_guess},
         "error_vs_fp64": {"after_steps": a.warmup + a.steps, "max_abs_diff_over_max_abs": err},
         "fp64_path": {"steps_per_s": a.steps / r64["elapsed"], "ms_per_step": 1e3 * r64["elapsed"] / a.steps,
                       "solver": "FP64 GPU multigrid (plain MG)", "solve_ms_mean": r64["solve_ms"],

@@ -65,6 +65,9 @@ typedef struct pg_config {
   int32_t mixed_precision;     /* 0: FP64 throughout (bitwise with the reference);
                                   1: FP32 interior stencils and tendency history, FP64 state, AB update,
                                   pressure correction, norms (requires A4 = 0, Nlay <= 8, Nx, Ny >= 8) */
+  uint32_t pi_guess;           /* initial guess of the GPU-MG solves (3, 4) inside pg_pressure_correct:
+                                  0 the previous pi (as the reference), 1 linear extrapolation of the
+                                  last two solutions, 2 quadratic extrapolation of the last three */
 } pg_config;
 
 /* Static inputs (state.c:1012-1047).  NULL hhs means a flat lid at 0;

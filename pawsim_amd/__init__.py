@@ -49,7 +49,7 @@ class PgConfig(ctypes.Structure):
         ("pressureSolver", ctypes.c_uint32), ("pi_tol", ctypes.c_double), ("SOR_rp", ctypes.c_double),
         ("omega_MG", ctypes.c_double), ("maxiters", ctypes.c_uint32), ("use_fullMG", ctypes.c_int32),
         ("pressureTiming", ctypes.c_int32), ("mg_nu1", ctypes.c_uint32), ("mg_nu2", ctypes.c_uint32),
-        ("tendency_kernel", ctypes.c_int32), ("mixed_precision", ctypes.c_int32),
+        ("tendency_kernel", ctypes.c_int32), ("mixed_precision", ctypes.c_int32), ("pi_guess", ctypes.c_uint32),
     ]
 
 
@@ -194,6 +194,7 @@ class Model:
         c.mg_nu2 = int(p.get("mg_nu2", 0))
         c.tendency_kernel = int(p.get("tendency_kernel", 0))
         c.mixed_precision = int(p.get("mixed_precision", 0))
+        c.pi_guess = int(p.get("pi_guess", 0))
         self.cfg = c
         self.shape = (c.Nlay, c.Nx, c.Ny)
         self._keep = [_c64(hhb), _c64(hhs), _c64(omegaz), _c64(gg), _c64(taux), _c64(tauy)]

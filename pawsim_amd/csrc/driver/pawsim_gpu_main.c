@@ -102,6 +102,7 @@ static bool read_config(const char *fname, drv_config *c)
     else if (!strcmp(key, "mg_nu2")) c->pg.mg_nu2 = (uint32_t)uv;
     else if (!strcmp(key, "tendency_kernel")) c->pg.tendency_kernel = (int32_t)iv;
     else if (!strcmp(key, "mixed_precision")) c->pg.mixed_precision = (int32_t)iv;
+    else if (!strcmp(key, "pi_guess")) c->pg.pi_guess = (uint32_t)uv;
     else if (!strcmp(key, "Lx")) c->pg.Lx = dv;
     else if (!strcmp(key, "Ly")) c->pg.Ly = dv;
     else if (!strcmp(key, "dt")) c->pg.dt = dv;

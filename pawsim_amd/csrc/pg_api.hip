@@ -91,6 +91,9 @@ int validate(const pg_config* c, std::string& why)
     return why = "unknown pressureSolver", PG_EARG;
   if (c->tendency_kernel != 0 && c->tendency_kernel != 1) return why = "tendency_kernel must be 0 or 1", PG_EARG;
   if (c->mixed_precision != 0 && c->mixed_precision != 1) return why = "mixed_precision must be 0 or 1", PG_EARG;
+  if (c->pi_guess > 2) return why = "pi_guess must be 0, 1 or 2", PG_EARG;
+  if (c->pi_guess && c->pressureSolver != PG_SOLVER_GPU_MG && c->pressureSolver != PG_SOLVER_GPU_MG_MIXED)
+    return why = "pi_guess needs a GPU multigrid solver (3 or 4)", PG_EARG;
   if (c->mixed_precision && (c->A4 > 0 || c->Nlay > 8 || c->Nx < 8 || c->Ny < 8 || c->tendency_kernel != 0))
     return why = "mixed_precision needs A4 = 0, Nlay <= 8, Nx and Ny >= 8 and the tiled tendency kernel", PG_EARG;
   return PG_OK;
@@ -234,6 +237,9 @@ static int create_body(const pg_config* cfg, const pg_static* st, pg_handle** ou
   *out = h;
   PG_CHECK(h, hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking));
   h->stream = h->own_stream;
+  PG_CHECK(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+  PG_CHECK(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+  PG_CHECK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
   PG_CHECK(h, hipEventCreate(&h->ev0));
   PG_CHECK(h, hipEventCreate(&h->ev1));
   const size_t lf = (size_t)G.L * h->fsz * sizeof(double);
@@ -314,6 +320,9 @@ void pg_destroy(pg_handle* h)
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
   if (h->mapped_host) (void)hipHostFree(h->mapped_host);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->side) (void)hipStreamDestroy(h->side);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
@@ -356,6 +365,7 @@ int pg_upload_pi(pg_handle* h, const double* pi)
 {
   if (!h || !pi) return PG_EARG;
   std::vector<double> stage;
+  h->pi_solves = 0;  // pi no longer continues the timestep's solutions
   int rc = upload_owned(h, h->pi, pi, h->G.ny, stage);
   if (rc) return rc;
   PG_CHECK(h, pg_fill(h, h->pi, 1, PG_HALO_SCALAR));
@@ -437,7 +447,9 @@ int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
   PG_CHECK(h, pg_launch_sub_mean(h, h->rhs));
   // the GPU multigrid gathers the owned rhs only
   if (solver == PG_SOLVER_SOR || solver == PG_SOLVER_MG_REPLICA) PG_CHECK(h, pg_fill(h, h->rhs, 1, PG_HALO_SCALAR));
+  h->pi_guess_now = true;
   int rc = run_solver(h, solver, st);
+  h->pi_guess_now = false;
   if (rc && rc != PG_ENOCONV) return rc;
   if (h->cfg.pressureTiming) PG_CHECK(h, pg_launch_norms(h, 0));
   PG_CHECK(h, pg_launch_correct(h));
@@ -464,6 +476,7 @@ int pg_solve_pressure(pg_handle* h, uint32_t solver, double* pi, const double* r
       solver != PG_SOLVER_GPU_MG_MIXED)
     return fail(h, PG_EARG, "unknown solver");
   std::vector<double> stage;
+  h->pi_solves = 0;  // pi no longer continues the timestep's solutions
   int rc = upload_owned(h, h->pi, pi, h->G.ny, stage);
   if (!rc) rc = upload_owned(h, h->rhs, rhs, h->G.ny, stage);
   if (rc) return rc;

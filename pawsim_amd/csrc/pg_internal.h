@@ -35,6 +35,7 @@ struct PgCtl {
   double alpha_num, alpha_den, alpha;  // minimal-residual length of the learned correction
   double sumsq0;         // defect norm^2 before the learned correction
   double worst_rel2;     // largest (||r|| / ||rhs||)^2 after a fixed-cycle solve
+  double pisum;          // sum of the GPU-MG solution (its mean is removed)
   unsigned long long maxbits; // bit pattern of a non-negative double max
   int iters;
   int done;
@@ -46,6 +47,8 @@ struct pg_handle {
   PgPhys P;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;          // edge-ring tendency, concurrent with the interior kernel
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   size_t fsz = 0;                      // sx * sy
   double *u = nullptr, *v = nullptr, *h = nullptr;   // [L][fsz] state, halos filled
   double *fu[3] = {}, *fv[3] = {}, *fh[3] = {};      // AB history buffers
@@ -67,6 +70,9 @@ struct pg_handle {
   bool mg_built = false;
   struct PgLearn* learn = nullptr;     // learned preconditioner (pg_learn.hip), optional
   int mg_pred = 0;                     // V-cycles the previous GPU-MG solve took
+  double* pi_hist = nullptr;           // [2][Nx*Ny] earlier GPU-MG solutions (initial-guess extrapolation)
+  int pi_solves = 0;                   // timestep solves since the history was last reset
+  bool pi_guess_now = false;           // the running solve belongs to a timestep (extrapolation allowed)
   double* diag = nullptr;              // diagnostics reduction scratch
   size_t ndiag = 0;
   int fixed_cycles = 0;                // > 0: GPU-MG solves run exactly this many cycles with no host reads
@@ -124,7 +130,9 @@ hipError_t pg_gpu_mg_vcycle(pg_handle* h);
 hipError_t pg_gpu_mg_vcycle32(pg_handle* h);
 hipError_t pg_gpu_mg_sweep(pg_handle* h);
 hipError_t pg_gather_to_mg(pg_handle* h);
+hipError_t pg_gather_guess(pg_handle* h);  // pg_gather_to_mg + extrapolated initial guess (cfg.pi_guess)
 hipError_t pg_scatter_from_mg(pg_handle* h);
+hipError_t pg_scatter_zero_mean(pg_handle* h);  // scatter minus the solution's mean (GPU MG)
 hipError_t pg_launch_coarse_global(pg_handle* h, int top, int nu1, int nu2, bool fp32);
 
 // ---- learned preconditioner (pg_learn.hip) ----

@@ -21,6 +21,7 @@
 #include "pg_internal.h"
 #include "pg_mg_common.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace {
@@ -103,6 +104,24 @@ __global__ void k_gather(PgGrid G, const double* gf, double* f)
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
   if (j >= G.ny) return;
   f[(size_t)i * G.ny + j] = gf[pg_at(G, i + G.g, j + G.g)];
+}
+// Gather pi and rhs; the initial guess extrapolates the last solutions
+// (pi = pi_n, h1 = pi_{n-1}, h2 = pi_{n-2}): order 1 2 pi_n - pi_{n-1},
+// order 2 3 pi_n - 3 pi_{n-1} + pi_{n-2}; the history then shifts by one.
+__global__ void k_gather_guess(PgGrid G, const double* gpi, const double* grhs, double* pi, double* rhs, double* h1,
+                               double* h2, int order)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= G.ny) return;
+  const size_t q = (size_t)i * G.ny + j, s = pg_at(G, i + G.g, j + G.g);
+  const double p0 = gpi[s], p1 = h1[q], p2 = h2[q];
+  double g = p0;
+  if (order == 1) g = 2.0 * p0 - p1;
+  else if (order == 2) g = 3.0 * p0 - 3.0 * p1 + p2;
+  pi[q] = g;
+  rhs[q] = grhs[s];
+  h2[q] = p1;
+  h1[q] = p0;
 }
 __global__ void k_scatter(PgGrid G, const double* f, double* gf)
 {
@@ -517,6 +536,9 @@ void pg_mg_free(pg_handle* h)
   }
   h->lev.clear();
   h->mg_built = false;
+  (void)hipFree(h->pi_hist);
+  h->pi_hist = nullptr;
+  h->pi_solves = 0;
   for (int q = 0; q < 2; q++) {
     (void)hipFree(h->tail_img[q]);
     h->tail_img[q] = nullptr;
@@ -583,6 +605,26 @@ hipError_t pg_gather_to_mg(pg_handle* h)
   const PgGrid& G = h->G;
   hipLaunchKernelGGL(k_gather, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->pi, top.pi);
   hipLaunchKernelGGL(k_gather, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->rhs, top.rhs);
+  return hipGetLastError();
+}
+
+hipError_t pg_gather_guess(pg_handle* h)
+{
+  const int want = h->pi_guess_now ? (int)h->cfg.pi_guess : 0;
+  if (want == 0) return pg_gather_to_mg(h);
+  MgLevel& top = h->lev.back();
+  const PgGrid& G = h->G;
+  const size_t N = (size_t)G.nx * G.ny;
+  if (!h->pi_hist) {
+    hipError_t e = hipMalloc(&h->pi_hist, 2 * N * sizeof(double));
+    if (e != hipSuccess) return e;
+    h->pi_solves = 0;
+  }
+  // solve k of a run: pi holds solution k-1, the history k-2 and k-3
+  const int order = std::min(want, std::max(h->pi_solves - 1, 0));
+  hipLaunchKernelGGL(k_gather_guess, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->pi, h->rhs, top.pi,
+                     top.rhs, h->pi_hist, h->pi_hist + N, order);
+  h->pi_solves++;
   return hipGetLastError();
 }
 
@@ -679,7 +721,7 @@ int pg_gpu_mg_solve(pg_handle* h, int mixed, int* iters, double* rel)
   const dim3 gg = level_grid(top);
   const int nb = gg.x * gg.y;
   double* sums = &h->ctl->sumsq;
-  PG_CHECK(h, pg_gather_to_mg(h));
+  PG_CHECK(h, pg_gather_guess(h));
   // ||rhs||^2
   hipLaunchKernelGGL(k_sumsq_rhs, dim3(256), dim3(256), 0, h->stream, top, h->partials);
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, h->stream, 256, h->partials, &h->ctl->rhs_sumsq);
@@ -707,6 +749,6 @@ int pg_gpu_mg_solve(pg_handle* h, int mixed, int* iters, double* rel)
   }
   *iters = it;
   *rel = r;
-  PG_CHECK(h, pg_scatter_from_mg(h));
+  PG_CHECK(h, pg_scatter_zero_mean(h));
   return PG_OK;
 }

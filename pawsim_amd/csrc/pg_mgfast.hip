@@ -41,6 +41,19 @@ __device__ __forceinline__ int wrapm(int i, int N)
   return i < 0 ? i + N : i;
 }
 
+// i mod N for -N <= i < 2N without a division (other values: wrapm).
+__device__ __forceinline__ int wrapf(int i, int N)
+{
+  int r = i < 0 ? i + N : i;
+  r = r >= N ? r - N : r;
+  return ((unsigned)r < (unsigned)N) ? r : wrapm(i, N);
+}
+
+// floor(idx / n) for 0 <= idx < 2^20, n <= 2^10: the float quotient of
+// idx + 1/2 stays at least 1/(2n) away from an integer, far beyond the
+// reciprocal's rounding error.
+__device__ __forceinline__ int fdiv(int idx, float inv_n) { return (int)(((float)idx + 0.5f) * inv_n); }
+
 // Bilinear weights of the 2:1 prolongation at fine cell (i, j) with the wall
 // and corner overrides of multigrid.c:377-442 (same values as the stored
 // weight tables; all are exact in FP32).  Order: wmm, wmp, wpm, wpp.
@@ -94,9 +107,11 @@ __device__ __forceinline__ void rb_pass(T* spi, const T* srhs, const T* sOw, con
                                         int ahi, int blo, int bhi, int par, int tid)
 {
   const int rows = ahi - alo + 1, half = (bhi - blo + 2) / 2;
+  const float invh = 1.0f / (float)half;
   for (int idx = tid; idx < rows * half; idx += NTH) {
-    const int a = alo + idx / half;
-    const int b = blo + ((par + a + blo) & 1) + 2 * (idx % half);
+    const int q = fdiv(idx, invh);
+    const int a = alo + q;
+    const int b = blo + ((par + a + blo) & 1) + 2 * (idx - q * half);
     if (b > bhi) continue;
     const int s = a * P + b;
     const T r = srO[s];
@@ -161,59 +176,110 @@ struct TileLds {
   T pi[RMAX * P], rhs[RMAX * P], Ow[RMAX * P], Os[RMAX * P], rO[RMAX * P];
 };
 
-// Stage the tile with halo H: coefficients over (R+1) x (C+1) cells, pi and
-// rhs over R x C, R = TI + 2H, C = FJ + 2H.  pi is the level's `src` array
-// minus the prolonged coarse correction when `coarse` is given, or zero.
+// Window of the coarse level that the 2:1 prolongation onto a tile with
+// halo H reads: coarse rows cbi .. cbi + CRn - 1 (unwrapped), likewise
+// columns.  Fine cell fi interpolates from coarse rows floor((fi - 1) / 2)
+// and the next one (interpolate_power2_MG, multigrid.c:827-862).
 template <typename T>
-__device__ void stage_tile(TileLds<T>& S, const MgLevel& f, const T* src, const T* coarse, int CY, int H, int i0, int j0,
-                           int wallEW, int wallNS, int tid)
+struct CoarseWin {
+  static constexpr int RMAX = Tile<T>::TI / 2 + HMAX + 2, CMAX = FJ / 2 + HMAX + 2;
+  int cbi, cbj, CRn, CCn;
+  __device__ CoarseWin(int H, int i0, int j0)
+  {
+    const int R = Tile<T>::TI + 2 * H, C = FJ + 2 * H;
+    cbi = (i0 - H - 1) >> 1;  // arithmetic shift: floor for negative values
+    cbj = (j0 - H - 1) >> 1;
+    CRn = ((i0 - H + R - 2) >> 1) + 2 - cbi;
+    CCn = ((j0 - H + C - 2) >> 1) + 2 - cbj;
+  }
+};
+
+// Stage the tile with halo H: coefficients over (R+1) x (C+1) cells, pi and
+// rhs over R x C, R = TI + 2H, C = FJ + 2H, pi from the level's `src` array
+// (or zero, ZERO).  With `coarse`, the coarse window W goes to cwin as well
+// (k_up).  Every load is issued before the first LDS store, indices clamped
+// to the last cell and wrapped onto the torus, so the whole stage is one
+// round trip to memory; the stores are guarded.
+template <typename T, bool ZERO>
+__device__ void stage_tile(TileLds<T>& S, const MgLevel& f, const T* src, int H, int i0, int j0, int tid,
+                           T* cwin = nullptr, const CoarseWin<T>* W = nullptr, const T* coarse = nullptr, int CX = 0,
+                           int CY = 0)
 {
   constexpr int P = TileLds<T>::P;
-  constexpr int U = 8;  // cells per thread whose loads are in flight together
-  const int R = TileLds<T>::TI + 2 * H, C = FJ + 2 * H;
+  // fine / coarse cells per thread: one batch covers the largest tile
+  constexpr int U = ((Tile<T>::TI + 2 * HMAX + 1) * (FJ + 2 * HMAX + 1) + NT - 1) / NT, UC = 2;
+  static_assert(CoarseWin<T>::RMAX * CoarseWin<T>::CMAX <= UC * NT, "coarse stage is one batch");
+  const int R = TileLds<T>::TI + 2 * H, C = FJ + 2 * H, C1 = C + 1;
+  const float inv = 1.0f / (float)C1;
   const int X = f.Nx, Y = f.Ny;
   const T* gOw = LvPtr<T>::Ow(f);
   const T* gOs = LvPtr<T>::Os(f);
   const T* grO = LvPtr<T>::rOsum(f);
   const T* grhs = LvPtr<T>::rhs(f);
-  const int total = (R + 1) * (C + 1);
-  for (int base = tid; base < total; base += U * NT) {
-    T ow[U], os[U], pv[U], rh[U], ro[U];
-    int sl[U];
-    bool in[U];
+  const int total = (R + 1) * C1;
+  T cv[UC];
+  int ctot = 0;
+  if (coarse) {
+    ctot = W->CRn * W->CCn;
+    const float cinv = 1.0f / (float)W->CCn;
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int idx = base + u * NT;
-      sl[u] = -1;
-      in[u] = false;
-      if (idx < total) {
-        const int a = idx / (C + 1), b = idx - (idx / (C + 1)) * (C + 1);
-        const int gi = wrapm(i0 - H + a, X), gj = wrapm(j0 - H + b, Y);
-        const size_t p = (size_t)gi * Y + gj;
-        sl[u] = a * P + b;
-        ow[u] = gOw[p];
-        os[u] = gOs[p];
-        in[u] = a < R && b < C;
-        if (in[u]) {
-          T v = src ? src[p] : (T)0;
-          if (coarse) v -= interp2<T>(coarse, CY, gi, gj, X, Y, wallEW, wallNS);
-          pv[u] = v;
-          rh[u] = grhs[p];
-          ro[u] = grO[p];
-        }
+    for (int u = 0; u < UC; u++) {
+      const int idx = min(tid + u * NT, ctot - 1);
+      const int r = fdiv(idx, cinv), c = idx - r * W->CCn;
+      cv[u] = coarse[(unsigned)(wrapf(W->cbi + r, CX) * CY + wrapf(W->cbj + c, CY))];
+    }
+  }
+  T ow[U], os[U], pv[U], rh[U], ro[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int idx = min(tid + u * NT, total - 1);
+    const int a = fdiv(idx, inv), b = idx - a * C1;
+    const unsigned p = (unsigned)(wrapf(i0 - H + a, X) * Y + wrapf(j0 - H + b, Y));
+    ow[u] = gOw[p];
+    os[u] = gOs[p];
+    rh[u] = grhs[p];
+    ro[u] = grO[p];
+    pv[u] = ZERO ? (T)0 : src[p];
+  }
+  if (coarse) {
+#pragma unroll
+    for (int u = 0; u < UC; u++)
+      if (tid + u * NT < ctot) cwin[tid + u * NT] = cv[u];
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int idx = tid + u * NT;
+    if (idx < total) {
+      const int a = fdiv(idx, inv), b = idx - a * C1;
+      const int sl = a * P + b;
+      S.Ow[sl] = ow[u];
+      S.Os[sl] = os[u];
+      if (a < R && b < C) {
+        S.pi[sl] = pv[u];
+        S.rhs[sl] = rh[u];
+        S.rO[sl] = ro[u];
       }
     }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (sl[u] < 0) continue;
-      S.Ow[sl[u]] = ow[u];
-      S.Os[sl[u]] = os[u];
-      if (in[u]) {
-        S.pi[sl[u]] = pv[u];
-        S.rhs[sl[u]] = rh[u];
-        S.rO[sl[u]] = ro[u];
-      }
-    }
+  }
+}
+
+// pi -= prolonged coarse correction over the R x C cells of the tile, with
+// the expression and weights of interp2.
+template <typename T>
+__device__ void apply_coarse(TileLds<T>& S, const T* cwin, const CoarseWin<T>& W, int H, int i0, int j0, int X, int Y,
+                             int wallEW, int wallNS, int tid)
+{
+  constexpr int P = TileLds<T>::P;
+  const int R = Tile<T>::TI + 2 * H, C = FJ + 2 * H;
+  const float inv = 1.0f / (float)C;
+  for (int idx = tid; idx < R * C; idx += NT) {
+    const int a = fdiv(idx, inv), b = idx - a * C;
+    const int fi = i0 - H + a, fj = j0 - H + b;
+    T w[4];
+    pow2_weights<T>(wrapf(fi, X), wrapf(fj, Y), X, Y, wallEW, wallNS, w);
+    const T* c0 = cwin + (((fi - 1) >> 1) - W.cbi) * W.CCn + (((fj - 1) >> 1) - W.cbj);
+    const T corr = w[0] * c0[0] + w[2] * c0[W.CCn] + w[1] * c0[1] + w[3] * c0[W.CCn + 1];
+    S.pi[a * P + b] -= corr;
   }
 }
 
@@ -227,7 +293,8 @@ __global__ __launch_bounds__(NT) void k_down(MgLevel f, MgLevel c, int nu, int z
   const int tid = threadIdx.x;
   const int H = 2 * nu + 1, R = TI + 2 * H, C = FJ + 2 * H;
   const int i0 = blockIdx.y * TI, j0 = blockIdx.x * FJ;
-  stage_tile<T>(S, f, zero_init ? nullptr : LvPtr<T>::pi(f), nullptr, 0, H, i0, j0, 0, 0, tid);
+  if (zero_init) stage_tile<T, true>(S, f, nullptr, H, i0, j0, tid);
+  else stage_tile<T, false>(S, f, LvPtr<T>::pi(f), H, i0, j0, tid);
   __syncthreads();
   const int base = (i0 + j0) & 1;  // parity of LDS (0, 0): 2H is even
   for (int pass = 0; pass < 2 * nu; pass++) {
@@ -276,7 +343,11 @@ __global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wal
   const int tid = threadIdx.x;
   const int H = 2 * nu + (norm ? 1 : 0), R = TI + 2 * H, C = FJ + 2 * H;
   const int i0 = blockIdx.y * TI, j0 = blockIdx.x * FJ;
-  stage_tile<T>(S, f, LvPtr<T>::tmp(f), LvPtr<T>::pi(c), c.Ny, H, i0, j0, wallEW, wallNS, tid);
+  __shared__ T cwin[CoarseWin<T>::RMAX * CoarseWin<T>::CMAX];
+  const CoarseWin<T> W(H, i0, j0);
+  stage_tile<T, false>(S, f, LvPtr<T>::tmp(f), H, i0, j0, tid, cwin, &W, LvPtr<T>::pi(c), c.Nx, c.Ny);
+  __syncthreads();
+  apply_coarse<T>(S, cwin, W, H, i0, j0, f.Nx, f.Ny, wallEW, wallNS, tid);
   __syncthreads();
   const int base = (i0 + j0) & 1;
   for (int pass = 0; pass < 2 * nu; pass++) {
@@ -288,23 +359,53 @@ __global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wal
   T* out = LvPtr<T>::pi(f);
   const int X = f.Nx, Y = f.Ny;
   double ss = 0;
-  for (int idx = tid; idx < TI * FJ; idx += NT) {
-    const int gi = i0 + idx / FJ, gj = j0 + idx % FJ;
-    if (gi >= X || gj >= Y) continue;
-    const int s = (H + idx / FJ) * P + H + idx % FJ;
-    const size_t p = (size_t)gi * Y + gj;
-    if (norm == 2) {
-      const size_t pE = (size_t)ip1(gi, X) * Y + gj, pW = (size_t)im1(gi, X) * Y + gj;
-      const size_t pN = (size_t)gi * Y + ip1(gj, Y), pS = (size_t)gi * Y + im1(gj, Y);
-      const double v = pcur[p] - 1.0 * (double)S.pi[s], vE = pcur[pE] - 1.0 * (double)S.pi[s + P];
-      const double vW = pcur[pW] - 1.0 * (double)S.pi[s - P], vN = pcur[pN] - 1.0 * (double)S.pi[s + 1];
-      const double vS = pcur[pS] - 1.0 * (double)S.pi[s - 1];
-      pout[p] = v;
-      const double r = f.Os[pN] * vN + f.Os[p] * vS + f.Ow[pE] * vE + f.Ow[p] * vW - f.Osum[p] * v - f.rhs[p];
-      f.rhs32[p] = (float)r;
-      ss += r * r;
-    } else {
-      out[p] = S.pi[s];
+  if (norm == 2) {
+    // FP64 loads of V cells per thread issued together (clamped indices)
+    constexpr int V = 2;
+    static_assert((TI * FJ) % (V * NT) == 0, "whole batches");
+    for (int base = tid; base < TI * FJ; base += V * NT) {
+      double pc[V], pe[V], pw[V], pn[V], ps[V], osn[V], osp[V], owe[V], owp[V], osm[V], rh[V];
+#pragma unroll
+      for (int u = 0; u < V; u++) {
+        const int idx = base + u * NT;
+        const int gi = min(i0 + idx / FJ, X - 1), gj = min(j0 + idx % FJ, Y - 1);
+        const unsigned p = (unsigned)(gi * Y + gj);
+        const unsigned pE = (unsigned)(ip1(gi, X) * Y + gj), pW = (unsigned)(im1(gi, X) * Y + gj);
+        const unsigned pN = (unsigned)(gi * Y + ip1(gj, Y)), pS = (unsigned)(gi * Y + im1(gj, Y));
+        pc[u] = pcur[p];
+        pe[u] = pcur[pE];
+        pw[u] = pcur[pW];
+        pn[u] = pcur[pN];
+        ps[u] = pcur[pS];
+        osn[u] = f.Os[pN];
+        osp[u] = f.Os[p];
+        owe[u] = f.Ow[pE];
+        owp[u] = f.Ow[p];
+        osm[u] = f.Osum[p];
+        rh[u] = f.rhs[p];
+      }
+#pragma unroll
+      for (int u = 0; u < V; u++) {
+        const int idx = base + u * NT;
+        const int gi = i0 + idx / FJ, gj = j0 + idx % FJ;
+        if (gi >= X || gj >= Y) continue;
+        const int s = (H + idx / FJ) * P + H + idx % FJ;
+        const unsigned p = (unsigned)(gi * Y + gj);
+        const double v = pc[u] - 1.0 * (double)S.pi[s], vE = pe[u] - 1.0 * (double)S.pi[s + P];
+        const double vW = pw[u] - 1.0 * (double)S.pi[s - P], vN = pn[u] - 1.0 * (double)S.pi[s + 1];
+        const double vS = ps[u] - 1.0 * (double)S.pi[s - 1];
+        pout[p] = v;
+        const double r = osn[u] * vN + osp[u] * vS + owe[u] * vE + owp[u] * vW - osm[u] * v - rh[u];
+        f.rhs32[p] = (float)r;
+        ss += r * r;
+      }
+    }
+  } else {
+    for (int idx = tid; idx < TI * FJ; idx += NT) {
+      const int gi = i0 + idx / FJ, gj = j0 + idx % FJ;
+      if (gi >= X || gj >= Y) continue;
+      const int s = (H + idx / FJ) * P + H + idx % FJ;
+      out[(size_t)gi * Y + gj] = S.pi[s];
       if (norm) {
         const double r = (double)lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s);
         ss += r * r;
@@ -588,6 +689,22 @@ __global__ __launch_bounds__(256) void k_sumsq(const double* x, size_t n, double
   block_partial(ss, part);
 }
 
+__global__ __launch_bounds__(256) void k_sum(const double* x, size_t n, double* part)
+{
+  double s = 0;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) s += x[p];
+  block_partial(s, part);
+}
+
+// owned cells of the ghosted field gf = f - sum / N
+__global__ void k_scatter_mean(PgGrid G, const double* f, const double* sum, double* gf)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= G.ny) return;
+  const double mean = *sum / ((double)G.nx * (double)G.ny);
+  gf[pg_at(G, i + G.g, j + G.g)] = f[(size_t)i * G.ny + j] - mean;
+}
+
 __global__ void k_copy64(const double* src, double* dst, size_t n)
 {
   for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) dst[p] = src[p];
@@ -643,7 +760,7 @@ static int tail_level(const pg_handle* h, bool* in_lds)
 
 template <typename T>
 static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top, int norm_top, bool* norm_done,
-                               const double* pcur = nullptr, double* pout = nullptr)
+                               const double* pcur = nullptr, double* pout = nullptr, bool want_sum = true)
 {
   const int top = (int)h->lev.size() - 1;
   bool lds = false;
@@ -691,7 +808,7 @@ static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top
     const int norm = (n == top) ? norm_top : 0;
     const dim3 grid((l.Ny + FJ - 1) / FJ, (l.Nx + TI - 1) / TI);
     hipLaunchKernelGGL(k_up<T>, grid, dim3(NT), 0, h->stream, l, h->lev[n - 1], nu2, wx, wy, norm, h->partials, pcur, pout);
-    if (norm)
+    if (norm && want_sum)  // the block partials are only summed when the norm is read
       hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, (int)(grid.x * grid.y), &h->ctl->sumsq,
                          h->mapped);
   }
@@ -716,6 +833,22 @@ hipError_t pg_launch_sumsq(pg_handle* h, const double* x, size_t n, double* out)
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, out,
                      out == &h->ctl->rhs_sumsq ? h->mapped + 1 : nullptr);
   return hipGetLastError();
+}
+
+// The GPU multigrid leaves the constant null-space component of pi free on
+// closed and periodic domains; like the reference's distributed multigrid
+// and SOR solves (pressure.c:2170-2172, 811), the solution's mean is removed.
+hipError_t pg_scatter_zero_mean(pg_handle* h)
+{
+  const MgLevel& top = h->lev.back();
+  const PgGrid& G = h->G;
+  hipLaunchKernelGGL(k_sum, dim3(256), dim3(256), 0, h->stream, top.pi, (size_t)top.Nx * top.Ny, h->partials);
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, &h->ctl->pisum, (double*)nullptr);
+  hipLaunchKernelGGL(k_scatter_mean, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, top.pi, &h->ctl->pisum,
+                     h->pi);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return pg_fill(h, h->pi, 1, PG_HALO_SCALAR);
 }
 
 hipError_t pg_launch_finish_sum(pg_handle* h, const double* part, int nb, double* out)
@@ -760,7 +893,7 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
     return pg_launch_defect(h, pcur, e, nullptr, pout, r32, &h->ctl->sumsq);
   };
   (void)nbd;
-  PG_CHECK(h, pg_gather_to_mg(h));
+  PG_CHECK(h, pg_gather_guess(h));
   hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, h->stream, top.rhs, N, h->partials);
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, &h->ctl->rhs_sumsq, h->mapped + 1);
   PG_CHECK(h, defect(top.pi, nullptr, nullptr, mixed ? top.rhs32 : nullptr));
@@ -794,14 +927,17 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
       r = (rr > 0) ? std::sqrt(ss / rr) : 0.0;
       if (r <= h->cfg.pi_tol || it >= maxit) break;
     }
+    // the residual norm is read after this cycle (convergence test, or the
+    // worst-residual record of a fixed-cycle solve)
+    const bool want_sum = fixed > 0 ? it + 1 >= fixed : (it + 1 >= first_check || it + 1 >= maxit);
     if (mixed) {
       bool fused_defect = false;
-      PG_CHECK(h, fused_vcycle<float>(h, nu1, nu2, 1, 2, &fused_defect, cur, alt));
+      PG_CHECK(h, fused_vcycle<float>(h, nu1, nu2, 1, 2, &fused_defect, cur, alt, want_sum));
       if (!fused_defect) PG_CHECK(h, defect(cur, top.pi32, alt, top.rhs32));
       std::swap(cur, alt);
     } else {
       bool norm_done = false;
-      PG_CHECK(h, fused_vcycle<double>(h, nu1, nu2, 0, 1, &norm_done));
+      PG_CHECK(h, fused_vcycle<double>(h, nu1, nu2, 0, 1, &norm_done, nullptr, nullptr, want_sum));
       if (!norm_done) PG_CHECK(h, defect(top.pi, nullptr, nullptr, nullptr));
     }
     PG_CHECK(h, hipGetLastError());
@@ -816,6 +952,6 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
   if (fixed <= 0) h->mg_pred = it;
   *iters = it;
   *rel = r;
-  PG_CHECK(h, pg_scatter_from_mg(h));
+  PG_CHECK(h, pg_scatter_zero_mean(h));
   return PG_OK;
 }

@@ -516,16 +516,30 @@ hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double
   return hipGetLastError();
 }
 
-hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double* dh)
+// The edge ring (a few thousand points, each a long chain of dependent
+// loads) runs on the handle's side stream, forked from and joined back into
+// the main stream around the interior kernel; the two write disjoint points.
+template <typename T>
+static hipError_t launch_edge_forked(pg_handle* h, T* du, T* dv, T* dh, hipError_t (*inner)(pg_handle*, T*, T*, T*))
 {
-  hipError_t e = pg_launch_tendency_inner(h, du, dv, dh);
-  if (e != hipSuccess) return e;
   PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
   const PgGrid& G = h->G;
   const int total = 5 * G.ny + (G.nx - 5) * 5;
-  hipLaunchKernelGGL(k_tendency_edge<double>, dim3((total + 255) / 256, G.L), dim3(256), 0, h->stream, G, h->P, I, du, dv,
-                     dh);
-  return hipGetLastError();
+  hipError_t e = hipEventRecord(h->ev_fork, h->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->ev_fork, 0);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_tendency_edge<T>, dim3((total + 255) / 256, G.L), dim3(256), 0, h->side, G, h->P, I, du, dv, dh);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->side);
+  if (e != hipSuccess) return e;
+  e = inner(h, du, dv, dh);
+  if (e != hipSuccess) return e;
+  return hipStreamWaitEvent(h->stream, h->ev_join, 0);
+}
+
+hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double* dh)
+{
+  return launch_edge_forked<double>(h, du, dv, dh, pg_launch_tendency_inner);
 }
 
 hipError_t pg_launch_tendency_inner32(pg_handle* h, float* du, float* dv, float* dh)
@@ -541,12 +555,5 @@ hipError_t pg_launch_tendency_inner32(pg_handle* h, float* du, float* dv, float*
 // Mixed precision: FP32 interior kernel + FP64 edge ring, FP32 tendencies.
 hipError_t pg_launch_tendency32(pg_handle* h, float* du, float* dv, float* dh)
 {
-  hipError_t e = pg_launch_tendency_inner32(h, du, dv, dh);
-  if (e != hipSuccess) return e;
-  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
-  const PgGrid& G = h->G;
-  const int total = 5 * G.ny + (G.nx - 5) * 5;
-  hipLaunchKernelGGL(k_tendency_edge<float>, dim3((total + 255) / 256, G.L), dim3(256), 0, h->stream, G, h->P, I, du, dv,
-                     dh);
-  return hipGetLastError();
+  return launch_edge_forked<float>(h, du, dv, dh, pg_launch_tendency_inner32);
 }

@@ -97,3 +97,9 @@ def test_gpu_mg_run_matches_converged_oracle(solver, tmp_path, oracle_exe):
             rel = np.max(np.abs(g[key] - o[key])) / np.max(np.abs(o[key]))
             print(f"[gpu mg parity] solver={solver} n={n} {key} rel={rel:.3e}", flush=True)
             assert rel <= 1e-12, (key, n, rel)
+        # pi is fixed up to a constant; the GPU solves remove its mean
+        gp, op = g["P"], o["P"] - np.mean(o["P"])
+        assert abs(np.mean(gp)) <= 1e-12 * np.max(np.abs(gp))
+        rel = np.max(np.abs(gp - op)) / np.max(np.abs(op))
+        print(f"[gpu mg parity] solver={solver} n={n} P (zero mean) rel={rel:.3e}", flush=True)
+        assert rel <= 1e-10, ("P", n, rel)
