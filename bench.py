@@ -56,7 +56,8 @@ def parse():
     p.add_argument("--tol", type=float, default=1e-8)
     p.add_argument("--mg-nu1", type=int, default=2, help="pre-smoothing sweeps of the GPU multigrid")
     p.add_argument("--mg-nu2", type=int, default=2, help="post-smoothing sweeps")
-    p.add_argument("--pi-guess", type=int, default=0,
+    p.add_argument("--mg-tail", type=int, default=0, help="largest level of the LDS coarse tail (0: library default)")
+    p.add_argument("--pi-guess", type=int, default=2,
                    help="initial guess of each solve: 0 previous pi, 1/2 linear/quadratic extrapolation")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-steps", type=int, default=12)
@@ -82,7 +83,10 @@ def make_model(a, mixed=1, solver=None, tol=None):
                         solver=a.solver if solver is None else solver, tol=a.tol if tol is None else tol,
                         A2=10.0, maxiters=200, mixed_precision=mixed, mg_nu1=a.mg_nu1, mg_nu2=a.mg_nu2,
                         pi_guess=a.pi_guess)
-    return setup, pawsim_amd.Model.from_setup(setup)
+    m = pawsim_amd.Model.from_setup(setup)
+    if a.mg_tail:
+        m.set_mg_tail(a.mg_tail)
+    return setup, m
 
 
 def interior_points(a):

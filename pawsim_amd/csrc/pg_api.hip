@@ -522,6 +522,13 @@ int pg_set_solve_cycles(pg_handle* h, uint32_t cycles)
   return PG_OK;
 }
 
+int pg_set_mg_tail(pg_handle* h, uint32_t cells)
+{
+  if (!h) return PG_EARG;
+  h->mg_tail_cells = cells ? (long)cells : 4096;
+  return PG_OK;
+}
+
 int pg_synchronize(pg_handle* h)
 {
   if (!h) return PG_EARG;

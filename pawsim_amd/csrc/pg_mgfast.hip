@@ -99,13 +99,31 @@ __device__ __forceinline__ T interp2(const T* c, int CY, int i, int j, int X, in
          + (T)w[3] * c[(size_t)ip * CY + jp];
 }
 
-// One colour pass over LDS rows [alo, ahi] x columns [blo, bhi] of a tile
-// with pitch P; `par` is the parity of (global i + global j) at LDS (0, 0)
-// plus the colour, so cell (a, b) is updated when (par + a + b) is even.
-template <typename T, int NTH>
-__device__ __forceinline__ void rb_pass(T* spi, const T* srhs, const T* sOw, const T* sOs, const T* srO, int P, int alo,
+// Tile layout in LDS: red-black split rows.  Cell (a, b) of a tile row lives
+// at a * P + (b & 1) * HP + (b >> 1), so the cells of one colour in a row
+// are contiguous, and so are their neighbours b +- 1 (the other half of the
+// row) and a +- 1 (the rows above and below): the lanes of a colour pass
+// read consecutive LDS words instead of every other one.
+struct TileIx {
+  int s, sN, sS;  // (a, b), (a, b + 1), (a, b - 1); (a +- 1, b) is s +- P
+};
+template <int P, int HP>
+__device__ __forceinline__ TileIx tix(int a, int b)
+{
+  const int h = b >> 1, pb = b & 1, row = a * P;
+  const int so = row + (pb ^ 1) * HP + h;
+  return TileIx{row + pb * HP + h, so + pb, so - (pb ^ 1)};
+}
+
+// One colour pass over tile rows [alo, ahi] x columns [blo, bhi]; `par` is
+// the parity of (global i + global j) at tile cell (0, 0) plus the colour,
+// so cell (a, b) is updated when (par + a + b) is even.  Multiply-adds may
+// fuse (the GPU multigrid is not a bitwise replica).
+template <typename T, int NTH, int P, int HP>
+__device__ __forceinline__ void rb_pass(T* spi, const T* srhs, const T* sOw, const T* sOs, const T* srO, int alo,
                                         int ahi, int blo, int bhi, int par, int tid)
 {
+#pragma clang fp contract(fast)
   const int rows = ahi - alo + 1, half = (bhi - blo + 2) / 2;
   const float invh = 1.0f / (float)half;
   for (int idx = tid; idx < rows * half; idx += NTH) {
@@ -113,22 +131,37 @@ __device__ __forceinline__ void rb_pass(T* spi, const T* srhs, const T* sOw, con
     const int a = alo + q;
     const int b = blo + ((par + a + blo) & 1) + 2 * (idx - q * half);
     if (b > bhi) continue;
-    const int s = a * P + b;
-    const T r = srO[s];
+    const TileIx t = tix<P, HP>(a, b);
+    const T r = srO[t.s];
     if (!(r == r) || r == (T)INFINITY) continue;
-    spi[s] = r * (sOs[s + 1] * spi[s + 1] + sOs[s] * spi[s - 1] + sOw[s + P] * spi[s + P] + sOw[s] * spi[s - P] - srhs[s]);
+    spi[t.s] = r * (sOs[t.sN] * spi[t.sN] + sOs[t.s] * spi[t.sS] + sOw[t.s + P] * spi[t.s + P] + sOw[t.s] * spi[t.s - P]
+                    - srhs[t.s]);
   }
 }
 
-// L pi - rhs at LDS cell s (residual_MG, multigrid.c:599-627); Osum is
+// L pi - rhs at a tile cell (residual_MG, multigrid.c:599-627); Osum is
 // rebuilt from the face coefficients exactly as the level construction
 // forms it (Ow + Ow_east + Os + Os_north).
-template <typename T>
-__device__ __forceinline__ T lds_residual(const T* spi, const T* srhs, const T* sOw, const T* sOs, int P, int s)
+template <typename T, int P>
+__device__ __forceinline__ T lds_residual(const T* spi, const T* srhs, const T* sOw, const T* sOs, const TileIx& t)
 {
-  const T osum = sOw[s] + sOw[s + P] + sOs[s] + sOs[s + 1];
-  return sOs[s + 1] * spi[s + 1] + sOs[s] * spi[s - 1] + sOw[s + P] * spi[s + P] + sOw[s] * spi[s - P] - osum * spi[s]
-         - srhs[s];
+#pragma clang fp contract(fast)
+  const T osum = sOw[t.s] + sOw[t.s + P] + sOs[t.s] + sOs[t.sN];
+  return sOs[t.sN] * spi[t.sN] + sOs[t.s] * spi[t.sS] + sOw[t.s + P] * spi[t.s + P] + sOw[t.s] * spi[t.s - P]
+         - osum * spi[t.s] - srhs[t.s];
+}
+
+// 1-D factors of the 2:1 prolongation weights at fine index i of an N-cell
+// axis: weight of coarse index floor((i - 1) / 2) and of the next one, with
+// the wall overrides of multigrid.c:377-442 (the 2-D weights of pow2_weights
+// are their products, corners included).
+template <typename T>
+__device__ __forceinline__ void pow2_factors(int i, int N, int wall, T& am, T& ap)
+{
+  am = (i & 1) ? (T)0.75 : (T)0.25;
+  ap = (i & 1) ? (T)0.25 : (T)0.75;
+  if (wall && i == 0) { am = 0; ap = 1; }
+  if (wall && i == N - 1) { am = 1; ap = 0; }
 }
 
 // Block partial of a grid-wide sum: part[block] = sum of v over the block.
@@ -171,8 +204,9 @@ __global__ __launch_bounds__(256) void k_finish_sum(const double* part, int nb, 
 template <typename T>
 struct TileLds {
   static constexpr int TI = Tile<T>::TI;
-  static constexpr int P = FJ + 2 * HMAX + 1;     // pitch
-  static constexpr int RMAX = TI + 2 * HMAX + 1;  // rows
+  static constexpr int HP = (FJ + 2 * HMAX + 2) / 2;  // one colour half of a row
+  static constexpr int P = 2 * HP;                    // pitch
+  static constexpr int RMAX = TI + 2 * HMAX + 1;      // rows
   T pi[RMAX * P], rhs[RMAX * P], Ow[RMAX * P], Os[RMAX * P], rO[RMAX * P];
 };
 
@@ -195,17 +229,17 @@ struct CoarseWin {
 };
 
 // Stage the tile with halo H: coefficients over (R+1) x (C+1) cells, pi and
-// rhs over R x C, R = TI + 2H, C = FJ + 2H, pi from the level's `src` array
-// (or zero, ZERO).  With `coarse`, the coarse window W goes to cwin as well
-// (k_up).  Every load is issued before the first LDS store, indices clamped
-// to the last cell and wrapped onto the torus, so the whole stage is one
-// round trip to memory; the stores are guarded.
-template <typename T, bool ZERO>
+// rhs over R x C, R = TI + 2H, C = FJ + 2H.  MODE 0: pi = 0; 1: pi = src;
+// 2: pi = src minus the 2:1 prolongation of `coarse` (its window W goes to
+// cwin first).  Every load is issued before the first LDS store, indices
+// clamped to the last cell and wrapped onto the torus, so the whole stage
+// is one round trip to memory; the stores are guarded.
+template <typename T, int MODE>
 __device__ void stage_tile(TileLds<T>& S, const MgLevel& f, const T* src, int H, int i0, int j0, int tid,
                            T* cwin = nullptr, const CoarseWin<T>* W = nullptr, const T* coarse = nullptr, int CX = 0,
-                           int CY = 0)
+                           int CY = 0, int wallEW = 0, int wallNS = 0)
 {
-  constexpr int P = TileLds<T>::P;
+  constexpr int P = TileLds<T>::P, HP = TileLds<T>::HP;
   // fine / coarse cells per thread: one batch covers the largest tile
   constexpr int U = ((Tile<T>::TI + 2 * HMAX + 1) * (FJ + 2 * HMAX + 1) + NT - 1) / NT, UC = 2;
   static_assert(CoarseWin<T>::RMAX * CoarseWin<T>::CMAX <= UC * NT, "coarse stage is one batch");
@@ -219,7 +253,7 @@ __device__ void stage_tile(TileLds<T>& S, const MgLevel& f, const T* src, int H,
   const int total = (R + 1) * C1;
   T cv[UC];
   int ctot = 0;
-  if (coarse) {
+  if (MODE == 2) {
     ctot = W->CRn * W->CCn;
     const float cinv = 1.0f / (float)W->CCn;
 #pragma unroll
@@ -239,9 +273,9 @@ __device__ void stage_tile(TileLds<T>& S, const MgLevel& f, const T* src, int H,
     os[u] = gOs[p];
     rh[u] = grhs[p];
     ro[u] = grO[p];
-    pv[u] = ZERO ? (T)0 : src[p];
+    pv[u] = MODE == 0 ? (T)0 : src[p];
   }
-  if (coarse) {
+  if (MODE == 2) {
 #pragma unroll
     for (int u = 0; u < UC; u++)
       if (tid + u * NT < ctot) cwin[tid + u * NT] = cv[u];
@@ -251,35 +285,35 @@ __device__ void stage_tile(TileLds<T>& S, const MgLevel& f, const T* src, int H,
     const int idx = tid + u * NT;
     if (idx < total) {
       const int a = fdiv(idx, inv), b = idx - a * C1;
-      const int sl = a * P + b;
+      const int sl = a * P + (b & 1) * HP + (b >> 1);
       S.Ow[sl] = ow[u];
       S.Os[sl] = os[u];
       if (a < R && b < C) {
-        S.pi[sl] = pv[u];
         S.rhs[sl] = rh[u];
         S.rO[sl] = ro[u];
+        if (MODE != 2) S.pi[sl] = pv[u];
       }
     }
   }
-}
-
-// pi -= prolonged coarse correction over the R x C cells of the tile, with
-// the expression and weights of interp2.
-template <typename T>
-__device__ void apply_coarse(TileLds<T>& S, const T* cwin, const CoarseWin<T>& W, int H, int i0, int j0, int X, int Y,
-                             int wallEW, int wallNS, int tid)
-{
-  constexpr int P = TileLds<T>::P;
-  const int R = Tile<T>::TI + 2 * H, C = FJ + 2 * H;
-  const float inv = 1.0f / (float)C;
-  for (int idx = tid; idx < R * C; idx += NT) {
-    const int a = fdiv(idx, inv), b = idx - a * C;
-    const int fi = i0 - H + a, fj = j0 - H + b;
-    T w[4];
-    pow2_weights<T>(wrapf(fi, X), wrapf(fj, Y), X, Y, wallEW, wallNS, w);
-    const T* c0 = cwin + (((fi - 1) >> 1) - W.cbi) * W.CCn + (((fj - 1) >> 1) - W.cbj);
-    const T corr = w[0] * c0[0] + w[2] * c0[W.CCn] + w[1] * c0[1] + w[3] * c0[W.CCn + 1];
-    S.pi[a * P + b] -= corr;
+  if (MODE == 2) {
+#pragma clang fp contract(fast)
+    __syncthreads();  // the coarse window is in LDS
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int idx = tid + u * NT;
+      if (idx < total) {
+        const int a = fdiv(idx, inv), b = idx - a * C1;
+        if (a < R && b < C) {
+          const int fi = i0 - H + a, fj = j0 - H + b;
+          T am, ap, bm, bp;
+          pow2_factors<T>(wrapf(fi, X), X, wallEW, am, ap);
+          pow2_factors<T>(wrapf(fj, Y), Y, wallNS, bm, bp);
+          const T* c0 = cwin + (((fi - 1) >> 1) - W->cbi) * W->CCn + (((fj - 1) >> 1) - W->cbj);
+          const T corr = am * (bm * c0[0] + bp * c0[1]) + ap * (bm * c0[W->CCn] + bp * c0[W->CCn + 1]);
+          S.pi[a * P + (b & 1) * HP + (b >> 1)] = pv[u] - corr;
+        }
+      }
+    }
   }
 }
 
@@ -288,19 +322,19 @@ __device__ void apply_coarse(TileLds<T>& S, const T* cwin, const CoarseWin<T>& W
 template <typename T>
 __global__ __launch_bounds__(NT) void k_down(MgLevel f, MgLevel c, int nu, int zero_init)
 {
-  constexpr int TI = Tile<T>::TI, P = TileLds<T>::P;
+  constexpr int TI = Tile<T>::TI, P = TileLds<T>::P, HP = TileLds<T>::HP;
   __shared__ TileLds<T> S;
   const int tid = threadIdx.x;
   const int H = 2 * nu + 1, R = TI + 2 * H, C = FJ + 2 * H;
   const int i0 = blockIdx.y * TI, j0 = blockIdx.x * FJ;
-  if (zero_init) stage_tile<T, true>(S, f, nullptr, H, i0, j0, tid);
-  else stage_tile<T, false>(S, f, LvPtr<T>::pi(f), H, i0, j0, tid);
+  if (zero_init) stage_tile<T, 0>(S, f, nullptr, H, i0, j0, tid);
+  else stage_tile<T, 1>(S, f, LvPtr<T>::pi(f), H, i0, j0, tid);
   __syncthreads();
-  const int base = (i0 + j0) & 1;  // parity of LDS (0, 0): 2H is even
+  const int base = (i0 + j0) & 1;  // parity of tile cell (0, 0): 2H is even
   for (int pass = 0; pass < 2 * nu; pass++) {
     const int color = pass & 1;
-    rb_pass<T, NT>(S.pi, S.rhs, S.Ow, S.Os, S.rO, P, pass + 1, R - 2 - pass, pass + 1, C - 2 - pass, (base + color) & 1,
-                   tid);
+    rb_pass<T, NT, P, HP>(S.pi, S.rhs, S.Ow, S.Os, S.rO, pass + 1, R - 2 - pass, pass + 1, C - 2 - pass,
+                          (base + color) & 1, tid);
     __syncthreads();
   }
   constexpr int CI = TI / 2, CJ = FJ / 2;
@@ -310,11 +344,11 @@ __global__ __launch_bounds__(NT) void k_down(MgLevel f, MgLevel c, int nu, int z
   for (int idx = tid; idx < CI * CJ; idx += NT) {
     const int ic = idx / CJ, jc = idx % CJ;
     if (ci0 + ic >= c.Nx || cj0 + jc >= c.Ny) continue;
-    const int s = (H + 2 * ic) * P + H + 2 * jc;
-    const T r00 = lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s);
-    const T r10 = lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s + P);
-    const T r01 = lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s + 1);
-    const T r11 = lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s + P + 1);
+    const int a = H + 2 * ic, b = H + 2 * jc;
+    const T r00 = lds_residual<T, P>(S.pi, S.rhs, S.Ow, S.Os, tix<P, HP>(a, b));
+    const T r10 = lds_residual<T, P>(S.pi, S.rhs, S.Ow, S.Os, tix<P, HP>(a + 1, b));
+    const T r01 = lds_residual<T, P>(S.pi, S.rhs, S.Ow, S.Os, tix<P, HP>(a, b + 1));
+    const T r11 = lds_residual<T, P>(S.pi, S.rhs, S.Ow, S.Os, tix<P, HP>(a + 1, b + 1));
     const size_t q = (size_t)(ci0 + ic) * c.Ny + cj0 + jc;
     crhs[q] = (T)0.25 * (r00 + r10 + r01 + r11);
     cpi[q] = 0;
@@ -322,7 +356,7 @@ __global__ __launch_bounds__(NT) void k_down(MgLevel f, MgLevel c, int nu, int z
   T* out = LvPtr<T>::tmp(f);
   for (int idx = tid; idx < TI * FJ; idx += NT) {
     const int gi = i0 + idx / FJ, gj = j0 + idx % FJ;
-    if (gi < f.Nx && gj < f.Ny) out[(size_t)gi * f.Ny + gj] = S.pi[(H + idx / FJ) * P + H + idx % FJ];
+    if (gi < f.Nx && gj < f.Ny) out[(size_t)gi * f.Ny + gj] = S.pi[tix<P, HP>(H + idx / FJ, H + idx % FJ).s];
   }
 }
 
@@ -338,22 +372,20 @@ template <typename T>
 __global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wallEW, int wallNS, int norm, double* part,
                                            const double* pcur, double* pout)
 {
-  constexpr int TI = Tile<T>::TI, P = TileLds<T>::P;
+  constexpr int TI = Tile<T>::TI, P = TileLds<T>::P, HP = TileLds<T>::HP;
   __shared__ TileLds<T> S;
   const int tid = threadIdx.x;
   const int H = 2 * nu + (norm ? 1 : 0), R = TI + 2 * H, C = FJ + 2 * H;
   const int i0 = blockIdx.y * TI, j0 = blockIdx.x * FJ;
   __shared__ T cwin[CoarseWin<T>::RMAX * CoarseWin<T>::CMAX];
   const CoarseWin<T> W(H, i0, j0);
-  stage_tile<T, false>(S, f, LvPtr<T>::tmp(f), H, i0, j0, tid, cwin, &W, LvPtr<T>::pi(c), c.Nx, c.Ny);
-  __syncthreads();
-  apply_coarse<T>(S, cwin, W, H, i0, j0, f.Nx, f.Ny, wallEW, wallNS, tid);
+  stage_tile<T, 2>(S, f, LvPtr<T>::tmp(f), H, i0, j0, tid, cwin, &W, LvPtr<T>::pi(c), c.Nx, c.Ny, wallEW, wallNS);
   __syncthreads();
   const int base = (i0 + j0) & 1;
   for (int pass = 0; pass < 2 * nu; pass++) {
     const int color = 1 ^ (pass & 1);
-    rb_pass<T, NT>(S.pi, S.rhs, S.Ow, S.Os, S.rO, P, pass + 1, R - 2 - pass, pass + 1, C - 2 - pass, (base + color) & 1,
-                   tid);
+    rb_pass<T, NT, P, HP>(S.pi, S.rhs, S.Ow, S.Os, S.rO, pass + 1, R - 2 - pass, pass + 1, C - 2 - pass,
+                          (base + color) & 1, tid);
     __syncthreads();
   }
   T* out = LvPtr<T>::pi(f);
@@ -389,11 +421,11 @@ __global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wal
         const int idx = base + u * NT;
         const int gi = i0 + idx / FJ, gj = j0 + idx % FJ;
         if (gi >= X || gj >= Y) continue;
-        const int s = (H + idx / FJ) * P + H + idx % FJ;
+        const TileIx t = tix<P, HP>(H + idx / FJ, H + idx % FJ);
         const unsigned p = (unsigned)(gi * Y + gj);
-        const double v = pc[u] - 1.0 * (double)S.pi[s], vE = pe[u] - 1.0 * (double)S.pi[s + P];
-        const double vW = pw[u] - 1.0 * (double)S.pi[s - P], vN = pn[u] - 1.0 * (double)S.pi[s + 1];
-        const double vS = ps[u] - 1.0 * (double)S.pi[s - 1];
+        const double v = pc[u] - 1.0 * (double)S.pi[t.s], vE = pe[u] - 1.0 * (double)S.pi[t.s + P];
+        const double vW = pw[u] - 1.0 * (double)S.pi[t.s - P], vN = pn[u] - 1.0 * (double)S.pi[t.sN];
+        const double vS = ps[u] - 1.0 * (double)S.pi[t.sS];
         pout[p] = v;
         const double r = osn[u] * vN + osp[u] * vS + owe[u] * vE + owp[u] * vW - osm[u] * v - rh[u];
         f.rhs32[p] = (float)r;
@@ -404,10 +436,10 @@ __global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wal
     for (int idx = tid; idx < TI * FJ; idx += NT) {
       const int gi = i0 + idx / FJ, gj = j0 + idx % FJ;
       if (gi >= X || gj >= Y) continue;
-      const int s = (H + idx / FJ) * P + H + idx % FJ;
-      out[(size_t)gi * Y + gj] = S.pi[s];
+      const TileIx t = tix<P, HP>(H + idx / FJ, H + idx % FJ);
+      out[(size_t)gi * Y + gj] = S.pi[t.s];
       if (norm) {
-        const double r = (double)lds_residual<T>(S.pi, S.rhs, S.Ow, S.Os, P, s);
+        const double r = (double)lds_residual<T, P>(S.pi, S.rhs, S.Ow, S.Os, t);
         ss += r * r;
       }
     }
@@ -419,9 +451,17 @@ __global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wal
 // coarse tail in LDS
 // ---------------------------------------------------------------------------
 
-// Level layout inside the tail's LDS: six X*Y arrays from `off`.
-struct TailLevel {
-  int X, Y, off;
+// Level layout inside the tail's LDS, passed by value from the host: per
+// level the work arrays pi, rhs, tmp (X*Y each) from woff[n], and in a
+// separate region from coef_base the coefficients Ow, Os, 1/Osum from
+// coff[n] (the region a packed global image is copied into).
+constexpr int TAIL_MAXLEV = 16;
+struct TailDims {
+  int m, mw;        // top level; levels 0..mw run in wavefront 0
+  int coef_base;    // element offset of the coefficient region (16-byte aligned)
+  int coef_words;   // elements of the coefficient image (multiple of 16 bytes)
+  int pow2;         // bit n: level n transfers 2:1 to level n - 1
+  int X[TAIL_MAXLEV], Y[TAIL_MAXLEV], woff[TAIL_MAXLEV], coff[TAIL_MAXLEV];
 };
 
 // Typed views formed from the LDS base inside the kernel, so the compiler
@@ -433,11 +473,12 @@ struct TV {
 };
 
 template <typename T>
-__device__ __forceinline__ TV<T> tview(T* mem, const TailLevel& L)
+__device__ __forceinline__ TV<T> tview(T* mem, const TailDims& D, int n)
 {
-  const int N = L.X * L.Y;
-  T* b = mem + L.off;
-  return TV<T>{L.X, L.Y, b, b + N, b + 2 * N, b + 3 * N, b + 4 * N, b + 5 * N};
+  const int N = D.X[n] * D.Y[n];
+  T* w = mem + D.woff[n];
+  T* c = mem + D.coef_base + D.coff[n];
+  return TV<T>{D.X[n], D.Y[n], w, w + N, w + 2 * N, c, c + N, c + 2 * N};
 }
 
 __device__ __forceinline__ int dec(int i, int N) { return i == 0 ? N - 1 : i - 1; }
@@ -525,12 +566,94 @@ __device__ void tail_smooth(const TV<T>& L, int wallEW, int wallNS, int first, i
   else tail_rb<T, S>(L, first, tid);
 }
 
+// Fast forms for levels with even X and Y (every 2:1 level): a colour pass
+// enumerates only the cells of its colour, indices come from shifts and
+// compares instead of divisions, and multiply-adds may fuse (the GPU
+// multigrid is not a bitwise replica).  The expressions are those of
+// tail_rb, the residual, restrict_cell and interp2.
+template <typename T, typename S>
+__device__ void tail_rb_even(const TV<T>& L, int first, int tid)
+{
+#pragma clang fp contract(fast)
+  const int X = L.X, Y = L.Y, HY = Y >> 1, n = X * HY;
+  const float inv = 1.0f / (float)HY;
+  for (int cc = 0; cc < 2; cc++) {
+    const int color = first ^ cc;
+    for (int idx = tid; idx < n; idx += S::NT) {
+      const int i = fdiv(idx, inv), j = 2 * (idx - i * HY) + ((i + color) & 1);
+      const int p = i * Y + j;
+      const T r = L.rO[p];
+      if (!(r == r) || r == (T)INFINITY) continue;
+      const int pE = (i == X - 1) ? j : p + Y, pW = (i == 0) ? (X - 1) * Y + j : p - Y;
+      const int pN = (j == Y - 1) ? p - (Y - 1) : p + 1, pS = (j == 0) ? p + (Y - 1) : p - 1;
+      L.pi[p] = r * (L.Os[pN] * L.pi[pN] + L.Os[p] * L.pi[pS] + L.Ow[pE] * L.pi[pE] + L.Ow[p] * L.pi[pW] - L.rhs[p]);
+    }
+    S::sync();
+  }
+}
+
+template <typename T, typename S>
+__device__ void tail_residual_even(const TV<T>& F, int tid)
+{
+#pragma clang fp contract(fast)
+  const int X = F.X, Y = F.Y;
+  const float inv = 1.0f / (float)Y;
+  for (int p = tid; p < X * Y; p += S::NT) {
+    const int i = fdiv(p, inv), j = p - i * Y;
+    const int pE = (i == X - 1) ? j : p + Y, pW = (i == 0) ? (X - 1) * Y + j : p - Y;
+    const int pN = (j == Y - 1) ? p - (Y - 1) : p + 1, pS = (j == 0) ? p + (Y - 1) : p - 1;
+    const T osum = F.Ow[p] + F.Ow[pE] + F.Os[p] + F.Os[pN];
+    F.tmp[p] = F.Os[pN] * F.pi[pN] + F.Os[p] * F.pi[pS] + F.Ow[pE] * F.pi[pE] + F.Ow[p] * F.pi[pW] - osum * F.pi[p] - F.rhs[p];
+  }
+}
+
+// 2:1 restriction of F.tmp into Cl.rhs (2 x 2 mean), Cl.pi = 0
+template <typename T, typename S>
+__device__ void tail_restrict_pow2(const TV<T>& F, const TV<T>& Cl, int tid)
+{
+  const int Y = F.Y, CY = Cl.Y;
+  const float inv = 1.0f / (float)CY;
+  for (int p = tid; p < Cl.X * CY; p += S::NT) {
+    const int ic = fdiv(p, inv), jc = p - ic * CY;
+    const int f00 = 2 * ic * Y + 2 * jc;
+    Cl.rhs[p] = (T)0.25 * (F.tmp[f00] + F.tmp[f00 + Y] + F.tmp[f00 + 1] + F.tmp[f00 + Y + 1]);
+    Cl.pi[p] = 0;
+  }
+}
+
+// F.pi -= 2:1 bilinear prolongation of Cl.pi
+template <typename T, typename S>
+__device__ void tail_interp_pow2(const TV<T>& F, const TV<T>& Cl, int wallEW, int wallNS, int tid)
+{
+#pragma clang fp contract(fast)
+  const int X = F.X, Y = F.Y, hx = Cl.X, hy = Cl.Y;
+  const float inv = 1.0f / (float)Y;
+  for (int p = tid; p < X * Y; p += S::NT) {
+    const int i = fdiv(p, inv), j = p - i * Y;
+    int im = (i - 1) >> 1, jm = (j - 1) >> 1;
+    if (im < 0) im += hx;
+    if (jm < 0) jm += hy;
+    const int ip = (im + 1 == hx) ? 0 : im + 1, jp = (jm + 1 == hy) ? 0 : jm + 1;
+    T w[4];
+    pow2_weights<T>(i, j, X, Y, wallEW, wallNS, w);
+    F.pi[p] -= w[0] * Cl.pi[im * hy + jm] + w[2] * Cl.pi[ip * hy + jm] + w[1] * Cl.pi[im * hy + jp] + w[3] * Cl.pi[ip * hy + jp];
+  }
+}
+
 // Down leg at level n: smoothing, residual, restriction into level n - 1.
 template <typename T, typename S>
-__device__ void tail_down(const MgLevel* lv, const TailLevel* LL, T* mem, int n, int nu1, int wallEW, int wallNS, int tid)
+__device__ void tail_down(const MgLevel* lv, const TailDims& D, T* mem, int n, int nu1, int wallEW, int wallNS, int tid)
 {
-  const TV<T> F = tview(mem, LL[n]);
-  const TV<T> Cl = tview(mem, LL[n - 1]);
+  const TV<T> F = tview(mem, D, n);
+  const TV<T> Cl = tview(mem, D, n - 1);
+  if (!(F.X & 1) && !(F.Y & 1) && ((D.pow2 >> n) & 1)) {
+    for (int s = 0; s < nu1; s++) tail_rb_even<T, S>(F, 0, tid);
+    tail_residual_even<T, S>(F, tid);
+    S::sync();
+    tail_restrict_pow2<T, S>(F, Cl, tid);
+    S::sync();
+    return;
+  }
   for (int s = 0; s < nu1; s++) tail_smooth<T, S>(F, wallEW, wallNS, 0, tid);
   const int X = F.X, Y = F.Y;
   tail_cells<S>(X, Y, tid, [&](int i, int j, int p) {
@@ -551,11 +674,17 @@ __device__ void tail_down(const MgLevel* lv, const TailLevel* LL, T* mem, int n,
 
 // Up leg at level n: prolongation of level n - 1, correction, smoothing.
 template <typename T, typename S>
-__device__ void tail_up(const MgLevel* lv, const TailLevel* LL, T* mem, int n, int nu2, int wallEW, int wallNS, int tid)
+__device__ void tail_up(const MgLevel* lv, const TailDims& D, T* mem, int n, int nu2, int wallEW, int wallNS, int tid)
 {
-  const TV<T> F = tview(mem, LL[n]);
-  const TV<T> Cl = tview(mem, LL[n - 1]);
-  const bool p2 = lv[n].pow2;
+  const TV<T> F = tview(mem, D, n);
+  const TV<T> Cl = tview(mem, D, n - 1);
+  const bool p2 = (D.pow2 >> n) & 1;
+  if (p2 && !(F.X & 1) && !(F.Y & 1)) {
+    tail_interp_pow2<T, S>(F, Cl, wallEW, wallNS, tid);
+    S::sync();
+    for (int s = 0; s < nu2; s++) tail_rb_even<T, S>(F, 1, tid);
+    return;
+  }
   const MgLevel& Fg = lv[n];
   const MgLevel& Cg = lv[n - 1];
   tail_cells<S>(F.X, F.Y, tid, [&](int i, int j, int p) {
@@ -567,78 +696,81 @@ __device__ void tail_up(const MgLevel* lv, const TailLevel* LL, T* mem, int n, i
   for (int s = 0; s < nu2; s++) tail_smooth<T, S>(F, wallEW, wallNS, 1, tid);
 }
 
-// Packed image of the tail's LDS for levels 0..m: per level six X*Y arrays
-// (pi, rhs, tmp zero; Ow, Os, 1/Osum from the level).
+// Packed coefficient image of the tail's levels 0..m: per level Ow, Os,
+// 1/Osum (X*Y each) at coff[n]; zero padding to coef_words.
 template <typename T>
-__global__ void k_tail_image(const MgLevel* lv, int m, T* img)
+__global__ void k_tail_image(const MgLevel* lv, TailDims D, T* img)
 {
-  int off = 0;
-  for (int n = 0; n <= m; n++) {
-    const int N = lv[n].Nx * lv[n].Ny;
-    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += gridDim.x * blockDim.x) {
-      img[off + idx] = 0;
-      img[off + N + idx] = 0;
-      img[off + 2 * N + idx] = 0;
-      img[off + 3 * N + idx] = LvPtr<T>::Ow(lv[n])[idx];
-      img[off + 4 * N + idx] = LvPtr<T>::Os(lv[n])[idx];
-      img[off + 5 * N + idx] = LvPtr<T>::rOsum(lv[n])[idx];
+  const int stride = gridDim.x * blockDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  int end = 0;
+  for (int n = 0; n <= D.m; n++) {
+    const int N = D.X[n] * D.Y[n];
+    T* c = img + D.coff[n];
+    for (int idx = t0; idx < N; idx += stride) {
+      c[idx] = LvPtr<T>::Ow(lv[n])[idx];
+      c[N + idx] = LvPtr<T>::Os(lv[n])[idx];
+      c[2 * N + idx] = LvPtr<T>::rOsum(lv[n])[idx];
     }
-    off += 6 * N;
+    end = D.coff[n] + 3 * N;
   }
+  for (int idx = end + t0; idx < D.coef_words; idx += stride) img[idx] = 0;
 }
 
 // V(nu1, nu2) over levels m .. 0 with every array in LDS.  pi of level m is
 // loaded (or zero) and written back; the transfers are the reference's
-// (2:1 or general).  Levels of at most 8 x 8 cells run in wavefront 0 alone.
+// (2:1 or general).  Levels 0..mw (at most 8 x 8 cells) run in wavefront 0
+// alone.  The coefficient image and the top level's pi and rhs are loaded
+// in one batch; the coarser levels' pi, rhs and tmp are written before they
+// are read (restriction, residual), so they need no initial values.
 template <typename T>
-__global__ __launch_bounds__(TAIL_THREADS) void k_tail(const MgLevel* lv, int m, int nu1, int nu2, int wallEW, int wallNS,
-                                                       int zero_init, const void* img, size_t bytes)
+__global__ __launch_bounds__(TAIL_THREADS) void k_tail(const MgLevel* lv, TailDims D, int nu1, int nu2, int wallEW,
+                                                       int wallNS, int zero_init, const void* img)
 {
   extern __shared__ __align__(16) double tail_smem[];
   T* mem = reinterpret_cast<T*>(tail_smem);
-  __shared__ TailLevel LL[32];
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    int off = 0;
-    for (int n = 0; n <= m; n++) {
-      LL[n] = TailLevel{lv[n].Nx, lv[n].Ny, off};
-      off += 6 * lv[n].Nx * lv[n].Ny;
-    }
-  }
-  __syncthreads();
+  const int tid = threadIdx.x, m = D.m;
   {
-    // the packed image (coefficients of every level, zero iterates) in one
-    // coalesced copy, then the top level's iterate and right-hand side
-    const int n16 = (int)(bytes / 16);
+    constexpr int UB = 4;
+    const int n16 = D.coef_words * (int)sizeof(T) / 16, Nm = D.X[m] * D.Y[m];
     const uint4* src = reinterpret_cast<const uint4*>(img);
-    uint4* dst = reinterpret_cast<uint4*>(tail_smem);
-    for (int q = tid; q < n16; q += TAIL_THREADS) dst[q] = src[q];
-    const unsigned* s4 = reinterpret_cast<const unsigned*>(img);
-    unsigned* d4 = reinterpret_cast<unsigned*>(tail_smem);
-    for (int q = 4 * n16 + tid; q < (int)(bytes / 4); q += TAIL_THREADS) d4[q] = s4[q];
-    __syncthreads();
-    const TV<T> L = tview(mem, LL[m]);
+    uint4* dst = reinterpret_cast<uint4*>(mem + D.coef_base);
     const T* gpi = LvPtr<T>::pi(lv[m]);
     const T* grhs = LvPtr<T>::rhs(lv[m]);
-    for (int idx = tid; idx < L.X * L.Y; idx += TAIL_THREADS) {
-      L.pi[idx] = zero_init ? (T)0 : gpi[idx];
-      L.rhs[idx] = grhs[idx];
+    const TV<T> L = tview(mem, D, m);
+    const int per = max(n16, Nm);
+    for (int b0 = 0; b0 < per; b0 += UB * TAIL_THREADS) {
+      uint4 v[UB];
+      T pv[UB], rv[UB];
+#pragma unroll
+      for (int u = 0; u < UB; u++) {
+        const int q = b0 + u * TAIL_THREADS + tid;
+        v[u] = src[min(q, n16 - 1)];
+        pv[u] = zero_init ? (T)0 : gpi[min(q, Nm - 1)];
+        rv[u] = grhs[min(q, Nm - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < UB; u++) {
+        const int q = b0 + u * TAIL_THREADS + tid;
+        if (q < n16) dst[q] = v[u];
+        if (q < Nm) {
+          L.pi[q] = pv[u];
+          L.rhs[q] = rv[u];
+        }
+      }
     }
   }
   __syncthreads();
-  // levels 0 .. mw fit one wavefront (at most 8 x 8 cells)
-  int mw = 0;
-  while (mw < m && LL[mw + 1].X <= 8 && LL[mw + 1].Y <= 8) mw++;
-  for (int n = m; n > mw; n--) tail_down<T, BlockScope>(lv, LL, mem, n, nu1, wallEW, wallNS, tid);
+  const int mw = D.mw;
+  for (int n = m; n > mw; n--) tail_down<T, BlockScope>(lv, D, mem, n, nu1, wallEW, wallNS, tid);
   if (tid < 64) {
-    for (int n = mw; n >= 1; n--) tail_down<T, WaveScope>(lv, LL, mem, n, nu1, wallEW, wallNS, tid);
-    if (tid == 0 && LL[0].X * LL[0].Y == 1) tview(mem, LL[0]).pi[0] = 0;
+    for (int n = mw; n >= 1; n--) tail_down<T, WaveScope>(lv, D, mem, n, nu1, wallEW, wallNS, tid);
+    if (tid == 0 && D.X[0] * D.Y[0] == 1) tview(mem, D, 0).pi[0] = 0;
     WaveScope::sync();
-    for (int n = 1; n <= mw; n++) tail_up<T, WaveScope>(lv, LL, mem, n, nu2, wallEW, wallNS, tid);
+    for (int n = 1; n <= mw; n++) tail_up<T, WaveScope>(lv, D, mem, n, nu2, wallEW, wallNS, tid);
   }
   __syncthreads();
-  for (int n = mw + 1; n <= m; n++) tail_up<T, BlockScope>(lv, LL, mem, n, nu2, wallEW, wallNS, tid);
-  const TV<T> Lm = tview(mem, LL[m]);
+  for (int n = mw + 1; n <= m; n++) tail_up<T, BlockScope>(lv, D, mem, n, nu2, wallEW, wallNS, tid);
+  const TV<T> Lm = tview(mem, D, m);
   T* gpi = LvPtr<T>::pi(lv[m]);
   for (int idx = tid; idx < Lm.X * Lm.Y; idx += TAIL_THREADS) gpi[idx] = Lm.pi[idx];
 }
@@ -732,12 +864,38 @@ bool odd_periodic(const pg_handle* h, const MgLevel& l)
   return (!h->G.wallx && (l.Nx & 1) && l.Nx > 1) || (!h->G.wally && (l.Ny & 1) && l.Ny > 1);
 }
 
+// LDS layout of a tail over levels 0..m; returns the LDS bytes (0 when the
+// hierarchy is too deep for TailDims).
+template <typename T>
+size_t tail_dims(const pg_handle* h, int m, TailDims* D)
+{
+  if (m >= TAIL_MAXLEV) return 0;
+  constexpr int A = 16 / (int)sizeof(T);  // elements per 16 bytes
+  TailDims d{};
+  d.m = m;
+  int w = 0, c = 0;
+  for (int n = 0; n <= m; n++) {
+    d.X[n] = h->lev[n].Nx;
+    d.Y[n] = h->lev[n].Ny;
+    d.woff[n] = w;
+    d.coff[n] = c;
+    w += 3 * d.X[n] * d.Y[n];
+    c += 3 * d.X[n] * d.Y[n];
+    if (h->lev[n].pow2) d.pow2 |= 1 << n;
+  }
+  d.coef_base = (w + A - 1) / A * A;
+  d.coef_words = (c + A - 1) / A * A;
+  d.mw = 0;
+  while (d.mw < m && d.X[d.mw + 1] <= 8 && d.Y[d.mw + 1] <= 8) d.mw++;
+  if (D) *D = d;
+  return (size_t)(d.coef_base + d.coef_words) * sizeof(T);
+}
+
 template <typename T>
 size_t tail_bytes(const pg_handle* h, int m)
 {
-  size_t cells = 0;
-  for (int n = 0; n <= m; n++) cells += (size_t)h->lev[n].Nx * h->lev[n].Ny;
-  return cells * 6 * sizeof(T);
+  const size_t b = tail_dims<T>(h, m, nullptr);
+  return b ? b : ~(size_t)0;
 }
 
 }  // namespace
@@ -775,14 +933,15 @@ static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top
   const MgLevel* dlev = reinterpret_cast<const MgLevel*>(h->partials + h->npartials);
   const int zi = (mt == top) ? zero_init_top : 0;
   if (lds) {
-    const size_t bytes = tail_bytes<T>(h, mt);
+    TailDims D;
+    const size_t bytes = tail_dims<T>(h, mt, &D);
     const int q = sizeof(T) == 8;
     if (h->tail_img_m[q] != mt) {
       (void)hipFree(h->tail_img[q]);
       h->tail_img[q] = nullptr;
-      hipError_t e = hipMalloc(&h->tail_img[q], bytes);
+      hipError_t e = hipMalloc(&h->tail_img[q], (size_t)D.coef_words * sizeof(T));
       if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(k_tail_image<T>, dim3(16), dim3(256), 0, h->stream, dlev, mt, (T*)h->tail_img[q]);
+      hipLaunchKernelGGL(k_tail_image<T>, dim3(16), dim3(256), 0, h->stream, dlev, D, (T*)h->tail_img[q]);
       h->tail_img_m[q] = mt;
     }
     static bool attr_set[2] = {false, false};
@@ -791,8 +950,8 @@ static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top
       if (e != hipSuccess) return e;
       attr_set[sizeof(T) == 8] = true;
     }
-    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(TAIL_THREADS), bytes, h->stream, dlev, mt, nu1, nu2, wx, wy, zi,
-                       (const void*)h->tail_img[q], bytes);
+    hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(TAIL_THREADS), bytes, h->stream, dlev, D, nu1, nu2, wx, wy, zi,
+                       (const void*)h->tail_img[q]);
   } else {
     if (zi) {
       if constexpr (sizeof(T) == 4) {
