@@ -31,9 +31,15 @@ constexpr int NT = 512;   // threads per fused-level workgroup
 constexpr int TAIL_THREADS = 1024;
 constexpr size_t TAIL_LDS_MAX = 150 * 1024;
 
-template <typename T> struct Tile;
-template <> struct Tile<float> { static constexpr int TI = 32; };
-template <> struct Tile<double> { static constexpr int TI = 16; };
+// Tile geometry of the fused level kernels: rows TI x columns FJ per
+// workgroup of NT threads.  Large tiles on levels with enough of them to
+// fill the chip; small tiles (four times as many workgroups, each with a
+// shorter serial chain) on the coarser levels, which are latency-bound.
+template <typename T, bool SMALL> struct Geo;
+template <> struct Geo<float, false> { static constexpr int TI = 32, FJ = 64, NT = 512; };
+template <> struct Geo<double, false> { static constexpr int TI = 16, FJ = 64, NT = 512; };
+template <> struct Geo<float, true> { static constexpr int TI = 16, FJ = 32, NT = 256; };
+template <> struct Geo<double, true> { static constexpr int TI = 8, FJ = 32, NT = 256; };
 
 __device__ __forceinline__ int wrapm(int i, int N)
 {
@@ -201,9 +207,9 @@ __global__ __launch_bounds__(256) void k_finish_sum(const double* part, int nb, 
 // fused fine-level kernels
 // ---------------------------------------------------------------------------
 
-template <typename T>
+template <typename T, bool SM>
 struct TileLds {
-  static constexpr int TI = Tile<T>::TI;
+  static constexpr int TI = Geo<T, SM>::TI, FJ = Geo<T, SM>::FJ;
   static constexpr int HP = (FJ + 2 * HMAX + 2) / 2;  // one colour half of a row
   static constexpr int P = 2 * HP;                    // pitch
   static constexpr int RMAX = TI + 2 * HMAX + 1;      // rows
@@ -214,13 +220,14 @@ struct TileLds {
 // halo H reads: coarse rows cbi .. cbi + CRn - 1 (unwrapped), likewise
 // columns.  Fine cell fi interpolates from coarse rows floor((fi - 1) / 2)
 // and the next one (interpolate_power2_MG, multigrid.c:827-862).
-template <typename T>
+template <typename T, bool SM>
 struct CoarseWin {
-  static constexpr int RMAX = Tile<T>::TI / 2 + HMAX + 2, CMAX = FJ / 2 + HMAX + 2;
+  static constexpr int TI = Geo<T, SM>::TI, FJ = Geo<T, SM>::FJ;
+  static constexpr int RMAX = TI / 2 + HMAX + 2, CMAX = FJ / 2 + HMAX + 2;
   int cbi, cbj, CRn, CCn;
   __device__ CoarseWin(int H, int i0, int j0)
   {
-    const int R = Tile<T>::TI + 2 * H, C = FJ + 2 * H;
+    const int R = TI + 2 * H, C = FJ + 2 * H;
     cbi = (i0 - H - 1) >> 1;  // arithmetic shift: floor for negative values
     cbj = (j0 - H - 1) >> 1;
     CRn = ((i0 - H + R - 2) >> 1) + 2 - cbi;
@@ -234,16 +241,17 @@ struct CoarseWin {
 // cwin first).  Every load is issued before the first LDS store, indices
 // clamped to the last cell and wrapped onto the torus, so the whole stage
 // is one round trip to memory; the stores are guarded.
-template <typename T, int MODE>
-__device__ void stage_tile(TileLds<T>& S, const MgLevel& f, const T* src, int H, int i0, int j0, int tid,
-                           T* cwin = nullptr, const CoarseWin<T>* W = nullptr, const T* coarse = nullptr, int CX = 0,
+template <typename T, bool SM, int MODE>
+__device__ void stage_tile(TileLds<T, SM>& S, const MgLevel& f, const T* src, int H, int i0, int j0, int tid,
+                           T* cwin = nullptr, const CoarseWin<T, SM>* W = nullptr, const T* coarse = nullptr, int CX = 0,
                            int CY = 0, int wallEW = 0, int wallNS = 0)
 {
-  constexpr int P = TileLds<T>::P, HP = TileLds<T>::HP;
+  constexpr int P = TileLds<T, SM>::P, HP = TileLds<T, SM>::HP;
+  constexpr int TI = Geo<T, SM>::TI, FJ = Geo<T, SM>::FJ, NT = Geo<T, SM>::NT;
   // fine / coarse cells per thread: one batch covers the largest tile
-  constexpr int U = ((Tile<T>::TI + 2 * HMAX + 1) * (FJ + 2 * HMAX + 1) + NT - 1) / NT, UC = 2;
-  static_assert(CoarseWin<T>::RMAX * CoarseWin<T>::CMAX <= UC * NT, "coarse stage is one batch");
-  const int R = TileLds<T>::TI + 2 * H, C = FJ + 2 * H, C1 = C + 1;
+  constexpr int U = ((TI + 2 * HMAX + 1) * (FJ + 2 * HMAX + 1) + NT - 1) / NT, UC = 2;
+  static_assert(CoarseWin<T, SM>::RMAX * CoarseWin<T, SM>::CMAX <= UC * NT, "coarse stage is one batch");
+  const int R = TI + 2 * H, C = FJ + 2 * H, C1 = C + 1;
   const float inv = 1.0f / (float)C1;
   const int X = f.Nx, Y = f.Ny;
   const T* gOw = LvPtr<T>::Ow(f);
@@ -319,16 +327,17 @@ __device__ void stage_tile(TileLds<T>& S, const MgLevel& f, const T* src, int H,
 
 // nu1 sweeps (colour 0 first) + residual + 2:1 restriction.  Reads f.pi
 // (or zero when zero_init), writes f.tmp (smoothed), c.rhs and c.pi = 0.
-template <typename T>
-__global__ __launch_bounds__(NT) void k_down(MgLevel f, MgLevel c, int nu, int zero_init)
+template <typename T, bool SM>
+__global__ __launch_bounds__((Geo<T, SM>::NT)) void k_down(MgLevel f, MgLevel c, int nu, int zero_init)
 {
-  constexpr int TI = Tile<T>::TI, P = TileLds<T>::P, HP = TileLds<T>::HP;
-  __shared__ TileLds<T> S;
+  constexpr int TI = Geo<T, SM>::TI, FJ = Geo<T, SM>::FJ, NT = Geo<T, SM>::NT;
+  constexpr int P = TileLds<T, SM>::P, HP = TileLds<T, SM>::HP;
+  __shared__ TileLds<T, SM> S;
   const int tid = threadIdx.x;
   const int H = 2 * nu + 1, R = TI + 2 * H, C = FJ + 2 * H;
   const int i0 = blockIdx.y * TI, j0 = blockIdx.x * FJ;
-  if (zero_init) stage_tile<T, 0>(S, f, nullptr, H, i0, j0, tid);
-  else stage_tile<T, 1>(S, f, LvPtr<T>::pi(f), H, i0, j0, tid);
+  if (zero_init) stage_tile<T, SM, 0>(S, f, nullptr, H, i0, j0, tid);
+  else stage_tile<T, SM, 1>(S, f, LvPtr<T>::pi(f), H, i0, j0, tid);
   __syncthreads();
   const int base = (i0 + j0) & 1;  // parity of tile cell (0, 0): 2H is even
   for (int pass = 0; pass < 2 * nu; pass++) {
@@ -368,18 +377,19 @@ __global__ __launch_bounds__(NT) void k_down(MgLevel f, MgLevel c, int nu, int z
 //    its FP64 defect r = L v - rhs goes to f.rhs32 with block partials of
 //    r^2 -- the same expressions as k_defect, fused.
 // Both need e one cell beyond the tile (halo one wider).
-template <typename T>
-__global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wallEW, int wallNS, int norm, double* part,
-                                           const double* pcur, double* pout)
+template <typename T, bool SM>
+__global__ __launch_bounds__((Geo<T, SM>::NT)) void k_up(MgLevel f, MgLevel c, int nu, int wallEW, int wallNS, int norm,
+                                                      double* part, const double* pcur, double* pout)
 {
-  constexpr int TI = Tile<T>::TI, P = TileLds<T>::P, HP = TileLds<T>::HP;
-  __shared__ TileLds<T> S;
+  constexpr int TI = Geo<T, SM>::TI, FJ = Geo<T, SM>::FJ, NT = Geo<T, SM>::NT;
+  constexpr int P = TileLds<T, SM>::P, HP = TileLds<T, SM>::HP;
+  __shared__ TileLds<T, SM> S;
   const int tid = threadIdx.x;
   const int H = 2 * nu + (norm ? 1 : 0), R = TI + 2 * H, C = FJ + 2 * H;
   const int i0 = blockIdx.y * TI, j0 = blockIdx.x * FJ;
-  __shared__ T cwin[CoarseWin<T>::RMAX * CoarseWin<T>::CMAX];
-  const CoarseWin<T> W(H, i0, j0);
-  stage_tile<T, 2>(S, f, LvPtr<T>::tmp(f), H, i0, j0, tid, cwin, &W, LvPtr<T>::pi(c), c.Nx, c.Ny, wallEW, wallNS);
+  __shared__ T cwin[CoarseWin<T, SM>::RMAX * CoarseWin<T, SM>::CMAX];
+  const CoarseWin<T, SM> W(H, i0, j0);
+  stage_tile<T, SM, 2>(S, f, LvPtr<T>::tmp(f), H, i0, j0, tid, cwin, &W, LvPtr<T>::pi(c), c.Nx, c.Ny, wallEW, wallNS);
   __syncthreads();
   const int base = (i0 + j0) & 1;
   for (int pass = 0; pass < 2 * nu; pass++) {
@@ -393,7 +403,7 @@ __global__ __launch_bounds__(NT) void k_up(MgLevel f, MgLevel c, int nu, int wal
   double ss = 0;
   if (norm == 2) {
     // FP64 loads of V cells per thread issued together (clamped indices)
-    constexpr int V = 2;
+    constexpr int V = (TI * FJ >= 2 * NT) ? 2 : 1;
     static_assert((TI * FJ) % (V * NT) == 0, "whole batches");
     for (int base = tid; base < TI * FJ; base += V * NT) {
       double pc[V], pe[V], pw[V], pn[V], ps[V], osn[V], osp[V], owe[V], owp[V], osm[V], rh[V];
@@ -900,6 +910,20 @@ size_t tail_bytes(const pg_handle* h, int m)
 
 }  // namespace
 
+template <typename T, bool SM>
+dim3 tile_grid(const MgLevel& l)
+{
+  return dim3((l.Ny + Geo<T, SM>::FJ - 1) / Geo<T, SM>::FJ, (l.Nx + Geo<T, SM>::TI - 1) / Geo<T, SM>::TI);
+}
+
+// small tiles where large ones would leave most compute units idle
+template <typename T>
+bool small_tiles(const MgLevel& l)
+{
+  const dim3 g = tile_grid<T, false>(l);
+  return (long)g.x * g.y < 256;
+}
+
 // Level from which the coarse tail runs, and whether it fits in LDS.
 template <typename T>
 static int tail_level(const pg_handle* h, bool* in_lds)
@@ -924,11 +948,14 @@ static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top
   bool lds = false;
   const int mt = tail_level<T>(h, &lds);
   const int wx = h->G.wallx, wy = h->G.wally;
-  constexpr int TI = Tile<T>::TI;
   for (int n = top; n > mt; n--) {
     const MgLevel& l = h->lev[n];
-    hipLaunchKernelGGL(k_down<T>, dim3((l.Ny + FJ - 1) / FJ, (l.Nx + TI - 1) / TI), dim3(NT), 0, h->stream, l, h->lev[n - 1],
-                       nu1, (n == top) ? zero_init_top : 0);
+    const int z = (n == top) ? zero_init_top : 0;
+    const bool sm = small_tiles<T>(l);
+    const dim3 grid = sm ? tile_grid<T, true>(l) : tile_grid<T, false>(l);
+    const dim3 block(sm ? Geo<T, true>::NT : Geo<T, false>::NT);
+    if (sm) hipLaunchKernelGGL((k_down<T, true>), grid, block, 0, h->stream, l, h->lev[n - 1], nu1, z);
+    else hipLaunchKernelGGL((k_down<T, false>), grid, block, 0, h->stream, l, h->lev[n - 1], nu1, z);
   }
   const MgLevel* dlev = reinterpret_cast<const MgLevel*>(h->partials + h->npartials);
   const int zi = (mt == top) ? zero_init_top : 0;
@@ -965,8 +992,15 @@ static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top
   for (int n = mt + 1; n <= top; n++) {
     const MgLevel& l = h->lev[n];
     const int norm = (n == top) ? norm_top : 0;
-    const dim3 grid((l.Ny + FJ - 1) / FJ, (l.Nx + TI - 1) / TI);
-    hipLaunchKernelGGL(k_up<T>, grid, dim3(NT), 0, h->stream, l, h->lev[n - 1], nu2, wx, wy, norm, h->partials, pcur, pout);
+    const bool sm = small_tiles<T>(l);
+    const dim3 grid = sm ? tile_grid<T, true>(l) : tile_grid<T, false>(l);
+    const dim3 block(sm ? Geo<T, true>::NT : Geo<T, false>::NT);
+    if (sm)
+      hipLaunchKernelGGL((k_up<T, true>), grid, block, 0, h->stream, l, h->lev[n - 1], nu2, wx, wy, norm, h->partials, pcur,
+                         pout);
+    else
+      hipLaunchKernelGGL((k_up<T, false>), grid, block, 0, h->stream, l, h->lev[n - 1], nu2, wx, wy, norm, h->partials,
+                         pcur, pout);
     if (norm && want_sum)  // the block partials are only summed when the norm is read
       hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, (int)(grid.x * grid.y), &h->ctl->sumsq,
                          h->mapped);
