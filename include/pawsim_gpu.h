@@ -200,7 +200,7 @@ int pg_learn_samples(const pg_handle *h);
 int pg_set_solve_cycles(pg_handle *h, uint32_t cycles);
 /* pg_set_mg_tail: the GPU multigrid hands every level of at most `cells`
  * cells (and everything coarser) to one workgroup with the arrays in LDS,
- * when they fit; 0 restores the default (4096). */
+ * when they fit; 0 restores the default (1024). */
 int pg_set_mg_tail(pg_handle *h, uint32_t cells);
 
 /* Ensemble: independent members (any grid sizes, GPU-MG solver, no

@@ -56,14 +56,16 @@ hipError_t fill_prognostic(pg_handle* h)
 }
 
 // The reference refreshes the prognostic halos after every change of u, v,
-// h.  A refresh depends only on the owned values, except on a periodic-x /
-// wall-y domain, where the wall-y phase skips the x-ghost columns and the
-// corner ghosts keep values from the previous refresh (field.c:137-198); the
-// residual and divergence norms read ghost values too.  Those cases refresh
-// at every reference point; otherwise a refresh is deferred to the next
-// reader of ghost values (the tendency, or the end of the correction) --
-// nothing between the AB update and the velocity correction reads them.
-bool exact_refresh(const pg_handle* h) { return (!h->G.wallx && h->G.wally) || h->cfg.pressureTiming; }
+// h.  On a closed domain (walls on all sides) nothing between the AB update
+// and the velocity correction reads a ghost value that the update changed:
+// the RHS reads the wall-face velocities, which stay zero, and builds the
+// face thickness with modular indices.  There the refresh is deferred to the
+// next reader (the tendency, or the end of the correction).  A periodic
+// direction makes the RHS read the periodic images of updated velocities,
+// and on a periodic-x / wall-y domain the corner ghosts keep values from the
+// previous refresh (field.c:137-198); the residual and divergence norms read
+// ghost values too.  Those cases refresh at every reference point.
+bool exact_refresh(const pg_handle* h) { return !(h->G.wallx && h->G.wally) || h->cfg.pressureTiming; }
 
 hipError_t refresh_after_change(pg_handle* h)
 {
@@ -525,7 +527,7 @@ int pg_set_solve_cycles(pg_handle* h, uint32_t cycles)
 int pg_set_mg_tail(pg_handle* h, uint32_t cells)
 {
   if (!h) return PG_EARG;
-  h->mg_tail_cells = cells ? (long)cells : 4096;
+  h->mg_tail_cells = cells ? (long)cells : 1024;
   return PG_OK;
 }
 

@@ -76,7 +76,7 @@ struct pg_handle {
   double* diag = nullptr;              // diagnostics reduction scratch
   size_t ndiag = 0;
   int fixed_cycles = 0;                // > 0: GPU-MG solves run exactly this many cycles with no host reads
-  long mg_tail_cells = 4096;           // largest level the LDS coarse tail starts from
+  long mg_tail_cells = 1024;           // largest level the LDS coarse tail starts from
   void* tail_img[2] = {};              // packed LDS image of the coarse-tail coefficients (FP32, FP64)
   int tail_img_m[2] = {-1, -1};        // tail level the image was built for
   bool fast_tendency = true;           // inner LDS kernel + edge ring (bitwise equal to the per-point kernel)

@@ -21,7 +21,9 @@
 #include "pg_internal.h"
 #include "pg_mg_common.h"
 
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace {
 
@@ -467,9 +469,10 @@ __global__ __launch_bounds__((Geo<T, SM>::NT)) void k_up(MgLevel f, MgLevel c, i
 // coff[n] (the region a packed global image is copied into).
 constexpr int TAIL_MAXLEV = 16;
 struct TailDims {
-  int m, mw;        // top level; levels 0..mw run in wavefront 0
+  int m, mw;        // top level; level mw (at most 8 x 8 cells) is solved exactly
   int coef_base;    // element offset of the coefficient region (16-byte aligned)
   int coef_words;   // elements of the coefficient image (multiple of 16 bytes)
+  int pinv_off;     // offset in the image of the pseudo-inverse of level mw's operator
   int pow2;         // bit n: level n transfers 2:1 to level n - 1
   int X[TAIL_MAXLEV], Y[TAIL_MAXLEV], woff[TAIL_MAXLEV], coff[TAIL_MAXLEV];
 };
@@ -726,10 +729,11 @@ __global__ void k_tail_image(const MgLevel* lv, TailDims D, T* img)
   for (int idx = end + t0; idx < D.coef_words; idx += stride) img[idx] = 0;
 }
 
-// V(nu1, nu2) over levels m .. 0 with every array in LDS.  pi of level m is
-// loaded (or zero) and written back; the transfers are the reference's
-// (2:1 or general).  Levels 0..mw (at most 8 x 8 cells) run in wavefront 0
-// alone.  The coefficient image and the top level's pi and rhs are loaded
+// V(nu1, nu2) over levels m .. mw with every array in LDS.  pi of level m
+// is loaded (or zero) and written back; the transfers are the reference's
+// (2:1 or general).  The first level of at most 8 x 8 cells (mw) is not
+// smoothed but solved exactly, by the pseudo-inverse of its operator (the
+// levels below it would only approximate that solution).  The coefficient image and the top level's pi and rhs are loaded
 // in one batch; the coarser levels' pi, rhs and tmp are written before they
 // are read (restriction, residual), so they need no initial values.
 template <typename T>
@@ -772,11 +776,17 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(const MgLevel* lv, TailDi
   __syncthreads();
   const int mw = D.mw;
   for (int n = m; n > mw; n--) tail_down<T, BlockScope>(lv, D, mem, n, nu1, wallEW, wallNS, tid);
-  if (tid < 64) {
-    for (int n = mw; n >= 1; n--) tail_down<T, WaveScope>(lv, D, mem, n, nu1, wallEW, wallNS, tid);
-    if (tid == 0 && D.X[0] * D.Y[0] == 1) tview(mem, D, 0).pi[0] = 0;
-    WaveScope::sync();
-    for (int n = 1; n <= mw; n++) tail_up<T, WaveScope>(lv, D, mem, n, nu2, wallEW, wallNS, tid);
+  {
+    // level mw exactly: pi = A+ rhs with the pseudo-inverse of its operator
+    // (symmetric, stored in the image; lane p forms row p)
+    const TV<T> Lw = tview(mem, D, mw);
+    const int N = Lw.X * Lw.Y;
+    const T* A = mem + D.coef_base + D.pinv_off;
+    if (tid < N) {
+      T acc = 0;
+      for (int q = 0; q < N; q++) acc += A[q * N + tid] * Lw.rhs[q];
+      Lw.pi[tid] = acc;
+    }
   }
   __syncthreads();
   for (int n = mw + 1; n <= m; n++) tail_up<T, BlockScope>(lv, D, mem, n, nu2, wallEW, wallNS, tid);
@@ -847,6 +857,30 @@ __global__ void k_scatter_mean(PgGrid G, const double* f, const double* sum, dou
   gf[pg_at(G, i + G.g, j + G.g)] = f[(size_t)i * G.ny + j] - mean;
 }
 
+// Owned index feeding ghosted index k of an axis (scalar halo kind): the
+// cell itself, its wall reflection (field.c:158-241) or its periodic image
+// (field.c:121-150).
+__device__ __forceinline__ int scalar_src(int k, int n, int g, int wall)
+{
+  const int o = k - g;
+  if (o >= 0 && o < n) return o;
+  if (wall) return o < 0 ? -1 - o : 2 * n - 1 - o;
+  return o < 0 ? o + n : o - n;
+}
+
+// The whole ghosted field gf = f - sum / N, halo included: every ghost cell
+// takes the owned cell the scalar fill would copy into it.  Used where that
+// fill depends on owned values only (not on a periodic-x / wall-y domain,
+// whose corner ghosts keep older values).
+__global__ void k_scatter_mean_ghost(PgGrid G, const double* f, const double* sum, double* gf)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= G.sy) return;
+  const double mean = *sum / ((double)G.nx * (double)G.ny);
+  const int io = scalar_src(i, G.nx, G.g, G.wallx), jo = scalar_src(j, G.ny, G.g, G.wally);
+  gf[(size_t)i * G.sy + j] = f[(size_t)io * G.ny + jo] - mean;
+}
+
 __global__ void k_copy64(const double* src, double* dst, size_t n)
 {
   for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) dst[p] = src[p];
@@ -874,6 +908,71 @@ bool odd_periodic(const pg_handle* h, const MgLevel& l)
   return (!h->G.wallx && (l.Nx & 1) && l.Nx > 1) || (!h->G.wally && (l.Ny & 1) && l.Ny > 1);
 }
 
+// Pseudo-inverse of the operator L pi = sum O pi_nb - Osum pi of level n
+// (at most 64 cells; the coefficients the smoother uses, with the modular
+// neighbours of the level), by cyclic Jacobi rotations of the symmetric
+// matrix: L+ = sum over |lambda| > 1e-10 max|lambda| of v v^T / lambda.
+int level_pinv(pg_handle* h, int n, std::vector<double>& out)
+{
+  const MgLevel& l = h->lev[n];
+  const int X = l.Nx, Y = l.Ny, N = X * Y;
+  std::vector<double> Ow(N), Os(N);
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  PG_CHECK(h, hipMemcpy(Ow.data(), l.Ow, N * sizeof(double), hipMemcpyDeviceToHost));
+  PG_CHECK(h, hipMemcpy(Os.data(), l.Os, N * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> a((size_t)N * N, 0.0), v((size_t)N * N, 0.0);
+  for (int i = 0; i < X; i++)
+    for (int j = 0; j < Y; j++) {
+      const int p = i * Y + j, e = ((i + 1) % X) * Y + j, w = ((i + X - 1) % X) * Y + j;
+      const int no = i * Y + (j + 1) % Y, so = i * Y + (j + Y - 1) % Y;
+      a[(size_t)p * N + no] += Os[no];
+      a[(size_t)p * N + so] += Os[p];
+      a[(size_t)p * N + e] += Ow[e];
+      a[(size_t)p * N + w] += Ow[p];
+      a[(size_t)p * N + p] -= Ow[p] + Ow[e] + Os[p] + Os[no];
+    }
+  for (int p = 0; p < N; p++) v[(size_t)p * N + p] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0, diag = 0;
+    for (int p = 0; p < N; p++)
+      for (int q = 0; q < N; q++) (p == q ? diag : off) += a[(size_t)p * N + q] * a[(size_t)p * N + q];
+    if (off <= 1e-30 * (diag + 1e-300)) break;
+    for (int p = 0; p < N - 1; p++)
+      for (int q = p + 1; q < N; q++) {
+        const double apq = a[(size_t)p * N + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double theta = (a[(size_t)q * N + q] - a[(size_t)p * N + p]) / (2 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+        const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < N; k++) {  // rotate rows/columns p, q
+          const double akp = a[(size_t)k * N + p], akq = a[(size_t)k * N + q];
+          a[(size_t)k * N + p] = c * akp - s * akq;
+          a[(size_t)k * N + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < N; k++) {
+          const double apk = a[(size_t)p * N + k], aqk = a[(size_t)q * N + k];
+          a[(size_t)p * N + k] = c * apk - s * aqk;
+          a[(size_t)q * N + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < N; k++) {
+          const double vkp = v[(size_t)k * N + p], vkq = v[(size_t)k * N + q];
+          v[(size_t)k * N + p] = c * vkp - s * vkq;
+          v[(size_t)k * N + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  double lmax = 0;
+  for (int k = 0; k < N; k++) lmax = std::max(lmax, std::fabs(a[(size_t)k * N + k]));
+  out.assign((size_t)N * N, 0.0);
+  for (int k = 0; k < N; k++) {
+    const double lam = a[(size_t)k * N + k];
+    if (std::fabs(lam) <= 1e-10 * lmax) continue;
+    for (int p = 0; p < N; p++)
+      for (int q = 0; q < N; q++) out[(size_t)p * N + q] += v[(size_t)p * N + k] * v[(size_t)q * N + k] / lam;
+  }
+  return PG_OK;
+}
+
 // LDS layout of a tail over levels 0..m; returns the LDS bytes (0 when the
 // hierarchy is too deep for TailDims).
 template <typename T>
@@ -893,10 +992,13 @@ size_t tail_dims(const pg_handle* h, int m, TailDims* D)
     c += 3 * d.X[n] * d.Y[n];
     if (h->lev[n].pow2) d.pow2 |= 1 << n;
   }
-  d.coef_base = (w + A - 1) / A * A;
-  d.coef_words = (c + A - 1) / A * A;
   d.mw = 0;
   while (d.mw < m && d.X[d.mw + 1] <= 8 && d.Y[d.mw + 1] <= 8) d.mw++;
+  const int nw = d.X[d.mw] * d.Y[d.mw];
+  d.pinv_off = (c + A - 1) / A * A;
+  c = d.pinv_off + nw * nw;
+  d.coef_base = (w + A - 1) / A * A;
+  d.coef_words = (c + A - 1) / A * A;
   if (D) *D = d;
   return (size_t)(d.coef_base + d.coef_words) * sizeof(T);
 }
@@ -969,6 +1071,12 @@ static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top
       hipError_t e = hipMalloc(&h->tail_img[q], (size_t)D.coef_words * sizeof(T));
       if (e != hipSuccess) return e;
       hipLaunchKernelGGL(k_tail_image<T>, dim3(16), dim3(256), 0, h->stream, dlev, D, (T*)h->tail_img[q]);
+      std::vector<double> pinv;
+      if (level_pinv(h, D.mw, pinv) != PG_OK) return hipErrorUnknown;
+      std::vector<T> pt(pinv.begin(), pinv.end());
+      e = hipMemcpyAsync((T*)h->tail_img[q] + D.pinv_off, pt.data(), pt.size() * sizeof(T), hipMemcpyHostToDevice, h->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) return e;
       h->tail_img_m[q] = mt;
     }
     static bool attr_set[2] = {false, false};
@@ -1037,6 +1145,12 @@ hipError_t pg_scatter_zero_mean(pg_handle* h)
   const PgGrid& G = h->G;
   hipLaunchKernelGGL(k_sum, dim3(256), dim3(256), 0, h->stream, top.pi, (size_t)top.Nx * top.Ny, h->partials);
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, &h->ctl->pisum, (double*)nullptr);
+  const bool stale_corners = !G.wallx && G.wally;
+  if (!stale_corners && !h->cfg.pressureTiming && G.nx >= G.g && G.ny >= G.g) {
+    hipLaunchKernelGGL(k_scatter_mean_ghost, dim3((G.sy + 255) / 256, G.sx), dim3(256), 0, h->stream, G, top.pi,
+                       &h->ctl->pisum, h->pi);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_scatter_mean, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, top.pi, &h->ctl->pisum,
                      h->pi);
   hipError_t e = hipGetLastError();

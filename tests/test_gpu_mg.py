@@ -81,25 +81,38 @@ def test_gpu_mg_reaches_relative_tolerance(solver, case):
             assert rel <= 2 * tol, rel
 
 
+RUN_DOMAINS = {
+    # closed basin; doubly periodic; wall-x / periodic-y; periodic-x / wall-y
+    "gyre64": dict(kind="gyre", nx=64, ny=64, A2=50.0),
+    "periodic48": dict(kind="periodic_box", nx=48, ny=48),
+    "wallx_pery48x32": dict(kind="periodic_box", nx=48, ny=32, useWallEW=1),
+    "chan64x32": dict(kind="channel", nx=64, ny=32, dx=20e3),
+}
+
+
 @pytest.mark.parametrize("solver", [pawsim_amd.SOLVER_GPU_MG, pawsim_amd.SOLVER_GPU_MG_MIXED])
-def test_gpu_mg_run_matches_converged_oracle(solver, tmp_path, oracle_exe):
-    """Gyre 64x64x2, 10 AB3 steps: GPU MG (rel. tol 1e-14) vs oracle gathered MG
-    (max-norm tol 1e-17, i.e. converged to round-off); u, v, h within 1e-12."""
-    common = dict(nx=64, ny=64, nt=10, A2=50.0)
-    s_gpu = inputs.gyre(solver=solver, tol=1e-14, maxiters=400, name="g", **common)
-    s_ora = inputs.gyre(solver=1, tol=1e-17, maxiters=3000, name="o", **common)
+@pytest.mark.parametrize("domain", sorted(RUN_DOMAINS))
+def test_gpu_mg_run_matches_converged_oracle(solver, domain, tmp_path, oracle_exe):
+    """10 AB3 steps: GPU MG (rel. tol 1e-14) vs oracle gathered MG (max-norm
+    tol 1e-17, i.e. converged to round-off); u, v, h within 1e-12 relative,
+    pi within 1e-10 modulo its mean (the GPU solves remove it)."""
+    spec = dict(RUN_DOMAINS[domain])
+    kind = spec.pop("kind")
+    s_gpu = getattr(inputs, kind)(solver=solver, tol=1e-14, maxiters=400, nt=10, name="g",
+                                  mixed_precision=0, **spec)
+    s_ora = getattr(inputs, kind)(solver=1, tol=1e-17, maxiters=3000, nt=10, name="o", **spec)
     run_exe(pawsim_amd.DRIVER_PATH, s_gpu.write(str(tmp_path / "gi")), str(tmp_path / "gpu"))
     run_exe(oracle_exe, s_ora.write(str(tmp_path / "oi")), str(tmp_path / "ora"))
+    nx, ny, nl = s_gpu.nx, s_gpu.ny, s_gpu.nlay
     for n in (1, 5, 10):
-        g = inputs.read_outputs(str(tmp_path / "gpu"), 64, 64, 2, n)
-        o = inputs.read_outputs(str(tmp_path / "ora"), 64, 64, 2, n)
+        g = inputs.read_outputs(str(tmp_path / "gpu"), nx, ny, nl, n)
+        o = inputs.read_outputs(str(tmp_path / "ora"), nx, ny, nl, n)
         for key in ("U", "V", "H"):
             rel = np.max(np.abs(g[key] - o[key])) / np.max(np.abs(o[key]))
-            print(f"[gpu mg parity] solver={solver} n={n} {key} rel={rel:.3e}", flush=True)
+            print(f"[gpu mg parity] {domain} solver={solver} n={n} {key} rel={rel:.3e}", flush=True)
             assert rel <= 1e-12, (key, n, rel)
-        # pi is fixed up to a constant; the GPU solves remove its mean
         gp, op = g["P"], o["P"] - np.mean(o["P"])
         assert abs(np.mean(gp)) <= 1e-12 * np.max(np.abs(gp))
         rel = np.max(np.abs(gp - op)) / np.max(np.abs(op))
-        print(f"[gpu mg parity] solver={solver} n={n} P (zero mean) rel={rel:.3e}", flush=True)
+        print(f"[gpu mg parity] {domain} solver={solver} n={n} P (zero mean) rel={rel:.3e}", flush=True)
         assert rel <= 1e-10, ("P", n, rel)
