@@ -136,12 +136,15 @@ int run_solver(pg_handle* h, uint32_t solver, pg_solve_stats* st)
   if (!fixed) PG_CHECK(h, hipEventRecord(h->ev0, h->stream));
   if (solver == PG_SOLVER_SOR) rc = pg_sor_solve(h, &iters);
   else if (solver == PG_SOLVER_MG_REPLICA) rc = pg_mg_replica_solve(h, &iters);
-  else if (pg_fused_ok(h)) rc = pg_fused_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel, &learned, &fallback, &ratio);
-  else rc = pg_gpu_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel);
+  const bool fused = solver >= PG_SOLVER_GPU_MG && pg_fused_ok(h);
+  if (fused) rc = pg_fused_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel, &learned, &fallback, &ratio);
+  else if (solver >= PG_SOLVER_GPU_MG) rc = pg_gpu_mg_solve(h, solver == PG_SOLVER_GPU_MG_MIXED, &iters, &rel);
   if (rc) return rc;
   float ms = 0;
   if (!fixed) {
-    PG_CHECK(h, hipEventRecord(h->ev1, h->stream));
+    // the fused solve records ev1 before its last (converged) check and has
+    // waited on it; the other solvers are timed here
+    if (!fused) PG_CHECK(h, hipEventRecord(h->ev1, h->stream));
     PG_CHECK(h, hipEventSynchronize(h->ev1));
     PG_CHECK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
   }

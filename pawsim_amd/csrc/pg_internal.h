@@ -65,7 +65,9 @@ struct pg_handle {
   PgCtl *ctl = nullptr;                // device control block
   PgCtl *ctl_host = nullptr;           // pinned mirror
   double* mapped_host = nullptr;       // coherent host memory the convergence sums are also stored to:
-  double* mapped = nullptr;            //   [0] ||r||^2, [1] ||rhs||^2 (host / device addresses)
+  double* mapped = nullptr;            //   [0] ||r||^2 after an even number of cycles, [1] ||rhs||^2,
+                                       //   [2] ||r||^2 after an odd number (host / device addresses)
+  double* mapped_r = nullptr;          // the slot the next residual sum goes to
   std::vector<MgLevel> lev;            // [0] = coarsest
   bool mg_built = false;
   struct PgLearn* learn = nullptr;     // learned preconditioner (pg_learn.hip), optional

@@ -1111,7 +1111,7 @@ static hipError_t fused_vcycle(pg_handle* h, int nu1, int nu2, int zero_init_top
                          pcur, pout);
     if (norm && want_sum)  // the block partials are only summed when the norm is read
       hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, (int)(grid.x * grid.y), &h->ctl->sumsq,
-                         h->mapped);
+                         h->mapped_r ? h->mapped_r : h->mapped);
   }
   if (norm_done) *norm_done = norm_top && mt < top;
   return hipGetLastError();
@@ -1124,7 +1124,7 @@ hipError_t pg_launch_defect(pg_handle* h, const double* pcur, const float* e, co
   const dim3 dg((top.Ny + DJ - 1) / DJ, (top.Nx + DI * DROWS - 1) / (DI * DROWS));
   hipLaunchKernelGGL(k_defect, dg, dim3(DJ, DI), 0, h->stream, top, pcur, e, escale, pout, rhs32, h->partials);
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, (int)(dg.x * dg.y), sumsq_out,
-                     sumsq_out == &h->ctl->sumsq ? h->mapped : nullptr);
+                     sumsq_out == &h->ctl->sumsq ? (h->mapped_r ? h->mapped_r : h->mapped) : nullptr);
   return hipGetLastError();
 }
 
@@ -1201,6 +1201,8 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
   };
   (void)nbd;
   PG_CHECK(h, pg_gather_guess(h));
+  auto slot = [&](int cycles) { return h->mapped + ((cycles & 1) ? 2 : 0); };  // residual after `cycles`
+  h->mapped_r = slot(0);
   hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, h->stream, top.rhs, N, h->partials);
   hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, &h->ctl->rhs_sumsq, h->mapped + 1);
   PG_CHECK(h, defect(top.pi, nullptr, nullptr, mixed ? top.rhs32 : nullptr));
@@ -1218,25 +1220,38 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
       alt = top.tmp;
     }
   }
+  // The host first looks at the residual after the previous solve's cycle
+  // count (one synchronisation when the count repeats).  The residual of the
+  // cycle before is finished too, in the other host-mapped slot: when it
+  // already met the tolerance the next solve is predicted one cycle shorter.
   const int maxit = (int)h->cfg.maxiters;
-  const int first_check = h->mg_pred > 1 ? h->mg_pred - 1 : 0;
-  int it = 0;
+  const int first_check = h->mg_pred;
+  int it = 0, next_pred = -1;
   double r = 0;
   const int fixed = h->fixed_cycles;
   for (;;) {
     if (fixed > 0) {
       if (it >= fixed) break;
     } else if (it >= first_check || it >= maxit) {
-      // the finishing kernels also store both sums in host-mapped memory
+      // ev1 marks the end of the cycles for the solve time (no extra wait)
+      PG_CHECK(h, hipEventRecord(h->ev1, h->stream));
       PG_CHECK(h, hipStreamSynchronize(h->stream));
       const double rr = *(volatile double*)(h->mapped_host + 1);
-      const double ss = *(volatile double*)(h->mapped_host);
+      const double ss = *(volatile double*)(h->mapped_host + ((it & 1) ? 2 : 0));
       r = (rr > 0) ? std::sqrt(ss / rr) : 0.0;
-      if (r <= h->cfg.pi_tol || it >= maxit) break;
+      if (r <= h->cfg.pi_tol || it >= maxit) {
+        if (it == first_check && it >= 1 && rr > 0) {
+          const double sp = *(volatile double*)(h->mapped_host + (((it - 1) & 1) ? 2 : 0));
+          if (std::sqrt(sp / rr) <= h->cfg.pi_tol) next_pred = it - 1;
+        }
+        break;
+      }
     }
-    // the residual norm is read after this cycle (convergence test, or the
-    // worst-residual record of a fixed-cycle solve)
-    const bool want_sum = fixed > 0 ? it + 1 >= fixed : (it + 1 >= first_check || it + 1 >= maxit);
+    // the residual norm is read after this cycle (convergence test, the
+    // look-back at the predicted count minus one, or the worst-residual
+    // record of a fixed-cycle solve)
+    const bool want_sum = fixed > 0 ? it + 1 >= fixed : (it + 1 >= first_check - 1 || it + 1 >= maxit);
+    h->mapped_r = slot(it + 1);
     if (mixed) {
       bool fused_defect = false;
       PG_CHECK(h, fused_vcycle<float>(h, nu1, nu2, 1, 2, &fused_defect, cur, alt, want_sum));
@@ -1256,7 +1271,8 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
     const int rc = pg_learn_end_capture(h);
     if (rc) return rc;
   }
-  if (fixed <= 0) h->mg_pred = it;
+  if (fixed <= 0) h->mg_pred = next_pred >= 0 ? next_pred : it;
+  h->mapped_r = nullptr;
   *iters = it;
   *rel = r;
   PG_CHECK(h, pg_scatter_zero_mean(h));
