@@ -338,6 +338,7 @@ const char* pg_last_error(const pg_handle* h) { return h ? h->err.c_str() : "nul
 int pg_upload_state(pg_handle* h, const double* u, const double* v, const double* hh)
 {
   if (!h || !u || !v || !hh) return PG_EARG;
+  h->thickness_done = false;  // a new h gets its own thickness fix
   std::vector<double> stage;
   const size_t n = (size_t)h->G.nx * h->G.ny;
   for (int k = 0; k < h->G.L; k++) {
@@ -421,7 +422,9 @@ int pg_step(pg_handle* h, uint32_t n)
     order = (n == 1) ? 1 : (n == 2 ? 2 : 3);
     shift = (n != 1);
   }
-  PG_CHECK(h, pg_launch_ab(h, order));
+  // mixed precision on a closed domain: the thickness fix of the following
+  // correction rides on the update (no refresh between them to keep)
+  PG_CHECK(h, pg_launch_ab(h, order, h->mixed && !exact_refresh(h)));
   if (shift) {  // shift_tendency_history (integrator.c:127-132) as a buffer rotation
     const int t0 = h->slot[0], t1 = h->slot[1], t2 = h->slot[2];
     h->slot[0] = t2;
@@ -441,8 +444,11 @@ int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
   *st = pg_solve_stats{};
   const uint32_t solver = h->cfg.pressureSolver;
   // thickness constraint, then halo refresh (pressure.c:525-561)
-  PG_CHECK(h, pg_launch_thickness(h));
-  PG_CHECK(h, refresh_after_change(h));
+  if (!h->thickness_done) {
+    PG_CHECK(h, pg_launch_thickness(h));
+    PG_CHECK(h, refresh_after_change(h));
+  }
+  h->thickness_done = false;
   // RHS with mean removal (pressure.c:570-606)
   PG_CHECK(h, pg_launch_rhs(h));
   // The bit-replica solvers need the serial-order mean of pressure.c:55-72;

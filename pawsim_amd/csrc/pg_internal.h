@@ -56,6 +56,7 @@ struct pg_handle {
   float *fu32[3] = {}, *fv32[3] = {}, *fh32[3] = {}; // AB history buffers in mixed precision
   int slot[3] = {0, 1, 2};             // buffers holding tendency, tendency_1, tendency_2
   bool halo_dirty = false;             // u, v, h halos are stale (refilled before the next reader)
+  bool thickness_done = false;         // the last AB update also applied the rigid-lid thickness fix
   double *hhb = nullptr, *hhs = nullptr, *omz = nullptr, *taux = nullptr, *tauy = nullptr;
   double *Hc = nullptr, *Hw = nullptr, *Hs = nullptr;  // ghosted static geometry
   double *pi = nullptr, *rhs = nullptr;                // ghosted
@@ -111,7 +112,7 @@ hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double*
 hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double* dh);
 hipError_t pg_launch_tendency32(pg_handle* h, float* du, float* dv, float* dh);
 hipError_t pg_launch_tendency_inner32(pg_handle* h, float* du, float* dv, float* dh);
-hipError_t pg_launch_ab(pg_handle* h, int order);
+hipError_t pg_launch_ab(pg_handle* h, int order, bool fuse_thickness = false);
 hipError_t pg_launch_thickness(pg_handle* h);
 hipError_t pg_launch_rhs(pg_handle* h);
 hipError_t pg_launch_serial_mean(pg_handle* h, double* f);

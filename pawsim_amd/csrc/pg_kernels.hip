@@ -149,20 +149,45 @@ struct Ab32Args {
   const float* f2[3];
   double c0, c1, c2;
   int order;
+  const double* Hc;  // non-null: h is updated a column at a time and the rigid-lid
+                     // thickness fix (pressure.c:525-561) follows in registers
 };
 
-__global__ __launch_bounds__(TJ* TI) void k_ab32(PgGrid G, Ab32Args A)
+__device__ __forceinline__ double ab32_value(const Ab32Args& A, int fld, size_t p)
 {
-  const int j = G.g + blockIdx.x * TJ + threadIdx.x;
-  const int i = G.g + blockIdx.y * TI + threadIdx.y;
-  const int k = blockIdx.z % G.L, fld = blockIdx.z / G.L;
-  if (i >= G.g + G.nx || j >= G.g + G.ny) return;
-  const size_t p = pg_lay(G, k) + pg_at(G, i, j);
   double x = A.x[fld][p];
   if (A.order == 1) x += A.c0 * (double)A.f0[fld][p];
   else if (A.order == 2) x += A.c0 * (double)A.f0[fld][p] - A.c1 * (double)A.f1[fld][p];
   else x += A.c0 * (double)A.f0[fld][p] - A.c1 * (double)A.f1[fld][p] + A.c2 * (double)A.f2[fld][p];
-  A.x[fld][p] = x;
+  return x;
+}
+
+// blockIdx.z: field * L + layer; with A.Hc, z = 2L is the whole h column
+__global__ __launch_bounds__(TJ* TI) void k_ab32(PgGrid G, Ab32Args A)
+{
+  const int j = G.g + blockIdx.x * TJ + threadIdx.x;
+  const int i = G.g + blockIdx.y * TI + threadIdx.y;
+  if (i >= G.g + G.nx || j >= G.g + G.ny) return;
+  if (A.Hc && blockIdx.z == 2 * G.L) {
+    constexpr int LMAX = 8;
+    const size_t q = pg_at(G, i, j);
+    double hk[LMAX];
+    double tot = 0;
+#pragma unroll
+    for (int k = 0; k < LMAX; k++)
+      if (k < G.L) {
+        hk[k] = ab32_value(A, 2, pg_lay(G, k) + q);
+        tot += hk[k];
+      }
+    const double herr = tot - A.Hc[q];
+#pragma unroll
+    for (int k = 0; k < LMAX; k++)
+      if (k < G.L) A.x[2][pg_lay(G, k) + q] = (tot == 0) ? hk[k] : hk[k] * (1 - herr / tot);
+    return;
+  }
+  const int k = blockIdx.z % G.L, fld = blockIdx.z / G.L;
+  const size_t p = pg_lay(G, k) + pg_at(G, i, j);
+  A.x[fld][p] = ab32_value(A, fld, p);
 }
 
 __global__ void k_widen(const float* src, double* dst, size_t n)
@@ -587,11 +612,12 @@ hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh)
   return hipGetLastError();
 }
 
-hipError_t pg_launch_ab(pg_handle* h, int order)
+hipError_t pg_launch_ab(pg_handle* h, int order, bool fuse_thickness)
 {
   const double dt = h->cfg.dt;
   if (h->mixed) {
     Ab32Args A;
+    A.Hc = (fuse_thickness && h->G.L <= 8) ? h->Hc : nullptr;
     float** fs[3] = {h->fu32, h->fv32, h->fh32};
     double* xs[3] = {h->u, h->v, h->h};
     for (int f = 0; f < 3; f++) {
@@ -608,7 +634,8 @@ hipError_t pg_launch_ab(pg_handle* h, int order)
     } else {
       A.c0 = 23 * dt / 12; A.c1 = 4 * dt / 3; A.c2 = 5 * dt / 12;
     }
-    hipLaunchKernelGGL(k_ab32, owned_grid(h->G, 3 * h->G.L), dim3(TJ, TI), 0, h->stream, h->G, A);
+    hipLaunchKernelGGL(k_ab32, owned_grid(h->G, A.Hc ? 2 * h->G.L + 1 : 3 * h->G.L), dim3(TJ, TI), 0, h->stream, h->G, A);
+    h->thickness_done = A.Hc != nullptr;
     return hipGetLastError();
   }
   AbArgs A;
