@@ -449,13 +449,22 @@ int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
     PG_CHECK(h, refresh_after_change(h));
   }
   h->thickness_done = false;
-  // RHS with mean removal (pressure.c:570-606)
-  PG_CHECK(h, pg_launch_rhs(h));
-  // The bit-replica solvers need the serial-order mean of pressure.c:55-72;
-  // the GPU multigrid is not bitwise and takes a tree-reduced mean.
-  if (solver == PG_SOLVER_SOR || solver == PG_SOLVER_MG_REPLICA) PG_CHECK(h, pg_launch_serial_mean(h, h->rhs));
-  else PG_CHECK(h, pg_launch_parallel_mean(h, h->rhs));
-  PG_CHECK(h, pg_launch_sub_mean(h, h->rhs));
+  // RHS with mean removal (pressure.c:570-606).  The bit-replica solvers
+  // need the serial-order mean of pressure.c:55-72; the GPU multigrid is not
+  // bitwise and takes a tree-reduced mean.  Without the norms (which read
+  // the ghosted rhs) the fused multigrid sums the mean inside the RHS kernel
+  // and removes it while gathering its finest level.
+  const bool gpu_mg = solver == PG_SOLVER_GPU_MG || solver == PG_SOLVER_GPU_MG_MIXED;
+  const long gather_blocks = (long)((h->G.ny + 255) / 256) * h->G.nx;
+  if (gpu_mg && pg_fused_ok(h) && !h->cfg.pressureTiming && gather_blocks <= h->npartials) {
+    PG_CHECK(h, pg_launch_rhs_mean(h));
+    h->rhs_mean_pending = true;
+  } else {
+    PG_CHECK(h, pg_launch_rhs(h));
+    if (!gpu_mg) PG_CHECK(h, pg_launch_serial_mean(h, h->rhs));
+    else PG_CHECK(h, pg_launch_parallel_mean(h, h->rhs));
+    PG_CHECK(h, pg_launch_sub_mean(h, h->rhs));
+  }
   // the GPU multigrid gathers the owned rhs only
   if (solver == PG_SOLVER_SOR || solver == PG_SOLVER_MG_REPLICA) PG_CHECK(h, pg_fill(h, h->rhs, 1, PG_HALO_SCALAR));
   h->pi_guess_now = true;

@@ -57,6 +57,7 @@ struct pg_handle {
   int slot[3] = {0, 1, 2};             // buffers holding tendency, tendency_1, tendency_2
   bool halo_dirty = false;             // u, v, h halos are stale (refilled before the next reader)
   bool thickness_done = false;         // the last AB update also applied the rigid-lid thickness fix
+  bool rhs_mean_pending = false;       // ctl->mean is still to be removed from rhs (by the MG gather)
   double *hhb = nullptr, *hhs = nullptr, *omz = nullptr, *taux = nullptr, *tauy = nullptr;
   double *Hc = nullptr, *Hw = nullptr, *Hs = nullptr;  // ghosted static geometry
   double *pi = nullptr, *rhs = nullptr;                // ghosted
@@ -115,6 +116,7 @@ hipError_t pg_launch_tendency_inner32(pg_handle* h, float* du, float* dv, float*
 hipError_t pg_launch_ab(pg_handle* h, int order, bool fuse_thickness = false);
 hipError_t pg_launch_thickness(pg_handle* h);
 hipError_t pg_launch_rhs(pg_handle* h);
+hipError_t pg_launch_rhs_mean(pg_handle* h);  // rhs + its parallel mean in ctl->mean
 hipError_t pg_launch_serial_mean(pg_handle* h, double* f);
 hipError_t pg_launch_parallel_mean(pg_handle* h, double* f);
 hipError_t pg_launch_sub_mean(pg_handle* h, double* f);
@@ -133,7 +135,10 @@ hipError_t pg_gpu_mg_vcycle(pg_handle* h);
 hipError_t pg_gpu_mg_vcycle32(pg_handle* h);
 hipError_t pg_gpu_mg_sweep(pg_handle* h);
 hipError_t pg_gather_to_mg(pg_handle* h);
-hipError_t pg_gather_guess(pg_handle* h);  // pg_gather_to_mg + extrapolated initial guess (cfg.pi_guess)
+// pg_gather_to_mg + extrapolated initial guess (cfg.pi_guess) + the pending
+// rhs mean removal; *sumsq_blocks > 0: that many partials of ||rhs||^2 wait
+// in h->partials
+hipError_t pg_gather_guess(pg_handle* h, int* sumsq_blocks = nullptr);
 hipError_t pg_scatter_from_mg(pg_handle* h);
 hipError_t pg_scatter_zero_mean(pg_handle* h);  // scatter minus the solution's mean (GPU MG)
 hipError_t pg_launch_coarse_global(pg_handle* h, int top, int nu1, int nu2, bool fp32);

@@ -252,13 +252,29 @@ __device__ __forceinline__ double divergence(const PgGrid& G, const double* u, c
 }
 
 // RHS of the pressure equation (pressure.c:570-606) before mean removal.
+// part non-null: each block also leaves the sum of its rhs values there
+// (the parallel mean of the GPU multigrid, finished by k_mean_final)
 __global__ __launch_bounds__(TJ* TI) void k_rhs(PgGrid G, const double* u, const double* v, const double* h,
-                                                double* rhs, double dxdt, double dydt)
+                                                double* rhs, double dxdt, double dydt, double* part)
 {
   const int j = blockIdx.x * TJ + threadIdx.x;
   const int i = blockIdx.y * TI + threadIdx.y;
-  if (i >= G.nx || j >= G.ny) return;
-  rhs[pg_at(G, i + G.g, j + G.g)] = 0 + divergence(G, u, v, h, i, j, dxdt, dydt);
+  double r = 0;
+  if (i < G.nx && j < G.ny) {
+    r = 0 + divergence(G, u, v, h, i, j, dxdt, dydt);
+    rhs[pg_at(G, i + G.g, j + G.g)] = r;
+  }
+  if (!part) return;
+  const int tid = threadIdx.x + threadIdx.y * TJ;
+  for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+  __shared__ double red[TJ * TI / 64];
+  if ((tid & 63) == 0) red[tid >> 6] = r;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0;
+    for (int q = 0; q < TJ * TI / 64; q++) s += red[q];
+    part[blockIdx.x + (size_t)blockIdx.y * gridDim.x] = s;
+  }
 }
 
 // Serial-order mean of the owned cells (pressure.c:55-72): the same
@@ -669,7 +685,18 @@ hipError_t pg_launch_thickness(pg_handle* h)
 hipError_t pg_launch_rhs(pg_handle* h)
 {
   const double dxdt = h->G.dx * h->cfg.dt, dydt = h->G.dy * h->cfg.dt;
-  hipLaunchKernelGGL(k_rhs, owned_grid(h->G, 1), dim3(TJ, TI), 0, h->stream, h->G, h->u, h->v, h->h, h->rhs, dxdt, dydt);
+  hipLaunchKernelGGL(k_rhs, owned_grid(h->G, 1), dim3(TJ, TI), 0, h->stream, h->G, h->u, h->v, h->h, h->rhs, dxdt, dydt,
+                     (double*)nullptr);
+  return hipGetLastError();
+}
+
+// RHS plus its parallel mean in ctl->mean (block partials in the RHS kernel)
+hipError_t pg_launch_rhs_mean(pg_handle* h)
+{
+  const double dxdt = h->G.dx * h->cfg.dt, dydt = h->G.dy * h->cfg.dt;
+  const dim3 grid = owned_grid(h->G, 1);
+  hipLaunchKernelGGL(k_rhs, grid, dim3(TJ, TI), 0, h->stream, h->G, h->u, h->v, h->h, h->rhs, dxdt, dydt, h->partials);
+  hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(64), 0, h->stream, h->G, (int)(grid.x * grid.y), h->partials, h->ctl);
   return hipGetLastError();
 }
 

@@ -105,23 +105,40 @@ __global__ void k_gather(PgGrid G, const double* gf, double* f)
   if (j >= G.ny) return;
   f[(size_t)i * G.ny + j] = gf[pg_at(G, i + G.g, j + G.g)];
 }
-// Gather pi and rhs; the initial guess extrapolates the last solutions
-// (pi = pi_n, h1 = pi_{n-1}, h2 = pi_{n-2}): order 1 2 pi_n - pi_{n-1},
-// order 2 3 pi_n - 3 pi_{n-1} + pi_{n-2}; the history then shifts by one.
-__global__ void k_gather_guess(PgGrid G, const double* gpi, const double* grhs, double* pi, double* rhs, double* h1,
-                               double* h2, int order)
+// Gather pi and rhs.  With a history (h1 non-null) the initial guess
+// extrapolates the last solutions (pi = pi_n, h1 = pi_{n-1}, h2 =
+// pi_{n-2}): order 1 2 pi_n - pi_{n-1}, order 2 3 pi_n - 3 pi_{n-1} +
+// pi_{n-2}; the history then shifts by one.  With `mean` the rhs mean
+// (pressure.c:603) is removed on the way and each block leaves the partial
+// sum of the squared result in part[block].
+__global__ __launch_bounds__(256) void k_gather_guess(PgGrid G, const double* gpi, const double* grhs, double* pi,
+                                                      double* rhs, double* h1, double* h2, int order, const double* mean,
+                                                      double* part)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
-  if (j >= G.ny) return;
-  const size_t q = (size_t)i * G.ny + j, s = pg_at(G, i + G.g, j + G.g);
-  const double p0 = gpi[s], p1 = h1[q], p2 = h2[q];
-  double g = p0;
-  if (order == 1) g = 2.0 * p0 - p1;
-  else if (order == 2) g = 3.0 * p0 - 3.0 * p1 + p2;
-  pi[q] = g;
-  rhs[q] = grhs[s];
-  h2[q] = p1;
-  h1[q] = p0;
+  double ss = 0;
+  if (j < G.ny) {
+    const size_t q = (size_t)i * G.ny + j, s = pg_at(G, i + G.g, j + G.g);
+    const double p0 = gpi[s];
+    double g = p0;
+    if (h1) {
+      const double p1 = h1[q], p2 = h2[q];
+      if (order == 1) g = 2.0 * p0 - p1;
+      else if (order == 2) g = 3.0 * p0 - 3.0 * p1 + p2;
+      h2[q] = p1;
+      h1[q] = p0;
+    }
+    pi[q] = g;
+    const double r = mean ? grhs[s] - *mean : grhs[s];
+    rhs[q] = r;
+    ss = r * r;
+  }
+  if (!part) return;
+  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x + (size_t)blockIdx.y * gridDim.x] = red[0] + red[1] + red[2] + red[3];
 }
 __global__ void k_scatter(PgGrid G, const double* f, double* gf)
 {
@@ -608,23 +625,34 @@ hipError_t pg_gather_to_mg(pg_handle* h)
   return hipGetLastError();
 }
 
-hipError_t pg_gather_guess(pg_handle* h)
+hipError_t pg_gather_guess(pg_handle* h, int* sumsq_blocks)
 {
   const int want = h->pi_guess_now ? (int)h->cfg.pi_guess : 0;
-  if (want == 0) return pg_gather_to_mg(h);
+  const bool mean = h->rhs_mean_pending;
+  h->rhs_mean_pending = false;
+  if (sumsq_blocks) *sumsq_blocks = 0;
+  if (want == 0 && !mean) return pg_gather_to_mg(h);
   MgLevel& top = h->lev.back();
   const PgGrid& G = h->G;
   const size_t N = (size_t)G.nx * G.ny;
-  if (!h->pi_hist) {
-    hipError_t e = hipMalloc(&h->pi_hist, 2 * N * sizeof(double));
-    if (e != hipSuccess) return e;
-    h->pi_solves = 0;
+  double *h1 = nullptr, *h2 = nullptr;
+  int order = 0;
+  if (want > 0) {
+    if (!h->pi_hist) {
+      hipError_t e = hipMalloc(&h->pi_hist, 2 * N * sizeof(double));
+      if (e != hipSuccess) return e;
+      h->pi_solves = 0;
+    }
+    // solve k of a run: pi holds solution k-1, the history k-2 and k-3
+    order = std::min(want, std::max(h->pi_solves - 1, 0));
+    h1 = h->pi_hist;
+    h2 = h->pi_hist + N;
+    h->pi_solves++;
   }
-  // solve k of a run: pi holds solution k-1, the history k-2 and k-3
-  const int order = std::min(want, std::max(h->pi_solves - 1, 0));
-  hipLaunchKernelGGL(k_gather_guess, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->pi, h->rhs, top.pi,
-                     top.rhs, h->pi_hist, h->pi_hist + N, order);
-  h->pi_solves++;
+  const dim3 grid((G.ny + 255) / 256, G.nx);
+  hipLaunchKernelGGL(k_gather_guess, grid, dim3(256), 0, h->stream, G, h->pi, h->rhs, top.pi, top.rhs, h1, h2, order,
+                     mean ? &h->ctl->mean : nullptr, mean ? h->partials : nullptr);
+  if (mean && sumsq_blocks) *sumsq_blocks = (int)(grid.x * grid.y);
   return hipGetLastError();
 }
 

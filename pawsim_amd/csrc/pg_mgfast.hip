@@ -1200,11 +1200,15 @@ int pg_fused_mg_solve(pg_handle* h, int mixed, int* iters, double* rel, int* lea
     return pg_launch_defect(h, pcur, e, nullptr, pout, r32, &h->ctl->sumsq);
   };
   (void)nbd;
-  PG_CHECK(h, pg_gather_guess(h));
+  int nrb = 0;
+  PG_CHECK(h, pg_gather_guess(h, &nrb));
   auto slot = [&](int cycles) { return h->mapped + ((cycles & 1) ? 2 : 0); };  // residual after `cycles`
   h->mapped_r = slot(0);
-  hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, h->stream, top.rhs, N, h->partials);
-  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, 256, &h->ctl->rhs_sumsq, h->mapped + 1);
+  if (nrb == 0) {
+    hipLaunchKernelGGL(k_sumsq, dim3(256), dim3(256), 0, h->stream, top.rhs, N, h->partials);
+    nrb = 256;
+  }
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(256), 0, h->stream, h->partials, nrb, &h->ctl->rhs_sumsq, h->mapped + 1);
   PG_CHECK(h, defect(top.pi, nullptr, nullptr, mixed ? top.rhs32 : nullptr));
   PG_CHECK(h, hipGetLastError());
   const bool capture = pg_learn_capturing(h);
