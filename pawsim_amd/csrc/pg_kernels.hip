@@ -190,6 +190,80 @@ __global__ __launch_bounds__(TJ* TI) void k_ab32(PgGrid G, Ab32Args A)
   A.x[fld][p] = ab32_value(A, fld, p);
 }
 
+// The prognostic halo fill in one parallel pass, where the fill reads owned
+// cells only (every domain except periodic-x / wall-y, whose corner ghosts
+// keep values of the previous fill).  Each cell of the halo band gets the
+// value k_fill leaves there: the y phase on owned rows, then the x phase on
+// every column, composed per cell.  Face zeroing and the -0.0 the
+// antisymmetric wall rule writes onto the far face are reproduced.
+__device__ __forceinline__ double fill_y(const double* f, const PgGrid& G, int kind, int i, int j)
+{
+  const int g = G.g, ny = G.ny;
+  const double* row = f + (size_t)i * G.sy;
+  if (!G.wally) {  // field.c:121-134
+    if (j < g) return row[ny + j];
+    if (j >= g + ny) return row[j - ny];
+    return row[j];
+  }
+  if (kind == PG_HALO_V) {  // field.c:158-198, v kind
+    if (j == g) return 0.0;
+    if (j == g + ny) return -0.0;
+    if (j < g) return -row[2 * g - j];
+    if (j > g + ny) return -row[2 * (g + ny) - j];
+    return row[j];
+  }
+  if (j < g) return row[2 * g - j - 1];
+  if (j >= g + ny) return row[2 * (g + ny) - j - 1];
+  return row[j];
+}
+
+__device__ __forceinline__ double fill_xy(const double* f, const PgGrid& G, int kind, int i, int j)
+{
+  const int g = G.g, nx = G.nx;
+  if (!G.wallx) {  // field.c:137-150
+    if (i < g) return fill_y(f, G, kind, nx + i, j);
+    if (i >= g + nx) return fill_y(f, G, kind, i - nx, j);
+    return fill_y(f, G, kind, i, j);
+  }
+  if (kind == PG_HALO_U) {  // field.c:204-241, u kind
+    if (i == g) return 0.0;
+    if (i == g + nx) return -0.0;
+    if (i < g) return -fill_y(f, G, kind, 2 * g - i, j);
+    if (i > g + nx) return -fill_y(f, G, kind, 2 * (g + nx) - i, j);
+    return fill_y(f, G, kind, i, j);
+  }
+  if (i < g) return fill_y(f, G, kind, 2 * g - i - 1, j);
+  if (i >= g + nx) return fill_y(f, G, kind, 2 * (g + nx) - i - 1, j);
+  return fill_y(f, G, kind, i, j);
+}
+
+// blockIdx.y: field * L + layer (u, v, h); x enumerates the band of rows
+// [0, g] and [g + nx, sx) (all columns) and, between them, columns [0, g]
+// and [g + ny, sy): every ghost cell and both wall faces.
+__global__ __launch_bounds__(256) void k_fill_direct(double* u, double* v, double* h, size_t stride, PgGrid G)
+{
+  const int fld = blockIdx.y / G.L, k = blockIdx.y - fld * G.L;
+  const int g = G.g, W = 2 * g + 1;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= W * (G.sy + G.nx - 1)) return;
+  int i, j;
+  if (e < W * G.sy) {
+    const int r = e / G.sy;
+    i = (r <= g) ? r : G.nx + r - 1;
+    j = e - r * G.sy;
+  } else {
+    const int f = e - W * G.sy, c = f % W;
+    i = g + 1 + f / W;
+    j = (c <= g) ? c : G.ny + c - 1;
+  }
+  const int kind = (fld == 0) ? PG_HALO_U : (fld == 1 ? PG_HALO_V : PG_HALO_SCALAR);
+  double* base = (fld == 0 ? u : (fld == 1 ? v : h)) + (size_t)k * stride;
+  const bool owned = i >= g && i < g + G.nx && j >= g && j < g + G.ny;
+  const bool face = (kind == PG_HALO_U && G.wallx && i == g) || (kind == PG_HALO_V && G.wally && j == g);
+  if (owned && !face) return;
+  base[(size_t)i * G.sy + j] = fill_xy(base, G, kind, i, j);
+}
+
 __global__ void k_widen(const float* src, double* dst, size_t n)
 {
   for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
@@ -600,8 +674,14 @@ hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind)
 
 hipError_t pg_fill_prognostic(pg_handle* h)
 {
-  FillSet S{{h->u, h->v, h->h}, {PG_HALO_U, PG_HALO_V, PG_HALO_SCALAR}, h->G.L};
-  hipLaunchKernelGGL(k_fill, dim3(3 * h->G.L), dim3(1024), 0, h->stream, S, h->fsz, h->G);
+  const PgGrid& G = h->G;
+  if (!(!G.wallx && G.wally) && G.nx > 2 * G.g && G.ny > 2 * G.g) {
+    const int T = (2 * G.g + 1) * (G.sy + G.nx - 1);
+    hipLaunchKernelGGL(k_fill_direct, dim3((T + 255) / 256, 3 * G.L), dim3(256), 0, h->stream, h->u, h->v, h->h, h->fsz, G);
+    return hipGetLastError();
+  }
+  FillSet S{{h->u, h->v, h->h}, {PG_HALO_U, PG_HALO_V, PG_HALO_SCALAR}, G.L};
+  hipLaunchKernelGGL(k_fill, dim3(3 * G.L), dim3(1024), 0, h->stream, S, h->fsz, G);
   return hipGetLastError();
 }
 

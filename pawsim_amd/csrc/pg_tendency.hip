@@ -542,6 +542,12 @@ static hipError_t launch_edge_forked(pg_handle* h, T* du, T* dv, T* dh, hipError
   PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
   const PgGrid& G = h->G;
   const int total = 5 * G.ny + (G.nx - 5) * 5;
+  if (!h->edge_fork) {
+    hipError_t e = inner(h, du, dv, dh);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tendency_edge<T>, dim3((total + 255) / 256, G.L), dim3(256), 0, h->stream, G, h->P, I, du, dv, dh);
+    return hipGetLastError();
+  }
   hipError_t e = hipEventRecord(h->ev_fork, h->stream);
   if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->ev_fork, 0);
   if (e != hipSuccess) return e;
