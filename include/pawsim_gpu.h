@@ -143,6 +143,17 @@ int pg_solve_pressure(pg_handle *h, uint32_t solver, double *pi, const double *r
  * diagnostics.c:483-703; AL81 thickness, no active buoyancy).  Tree-ordered
  * sums: equal to the reference's serial sums up to rounding. */
 int pg_energy(pg_handle *h, double *KE, double *PE, double *Z);
+/* Online averages: pawsim_diagnostics_accumulate_averages and the
+ * normalise/write/reset of pawsim_diagnostics_write_averages_if_due
+ * (diagnostics.c:307-470).  pg_average_add adds weight times the current
+ * u, v, h, Montgomery potential + pi, diapycnal velocity (zero here), h_w u,
+ * h_s v, h_w u^2, h_s v^2 and h_q u_q v_q of every layer, the diapycnal
+ * velocity of layer Nlay and pi to device sums (weight = the window factor
+ * times dt, diagnostics.c:317-334).  pg_average_take multiplies the sums by
+ * scale, copies them to out[(10 Nlay + 2)][Nx][Ny] in the order
+ * U V H M W HU HV HUU HVV HUV per layer, W of layer Nlay, P, and resets them. */
+int pg_average_add(pg_handle *h, double weight);
+int pg_average_take(pg_handle *h, double scale, double *out);
 
 /* Model time (ctx->t) and stream control. */
 double pg_time(const pg_handle *h);

@@ -38,6 +38,7 @@ typedef struct or_config {
   real Lx, Ly, dt, tmin, tmax, savefrequency, h0, hsml, hbbl, A2, A4, A2smag, A4smag;
   real linDrag, quadDrag, linDragSurf, quadDragSurf, pi_tol, SOR_rp, omega_MG;
   real savefreqEZ;            /* energy/enstrophy diagnostics interval (config.c:206) */
+  real savefreqAvg;           /* online-average window (config.c:200) */
   bool wallEW, wallNS, useRL, useFullMG, pressureTiming, writeInit, useWind;
   bool unsupported;           /* some active feature outside the restated path */
   char unsupported_what[128];
@@ -81,6 +82,9 @@ typedef struct or_model {
   uint solves, iters_total, n_saves;
   uint n_EZ;                        /* EZ diagnostics schedule (pawsim.c:434-438) */
   real t_next_EZ;
+  uint n_avg;                       /* online-average schedule (pawsim.c:429-433) */
+  real t_next_avg;
+  or_field *avg;                    /* [10 per layer: U V H M W HU HV HUU HVV HUV] [W of layer L] [P] */
   double solve_seconds;             /* wall time inside the solver only */
   double correct_seconds;           /* wall time of whole corrections */
   int last_iters;
@@ -92,6 +96,14 @@ typedef struct or_model {
 void or_energy(const or_model *m, double *KE, double *PE, double *Z);
 /* EZdiags.dat line(s) due at the current time (diagnostics.c:493-699). */
 bool or_write_EZ_if_due(or_model *m, const char *outdir);
+/* Online averages (diagnostics.c:307-470): add the current state with the
+ * window weight, and write and reset the sums at the end of a window. */
+bool or_accumulate_averages(or_model *m);
+bool or_write_averages_if_due(or_model *m, const char *outdir);
+void or_averages_free(or_model *m);
+/* pawsim_work_load_state + face thickness without ghosts + eta + Bernoulli
+ * (the work arrays diagnostics.c:320-325 recompute). */
+void or_average_work(or_model *m);
 
 /* ---- oracle_io.c ---- */
 void or_config_defaults(or_config *c);

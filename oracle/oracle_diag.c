@@ -92,3 +92,105 @@ bool or_write_EZ_if_due(or_model *m, const char *outdir)
   }
   return true;
 }
+
+/* ---- online averages (diagnostics.c:187-470) ---- */
+
+/* Field order: per layer U V H M W HU HV HUU HVV HUV, then W of layer L,
+ * then P (write_average_state, diagnostics.c:376-425). */
+enum { AV_U, AV_V, AV_H, AV_M, AV_W, AV_HU, AV_HV, AV_HUU, AV_HVV, AV_HUV, AV_PER_LAYER };
+
+static bool averages_alloc(or_model *m)
+{
+  uint q, n = AV_PER_LAYER * m->L + 2;
+  if (m->avg) return true;
+  m->avg = calloc(n, sizeof(or_field));
+  if (!m->avg) return false;
+  for (q = 0; q < n; q++)
+    if (!or_field_alloc(&m->avg[q], m->cfg.Nx, m->cfg.Ny, m->g)) return false;
+  return true;
+}
+
+void or_averages_free(or_model *m)
+{
+  uint q, n = AV_PER_LAYER * m->L + 2;
+  if (!m->avg) return;
+  for (q = 0; q < n; q++) or_field_free(&m->avg[q]);
+  free(m->avg);
+  m->avg = NULL;
+}
+
+#define AVG(q, k) (&m->avg[(k) * AV_PER_LAYER + (q)])
+
+/* pawsim_diagnostics_accumulate_averages (diagnostics.c:307-372); no
+ * tracer, no diapycnal velocity (its averages stay zero). */
+bool or_accumulate_averages(or_model *m)
+{
+  uint i, j, k;
+  const uint g = m->g, L = m->L;
+  real avg_fac;
+  if (m->cfg.savefreqAvg <= 0) return true;
+  if (!averages_alloc(m)) return false;
+  avg_fac = (m->t < m->t_next_avg) ? 1 : 1 - (m->t - m->t_next_avg) / m->cfg.dt;
+  or_average_work(m);
+  for (k = 0; k < L; k++) {
+    const or_field *U = &m->u[k], *V = &m->v[k], *H = &m->h[k];
+    for (i = g; i < g + m->cfg.Nx; i++)
+      for (j = g; j < g + m->cfg.Ny; j++) {
+        const uint im1 = i - 1, jm1 = j - 1;
+        const real dtfac = avg_fac * m->cfg.dt;
+        const real hq = 0.25 * (OF(H, i, j) + OF(H, im1, j) + OF(H, i, jm1) + OF(H, im1, jm1));
+        const real uq = 0.5 * (OF(U, i, j) + OF(U, i, jm1));
+        const real vq = 0.5 * (OF(V, i, j) + OF(V, im1, j));
+        OF(AVG(AV_U, k), i, j) += dtfac * OF(U, i, j);
+        OF(AVG(AV_V, k), i, j) += dtfac * OF(V, i, j);
+        OF(AVG(AV_H, k), i, j) += dtfac * OF(H, i, j);
+        OF(AVG(AV_M, k), i, j) += dtfac * (OF(&m->MM[k], i, j) + (m->cfg.useRL ? OF(&m->pi, i, j) : 0));
+        OF(AVG(AV_W, k), i, j) += dtfac * 0.0;
+        OF(AVG(AV_HU, k), i, j) += dtfac * OF(&m->hwest[k], i, j) * OF(U, i, j);
+        OF(AVG(AV_HV, k), i, j) += dtfac * OF(&m->hsouth[k], i, j) * OF(V, i, j);
+        OF(AVG(AV_HUU, k), i, j) += dtfac * OF(&m->hwest[k], i, j) * OF(U, i, j) * OF(U, i, j);
+        OF(AVG(AV_HVV, k), i, j) += dtfac * OF(&m->hsouth[k], i, j) * OF(V, i, j) * OF(V, i, j);
+        OF(AVG(AV_HUV, k), i, j) += dtfac * hq * uq * vq;
+      }
+  }
+  for (i = g; i < g + m->cfg.Nx; i++)
+    for (j = g; j < g + m->cfg.Ny; j++) {
+      const real dtfac = avg_fac * m->cfg.dt;
+      OF(&m->avg[AV_PER_LAYER * L], i, j) += dtfac * 0.0;
+      OF(&m->avg[AV_PER_LAYER * L + 1], i, j) += dtfac * OF(&m->pi, i, j);
+    }
+  return true;
+}
+
+static bool write_avg(const char *outdir, const char *prefix, int k, uint n, const or_field *f)
+{
+  char path[1024];
+  if (k >= 0) snprintf(path, sizeof(path), "%s/%s%d_n=%u.dat", outdir, prefix, k, n);
+  else snprintf(path, sizeof(path), "%s/%s_n=%u.dat", outdir, prefix, n);
+  return or_write_field(path, f);
+}
+
+/* pawsim_diagnostics_write_averages_if_due (diagnostics.c:428-470). */
+bool or_write_averages_if_due(or_model *m, const char *outdir)
+{
+  static const char *names[AV_PER_LAYER] = {"U_avg", "V_avg", "H_avg", "M_avg", "W_avg",
+                                            "HU_avg", "HV_avg", "HUU_avg", "HVV_avg", "HUV_avg"};
+  uint q, k, i, j;
+  const uint L = m->L, nf = AV_PER_LAYER * L + 2;
+  if (m->cfg.savefreqAvg <= 0 || m->t < m->t_next_avg) return true;
+  if (!averages_alloc(m)) return false;
+  for (q = 0; q < nf; q++) {
+    or_field *f = &m->avg[q];
+    for (i = f->g; i < f->g + f->nx; i++)
+      for (j = f->g; j < f->g + f->ny; j++) OF(f, i, j) *= 1 / m->cfg.savefreqAvg;
+  }
+  for (k = 0; k < L; k++)
+    for (q = 0; q < AV_PER_LAYER; q++)
+      if (!write_avg(outdir, names[q], (int)k, m->n_avg, AVG(q, k))) return false;
+  if (!write_avg(outdir, "W_avg", (int)L, m->n_avg, &m->avg[AV_PER_LAYER * L])) return false;
+  if (m->cfg.useRL && !write_avg(outdir, "P_avg", -1, m->n_avg, &m->avg[AV_PER_LAYER * L + 1])) return false;
+  for (q = 0; q < nf; q++) or_field_zero(&m->avg[q]);
+  m->n_avg++;
+  m->t_next_avg = m->n_avg * m->cfg.savefreqAvg;
+  return true;
+}

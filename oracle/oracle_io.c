@@ -105,7 +105,8 @@ bool or_config_read(const char *fname, or_config *c, FILE *err)
       flag_unsupported(c, key);
     }
     else if (!strcmp(key, "savefreqEZ")) c->savefreqEZ = strtod(val, NULL);
-    else if (!strcmp(key, "convDiff") || !strcmp(key, "savefreqAvg")
+    else if (!strcmp(key, "savefreqAvg")) c->savefreqAvg = strtod(val, NULL);
+    else if (!strcmp(key, "convDiff")
              || !strcmp(key, "savefreqUMom") || !strcmp(key, "savefreqVMom") || !strcmp(key, "savefreqThic")
              || !strcmp(key, "savefreqEnergy") || !strcmp(key, "savefreqTracer")) {
       if (strtod(val, NULL) != 0) flag_unsupported(c, key);

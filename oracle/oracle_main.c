@@ -73,6 +73,10 @@ int main(int argc, char **argv)
   if (m.cfg.writeInit && !or_write_init_diagnostics(&m, argv[2])) return 1;
   if (!or_write_state(&m, argv[2], 0)) return 1;
   m.n_saves = 1;
+  if (m.cfg.savefreqAvg > 0) {  /* pawsim.c:429-433 */
+    m.n_avg = (uint)round(m.cfg.tmin / m.cfg.savefreqAvg) + 1;
+    m.t_next_avg = m.cfg.tmin + m.cfg.savefreqAvg;
+  }
   if (m.cfg.savefreqEZ > 0) {  /* pawsim.c:434-438 */
     m.n_EZ = (uint)round(m.cfg.tmin / m.cfg.savefreqEZ) + 1;
     m.t_next_EZ = m.cfg.tmin + m.cfg.savefreqEZ;
@@ -89,11 +93,14 @@ int main(int argc, char **argv)
     tstep += now_s() - ts;
     if (iters == m.cfg.maxiters) fprintf(stderr, "WARNING: PAWSIM pressure solve reached maxiters at n=%u\n", n);
     if (!write_scheduled(&m, argv[2])) return 1;
+    /* pawsim.c:501-503 */
+    if (!or_accumulate_averages(&m) || !or_write_averages_if_due(&m, argv[2])) return 1;
     if (!or_write_EZ_if_due(&m, argv[2])) return 1;
   }
   printf("ORACLE timing: steps=%u step_seconds=%0.6f solve_seconds=%0.6f correct_seconds=%0.6f total_seconds=%0.6f mean_iters=%0.2f\n",
          m.cfg.Nt, tstep, m.solve_seconds, m.correct_seconds, now_s() - tall,
          m.solves ? (double)m.iters_total / m.solves : 0.0);
+  or_averages_free(&m);
   or_model_free(&m);
   return 0;
 }

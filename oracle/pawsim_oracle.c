@@ -966,6 +966,14 @@ static void face_thickness_no_ghost(or_model *m)
   }
 }
 
+void or_average_work(or_model *m)
+{
+  load_state(m);
+  face_thickness_no_ghost(m);
+  calc_eta(m);
+  calc_bernoulli(m);
+}
+
 /* pressure.c:570-606. */
 static void build_rhs(or_model *m)
 {

@@ -28,7 +28,7 @@ PG_OK, PG_EARG, PG_EHIP, PG_ENOCONV, PG_ENODEV = 0, -1, -2, -3, -4
 ABI_SYMBOLS = (
     "pg_create", "pg_destroy", "pg_last_error", "pg_upload_state", "pg_download_state",
     "pg_upload_pi", "pg_download_pi", "pg_tendency", "pg_step", "pg_pressure_correct",
-    "pg_solve_pressure", "pg_refresh_halos", "pg_energy", "pg_time", "pg_set_stream", "pg_synchronize",
+    "pg_solve_pressure", "pg_refresh_halos", "pg_energy", "pg_average_add", "pg_average_take", "pg_time", "pg_set_stream", "pg_synchronize",
     "pg_bench_kernel",
     "pg_net_param_count", "pg_net_create", "pg_net_destroy", "pg_net_set_params", "pg_net_get_params",
     "pg_net_forward", "pg_net_loss_grad", "pg_net_adam_step",
@@ -112,6 +112,8 @@ def load_library() -> ctypes.CDLL:
     lib.pg_solve_pressure.argtypes = [ctypes.c_void_p, ctypes.c_uint32, d, d, P(PgSolveStats)]
     lib.pg_refresh_halos.argtypes = [ctypes.c_void_p]
     lib.pg_energy.argtypes = [ctypes.c_void_p, d, d, d]
+    lib.pg_average_add.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    lib.pg_average_take.argtypes = [ctypes.c_void_p, ctypes.c_double, d]
     lib.pg_time.argtypes = [ctypes.c_void_p]
     lib.pg_time.restype = ctypes.c_double
     lib.pg_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -269,6 +271,23 @@ class Model:
         z = np.zeros(self.shape[0])
         self._check(self.lib.pg_energy(self.h, ke, pe, z.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "pg_energy")
         return float(ke[0]), float(pe[0]), z
+
+    def average_add(self, weight: float) -> None:
+        """Add weight x the current fields to the online-average sums."""
+        self._check(self.lib.pg_average_add(self.h, weight), "pg_average_add")
+
+    def average_take(self, scale: float) -> dict:
+        """Scaled sums {"U_avg": [L, Nx, Ny], ..., "W_avg": [L + 1, Nx, Ny], "P_avg": [Nx, Ny]}; resets them."""
+        L, nx, ny = self.shape
+        out = np.empty((10 * L + 2, nx, ny))
+        self._check(self.lib.pg_average_take(self.h, scale, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "pg_average_take")
+        names = ("U_avg", "V_avg", "H_avg", "M_avg", "W_avg", "HU_avg", "HV_avg", "HUU_avg", "HVV_avg", "HUV_avg")
+        per = out[:10 * L].reshape(L, 10, nx, ny)
+        res = {n: per[:, q].copy() for q, n in enumerate(names)}
+        res["W_avg"] = np.concatenate([res["W_avg"], out[10 * L][None]], axis=0)
+        res["P_avg"] = out[10 * L + 1].copy()
+        return res
 
     def refresh_halos(self) -> None:
         self._check(self.lib.pg_refresh_halos(self.h), "pg_refresh_halos")

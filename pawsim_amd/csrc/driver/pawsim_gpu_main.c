@@ -27,6 +27,7 @@ typedef struct drv_config {
   unsigned Nt, tauNrecs;
   double tmin, tmax, savefrequency, hsml, hbbl, A2smag, A4smag, linDragSurf, quadDragSurf;
   double savefreqEZ;
+  double savefreqAvg;
   bool useRL, writeInit, useMG, solverGiven;
   char unsupported[128];
   char dir[512];
@@ -150,7 +151,9 @@ static bool read_config(const char *fname, drv_config *c)
       note_unsupported(c, key);
     } else if (!strcmp(key, "savefreqEZ")) {
       c->savefreqEZ = dv;
-    } else if (!strcmp(key, "convDiff") || !strcmp(key, "savefreqAvg")
+    } else if (!strcmp(key, "savefreqAvg")) {
+      c->savefreqAvg = dv;
+    } else if (!strcmp(key, "convDiff")
                || !strcmp(key, "savefreqUMom") || !strcmp(key, "savefreqVMom") || !strcmp(key, "savefreqThic")
                || !strcmp(key, "savefreqEnergy") || !strcmp(key, "savefreqTracer")) {
       if (dv != 0) note_unsupported(c, key);
@@ -220,7 +223,43 @@ typedef struct model {
   double solve_ms;
   unsigned n_EZ;
   double t_next_EZ;
+  unsigned n_avg;
+  double t_next_avg;
+  double *avg;  /* (10 Nlay + 2) Nx x Ny */
 } model;
+
+/* pawsim_diagnostics_accumulate_averages + write_averages_if_due
+ * (diagnostics.c:307-470): the window weight of this step, then at the end
+ * of a window the normalised sums as U_avg{k}_n=N.dat ... P_avg_n=N.dat. */
+static bool averages(model *M, const char *dir)
+{
+  static const char *names[10] = {"U_avg", "V_avg", "H_avg", "M_avg", "W_avg",
+                                  "HU_avg", "HV_avg", "HUU_avg", "HVV_avg", "HUV_avg"};
+  const double dt = M->c.pg.dt, freq = M->c.savefreqAvg;
+  const unsigned Nx = M->c.pg.Nx, Ny = M->c.pg.Ny, L = M->c.pg.Nlay;
+  const size_t N = (size_t)Nx * Ny;
+  char name[256];
+  if (freq <= 0) return true;
+  {
+    const double avg_fac = (M->t < M->t_next_avg) ? 1 : 1 - (M->t - M->t_next_avg) / dt;
+    if (pg_average_add(M->h, avg_fac * dt) != PG_OK) return false;
+  }
+  if (M->t < M->t_next_avg) return true;
+  if (!M->avg && !(M->avg = malloc((10 * (size_t)L + 2) * N * sizeof(double)))) return false;
+  if (pg_average_take(M->h, 1 / freq, M->avg) != PG_OK) return false;
+  for (unsigned k = 0; k < L; k++)
+    for (unsigned q = 0; q < 10; q++) {
+      snprintf(name, sizeof(name), "%s%u_n=%u.dat", names[q], k, M->n_avg);
+      if (!write_field(dir, name, M->avg + ((size_t)k * 10 + q) * N, Nx, Ny)) return false;
+    }
+  snprintf(name, sizeof(name), "W_avg%u_n=%u.dat", L, M->n_avg);
+  if (!write_field(dir, name, M->avg + (size_t)10 * L * N, Nx, Ny)) return false;
+  snprintf(name, sizeof(name), "P_avg_n=%u.dat", M->n_avg);  /* rigid lid (useRL) */
+  if (!write_field(dir, name, M->avg + ((size_t)10 * L + 1) * N, Nx, Ny)) return false;
+  M->n_avg++;
+  M->t_next_avg = M->n_avg * freq;
+  return true;
+}
 
 /* pawsim_diagnostics_write_EZ_if_due (diagnostics.c:483-703): one line per
  * due time -- the time, KE, PE, KE + PE and the enstrophy of every layer. */
@@ -393,6 +432,10 @@ int main(int argc, char **argv)
     return 1;
   }
   M.n_saves = 1;
+  if (c->savefreqAvg > 0) { /* pawsim.c:429-433 */
+    M.n_avg = (unsigned)round(c->tmin / c->savefreqAvg) + 1;
+    M.t_next_avg = c->tmin + c->savefreqAvg;
+  }
   if (c->savefreqEZ > 0) { /* pawsim.c:434-438 */
     M.n_EZ = (unsigned)round(c->tmin / c->savefreqEZ) + 1;
     M.t_next_EZ = c->tmin + c->savefreqEZ;
@@ -422,7 +465,8 @@ int main(int argc, char **argv)
         M.n_saves++;
       }
     }
-    if (!write_EZ_if_due(&M, argv[2])) {
+    /* pawsim.c:501-503 */
+    if (!averages(&M, argv[2]) || !write_EZ_if_due(&M, argv[2])) {
       fprintf(stderr, "ERROR: PAWSIM diagnostic averaging failed at n=%u\n", n);
       return 1;
     }
@@ -431,6 +475,6 @@ int main(int argc, char **argv)
          c->Nt, t_loop, M.solve_ms, now_s() - t_all, M.solves ? (double)M.iters_total / M.solves : 0.0);
   pg_destroy(M.h);
   free(hhb); free(hhs); free(om); free(gg); free(taux); free(tauy);
-  free(M.u); free(M.v); free(M.hh); free(M.pi); free(M.zero);
+  free(M.u); free(M.v); free(M.hh); free(M.pi); free(M.zero); free(M.avg);
   return 0;
 }

@@ -324,6 +324,7 @@ void pg_destroy(pg_handle* h)
   (void)hipFree(h->diag);
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
   if (h->mapped_host) (void)hipHostFree(h->mapped_host);
+  if (h->avg) (void)hipFree(h->avg);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
   if (h->ev_join) (void)hipEventDestroy(h->ev_join);

@@ -100,7 +100,99 @@ __global__ __launch_bounds__(64) void k_energy_finish(const double* part, int nb
   }
 }
 
+// ---------------------------------------------------------------------------
+// Online averages (pawsim_diagnostics_accumulate_averages,
+// diagnostics.c:307-372): per owned cell, with the reference's expressions
+// and association, so the sums equal the serial ones bit for bit.  Sum
+// order in avg: per layer U V H M W HU HV HUU HVV HUV, then W of layer L,
+// then P; each an Nx x Ny array (j contiguous).  W stays zero (no
+// diapycnal velocity on this path).
+// ---------------------------------------------------------------------------
+constexpr int AV_PER_LAYER = 10, AV_LMAX = 16;
+
+__global__ __launch_bounds__(256) void k_avg_accumulate(PgGrid G, PgPhys P, const double* u, const double* v,
+                                                        const double* hh, const double* hhb, const double* pi,
+                                                        double* avg, double dtfac, int useRL)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= G.ny) return;
+  const int L = G.L, gi = i + G.g, gj = j + G.g;
+  const size_t N = (size_t)G.nx * G.ny, q = (size_t)i * G.ny + j, c = pg_at(G, gi, gj);
+  auto H = [&](int k, int a, int b) { return hh[pg_lay(G, k) + pg_at(G, a, b)]; };
+  auto U = [&](int k, int a, int b) { return u[pg_lay(G, k) + pg_at(G, a, b)]; };
+  auto V = [&](int k, int a, int b) { return v[pg_lay(G, k) + pg_at(G, a, b)]; };
+  // interface heights bottom-up (work.c:1173-1203)
+  double eta[AV_LMAX + 1];
+  eta[L] = hhb[c];
+  for (int k = L - 1; k >= 0; k--) eta[k] = eta[k + 1] + H(k, gi, gj);
+  const int im = (i + G.nx - 1) % G.nx, jm = (j + G.ny - 1) % G.ny;  // modular faces (work.c:865-896)
+  double pp = 0;
+  for (int k = 0; k < L; k++) {
+    // Montgomery potential MM_B (work.c:1437-1504)
+    if (k > 0) pp = pp + P.gg[k] * eta[k];
+    double mm = pp;
+    const double hk = H(k, gi, gj);
+    if (P.h0 != 0) mm -= P.geff[k] * (P.h0 * P.h0 * P.h0 * P.h0) / (hk * hk * hk) / 3;
+    const double hw = 0.5 * (hk + H(k, im + G.g, gj)), hs = 0.5 * (hk + H(k, gi, jm + G.g));
+    const double uk = U(k, gi, gj), vk = V(k, gi, gj);
+    const double hq = 0.25 * (hk + H(k, gi - 1, gj) + H(k, gi, gj - 1) + H(k, gi - 1, gj - 1));
+    const double uq = 0.5 * (uk + U(k, gi, gj - 1));
+    const double vq = 0.5 * (vk + V(k, gi - 1, gj));
+    double* a = avg + (size_t)k * AV_PER_LAYER * N + q;
+    a[0 * N] += dtfac * uk;
+    a[1 * N] += dtfac * vk;
+    a[2 * N] += dtfac * hk;
+    a[3 * N] += dtfac * (mm + (useRL ? pi[c] : 0));
+    a[5 * N] += dtfac * hw * uk;
+    a[6 * N] += dtfac * hs * vk;
+    a[7 * N] += dtfac * hw * uk * uk;
+    a[8 * N] += dtfac * hs * vk * vk;
+    a[9 * N] += dtfac * hq * uq * vq;
+  }
+  avg[((size_t)AV_PER_LAYER * L + 1) * N + q] += dtfac * pi[c];
+}
+
+__global__ void k_scale(double* a, size_t n, double s)
+{
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) a[p] *= s;
+}
+
 }  // namespace
+
+extern "C" int pg_average_add(pg_handle* h, double weight)
+{
+  if (!h) return PG_EARG;
+  const PgGrid& G = h->G;
+  if (G.L > AV_LMAX) return PG_EARG;
+  const size_t n = ((size_t)AV_PER_LAYER * G.L + 2) * G.nx * G.ny;
+  if (!h->avg) {
+    PG_CHECK(h, hipMalloc(&h->avg, n * sizeof(double)));
+    PG_CHECK(h, hipMemsetAsync(h->avg, 0, n * sizeof(double), h->stream));
+  }
+  PG_CHECK(h, pg_ensure_halos(h));
+  hipLaunchKernelGGL(k_avg_accumulate, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->P, h->u, h->v, h->h,
+                     h->hhb, h->pi, h->avg, weight, 1);
+  PG_CHECK(h, hipGetLastError());
+  return PG_OK;
+}
+
+extern "C" int pg_average_take(pg_handle* h, double scale, double* out)
+{
+  if (!h || !out) return PG_EARG;
+  const PgGrid& G = h->G;
+  const size_t n = ((size_t)AV_PER_LAYER * G.L + 2) * G.nx * G.ny;
+  if (!h->avg) {  // nothing accumulated yet: zero sums
+    for (size_t p = 0; p < n; p++) out[p] = 0;
+    return PG_OK;
+  }
+  // scale_field_interior (diagnostics.c:173-185): a *= scale, then write and reset
+  hipLaunchKernelGGL(k_scale, dim3(1024), dim3(256), 0, h->stream, h->avg, n, scale);
+  PG_CHECK(h, hipGetLastError());
+  PG_CHECK(h, hipMemcpyAsync(out, h->avg, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  PG_CHECK(h, hipMemsetAsync(h->avg, 0, n * sizeof(double), h->stream));
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
 
 extern "C" int pg_energy(pg_handle* h, double* KE, double* PE, double* Z)
 {

@@ -79,6 +79,7 @@ struct pg_handle {
   int pi_solves = 0;                   // timestep solves since the history was last reset
   bool pi_guess_now = false;           // the running solve belongs to a timestep (extrapolation allowed)
   double* diag = nullptr;              // diagnostics reduction scratch
+  double* avg = nullptr;               // online-average sums (pg_average_add)
   size_t ndiag = 0;
   int fixed_cycles = 0;                // > 0: GPU-MG solves run exactly this many cycles with no host reads
   long mg_tail_cells = 1024;           // largest level the LDS coarse tail starts from

@@ -300,3 +300,24 @@ def read_outputs(outdir: str, nx: int, ny: int, nlay: int, n: int, with_p: bool 
     if with_p:
         out["P"] = read_2d(os.path.join(outdir, f"P_n={n}.dat"), nx, ny)
     return out
+
+
+AVG_PREFIXES = ("U_avg", "V_avg", "H_avg", "M_avg", "W_avg", "HU_avg", "HV_avg", "HUU_avg", "HVV_avg", "HUV_avg")
+
+
+def read_averages(outdir: str, nx: int, ny: int, nlay: int) -> Dict[int, Dict[str, np.ndarray]]:
+    """Online-average records in outdir: {n: {prefix: [layers, Nx, Ny], "P_avg": [Nx, Ny]}}.
+    W_avg has Nlay + 1 layers (interfaces), the others Nlay."""
+    found = {}
+    for name in os.listdir(outdir):
+        if name.startswith("U_avg0_n=") and name.endswith(".dat"):
+            found[int(name[len("U_avg0_n="):-4])] = {}
+    for n in found:
+        for pre in AVG_PREFIXES:
+            layers = nlay + 1 if pre == "W_avg" else nlay
+            found[n][pre] = np.stack([read_2d(os.path.join(outdir, f"{pre}{k}_n={n}.dat"), nx, ny)
+                                      for k in range(layers)])
+        p = os.path.join(outdir, f"P_avg_n={n}.dat")
+        if os.path.exists(p):
+            found[n]["P_avg"] = read_2d(p, nx, ny)
+    return found

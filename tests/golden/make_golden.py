@@ -56,6 +56,9 @@ def cases():
         # energy / enstrophy diagnostics (EZdiags.dat)
         "box24_ez": inputs.gyre(nx=24, ny=24, nt=8, A2=50.0, h0=20.0, savefreqEZ=600.0, **T),
         "chan32x16_ez": inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1, savefreqEZ=200.0, **T),
+        # online averages (U_avg ... P_avg); windows that end inside a step
+        "box24_avg": inputs.gyre(nx=24, ny=24, nt=8, A2=50.0, h0=20.0, savefreqAvg=750.0, **T),
+        "chan32x16_avg": inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1, savefreqAvg=500.0, **T),
     }
 
 
@@ -96,6 +99,12 @@ def make_case(name: str, setup: inputs.Setup, outdir: str) -> str:
         for f in INIT_FIELDS:
             arrays[f"init_{f}"] = np.stack([
                 inputs.read_2d(os.path.join(res, f"init_{f}{k}_n=0.dat"), nx, ny) for k in range(nl)])
+        avg = inputs.read_averages(res, nx, ny, nl)
+        if avg:
+            arrays["avg_saves"] = np.array(sorted(avg))
+            for n, fields in avg.items():
+                for key, val in fields.items():
+                    arrays[f"avg_{key}_n{n}"] = val
         ez = os.path.join(res, "EZdiags.dat")
         if os.path.exists(ez):
             with open(ez) as f:

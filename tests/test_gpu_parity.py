@@ -32,10 +32,16 @@ def test_gpu_matches_reference_golden(name, tmp_path):
     assert parse_iters(stdout) == [int(x) for x in z["solves"][:, 1]]
     if "ezdiags" in z.files:  # energy / enstrophy: same times, sums to rounding
         assert ez_close(read_ez(os.path.join(out, "EZdiags.dat")), z["ezdiags"])
+    if "avg_saves" in z.files:  # online averages: per-cell sums, bit for bit
+        got = inputs.read_averages(out, setup.nx, setup.ny, setup.nlay)
+        assert sorted(got) == [int(n) for n in z["avg_saves"]]
+        for n, fields in got.items():
+            for key, arr in fields.items():
+                assert same_bits(arr, z[f"avg_{key}_n{n}"]), f"{key} average {n}"
 
 
 TRAJECTORY_CASES = {
-    "gyre128_sor": dict(kind="gyre", nx=128, ny=128, nt=12, solver=0, savefreqEZ=900.0),
+    "gyre128_sor": dict(kind="gyre", nx=128, ny=128, nt=12, solver=0, savefreqEZ=900.0, savefreqAvg=1000.0),
     "gyre64_mg": dict(kind="gyre", nx=64, ny=64, nt=12, solver=1, A2=50.0, quadDragCoeff=2e-3),
     "chan64x32_mg": dict(kind="channel", nx=64, ny=32, nt=12, dx=20e3, solver=1, linDragCoeff=1e-3),
     "chan48x24_sor_quad": dict(kind="channel", nx=48, ny=24, nt=10, dx=20e3, solver=0, SOR_rp=1.8, quadDragCoeff=2e-3),
@@ -50,7 +56,7 @@ TRAJECTORY_CASES = {
     # tiled interior kernel across several tiles, every momentum scheme
     "jets136x72_tw81_a2": dict(kind="jets", nx=136, ny=72, nlay=3, dx=20e3, nt=6, solver=1, A2=200.0,
                                H_rest=(500.0, 1500.0, 2000.0),
-                               quadDragCoeff=2e-3, momentumScheme=3, savefreqEZ=300.0),
+                               quadDragCoeff=2e-3, momentumScheme=3, savefreqEZ=300.0, savefreqAvg=500.0),
     "jets80x136_hk83": dict(kind="jets", nx=80, ny=136, nlay=2, dx=20e3, nt=6, solver=1, momentumScheme=1,
                             H_rest=(1000.0, 3000.0),
                             h0=50.0, linDragCoeff=1e-3),
@@ -80,6 +86,13 @@ def test_gpu_matches_oracle_every_step(name, tmp_path, oracle_exe):
             d = np.max(np.abs(g[key] - o[key]))
             assert same_bits(g[key], o[key]), f"first mismatch: {key} at step {n}, max|diff|={d:.3e}"
     assert parse_iters(gout) == parse_iters(oout)  # empty without the norms
+    if "savefreqAvg" in spec:
+        ag = inputs.read_averages(str(tmp_path / "gpu"), setup.nx, setup.ny, setup.nlay)
+        ao = inputs.read_averages(str(tmp_path / "ora"), setup.nx, setup.ny, setup.nlay)
+        assert len(ao) >= 2 and sorted(ag) == sorted(ao)
+        for n in ao:
+            for key in ao[n]:
+                assert same_bits(ag[n][key], ao[n][key]), (key, n)
     if "savefreqEZ" in spec:
         ez_g = read_ez(str(tmp_path / "gpu" / "EZdiags.dat"))
         ez_o = read_ez(str(tmp_path / "ora" / "EZdiags.dat"))

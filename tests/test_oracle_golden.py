@@ -9,7 +9,8 @@ exactly.  The cases cover SOR / gathered MG / full MG / general (non 2:1)
 transfers, walls / periodic / mixed boundaries, odd grids, A2 and A4
 viscosity, linear and quadratic drag, h0, AB1/AB2/AB3 and all four
 momentum schemes; two cases also pin the energy / enstrophy diagnostics
-(EZdiags.dat, diagnostics.c:483-703).
+(EZdiags.dat, diagnostics.c:483-703) and two the online averages
+(U_avg ... P_avg, diagnostics.c:307-470).
 """
 import os
 
@@ -34,6 +35,12 @@ def test_oracle_matches_reference(name, oracle_exe, tmp_path):
     for f in ("dudt", "dvdt", "dhdt", "qq", "KE", "MM", "hwest", "alpha", "mu"):
         got = np.stack([inputs.read_2d(os.path.join(out, f"init_{f}{k}_n=0.dat"), nx, ny) for k in range(nl)])
         assert same_bits(got, z[f"init_{f}"]), f"init_{f}"
+    if "avg_saves" in z.files:  # online averages, bit for bit
+        got = inputs.read_averages(out, nx, ny, nl)
+        assert sorted(got) == [int(n) for n in z["avg_saves"]]
+        for n, fields in got.items():
+            for key, arr in fields.items():
+                assert same_bits(arr, z[f"avg_{key}_n{n}"]), f"{key} average {n}"
     if "ezdiags" in z.files:  # energy / enstrophy diagnostics, bit for bit
         got = read_ez(os.path.join(out, "EZdiags.dat"))
         assert got.shape == z["ezdiags"].shape
