@@ -50,7 +50,7 @@ typedef struct pg_config {
   double Lx, Ly, dt;
   uint32_t timeSteppingScheme; /* 3 AB1, 4 AB2, 5 AB3 (defs.h:18-24) */
   uint32_t momentumScheme;     /* 0 AL81, 1 HK83, 2 S75e, 3 TW81 (defs.h:27-30) */
-  uint32_t thicknessScheme;    /* 0 AL81 (only centred face thickness is supported) */
+  uint32_t thicknessScheme;    /* 0 AL81, 1 HK83, 2 UP3, 3 KT00 (defs.h:35-38) */
   int32_t useWallEW, useWallNS;
   double h0, A2, A4, linDragCoeff, quadDragCoeff;
   uint32_t pressureSolver;     /* PG_SOLVER_* */
@@ -68,6 +68,9 @@ typedef struct pg_config {
   uint32_t pi_guess;           /* initial guess of the GPU-MG solves (3, 4) inside pg_pressure_correct:
                                   0 the previous pi (as the reference), 1 linear extrapolation of the
                                   last two solutions, 2 quadratic extrapolation of the last three */
+  double KT00_sigma;           /* KT00 limiter slope factor, > 0 when thicknessScheme = 3
+                                  (config.c:73: the reference's default is 1.4) */
+  double A2smag, A4smag;       /* Smagorinsky coefficients (work.c:1538-1610); 0 = off */
 } pg_config;
 
 /* Static inputs (state.c:1012-1047).  NULL hhs means a flat lid at 0;

@@ -14,7 +14,8 @@
  *
  * Pinning: every function here is checked against golden vectors written by
  * the reference serial executable (tests/golden/), and the multigrid
- * transfer operators additionally against closed forms (tests/).
+ * operator, transfers and V-cycle and the KT00 limiter additionally against
+ * closed forms (tests/test_oracle_units.py).
  */
 #ifndef PAWSIM_ORACLE_H
 #define PAWSIM_ORACLE_H
@@ -29,7 +30,7 @@ typedef unsigned int uint;
 /* Scheme identifiers (defs.h:14-47). */
 enum { OR_AB1 = 3, OR_AB2 = 4, OR_AB3 = 5 };
 enum { OR_MOM_AL81 = 0, OR_MOM_HK83 = 1, OR_MOM_S75E = 2, OR_MOM_TW81 = 3 };
-enum { OR_THK_AL81 = 0 };
+enum { OR_THK_AL81 = 0, OR_THK_HK83 = 1, OR_THK_UP3 = 2, OR_THK_KT00 = 3 };
 enum { OR_SOLVER_SOR = 0, OR_SOLVER_MG = 1 };
 
 /* Parsed parameter file (config.c:42-122 defaults, 172-282 keys). */
@@ -39,6 +40,7 @@ typedef struct or_config {
   real linDrag, quadDrag, linDragSurf, quadDragSurf, pi_tol, SOR_rp, omega_MG;
   real savefreqEZ;            /* energy/enstrophy diagnostics interval (config.c:206) */
   real savefreqAvg;           /* online-average window (config.c:200) */
+  real KT00_sigma;            /* KT00 limiter slope factor (config.c:73, 207) */
   bool wallEW, wallNS, useRL, useFullMG, pressureTiming, writeInit, useWind;
   bool unsupported;           /* some active feature outside the restated path */
   char unsupported_what[128];
@@ -76,6 +78,7 @@ typedef struct or_model {
   or_field hhs_w, hhb_w, hhs_west, hhs_south, hhb_west, hhb_south, Hc, Hw, Hs;
   or_field taux_w, tauy_w, usq_bot, vsq_bot, uabs_bot;
   or_field DD_T, DD_S, DD4_T, DD4_S, A2_h, A2_q, A4s_h, A4s_q, UU4, VV4;
+  or_field thk_x, thk_y;            /* d2hx/d2hy (UP3) or dh_dx/dh_dy (KT00) (work.c:630-667) */
   or_field pi, rhs;
   /* pressure diagnostics (context.h:52-61) */
   real res_max, res_rms, div_max, div_rms;
@@ -113,6 +116,8 @@ bool or_read_matrix(const char *path, real *a, uint m, uint n, uint layers, FILE
 bool or_write_field(const char *path, const or_field *f);
 
 /* ---- pawsim_oracle.c ---- */
+#define OR_PI 3.14159265358979323846 /* defs.h:46 */
+real or_minmod(real a, real b, real c);
 bool or_field_alloc(or_field *f, uint nx, uint ny, uint g);
 void or_field_free(or_field *f);
 void or_field_zero(or_field *f);

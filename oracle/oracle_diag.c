@@ -2,7 +2,7 @@
  * oracle_diag.c -- CPU restatement of the energy / enstrophy diagnostics.
  *
  * TEST INFRASTRUCTURE ONLY (the oracle): restates diagnostics.c:483-703 for
- * the configurations the timestep path supports (AL81 thickness scheme, no
+ * the configurations the timestep path supports (every thickness scheme, no
  * active buoyancy), in the reference's loop order and FP64 association, so
  * that its output equals the reference serial build's EZdiags.dat.
  */
@@ -49,9 +49,37 @@ void or_energy(const or_model *m, double *KE, double *PE, double *Z)
           pv = (2 * OF(&m->omz, i, j) + zeta) / h_q;
           Z[k] += dx * dy * 0.5 * h_q * SQ(pv);
         }
-        KEdens = 0.125 * dx * dy
-                 * ((OF(H, i, j) + OF(H, ip1, j)) * SQ(OF(U, ip1, j)) + (OF(H, im1, j) + OF(H, i, j)) * SQ(OF(U, i, j))
-                    + (OF(H, i, j) + OF(H, i, jp1)) * SQ(OF(V, i, jp1)) + (OF(H, i, jm1) + OF(H, i, j)) * SQ(OF(V, i, j)));
+        if (m->cfg.thkScheme == OR_THK_HK83) {
+          /* diagnostics.c:581-642: HK83 weights u^2 and v^2 over the
+           * neighbouring faces along the face, one-sided on wall rows */
+          real hE = OF(H, i, j) + OF(H, ip1, j), hW = OF(H, im1, j) + OF(H, i, j);
+          real hN = OF(H, i, j) + OF(H, i, jp1), hS = OF(H, i, jm1) + OF(H, i, j);
+          KEdens = 0;
+          if (m->cfg.wallNS && gj == 0)
+            KEdens += hE * (1.25 * SQ(OF(U, ip1, j)) + 0.25 * SQ(OF(U, ip1, jp1)))
+                      + hW * (1.25 * SQ(OF(U, i, j)) + 0.25 * SQ(OF(U, i, jp1)));
+          else if (m->cfg.wallNS && gj == ny - 1)
+            KEdens += hE * (1.25 * SQ(OF(U, ip1, j)) + 0.25 * SQ(OF(U, ip1, jm1)))
+                      + hW * (1.25 * SQ(OF(U, i, j)) + 0.25 * SQ(OF(U, i, jm1)));
+          else
+            KEdens += hE * (SQ(OF(U, ip1, j)) + 0.25 * SQ(OF(U, ip1, jp1)) + 0.25 * SQ(OF(U, ip1, jm1)))
+                      + hW * (SQ(OF(U, i, j)) + 0.25 * SQ(OF(U, i, jp1)) + 0.25 * SQ(OF(U, i, jm1)));
+          if (m->cfg.wallEW && gi == 0)
+            KEdens += hN * (1.25 * SQ(OF(V, i, jp1)) + 0.25 * SQ(OF(V, ip1, jp1)))
+                      + hS * (1.25 * SQ(OF(V, i, j)) + 0.25 * SQ(OF(V, ip1, j)));
+          else if (m->cfg.wallEW && gi == nx - 1)
+            KEdens += hN * (1.25 * SQ(OF(V, i, jp1)) + 0.25 * SQ(OF(V, im1, jp1)))
+                      + hS * (1.25 * SQ(OF(V, i, j)) + 0.25 * SQ(OF(V, im1, j)));
+          else
+            KEdens += hN * (SQ(OF(V, i, jp1)) + 0.25 * SQ(OF(V, ip1, jp1)) + 0.25 * SQ(OF(V, im1, jp1)))
+                      + hS * (SQ(OF(V, i, j)) + 0.25 * SQ(OF(V, ip1, j)) + 0.25 * SQ(OF(V, im1, j)));
+          KEdens *= dx * dy / 12.0;
+        } else {
+          /* diagnostics.c:566-579 (AL81, UP3, KT00) */
+          KEdens = 0.125 * dx * dy
+                   * ((OF(H, i, j) + OF(H, ip1, j)) * SQ(OF(U, ip1, j)) + (OF(H, im1, j) + OF(H, i, j)) * SQ(OF(U, i, j))
+                      + (OF(H, i, j) + OF(H, i, jp1)) * SQ(OF(V, i, jp1)) + (OF(H, i, jm1) + OF(H, i, j)) * SQ(OF(V, i, j)));
+        }
         gtot = m->geff[k];
         local_KE += KEdens;
         local_PE += dx * dy * gtot * OF(H, i, j) * (eta[k] - 0.5 * OF(H, i, j));

@@ -19,6 +19,7 @@ void or_config_defaults(or_config *c)
   c->tsScheme = OR_AB3;
   c->momScheme = OR_MOM_TW81;
   c->thkScheme = OR_THK_AL81;
+  c->KT00_sigma = 1.4;
   c->tmin = -1;
   c->pi_tol = 1e-10;
   c->SOR_rp = 1.0;
@@ -83,7 +84,7 @@ bool or_config_read(const char *fname, or_config *c, FILE *err)
     U("thicknessScheme", c->thkScheme); U("maxiters", c->maxiters); U("tauNrecs", c->tauNrecs);
     R("Lx", c->Lx); R("Ly", c->Ly); R("dt", c->dt); R("tmin", c->tmin); R("tmax", c->tmax);
     R("savefrequency", c->savefrequency); R("h0", c->h0); R("hsml", c->hsml); R("hbbl", c->hbbl);
-    R("A2", c->A2); R("A4", c->A4); R("A2smag", c->A2smag); R("A4smag", c->A4smag);
+    R("A2", c->A2); R("A4", c->A4); R("A2smag", c->A2smag); R("A4smag", c->A4smag); R("KT00_sigma", c->KT00_sigma);
     R("linDragCoeff", c->linDrag); R("quadDragCoeff", c->quadDrag);
     R("linDragSurf", c->linDragSurf); R("quadDragSurf", c->quadDragSurf);
     R("tol", c->pi_tol); R("SOR_rp", c->SOR_rp); R("SOR_rp_min", c->SOR_rp); R("omega_MG", c->omega_MG);
@@ -121,10 +122,9 @@ bool or_config_read(const char *fname, or_config *c, FILE *err)
   if (!solverGiven) c->solver = useMG ? OR_SOLVER_MG : OR_SOLVER_SOR;
   c->useWind = (c->tauxFile[0] != '\0') || (c->tauyFile[0] != '\0');
   /* Features outside the restated path. */
-  if (c->thkScheme != OR_THK_AL81) flag_unsupported(c, "thicknessScheme");
+  if (c->thkScheme > OR_THK_KT00) flag_unsupported(c, "thicknessScheme");
   if (c->hsml > 0) flag_unsupported(c, "hsml");
   if (c->hbbl > 0) flag_unsupported(c, "hbbl");
-  if (c->A2smag > 0 || c->A4smag > 0) flag_unsupported(c, "Smagorinsky");
   if (c->linDragSurf > 0 || c->quadDragSurf > 0) flag_unsupported(c, "surface drag");
   if (c->tauNrecs != 1) flag_unsupported(c, "tauNrecs");
   if (!c->useRL) flag_unsupported(c, "useRL 0");

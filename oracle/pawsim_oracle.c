@@ -269,7 +269,7 @@ bool or_model_init(or_model *m, const char *infile, FILE *err)
 #define W2(f) or_field_alloc(&m->f, nx, ny, g)
   W2(hhs_w); W2(hhb_w); W2(hhs_west); W2(hhs_south); W2(hhb_west); W2(hhb_south);
   W2(Hc); W2(Hw); W2(Hs); W2(taux_w); W2(tauy_w); W2(usq_bot); W2(vsq_bot); W2(uabs_bot);
-  W2(DD_T); W2(DD_S); W2(DD4_T); W2(DD4_S); W2(A2_h); W2(A2_q); W2(A4s_h); W2(A4s_q);
+  W2(DD_T); W2(DD_S); W2(DD4_T); W2(DD4_S); W2(A2_h); W2(A2_q); W2(A4s_h); W2(A4s_q); W2(thk_x); W2(thk_y);
   W2(UU4); W2(VV4); W2(pi); W2(rhs);
 #undef W2
   m->gg = calloc(L, sizeof(real));
@@ -349,6 +349,7 @@ void or_model_free(or_model *m)
   or_field_free(&m->taux_w); or_field_free(&m->tauy_w); or_field_free(&m->usq_bot);
   or_field_free(&m->vsq_bot); or_field_free(&m->uabs_bot);
   or_field_free(&m->DD_T); or_field_free(&m->DD_S); or_field_free(&m->DD4_T); or_field_free(&m->DD4_S);
+  or_field_free(&m->thk_x); or_field_free(&m->thk_y);
   or_field_free(&m->A2_h); or_field_free(&m->A2_q); or_field_free(&m->A4s_h); or_field_free(&m->A4s_q);
   or_field_free(&m->UU4); or_field_free(&m->VV4); or_field_free(&m->pi); or_field_free(&m->rhs);
   free(m->gg);
@@ -435,12 +436,83 @@ void or_forcing_update(or_model *m)
 /* tendency core (tendency.c:49-75)                                     */
 /* ------------------------------------------------------------------ */
 
-/* work.c:1206-1224 with the AL81 branch of 566-611. */
+/* Kurganov-Tadmor limiter of three slopes (defs.c:994-1016): the one
+ * nearest zero when all three share a sign, else zero. */
+real or_minmod(real a, real b, real c)
+{
+  if (a > 0 && b > 0 && c > 0) {
+    real r = a < b ? a : b;
+    return r < c ? r : c;
+  }
+  if (a < 0 && b < 0 && c < 0) {
+    real r = a > b ? a : b;
+    return r > c ? r : c;
+  }
+  return 0;
+}
+
+/* Velocity-dependent face thickness on [1,sx-1) x [1,sy-1) from ghosted
+ * inputs (work.c:566-683): AL81 centred, HK83 six-point weighted, UP3
+ * centred minus 1/6 of the upwind second difference, KT00 upwind limited
+ * reconstruction (its south state adds the half slope on both sides, as the
+ * reference does at work.c:676-677). */
+static void face_thickness_layer(or_model *m, const or_field *hh, const or_field *uu, const or_field *vv,
+                                 or_field *hw, or_field *hs)
+{
+  uint i, j, sx = hh->sx, sy = hh->sy;
+  const real sig = m->cfg.KT00_sigma;
+  or_field *X = &m->thk_x, *Y = &m->thk_y;
+  switch (m->cfg.thkScheme) {
+    case OR_THK_HK83:
+      for (i = 1; i < sx - 1; i++)
+        for (j = 1; j < sy - 1; j++) {
+          OF(hs, i, j) = (1.0 / 3) * (OF(hh, i, j) + OF(hh, i, j - 1)
+                                      + 0.25 * (OF(hh, i + 1, j) + OF(hh, i - 1, j) + OF(hh, i + 1, j - 1) + OF(hh, i - 1, j - 1)));
+          OF(hw, i, j) = (1.0 / 3) * (OF(hh, i, j) + OF(hh, i - 1, j)
+                                      + 0.25 * (OF(hh, i, j + 1) + OF(hh, i - 1, j + 1) + OF(hh, i, j - 1) + OF(hh, i - 1, j - 1)));
+        }
+      return;
+    case OR_THK_UP3:
+      for (i = 1; i < sx - 1; i++)
+        for (j = 1; j < sy - 1; j++) {
+          OF(X, i, j) = OF(hh, i + 1, j) - 2 * OF(hh, i, j) + OF(hh, i - 1, j);
+          OF(Y, i, j) = OF(hh, i, j + 1) - 2 * OF(hh, i, j) + OF(hh, i, j - 1);
+        }
+      for (i = 1; i < sx - 1; i++)
+        for (j = 1; j < sy - 1; j++) {
+          OF(hs, i, j) = 0.5 * (OF(hh, i, j) + OF(hh, i, j - 1));
+          OF(hs, i, j) -= (1 / 6.0) * ((OF(vv, i, j) > 0) ? OF(Y, i, j - 1) : OF(Y, i, j));
+          OF(hw, i, j) = 0.5 * (OF(hh, i, j) + OF(hh, i - 1, j));
+          OF(hw, i, j) -= (1 / 6.0) * ((OF(uu, i, j) > 0) ? OF(X, i - 1, j) : OF(X, i, j));
+        }
+      return;
+    case OR_THK_KT00:
+      for (i = 1; i < sx - 1; i++)
+        for (j = 1; j < sy - 1; j++) {
+          OF(X, i, j) = or_minmod(sig * (OF(hh, i + 1, j) - OF(hh, i, j)), 0.5 * (OF(hh, i + 1, j) - OF(hh, i - 1, j)),
+                                  sig * (OF(hh, i, j) - OF(hh, i - 1, j)));
+          OF(Y, i, j) = or_minmod(sig * (OF(hh, i, j + 1) - OF(hh, i, j)), 0.5 * (OF(hh, i, j + 1) - OF(hh, i, j - 1)),
+                                  sig * (OF(hh, i, j) - OF(hh, i, j - 1)));
+        }
+      for (i = 1; i < sx - 1; i++)
+        for (j = 1; j < sy - 1; j++) {
+          real xm = OF(hh, i - 1, j) + 0.5 * OF(X, i - 1, j), xp = OF(hh, i, j) - 0.5 * OF(X, i, j);
+          real ym = OF(hh, i, j - 1) + 0.5 * OF(Y, i, j - 1), yp = OF(hh, i, j) + 0.5 * OF(Y, i, j);
+          OF(hw, i, j) = (OF(uu, i, j) > 0) ? xm : xp;
+          OF(hs, i, j) = (OF(vv, i, j) > 0) ? ym : yp;
+        }
+      return;
+    default:
+      face_avg(hh, hw, hs);
+  }
+}
+
+/* work.c:1206-1224. */
 static void calc_face_thickness(or_model *m)
 {
   uint k;
   for (k = 0; k < m->L; k++) {
-    face_avg(&m->hw[k], &m->hwest[k], &m->hsouth[k]);
+    face_thickness_layer(m, &m->hw[k], &m->uw[k], &m->vw[k], &m->hwest[k], &m->hsouth[k]);
     or_fill(&m->hwest[k], m, OR_HALO_SCALAR);
     or_fill(&m->hsouth[k], m, OR_HALO_SCALAR);
   }
@@ -582,12 +654,15 @@ static void calc_mass_fluxes(or_model *m, uint k)
   or_fill_internal(hvv, m);
 }
 
-/* work.c:1520-1857: momentum RHS (constant A2/A4; no Smagorinsky, no buoyancy). */
+/* work.c:1520-1857: momentum RHS (constant or Smagorinsky A2/A4; no buoyancy). */
 static void calc_momentum(or_model *m)
 {
   uint k, i, j, g = m->g;
   const real dx = m->dx, dy = m->dy;
-  const bool useA2 = m->cfg.A2 > 0, useA4 = m->cfg.A4 > 0;
+  const bool useA2 = m->cfg.A2 > 0 || m->cfg.A2smag > 0, useA4 = m->cfg.A4 > 0 || m->cfg.A4smag > 0;
+  const bool useSmag = m->cfg.A2smag > 0 || m->cfg.A4smag > 0;
+  const real dmax = fmax(dx, dy), s2 = m->cfg.A2smag * dmax / OR_PI, s4 = m->cfg.A4smag / OR_PI;
+  const real A2fac = s2 * s2, A4fac = (s4 * s4) * (dmax * dmax * dmax * dmax) / 8;
   const real A4c = sqrt(m->cfg.A4);
   const uint sch = m->cfg.momScheme;
   for (k = 0; k < m->L; k++) {
@@ -608,6 +683,20 @@ static void calc_momentum(or_model *m)
           OF(&m->DD_T, i, j) = (OF(u, i + 1, j) - OF(u, i, j)) / dx - (OF(v, i, j + 1) - OF(v, i, j)) / dy;
           OF(&m->DD_S, i, j) = (OF(u, i, j) - OF(u, i, j - 1)) / dy + (OF(v, i, j) - OF(v, i - 1, j)) / dx;
         }
+      if (useSmag) {
+        or_field *DT = &m->DD_T, *DS = &m->DD_S;
+        for (i = 1; i < m->DD_T.sx - 1; i++)
+          for (j = 1; j < m->DD_T.sy - 1; j++) {
+            real tq = 0.25 * (OF(DT, i, j) + OF(DT, i - 1, j) + OF(DT, i, j - 1) + OF(DT, i - 1, j - 1));
+            real sh = 0.25 * (OF(DS, i, j) + OF(DS, i + 1, j) + OF(DS, i, j + 1) + OF(DS, i + 1, j + 1));
+            real DDq = sqrt(tq * tq + OF(DS, i, j) * OF(DS, i, j));
+            real DDh = sqrt(OF(DT, i, j) * OF(DT, i, j) + sh * sh);
+            OF(&m->A2_q, i, j) = fmax(m->cfg.A2, A2fac * DDq);
+            OF(&m->A2_h, i, j) = fmax(m->cfg.A2, A2fac * DDh);
+            OF(&m->A4s_q, i, j) = sqrt(fmax(m->cfg.A4, A4fac * DDq));
+            OF(&m->A4s_h, i, j) = sqrt(fmax(m->cfg.A4, A4fac * DDh));
+          }
+      } else
       for (i = 1; i < m->DD_T.sx - 1; i++)
         for (j = 1; j < m->DD_T.sy - 1; j++) {
           if (useA2) {
@@ -944,23 +1033,70 @@ static void correct_thickness(or_model *m)
   fill_prognostic(m, m->u, m->v, m->h);
 }
 
-/* work.c:865-896 + 997-1118: AL81 face thickness with modular wrap on the
- * no-ghost global arrays, scattered back and scalar-filled. */
+/* work.c:865-1032 + 1037-1118: face thickness on the no-ghost global
+ * arrays, every neighbour index taken modulo Nx / Ny (also across walls),
+ * except the HK83 wall rows, which use their own two-sided weights; then
+ * scattered back and scalar-filled.  The UP3/KT00 second differences and
+ * slopes are evaluated at every owned cell with the same modular wrap. */
 static void face_thickness_no_ghost(or_model *m)
 {
   uint k, i, j, g = m->g, Nx = m->cfg.Nx, Ny = m->cfg.Ny;
+  const uint sch = m->cfg.thkScheme;
+  const real sig = m->cfg.KT00_sigma;
   for (k = 0; k < m->L; k++) {
     or_field *hh = &m->hw[k], *hW = &m->hwest[k], *hS = &m->hsouth[k];
+    or_field *X = &m->thk_x, *Y = &m->thk_y;
+#define GH(a, b) OF(hh, ((a) % Nx) + g, ((b) % Ny) + g)
     or_field_zero(hW);
     or_field_zero(hS);
+    if (sch == OR_THK_UP3 || sch == OR_THK_KT00)
+      for (i = 0; i < Nx; i++) {
+        uint im1 = (i + Nx - 1) % Nx, ip1 = (i + 1) % Nx;
+        for (j = 0; j < Ny; j++) {
+          uint jm1 = (j + Ny - 1) % Ny, jp1 = (j + 1) % Ny;
+          if (sch == OR_THK_UP3) {
+            OF(X, i + g, j + g) = GH(ip1, j) - 2 * GH(i, j) + GH(im1, j);
+            OF(Y, i + g, j + g) = GH(i, jp1) - 2 * GH(i, j) + GH(i, jm1);
+          } else {
+            OF(X, i + g, j + g) = or_minmod(sig * (GH(ip1, j) - GH(i, j)), 0.5 * (GH(ip1, j) - GH(im1, j)), sig * (GH(i, j) - GH(im1, j)));
+            OF(Y, i + g, j + g) = or_minmod(sig * (GH(i, jp1) - GH(i, j)), 0.5 * (GH(i, jp1) - GH(i, jm1)), sig * (GH(i, j) - GH(i, jm1)));
+          }
+        }
+      }
     for (i = 0; i < Nx; i++) {
-      uint im1 = (i + Nx - 1) % Nx;
+      uint im1 = (i + Nx - 1) % Nx, ip1 = (i + 1) % Nx;
       for (j = 0; j < Ny; j++) {
-        uint jm1 = (j + Ny - 1) % Ny;
-        OF(hW, i + g, j + g) = 0.5 * (OF(hh, i + g, j + g) + OF(hh, im1 + g, j + g));
-        OF(hS, i + g, j + g) = 0.5 * (OF(hh, i + g, j + g) + OF(hh, i + g, jm1 + g));
+        uint jm1 = (j + Ny - 1) % Ny, jp1 = (j + 1) % Ny;
+        real *w = &OF(hW, i + g, j + g), *s = &OF(hS, i + g, j + g);
+        real uij = OF(&m->uw[k], i + g, j + g), vij = OF(&m->vw[k], i + g, j + g);
+        if (sch == OR_THK_HK83) {
+          if (m->cfg.wallEW && i == 0)
+            *s = (1.0 / 3) * (1.25 * (GH(i, j) + GH(i, jm1)) + 0.25 * (GH(ip1, j) + GH(ip1, jm1)));
+          else if (m->cfg.wallEW && i == Nx - 1)
+            *s = (1.0 / 3) * (1.25 * (GH(i, j) + GH(i, jm1)) + 0.25 * (GH(im1, j) + GH(im1, jm1)));
+          else
+            *s = (1.0 / 3) * (GH(i, j) + GH(i, jm1) + 0.25 * (GH(ip1, j) + GH(im1, j) + GH(ip1, jm1) + GH(im1, jm1)));
+          if (m->cfg.wallNS && j == 0)
+            *w = (1.0 / 3) * (1.25 * (GH(i, j) + GH(im1, j)) + 0.25 * (GH(i, jp1) + GH(im1, jp1)));
+          else if (m->cfg.wallNS && j == Ny - 1)
+            *w = (1.0 / 3) * (1.25 * (GH(i, j) + GH(im1, j)) + 0.25 * (GH(i, jm1) + GH(im1, jm1)));
+          else
+            *w = (1.0 / 3) * (GH(i, j) + GH(im1, j) + 0.25 * (GH(i, jp1) + GH(im1, jp1) + GH(i, jm1) + GH(im1, jm1)));
+        } else if (sch == OR_THK_UP3) {
+          *s = 0.5 * (GH(i, j) + GH(i, jm1)) - (1 / 6.0) * ((vij > 0) ? OF(Y, i + g, jm1 + g) : OF(Y, i + g, j + g));
+          *w = 0.5 * (GH(i, j) + GH(im1, j)) - (1 / 6.0) * ((uij > 0) ? OF(X, im1 + g, j + g) : OF(X, i + g, j + g));
+        } else if (sch == OR_THK_KT00) {
+          real xm = GH(im1, j) + 0.5 * OF(X, im1 + g, j + g), xp = GH(i, j) - 0.5 * OF(X, i + g, j + g);
+          real ym = GH(i, jm1) + 0.5 * OF(Y, i + g, jm1 + g), yp = GH(i, j) + 0.5 * OF(Y, i + g, j + g);
+          *w = (uij > 0) ? xm : xp;
+          *s = (vij > 0) ? ym : yp;
+        } else {
+          *w = 0.5 * (GH(i, j) + GH(im1, j));
+          *s = 0.5 * (GH(i, j) + GH(i, jm1));
+        }
       }
     }
+#undef GH
     or_fill(hW, m, OR_HALO_SCALAR);
     or_fill(hS, m, OR_HALO_SCALAR);
   }

@@ -50,6 +50,7 @@ class PgConfig(ctypes.Structure):
         ("omega_MG", ctypes.c_double), ("maxiters", ctypes.c_uint32), ("use_fullMG", ctypes.c_int32),
         ("pressureTiming", ctypes.c_int32), ("mg_nu1", ctypes.c_uint32), ("mg_nu2", ctypes.c_uint32),
         ("tendency_kernel", ctypes.c_int32), ("mixed_precision", ctypes.c_int32), ("pi_guess", ctypes.c_uint32),
+        ("KT00_sigma", ctypes.c_double), ("A2smag", ctypes.c_double), ("A4smag", ctypes.c_double),
     ]
 
 
@@ -198,6 +199,9 @@ class Model:
         c.tendency_kernel = int(p.get("tendency_kernel", 0))
         c.mixed_precision = int(p.get("mixed_precision", 0))
         c.pi_guess = int(p.get("pi_guess", 0))
+        c.KT00_sigma = float(p.get("KT00_sigma", 1.4))  # config.c:73
+        c.A2smag = float(p.get("A2smag", 0.0))
+        c.A4smag = float(p.get("A4smag", 0.0))
         self.cfg = c
         self.shape = (c.Nlay, c.Nx, c.Ny)
         self._keep = [_c64(hhb), _c64(hhs), _c64(omegaz), _c64(gg), _c64(taux), _c64(tauy)]

@@ -25,7 +25,7 @@
 typedef struct drv_config {
   pg_config pg;
   unsigned Nt, tauNrecs;
-  double tmin, tmax, savefrequency, hsml, hbbl, A2smag, A4smag, linDragSurf, quadDragSurf;
+  double tmin, tmax, savefrequency, hsml, hbbl, linDragSurf, quadDragSurf;
   double savefreqEZ;
   double savefreqAvg;
   bool useRL, writeInit, useMG, solverGiven;
@@ -54,6 +54,7 @@ static void defaults(drv_config *c)
   c->pg.timeSteppingScheme = 5;
   c->pg.momentumScheme = 3;
   c->pg.thicknessScheme = 0;
+  c->pg.KT00_sigma = 1.4;
   c->pg.pi_tol = 1e-10;
   c->pg.SOR_rp = 1.0;
   c->pg.omega_MG = 2.0 / 3.0;
@@ -115,8 +116,9 @@ static bool read_config(const char *fname, drv_config *c)
     else if (!strcmp(key, "hbbl")) c->hbbl = dv;
     else if (!strcmp(key, "A2")) c->pg.A2 = dv;
     else if (!strcmp(key, "A4")) c->pg.A4 = dv;
-    else if (!strcmp(key, "A2smag")) c->A2smag = dv;
-    else if (!strcmp(key, "A4smag")) c->A4smag = dv;
+    else if (!strcmp(key, "A2smag")) c->pg.A2smag = dv;
+    else if (!strcmp(key, "A4smag")) c->pg.A4smag = dv;
+    else if (!strcmp(key, "KT00_sigma")) c->pg.KT00_sigma = dv;
     else if (!strcmp(key, "linDragCoeff")) c->pg.linDragCoeff = dv;
     else if (!strcmp(key, "quadDragCoeff")) c->pg.quadDragCoeff = dv;
     else if (!strcmp(key, "linDragSurf")) c->linDragSurf = dv;
@@ -163,7 +165,6 @@ static bool read_config(const char *fname, drv_config *c)
   if (!c->solverGiven) c->pg.pressureSolver = c->useMG ? PG_SOLVER_MG_REPLICA : PG_SOLVER_SOR;  /* config.c:289-297 */
   if (c->hsml > 0) note_unsupported(c, "hsml");
   if (c->hbbl > 0) note_unsupported(c, "hbbl");
-  if (c->A2smag > 0 || c->A4smag > 0) note_unsupported(c, "Smagorinsky viscosity");
   if (c->linDragSurf > 0 || c->quadDragSurf > 0) note_unsupported(c, "surface drag");
   if (c->tauNrecs != 1) note_unsupported(c, "tauNrecs");
   if (!c->useRL) note_unsupported(c, "useRL 0");

@@ -76,6 +76,28 @@ hipError_t refresh_after_change(pg_handle* h)
 
 hipError_t ensure_halos(pg_handle* h) { return h->halo_dirty ? fill_prognostic(h) : hipSuccess; }
 
+// The reference's tendency reads work copies of u and v refreshed by their
+// own halo fill at every pawsim_work_load_state (work.c:538-559).  On a
+// periodic-x / wall-y domain that fill leaves the corner ghosts one load
+// behind (field.c:137-198), and the Smagorinsky strain averages read those
+// corners.  After a pressure correction the state's corners hold what the
+// work copies hold at the next tendency: the wall images of the state the
+// RHS loaded, or, with the corrected divergence (which reloads the work
+// copies), of the corrected state.  When the reference loads the current
+// state once more before the tendency (the averages, the start of a run, a
+// tendency just before), one more fill of u and v moves the current wall
+// images into the corners.
+hipError_t sync_work_corners(pg_handle* h)
+{
+  const PgGrid& G = h->G;
+  const bool due = h->corner_fill_due;
+  h->corner_fill_due = true;  // the tendency about to run is itself a load of the current state
+  if (!h->P.useSmag || G.wallx || !G.wally || !due) return hipSuccess;
+  hipError_t e = pg_fill(h, h->u, G.L, PG_HALO_U);
+  if (e == hipSuccess) e = pg_fill(h, h->v, G.L, PG_HALO_V);
+  return e;
+}
+
 int validate(const pg_config* c, std::string& why)
 {
   if (c->Nx < 4 || c->Ny < 4) return why = "Nx and Ny must be at least 4", PG_EARG;
@@ -83,8 +105,11 @@ int validate(const pg_config* c, std::string& why)
   if (c->Ly <= 0 || c->dt <= 0) return why = "Ly and dt must be positive", PG_EARG;
   if (c->timeSteppingScheme < 3 || c->timeSteppingScheme > 5) return why = "only AB1, AB2 and AB3 are supported", PG_EARG;
   if (c->momentumScheme > 3) return why = "unknown momentumScheme", PG_EARG;
-  if (c->thicknessScheme != 0) return why = "only the AL81 thickness scheme is supported", PG_EARG;
-  if (c->h0 < 0 || c->A2 < 0 || c->A4 < 0 || c->linDragCoeff < 0 || c->quadDragCoeff < 0)
+  if (c->thicknessScheme > 3) return why = "unknown thicknessScheme", PG_EARG;
+  if (c->thicknessScheme == 3 && !(c->KT00_sigma > 0))
+    return why = "KT00_sigma must be positive with the KT00 thickness scheme (the reference's default is 1.4)", PG_EARG;
+  if (c->h0 < 0 || c->A2 < 0 || c->A4 < 0 || c->linDragCoeff < 0 || c->quadDragCoeff < 0 || c->A2smag < 0
+      || c->A4smag < 0)
     return why = "negative physical coefficient", PG_EARG;
   if (c->pi_tol <= 0 || c->SOR_rp <= 0 || c->SOR_rp >= 2 || c->omega_MG <= 0 || c->maxiters == 0)
     return why = "invalid pressure solver parameters", PG_EARG;
@@ -96,8 +121,10 @@ int validate(const pg_config* c, std::string& why)
   if (c->pi_guess > 2) return why = "pi_guess must be 0, 1 or 2", PG_EARG;
   if (c->pi_guess && c->pressureSolver != PG_SOLVER_GPU_MG && c->pressureSolver != PG_SOLVER_GPU_MG_MIXED)
     return why = "pi_guess needs a GPU multigrid solver (3 or 4)", PG_EARG;
-  if (c->mixed_precision && (c->A4 > 0 || c->Nlay > 8 || c->Nx < 8 || c->Ny < 8 || c->tendency_kernel != 0))
-    return why = "mixed_precision needs A4 = 0, Nlay <= 8, Nx and Ny >= 8 and the tiled tendency kernel", PG_EARG;
+  if (c->mixed_precision && (c->A4 > 0 || c->Nlay > 8 || c->Nx < 8 || c->Ny < 8 || c->tendency_kernel != 0
+                             || c->thicknessScheme != 0 || c->A2smag > 0 || c->A4smag > 0))
+    return why = "mixed_precision needs A4 = 0, Nlay <= 8, Nx and Ny >= 8, the tiled tendency kernel, "
+                 "the AL81 thickness scheme and no Smagorinsky viscosity", PG_EARG;
   return PG_OK;
 }
 
@@ -219,14 +246,13 @@ static int create_body(const pg_config* cfg, const pg_static* st, pg_handle** ou
   G.dy = c.Ly / c.Ny;
   PgPhys& P = h->P;
   P.mom = (int)c.momentumScheme;
-  P.useA2 = c.A2 > 0;
-  P.useA4 = c.A4 > 0;
+  P.thk = (int)c.thicknessScheme;
+  P.kt00 = c.KT00_sigma;
+  pg_set_viscosity(P, c.A2, c.A4, c.A2smag, c.A4smag, G.dx, G.dy);
   P.useWind = (st->taux != nullptr) || (st->tauy != nullptr);
   P.useLin = c.linDragCoeff > 0;
   P.useQuad = c.quadDragCoeff > 0;
   P.h0 = c.h0;
-  P.A2 = c.A2;
-  P.A4c = std::sqrt(c.A4);
   P.lin = c.linDragCoeff;
   P.quad = c.quadDragCoeff;
   for (int k = 0; k < G.L; k++) P.gg[k] = st->gg[k];
@@ -349,6 +375,7 @@ int pg_upload_state(pg_handle* h, const double* u, const double* v, const double
     if (rc) return rc;
   }
   PG_CHECK(h, fill_prognostic(h));
+  h->corner_fill_due = true;
   PG_CHECK(h, hipStreamSynchronize(h->stream));
   return PG_OK;
 }
@@ -391,6 +418,7 @@ int pg_tendency(pg_handle* h, double* du, double* dv, double* dh)
   if (!h) return PG_EARG;
   const int s = h->slot[0];
   PG_CHECK(h, ensure_halos(h));
+  PG_CHECK(h, sync_work_corners(h));
   PG_CHECK(h, pg_launch_tendency_slot(h, s));
   if (h->mixed) PG_CHECK(h, pg_widen_slot(h, s));
   std::vector<double> stage;
@@ -410,6 +438,8 @@ int pg_step(pg_handle* h, uint32_t n)
   if (!h || n == 0) return PG_EARG;
   const int s = h->slot[0];
   PG_CHECK(h, ensure_halos(h));
+  if (n == 1) h->corner_fill_due = true;  // the run's first step follows a load of the initial state (pawsim.c:289)
+  PG_CHECK(h, sync_work_corners(h));
   PG_CHECK(h, pg_launch_tendency_slot(h, s));
   const uint32_t scheme = h->cfg.timeSteppingScheme;
   bool shift = true;
@@ -475,6 +505,7 @@ int pg_pressure_correct(pg_handle* h, pg_solve_stats* st)
   if (h->cfg.pressureTiming) PG_CHECK(h, pg_launch_norms(h, 0));
   PG_CHECK(h, pg_launch_correct(h));
   PG_CHECK(h, fill_prognostic(h));
+  h->corner_fill_due = false;
   if (h->cfg.pressureTiming) {
     PG_CHECK(h, pg_launch_norms(h, 1));
     // the divergence diagnostic reloads the work arrays (pressure.c:703)

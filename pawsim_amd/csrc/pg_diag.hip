@@ -1,12 +1,16 @@
 // pg_diag.hip -- energy and potential-enstrophy diagnostics on the device
-// (pawsim_diagnostics_write_EZ_if_due, diagnostics.c:483-703; AL81
-// thickness scheme, no active buoyancy).
+// (pawsim_diagnostics_write_EZ_if_due, diagnostics.c:483-703; no active
+// buoyancy).
 //
 // Per owned cell and layer (FP64, the reference's expressions):
 //   h_q, zeta: 4-point average and relative vorticity, or the wall-row forms
 //              (2-point h_q, zeta = 0) on the first row / column of a wall;
 //   Z_k  += dx dy / 2 h_q q^2,  q = (2 Omega_z + zeta) / h_q;
-//   KE   += dx dy / 8 [(h + h_E) u_E^2 + (h_W + h) u^2 + (h + h_N) v_N^2 + (h_S + h) v^2];
+//   KE   += dx dy / 8 [(h + h_E) u_E^2 + (h_W + h) u^2 + (h + h_N) v_N^2 + (h_S + h) v^2]
+//           (AL81, UP3, KT00 thickness), or with HK83 thickness each squared
+//           face velocity also takes 1/4 of the squares of its two
+//           neighbours along the face, one-sided (5/4, 1/4) on wall rows,
+//           and the sum is scaled by dx dy / 12;
 //   PE   += dx dy g'_k h (eta_k - h / 2) + dx dy g'_k h0^4 / h^2 / 6,
 //           eta_k = hhb + sum_{m >= k} h_m.
 // The sums are block partials added in a fixed order (deterministic; the
@@ -48,9 +52,32 @@ __global__ __launch_bounds__(EJ* EI) void k_energy(PgGrid G, PgPhys P, const dou
       const double pv = (2 * omz[pg_at(G, i, j)] + zeta) / h_q;
       z = dx * dy * 0.5 * h_q * (pv * pv);
     }
-    ke = 0.125 * dx * dy
-         * ((H(i, j) + H(i + 1, j)) * (U(i + 1, j) * U(i + 1, j)) + (H(i - 1, j) + H(i, j)) * (U(i, j) * U(i, j))
-            + (H(i, j) + H(i, j + 1)) * (V(i, j + 1) * V(i, j + 1)) + (H(i, j - 1) + H(i, j)) * (V(i, j) * V(i, j)));
+    if (P.thk == PG_THK_HK83) {
+      // diagnostics.c:581-642
+      auto S = [](double x) { return x * x; };
+      const double hE = H(i, j) + H(i + 1, j), hW = H(i - 1, j) + H(i, j);
+      const double hN = H(i, j) + H(i, j + 1), hS = H(i, j - 1) + H(i, j);
+      ke = 0;
+      if (G.wally && gj == 0)
+        ke += hE * (1.25 * S(U(i + 1, j)) + 0.25 * S(U(i + 1, j + 1))) + hW * (1.25 * S(U(i, j)) + 0.25 * S(U(i, j + 1)));
+      else if (G.wally && gj == G.ny - 1)
+        ke += hE * (1.25 * S(U(i + 1, j)) + 0.25 * S(U(i + 1, j - 1))) + hW * (1.25 * S(U(i, j)) + 0.25 * S(U(i, j - 1)));
+      else
+        ke += hE * (S(U(i + 1, j)) + 0.25 * S(U(i + 1, j + 1)) + 0.25 * S(U(i + 1, j - 1)))
+              + hW * (S(U(i, j)) + 0.25 * S(U(i, j + 1)) + 0.25 * S(U(i, j - 1)));
+      if (G.wallx && gi == 0)
+        ke += hN * (1.25 * S(V(i, j + 1)) + 0.25 * S(V(i + 1, j + 1))) + hS * (1.25 * S(V(i, j)) + 0.25 * S(V(i + 1, j)));
+      else if (G.wallx && gi == G.nx - 1)
+        ke += hN * (1.25 * S(V(i, j + 1)) + 0.25 * S(V(i - 1, j + 1))) + hS * (1.25 * S(V(i, j)) + 0.25 * S(V(i - 1, j)));
+      else
+        ke += hN * (S(V(i, j + 1)) + 0.25 * S(V(i + 1, j + 1)) + 0.25 * S(V(i - 1, j + 1)))
+              + hS * (S(V(i, j)) + 0.25 * S(V(i + 1, j)) + 0.25 * S(V(i - 1, j)));
+      ke *= dx * dy / 12.0;
+    } else {
+      ke = 0.125 * dx * dy
+           * ((H(i, j) + H(i + 1, j)) * (U(i + 1, j) * U(i + 1, j)) + (H(i - 1, j) + H(i, j)) * (U(i, j) * U(i, j))
+              + (H(i, j) + H(i, j + 1)) * (V(i, j + 1) * V(i, j + 1)) + (H(i, j - 1) + H(i, j)) * (V(i, j) * V(i, j)));
+    }
     // eta_k bottom-up (work.c:1173-1203)
     const size_t p = pg_at(G, i, j);
     double eta = hhb[p];
@@ -125,7 +152,6 @@ __global__ __launch_bounds__(256) void k_avg_accumulate(PgGrid G, PgPhys P, cons
   double eta[AV_LMAX + 1];
   eta[L] = hhb[c];
   for (int k = L - 1; k >= 0; k--) eta[k] = eta[k + 1] + H(k, gi, gj);
-  const int im = (i + G.nx - 1) % G.nx, jm = (j + G.ny - 1) % G.ny;  // modular faces (work.c:865-896)
   double pp = 0;
   for (int k = 0; k < L; k++) {
     // Montgomery potential MM_B (work.c:1437-1504)
@@ -133,7 +159,8 @@ __global__ __launch_bounds__(256) void k_avg_accumulate(PgGrid G, PgPhys P, cons
     double mm = pp;
     const double hk = H(k, gi, gj);
     if (P.h0 != 0) mm -= P.geff[k] * (P.h0 * P.h0 * P.h0 * P.h0) / (hk * hk * hk) / 3;
-    const double hw = 0.5 * (hk + H(k, im + G.g, gj)), hs = 0.5 * (hk + H(k, gi, jm + G.g));
+    // face thickness without ghosts (work.c:865-1032)
+    const double hw = pg_hW_ng(G, P, hh, u, k, i, j), hs = pg_hS_ng(G, P, hh, v, k, i, j);
     const double uk = U(k, gi, gj), vk = V(k, gi, gj);
     const double hq = 0.25 * (hk + H(k, gi - 1, gj) + H(k, gi, gj - 1) + H(k, gi - 1, gj - 1));
     const double uq = 0.5 * (uk + U(k, gi, gj - 1));
@@ -170,6 +197,7 @@ extern "C" int pg_average_add(pg_handle* h, double weight)
     PG_CHECK(h, hipMemsetAsync(h->avg, 0, n * sizeof(double), h->stream));
   }
   PG_CHECK(h, pg_ensure_halos(h));
+  h->corner_fill_due = true;  // the averages reload the work arrays (diagnostics.c:320)
   hipLaunchKernelGGL(k_avg_accumulate, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->P, h->u, h->v, h->h,
                      h->hhb, h->pi, h->avg, weight, 1);
   PG_CHECK(h, hipGetLastError());

@@ -58,6 +58,7 @@ struct pg_handle {
   int slot[3] = {0, 1, 2};             // buffers holding tendency, tendency_1, tendency_2
   bool halo_dirty = false;             // u, v, h halos are stale (refilled before the next reader)
   bool thickness_done = false;         // the last AB update also applied the rigid-lid thickness fix
+  bool corner_fill_due = true;         // u, v corner ghosts lag the reference's work copies by one fill
   bool rhs_mean_pending = false;       // ctl->mean is still to be removed from rhs (by the MG gather)
   double *hhb = nullptr, *hhs = nullptr, *omz = nullptr, *taux = nullptr, *tauy = nullptr;
   double *Hc = nullptr, *Hw = nullptr, *Hs = nullptr;  // ghosted static geometry

@@ -287,39 +287,37 @@ __global__ __launch_bounds__(TJ* TI) void k_thickness(PgGrid G, double* h, const
   for (int k = 0; k < G.L; k++) h[pg_lay(G, k) + p] *= (1 - herr / tot);
 }
 
-// No-ghost face thickness with modular wrap (work.c:865-896), read back
+// No-ghost face thickness with modular wrap (work.c:865-1032), read back
 // through the scalar halo fill of work.c:1112-1113 for the one ghost face
 // (i = Nx or j = Ny) the divergence stencil touches.
-__device__ __forceinline__ double face_w_ng(const PgGrid& G, const double* h, int k, int i, int j)
+__device__ __forceinline__ double face_w_ng(const PgGrid& G, const PgPhys& P, const double* h, const double* u, int k,
+                                            int i, int j)
 {
   // i, j: owned indices 0..Nx (i == Nx is the east ghost face)
   if (i == G.nx) i = G.wallx ? G.nx - 1 : 0;
-  const int im1 = (i + G.nx - 1) % G.nx;
-  const size_t L = pg_lay(G, k);
-  return 0.5 * (h[L + pg_at(G, i + G.g, j + G.g)] + h[L + pg_at(G, im1 + G.g, j + G.g)]);
+  return pg_hW_ng(G, P, h, u, k, i, j);
 }
-__device__ __forceinline__ double face_s_ng(const PgGrid& G, const double* h, int k, int i, int j)
+__device__ __forceinline__ double face_s_ng(const PgGrid& G, const PgPhys& P, const double* h, const double* v, int k,
+                                            int i, int j)
 {
   if (j == G.ny) j = G.wally ? G.ny - 1 : 0;
-  const int jm1 = (j + G.ny - 1) % G.ny;
-  const size_t L = pg_lay(G, k);
-  return 0.5 * (h[L + pg_at(G, i + G.g, j + G.g)] + h[L + pg_at(G, i + G.g, jm1 + G.g)]);
+  return pg_hS_ng(G, P, h, v, k, i, j);
 }
 
 // Barotropic divergence sum_k div(h u) at owned point (i,j): the RHS form
 // of pressure.c:585-597 (scale = dx*dt, dy*dt) or the diagnostic form of
 // pressure.c:716-726 (scale = dx, dy).
-__device__ __forceinline__ double divergence(const PgGrid& G, const double* u, const double* v, const double* h,
-                                             int i, int j, double sx_, double sy_)
+__device__ __forceinline__ double divergence(const PgGrid& G, const PgPhys& P, const double* u, const double* v,
+                                             const double* h, int i, int j, double sx_, double sy_)
 {
   const int il = i + G.g, jl = j + G.g;
   double acc = 0;
   for (int k = 0; k < G.L; k++) {
     const size_t L = pg_lay(G, k);
-    const double hu_w = u[L + pg_at(G, il, jl)] * face_w_ng(G, h, k, i, j);
-    const double hu_e = u[L + pg_at(G, il + 1, jl)] * face_w_ng(G, h, k, i + 1, j);
-    const double hv_s = v[L + pg_at(G, il, jl)] * face_s_ng(G, h, k, i, j);
-    const double hv_n = v[L + pg_at(G, il, jl + 1)] * face_s_ng(G, h, k, i, j + 1);
+    const double hu_w = u[L + pg_at(G, il, jl)] * face_w_ng(G, P, h, u, k, i, j);
+    const double hu_e = u[L + pg_at(G, il + 1, jl)] * face_w_ng(G, P, h, u, k, i + 1, j);
+    const double hv_s = v[L + pg_at(G, il, jl)] * face_s_ng(G, P, h, v, k, i, j);
+    const double hv_n = v[L + pg_at(G, il, jl + 1)] * face_s_ng(G, P, h, v, k, i, j + 1);
     acc += (hu_e - hu_w) / sx_ + (hv_n - hv_s) / sy_;
   }
   return acc;
@@ -328,14 +326,14 @@ __device__ __forceinline__ double divergence(const PgGrid& G, const double* u, c
 // RHS of the pressure equation (pressure.c:570-606) before mean removal.
 // part non-null: each block also leaves the sum of its rhs values there
 // (the parallel mean of the GPU multigrid, finished by k_mean_final)
-__global__ __launch_bounds__(TJ* TI) void k_rhs(PgGrid G, const double* u, const double* v, const double* h,
+__global__ __launch_bounds__(TJ* TI) void k_rhs(PgGrid G, PgPhys P, const double* u, const double* v, const double* h,
                                                 double* rhs, double dxdt, double dydt, double* part)
 {
   const int j = blockIdx.x * TJ + threadIdx.x;
   const int i = blockIdx.y * TI + threadIdx.y;
   double r = 0;
   if (i < G.nx && j < G.ny) {
-    r = 0 + divergence(G, u, v, h, i, j, dxdt, dydt);
+    r = 0 + divergence(G, P, u, v, h, i, j, dxdt, dydt);
     rhs[pg_at(G, i + G.g, j + G.g)] = r;
   }
   if (!part) return;
@@ -461,7 +459,7 @@ __device__ __forceinline__ void block_max_sum(double& mx, double& ss)
 // Residual norms of rhs - L(pi) (pressure.c:652-690) [which = 0] and
 // barotropic divergence norms after the projection (pressure.c:693-737)
 // [which = 1]: per-block partial max / sum of squares.
-__global__ __launch_bounds__(TJ* TI) void k_norm_partials(PgGrid G, int which, const double* pi, const double* rhs,
+__global__ __launch_bounds__(TJ* TI) void k_norm_partials(PgGrid G, PgPhys P, int which, const double* pi, const double* rhs,
                                                           const double* Hw, const double* Hs, const double* u,
                                                           const double* v, const double* h, double* part)
 {
@@ -479,7 +477,7 @@ __global__ __launch_bounds__(TJ* TI) void k_norm_partials(PgGrid G, int which, c
                          + On * (pi[pg_at(G, il, jl + 1)] - pc) + Os * (pi[pg_at(G, il, jl - 1)] - pc);
       r = rhs[pg_at(G, il, jl)] - Lpi;
     } else {
-      r = divergence(G, u, v, h, i, j, G.dx, G.dy);
+      r = divergence(G, P, u, v, h, i, j, G.dx, G.dy);
     }
     mx = fabs(r);
     ss = r * r;
@@ -765,7 +763,7 @@ hipError_t pg_launch_thickness(pg_handle* h)
 hipError_t pg_launch_rhs(pg_handle* h)
 {
   const double dxdt = h->G.dx * h->cfg.dt, dydt = h->G.dy * h->cfg.dt;
-  hipLaunchKernelGGL(k_rhs, owned_grid(h->G, 1), dim3(TJ, TI), 0, h->stream, h->G, h->u, h->v, h->h, h->rhs, dxdt, dydt,
+  hipLaunchKernelGGL(k_rhs, owned_grid(h->G, 1), dim3(TJ, TI), 0, h->stream, h->G, h->P, h->u, h->v, h->h, h->rhs, dxdt, dydt,
                      (double*)nullptr);
   return hipGetLastError();
 }
@@ -775,7 +773,7 @@ hipError_t pg_launch_rhs_mean(pg_handle* h)
 {
   const double dxdt = h->G.dx * h->cfg.dt, dydt = h->G.dy * h->cfg.dt;
   const dim3 grid = owned_grid(h->G, 1);
-  hipLaunchKernelGGL(k_rhs, grid, dim3(TJ, TI), 0, h->stream, h->G, h->u, h->v, h->h, h->rhs, dxdt, dydt, h->partials);
+  hipLaunchKernelGGL(k_rhs, grid, dim3(TJ, TI), 0, h->stream, h->G, h->P, h->u, h->v, h->h, h->rhs, dxdt, dydt, h->partials);
   hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(64), 0, h->stream, h->G, (int)(grid.x * grid.y), h->partials, h->ctl);
   return hipGetLastError();
 }
@@ -809,7 +807,7 @@ hipError_t pg_launch_norms(pg_handle* h, int which)
 {
   dim3 grid = owned_grid(h->G, 1);
   const int nb = grid.x * grid.y;
-  hipLaunchKernelGGL(k_norm_partials, grid, dim3(TJ, TI), 0, h->stream, h->G, which, h->pi, h->rhs, h->Hw, h->Hs,
+  hipLaunchKernelGGL(k_norm_partials, grid, dim3(TJ, TI), 0, h->stream, h->G, h->P, which, h->pi, h->rhs, h->Hw, h->Hs,
                      h->u, h->v, h->h, h->partials);
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(64), 0, h->stream, nb, h->partials, h->ctl, which);
   return hipGetLastError();

@@ -39,6 +39,7 @@
 #define PG_MAXL 16
 
 enum { PG_MOM_AL81 = 0, PG_MOM_HK83 = 1, PG_MOM_S75E = 2, PG_MOM_TW81 = 3 };
+enum { PG_THK_AL81 = 0, PG_THK_HK83 = 1, PG_THK_UP3 = 2, PG_THK_KT00 = 3 };
 
 // Geometry and physics parameters, passed to kernels by value.
 struct PgGrid {
@@ -50,10 +51,31 @@ struct PgGrid {
 
 struct PgPhys {
   int mom;                 // momentum scheme
+  int thk;                 // thickness scheme
   int useA2, useA4, useWind, useLin, useQuad;
+  int useSmag;             // Smagorinsky coefficients (A2smag or A4smag > 0)
   double h0, A2, A4c, lin, quad;
+  double A4;               // constant biharmonic coefficient (A4c = sqrt(A4))
+  double A2fac, A4fac;     // Smagorinsky factors (work.c:1542-1543)
+  double kt00;             // KT00 limiter slope factor
   double gg[PG_MAXL], geff[PG_MAXL];
 };
+
+// Viscosity switches and coefficients (work.c:1538-1544): A2 and A4 terms
+// are on when the constant or the Smagorinsky coefficient is positive.
+inline void pg_set_viscosity(PgPhys& P, double A2, double A4, double A2smag, double A4smag, double dx, double dy)
+{
+  P.useA2 = (A2 > 0) || (A2smag > 0);
+  P.useA4 = (A4 > 0) || (A4smag > 0);
+  P.useSmag = (A2smag > 0) || (A4smag > 0);
+  P.A2 = A2;
+  P.A4 = A4;
+  P.A4c = sqrt(A4);
+  const double pi = 3.14159265358979323846;  // defs.h:46
+  const double dmax = fmax(dx, dy), s2 = A2smag * dmax / pi, s4 = A4smag / pi;
+  P.A2fac = s2 * s2;
+  P.A4fac = (s4 * s4) * (dmax * dmax * dmax * dmax) / 8;
+}
 
 // Read-only inputs of the tendency: ghost-filled state and static fields.
 struct PgIn {
@@ -99,6 +121,20 @@ PG_HD int pg_smap_y(const PgGrid& G, int j)
   return pg_wrap(j, G.g, G.ny);
 }
 
+// Scalar-kind map of an array the stencil computes over [1, s-1) and then
+// fills (h_west, h_south): as pg_smap_*, except on a periodic-x / wall-y
+// domain, where the periodic x copy runs before the wall y fill and moves
+// the computed ghost rows of the owned columns into the corners
+// (field.c:137-198), so a corner holds the formula at the wrapped column
+// and the unmapped ghost row (rows 1..g-1 and g+ny..sy-2; the stencils
+// never read rows 0 and sy-1 of a corner).
+PG_HD void pg_fmap(const PgGrid& G, int& i, int& j)
+{
+  const bool ghost_col = i < G.g || i >= G.g + G.nx;
+  i = pg_smap_x(G, i);
+  if (!(ghost_col && !G.wallx && G.wally)) j = pg_smap_y(G, j);
+}
+
 // Internal-kind map (field.c:416-426): the serial exchange copies periodic y
 // ghosts on owned columns only, then periodic x ghosts over full columns;
 // every other position keeps the value computed from the ghost inputs.
@@ -115,18 +151,138 @@ PG_HD bool pg_in_core(const PgGrid& G, int i, int j) { return i >= 1 && i < G.sx
 #define PG_V(k, i, j) (I.v[pg_lay(G, k) + pg_at(G, i, j)])
 #define PG_H(k, i, j) (I.h[pg_lay(G, k) + pg_at(G, i, j)])
 
-// h_west / h_south, AL81 centred average (work.c:588-600), scalar kind.
-PG_HD double pg_hW(const PgGrid& G, const PgIn& I, int k, int i, int j)
+// Kurganov-Tadmor limiter of three slopes (defs.c:994-1016): the value
+// nearest zero when all three have the same strict sign, else zero.
+PG_HD double pg_minmod(double a, double b, double c)
 {
-  i = pg_smap_x(G, i);
-  j = pg_smap_y(G, j);
-  return 0.5 * (PG_H(k, i, j) + PG_H(k, i - 1, j));
+  if (a > 0 && b > 0 && c > 0) return fmin(fmin(a, b), c);
+  if (a < 0 && b < 0 && c < 0) return fmax(fmax(a, b), c);
+  return 0.0;
 }
-PG_HD double pg_hS(const PgGrid& G, const PgIn& I, int k, int i, int j)
+
+// h_west / h_south of the tendency (work.c:566-683 with velocities),
+// scalar kind (pg_fmap): the value at a ghost position is the one at the
+// mapped position, where the stencil reads the ghost-filled h (and u or v
+// for the upwind direction).  The second differences and slopes of UP3 / KT00 are
+// evaluated at their own positions from h directly.
+PG_HD double pg_hW(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
 {
-  i = pg_smap_x(G, i);
-  j = pg_smap_y(G, j);
-  return 0.5 * (PG_H(k, i, j) + PG_H(k, i, j - 1));
+  pg_fmap(G, i, j);
+  switch (P.thk) {
+    case PG_THK_HK83:
+      return (1.0 / 3) * (PG_H(k, i, j) + PG_H(k, i - 1, j)
+                          + 0.25 * (PG_H(k, i, j + 1) + PG_H(k, i - 1, j + 1) + PG_H(k, i, j - 1) + PG_H(k, i - 1, j - 1)));
+    case PG_THK_UP3: {
+      const int c = (PG_U(k, i, j) > 0) ? i - 1 : i;
+      const double d2 = PG_H(k, c + 1, j) - 2 * PG_H(k, c, j) + PG_H(k, c - 1, j);
+      const double w = 0.5 * (PG_H(k, i, j) + PG_H(k, i - 1, j));
+      return w - (1 / 6.0) * d2;
+    }
+    case PG_THK_KT00: {
+      const double s = P.kt00;
+      if (PG_U(k, i, j) > 0) {
+        const int c = i - 1;
+        const double d = pg_minmod(s * (PG_H(k, c + 1, j) - PG_H(k, c, j)), 0.5 * (PG_H(k, c + 1, j) - PG_H(k, c - 1, j)),
+                                   s * (PG_H(k, c, j) - PG_H(k, c - 1, j)));
+        return PG_H(k, c, j) + 0.5 * d;
+      }
+      const double d = pg_minmod(s * (PG_H(k, i + 1, j) - PG_H(k, i, j)), 0.5 * (PG_H(k, i + 1, j) - PG_H(k, i - 1, j)),
+                                 s * (PG_H(k, i, j) - PG_H(k, i - 1, j)));
+      return PG_H(k, i, j) - 0.5 * d;
+    }
+    default:
+      return 0.5 * (PG_H(k, i, j) + PG_H(k, i - 1, j));
+  }
+}
+PG_HD double pg_hS(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  pg_fmap(G, i, j);
+  switch (P.thk) {
+    case PG_THK_HK83:
+      return (1.0 / 3) * (PG_H(k, i, j) + PG_H(k, i, j - 1)
+                          + 0.25 * (PG_H(k, i + 1, j) + PG_H(k, i - 1, j) + PG_H(k, i + 1, j - 1) + PG_H(k, i - 1, j - 1)));
+    case PG_THK_UP3: {
+      // row 0 of the second difference is outside its computed range and
+      // stays zero (a corner's row 1 would read it)
+      const int c = (PG_V(k, i, j) > 0) ? j - 1 : j;
+      const double d2 = (c >= 1) ? PG_H(k, i, c + 1) - 2 * PG_H(k, i, c) + PG_H(k, i, c - 1) : 0.0;
+      const double s = 0.5 * (PG_H(k, i, j) + PG_H(k, i, j - 1));
+      return s - (1 / 6.0) * d2;
+    }
+    case PG_THK_KT00: {
+      // both states add the half slope (work.c:673-677)
+      const double s = P.kt00;
+      const int c = (PG_V(k, i, j) > 0) ? j - 1 : j;
+      const double d = (c >= 1) ? pg_minmod(s * (PG_H(k, i, c + 1) - PG_H(k, i, c)),
+                                            0.5 * (PG_H(k, i, c + 1) - PG_H(k, i, c - 1)), s * (PG_H(k, i, c) - PG_H(k, i, c - 1)))
+                                : 0.0;
+      return PG_H(k, i, c) + 0.5 * d;
+    }
+    default:
+      return 0.5 * (PG_H(k, i, j) + PG_H(k, i, j - 1));
+  }
+}
+
+// h_west / h_south of the pressure RHS, the corrected divergence and the
+// averages (work.c:865-1032, pawsim_work_calc_face_thickness_no_ghost): the
+// stencil on the owned cells alone, every neighbour index taken modulo Nx /
+// Ny (also across walls), except the HK83 rows next to a wall, which use
+// one-sided weights.  i, j are owned indices; h, u, v ghosted layer arrays.
+PG_HD double pg_hng(const PgGrid& G, const double* h, int k, int i, int j)
+{
+  return h[pg_lay(G, k) + pg_at(G, i + G.g, j + G.g)];
+}
+PG_HD double pg_hW_ng(const PgGrid& G, const PgPhys& P, const double* h, const double* u, int k, int i, int j)
+{
+  const int nx = G.nx, ny = G.ny;
+  const int im1 = (i + nx - 1) % nx, jm1 = (j + ny - 1) % ny, jp1 = (j + 1) % ny;
+#define HN(a, b) pg_hng(G, h, k, a, b)
+  switch (P.thk) {
+    case PG_THK_HK83:
+      if (G.wally && j == 0) return (1.0 / 3) * (1.25 * (HN(i, j) + HN(im1, j)) + 0.25 * (HN(i, jp1) + HN(im1, jp1)));
+      if (G.wally && j == ny - 1) return (1.0 / 3) * (1.25 * (HN(i, j) + HN(im1, j)) + 0.25 * (HN(i, jm1) + HN(im1, jm1)));
+      return (1.0 / 3) * (HN(i, j) + HN(im1, j) + 0.25 * (HN(i, jp1) + HN(im1, jp1) + HN(i, jm1) + HN(im1, jm1)));
+    case PG_THK_UP3: {
+      const int c = (u[pg_lay(G, k) + pg_at(G, i + G.g, j + G.g)] > 0) ? im1 : i;
+      const double d2 = HN((c + 1) % nx, j) - 2 * HN(c, j) + HN((c + nx - 1) % nx, j);
+      return 0.5 * (HN(i, j) + HN(im1, j)) - (1 / 6.0) * d2;
+    }
+    case PG_THK_KT00: {
+      const bool up = u[pg_lay(G, k) + pg_at(G, i + G.g, j + G.g)] > 0;
+      const int c = up ? im1 : i, cp = (c + 1) % nx, cm = (c + nx - 1) % nx;
+      const double s = P.kt00;
+      const double d = pg_minmod(s * (HN(cp, j) - HN(c, j)), 0.5 * (HN(cp, j) - HN(cm, j)), s * (HN(c, j) - HN(cm, j)));
+      return up ? HN(c, j) + 0.5 * d : HN(c, j) - 0.5 * d;
+    }
+    default:
+      return 0.5 * (HN(i, j) + HN(im1, j));
+  }
+}
+PG_HD double pg_hS_ng(const PgGrid& G, const PgPhys& P, const double* h, const double* v, int k, int i, int j)
+{
+  const int nx = G.nx, ny = G.ny;
+  const int im1 = (i + nx - 1) % nx, ip1 = (i + 1) % nx, jm1 = (j + ny - 1) % ny;
+  switch (P.thk) {
+    case PG_THK_HK83:
+      if (G.wallx && i == 0) return (1.0 / 3) * (1.25 * (HN(i, j) + HN(i, jm1)) + 0.25 * (HN(ip1, j) + HN(ip1, jm1)));
+      if (G.wallx && i == nx - 1) return (1.0 / 3) * (1.25 * (HN(i, j) + HN(i, jm1)) + 0.25 * (HN(im1, j) + HN(im1, jm1)));
+      return (1.0 / 3) * (HN(i, j) + HN(i, jm1) + 0.25 * (HN(ip1, j) + HN(im1, j) + HN(ip1, jm1) + HN(im1, jm1)));
+    case PG_THK_UP3: {
+      const int c = (v[pg_lay(G, k) + pg_at(G, i + G.g, j + G.g)] > 0) ? jm1 : j;
+      const double d2 = HN(i, (c + 1) % ny) - 2 * HN(i, c) + HN(i, (c + ny - 1) % ny);
+      return 0.5 * (HN(i, j) + HN(i, jm1)) - (1 / 6.0) * d2;
+    }
+    case PG_THK_KT00: {
+      // both states add the half slope (work.c:1018-1019)
+      const int c = (v[pg_lay(G, k) + pg_at(G, i + G.g, j + G.g)] > 0) ? jm1 : j, cp = (c + 1) % ny, cm = (c + ny - 1) % ny;
+      const double s = P.kt00;
+      const double d = pg_minmod(s * (HN(i, cp) - HN(i, c)), 0.5 * (HN(i, cp) - HN(i, cm)), s * (HN(i, c) - HN(i, cm)));
+      return HN(i, c) + 0.5 * d;
+    }
+    default:
+      return 0.5 * (HN(i, j) + HN(i, jm1));
+  }
+#undef HN
 }
 
 // Montgomery potential (work.c:1173-1203 eta, 1431-1443 pp, 1499-1504 MM),
@@ -189,15 +345,15 @@ PG_HD double pg_qq(const PgGrid& G, const PgIn& I, int k, int i, int j)
 }
 
 // Mass fluxes (work.c:176-182), internal kind; face thickness is scalar kind.
-PG_HD double pg_huu(const PgGrid& G, const PgIn& I, int k, int i, int j)
+PG_HD double pg_huu(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
 {
   pg_imap(G, i, j);
-  return PG_U(k, i, j) * pg_hW(G, I, k, i, j);
+  return PG_U(k, i, j) * pg_hW(G, P, I, k, i, j);
 }
-PG_HD double pg_hvv(const PgGrid& G, const PgIn& I, int k, int i, int j)
+PG_HD double pg_hvv(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
 {
   pg_imap(G, i, j);
-  return PG_V(k, i, j) * pg_hS(G, I, k, i, j);
+  return PG_V(k, i, j) * pg_hS(G, P, I, k, i, j);
 }
 
 // PV interpolation coefficients (work.c:1294-1404), internal kind: the
@@ -244,29 +400,63 @@ PG_HD double pg_DDS(const PgGrid& G, const PgIn& I, int k, int i, int j)
   return (PG_U(k, i, j) - PG_U(k, i, j - 1)) / G.dy + (PG_V(k, i, j) - PG_V(k, i - 1, j)) / G.dx;
 }
 
-// Constant viscosity coefficient arrays (work.c:1614-1631): set on the core range only.
-PG_HD double pg_coef(const PgGrid& G, double c, int i, int j) { return pg_in_core(G, i, j) ? c : 0.0; }
+// Viscosity coefficient arrays A2_h, A2_q, A4sqrt_h, A4sqrt_q, set on the
+// core range only: constants (work.c:1614-1631), or with Smagorinsky
+// (work.c:1587-1610) the larger of the constant and the factor times the
+// strain magnitude, DD_T averaged to q points or DD_S averaged to h points.
+PG_HD double pg_DDq(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  const double t = 0.25 * (pg_DDT(G, I, k, i, j) + pg_DDT(G, I, k, i - 1, j) + pg_DDT(G, I, k, i, j - 1)
+                           + pg_DDT(G, I, k, i - 1, j - 1));
+  const double s = pg_DDS(G, I, k, i, j);
+  return sqrt(t * t + s * s);
+}
+PG_HD double pg_DDh(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  const double t = pg_DDT(G, I, k, i, j);
+  const double s = 0.25 * (pg_DDS(G, I, k, i, j) + pg_DDS(G, I, k, i + 1, j) + pg_DDS(G, I, k, i, j + 1)
+                           + pg_DDS(G, I, k, i + 1, j + 1));
+  return sqrt(t * t + s * s);
+}
+PG_HD double pg_A2h(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0.0;
+  return P.useSmag ? fmax(P.A2, P.A2fac * pg_DDh(G, I, k, i, j)) : P.A2;
+}
+PG_HD double pg_A2q(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0.0;
+  return P.useSmag ? fmax(P.A2, P.A2fac * pg_DDq(G, I, k, i, j)) : P.A2;
+}
+PG_HD double pg_A4h(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0.0;
+  return P.useSmag ? sqrt(fmax(P.A4, P.A4fac * pg_DDh(G, I, k, i, j))) : P.A4c;
+}
+PG_HD double pg_A4q(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0.0;
+  return P.useSmag ? sqrt(fmax(P.A4, P.A4fac * pg_DDq(G, I, k, i, j))) : P.A4c;
+}
 
 // Biharmonic intermediate fields (work.c:1633-1661), zero outside [1, s-1).
 PG_HD double pg_UU4(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
 {
   if (!pg_in_core(G, i, j)) return 0;
-  const double c = P.A4c;
-  return ((pg_coef(G, c, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
-           - pg_coef(G, c, i - 1, j) * PG_H(k, i - 1, j) * pg_DDT(G, I, k, i - 1, j)) / G.dx
-          + (pg_coef(G, c, i, j + 1) * pg_hq(G, I, k, i, j + 1) * pg_DDS(G, I, k, i, j + 1)
-             - pg_coef(G, c, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / G.dy)
-         / pg_hW(G, I, k, i, j);
+  return ((pg_A4h(G, P, I, k, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
+           - pg_A4h(G, P, I, k, i - 1, j) * PG_H(k, i - 1, j) * pg_DDT(G, I, k, i - 1, j)) / G.dx
+          + (pg_A4q(G, P, I, k, i, j + 1) * pg_hq(G, I, k, i, j + 1) * pg_DDS(G, I, k, i, j + 1)
+             - pg_A4q(G, P, I, k, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / G.dy)
+         / pg_hW(G, P, I, k, i, j);
 }
 PG_HD double pg_VV4(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
 {
   if (!pg_in_core(G, i, j)) return 0;
-  const double c = P.A4c;
-  return ((pg_coef(G, c, i + 1, j) * pg_hq(G, I, k, i + 1, j) * pg_DDS(G, I, k, i + 1, j)
-           - pg_coef(G, c, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / G.dx
-          - (pg_coef(G, c, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
-             - pg_coef(G, c, i, j - 1) * PG_H(k, i, j - 1) * pg_DDT(G, I, k, i, j - 1)) / G.dy)
-         / pg_hS(G, I, k, i, j);
+  return ((pg_A4q(G, P, I, k, i + 1, j) * pg_hq(G, I, k, i + 1, j) * pg_DDS(G, I, k, i + 1, j)
+           - pg_A4q(G, P, I, k, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / G.dx
+          - (pg_A4h(G, P, I, k, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
+             - pg_A4h(G, P, I, k, i, j - 1) * PG_H(k, i, j - 1) * pg_DDT(G, I, k, i, j - 1)) / G.dy)
+         / pg_hS(G, P, I, k, i, j);
 }
 PG_HD double pg_DD4T(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
 {
@@ -371,13 +561,13 @@ PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, in
     }
   }
   // Mass fluxes on the needed faces.
-  const double huu_00 = pg_huu(G, I, k, i, j), huu_p0 = pg_huu(G, I, k, i + 1, j);
-  const double huu_m0 = pg_huu(G, I, k, i - 1, j), huu_0m = pg_huu(G, I, k, i, j - 1);
-  const double huu_pm = pg_huu(G, I, k, i + 1, j - 1);
-  const double hvv_00 = pg_hvv(G, I, k, i, j), hvv_0p = pg_hvv(G, I, k, i, j + 1);
-  const double hvv_m0 = pg_hvv(G, I, k, i - 1, j), hvv_mp = pg_hvv(G, I, k, i - 1, j + 1);
-  const double hvv_0m = pg_hvv(G, I, k, i, j - 1);
-  const double hwest = pg_hW(G, I, k, i, j), hsouth = pg_hS(G, I, k, i, j);
+  const double huu_00 = pg_huu(G, P, I, k, i, j), huu_p0 = pg_huu(G, P, I, k, i + 1, j);
+  const double huu_m0 = pg_huu(G, P, I, k, i - 1, j), huu_0m = pg_huu(G, P, I, k, i, j - 1);
+  const double huu_pm = pg_huu(G, P, I, k, i + 1, j - 1);
+  const double hvv_00 = pg_hvv(G, P, I, k, i, j), hvv_0p = pg_hvv(G, P, I, k, i, j + 1);
+  const double hvv_m0 = pg_hvv(G, P, I, k, i - 1, j), hvv_mp = pg_hvv(G, P, I, k, i - 1, j + 1);
+  const double hvv_0m = pg_hvv(G, P, I, k, i, j - 1);
+  const double hwest = pg_hW(G, P, I, k, i, j), hsouth = pg_hS(G, P, I, k, i, j);
   const double MM00 = pg_MM(G, P, I, k, i, j), KE00 = pg_KE(G, P, I, k, i, j);
 
   // ---- u momentum (work.c:1681-1752) ----
@@ -397,16 +587,16 @@ PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, in
     double a2 = 0, a4 = 0;
     const double buoy = 0;
     if (P.useA2) {
-      a2 = ((pg_coef(G, P.A2, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
-             - pg_coef(G, P.A2, i - 1, j) * PG_H(k, i - 1, j) * pg_DDT(G, I, k, i - 1, j)) / dx
-            + (pg_coef(G, P.A2, i, j + 1) * pg_hq(G, I, k, i, j + 1) * pg_DDS(G, I, k, i, j + 1)
-               - pg_coef(G, P.A2, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / dy);
+      a2 = ((pg_A2h(G, P, I, k, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
+             - pg_A2h(G, P, I, k, i - 1, j) * PG_H(k, i - 1, j) * pg_DDT(G, I, k, i - 1, j)) / dx
+            + (pg_A2q(G, P, I, k, i, j + 1) * pg_hq(G, I, k, i, j + 1) * pg_DDS(G, I, k, i, j + 1)
+               - pg_A2q(G, P, I, k, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / dy);
     }
     if (P.useA4) {
-      a4 = -((pg_coef(G, P.A4c, i, j) * PG_H(k, i, j) * pg_DD4T(G, P, I, k, i, j)
-              - pg_coef(G, P.A4c, i - 1, j) * PG_H(k, i - 1, j) * pg_DD4T(G, P, I, k, i - 1, j)) / dx
-             + (pg_coef(G, P.A4c, i, j + 1) * pg_hq(G, I, k, i, j + 1) * pg_DD4S(G, P, I, k, i, j + 1)
-                - pg_coef(G, P.A4c, i, j) * pg_hq(G, I, k, i, j) * pg_DD4S(G, P, I, k, i, j)) / dy);
+      a4 = -((pg_A4h(G, P, I, k, i, j) * PG_H(k, i, j) * pg_DD4T(G, P, I, k, i, j)
+              - pg_A4h(G, P, I, k, i - 1, j) * PG_H(k, i - 1, j) * pg_DD4T(G, P, I, k, i - 1, j)) / dx
+             + (pg_A4q(G, P, I, k, i, j + 1) * pg_hq(G, I, k, i, j + 1) * pg_DD4S(G, P, I, k, i, j + 1)
+                - pg_A4q(G, P, I, k, i, j) * pg_hq(G, I, k, i, j) * pg_DD4S(G, P, I, k, i, j)) / dy);
     }
     ru = gM + gK + qf + ((hwest != 0) ? (a2 + a4 + buoy) / hwest : 0);
     // forcing (forcing.c:244-358): wind on the top layer, drag on the bottom layer
@@ -444,16 +634,16 @@ PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, in
     double a2 = 0, a4 = 0;
     const double buoy = 0;
     if (P.useA2) {
-      a2 = ((pg_coef(G, P.A2, i + 1, j) * pg_hq(G, I, k, i + 1, j) * pg_DDS(G, I, k, i + 1, j)
-             - pg_coef(G, P.A2, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / dx
-            - (pg_coef(G, P.A2, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
-               - pg_coef(G, P.A2, i, j - 1) * PG_H(k, i, j - 1) * pg_DDT(G, I, k, i, j - 1)) / dy);
+      a2 = ((pg_A2q(G, P, I, k, i + 1, j) * pg_hq(G, I, k, i + 1, j) * pg_DDS(G, I, k, i + 1, j)
+             - pg_A2q(G, P, I, k, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / dx
+            - (pg_A2h(G, P, I, k, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
+               - pg_A2h(G, P, I, k, i, j - 1) * PG_H(k, i, j - 1) * pg_DDT(G, I, k, i, j - 1)) / dy);
     }
     if (P.useA4) {
-      a4 = -((pg_coef(G, P.A4c, i + 1, j) * pg_hq(G, I, k, i + 1, j) * pg_DD4S(G, P, I, k, i + 1, j)
-              - pg_coef(G, P.A4c, i, j) * pg_hq(G, I, k, i, j) * pg_DD4S(G, P, I, k, i, j)) / dx
-             - (pg_coef(G, P.A4c, i, j) * PG_H(k, i, j) * pg_DD4T(G, P, I, k, i, j)
-                - pg_coef(G, P.A4c, i, j - 1) * PG_H(k, i, j - 1) * pg_DD4T(G, P, I, k, i, j - 1)) / dy);
+      a4 = -((pg_A4q(G, P, I, k, i + 1, j) * pg_hq(G, I, k, i + 1, j) * pg_DD4S(G, P, I, k, i + 1, j)
+              - pg_A4q(G, P, I, k, i, j) * pg_hq(G, I, k, i, j) * pg_DD4S(G, P, I, k, i, j)) / dx
+             - (pg_A4h(G, P, I, k, i, j) * PG_H(k, i, j) * pg_DD4T(G, P, I, k, i, j)
+                - pg_A4h(G, P, I, k, i, j - 1) * PG_H(k, i, j - 1) * pg_DD4T(G, P, I, k, i, j - 1)) / dy);
     }
     rv = gM + gK + qf + ((hsouth != 0) ? (a2 + a4 + buoy) / hsouth : 0);
     const double fs = (k == 0) ? 1.0 : 0.0, fb = (k == G.L - 1) ? 1.0 : 0.0;

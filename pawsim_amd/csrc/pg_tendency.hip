@@ -517,10 +517,12 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, Pg
 
 }  // namespace
 
-// True if the fast pair of kernels serves this configuration.
+// True if the fast pair of kernels serves this configuration (constant
+// Laplacian viscosity and centred face thickness; everything else goes
+// through the per-point kernel).
 bool pg_tendency_fast_ok(const pg_handle* h)
 {
-  return !h->P.useA4 && h->G.nx >= 8 && h->G.ny >= 8 && h->G.L <= 8;
+  return !h->P.useA4 && !h->P.useSmag && h->P.thk == PG_THK_AL81 && h->G.nx >= 8 && h->G.ny >= 8 && h->G.L <= 8;
 }
 
 hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double* dh)

@@ -1,7 +1,9 @@
 // Host check of the device stencil math (pawsim_amd/csrc/pg_point.h)
 // against the CPU oracle: run the oracle for a few steps, then evaluate
-// pg_tendency_point at every owned point of every layer on the oracle's
-// ghost-filled state and compare bit for bit with the oracle's tendency.
+// pg_tendency_point at every owned point of every layer on the ghost-filled
+// work copies of the state that the oracle's tendency read, and compare bit
+// for bit with the oracle's tendency; then the face thickness arrays, with
+// and without ghosts, against the point functions.
 //
 //   point_vs_oracle <input_file> <nsteps>
 // prints "points=<n> mismatches=<m> maxabs=<d>" and exits 0 if m == 0.
@@ -45,14 +47,13 @@ int main(int argc, char** argv)
   PgPhys P;
   memset(&P, 0, sizeof(P));
   P.mom = (int)m.cfg.momScheme;
-  P.useA2 = m.cfg.A2 > 0;
-  P.useA4 = m.cfg.A4 > 0;
+  P.thk = (int)m.cfg.thkScheme;
+  P.kt00 = m.cfg.KT00_sigma;
+  pg_set_viscosity(P, m.cfg.A2, m.cfg.A4, m.cfg.A2smag, m.cfg.A4smag, G.dx, G.dy);
   P.useWind = m.cfg.useWind;
   P.useLin = m.cfg.linDrag > 0;
   P.useQuad = m.cfg.quadDrag > 0;
   P.h0 = m.cfg.h0;
-  P.A2 = m.cfg.A2;
-  P.A4c = sqrt(m.cfg.A4);
   P.lin = m.cfg.linDrag;
   P.quad = m.cfg.quadDrag;
   for (int k = 0; k < G.L; k++) {
@@ -64,9 +65,11 @@ int main(int argc, char** argv)
   double* v = (double*)malloc(G.L * fs * sizeof(double));
   double* h = (double*)malloc(G.L * fs * sizeof(double));
   for (int k = 0; k < G.L; k++) {
-    memcpy(u + k * fs, m.u[k].d, fs * sizeof(double));
-    memcpy(v + k * fs, m.v[k].d, fs * sizeof(double));
-    memcpy(h + k * fs, m.h[k].d, fs * sizeof(double));
+    // the work copies the tendency read (pawsim_work_load_state): their
+    // ghost corners on a periodic-x / wall-y domain lag one load behind
+    memcpy(u + k * fs, m.uw[k].d, fs * sizeof(double));
+    memcpy(v + k * fs, m.vw[k].d, fs * sizeof(double));
+    memcpy(h + k * fs, m.hw[k].d, fs * sizeof(double));
   }
   PgIn I{u, v, h, m.hhb.d, m.omz.d, m.taux.d, m.tauy.d};
   long pts = 0, bad = 0;
@@ -82,6 +85,46 @@ int main(int argc, char** argv)
           pts++;
           if (memcmp(&ref[q], &got[q], sizeof(double)) != 0 && !(ref[q] == 0 && got[q] == 0)) {
             if (bad < 5) fprintf(stderr, "field %d k=%d i=%d j=%d ref=%.17g got=%.17g\n", q, k, i, j, ref[q], got[q]);
+            bad++;
+            maxabs = fmax(maxabs, fabs(ref[q] - got[q]));
+          }
+        }
+      }
+  // the tendency's face thickness arrays, halo fill included, over the
+  // range the stencil computes ([1, s-1) in both directions)
+  for (int k = 0; k < G.L; k++)
+    for (int i = 1; i < G.sx - 1; i++)
+      for (int j = 1; j < G.sy - 1; j++) {
+        const double ref[2] = {OF(&m.hwest[k], i, j), OF(&m.hsouth[k], i, j)};
+        const double got[2] = {pg_hW(G, P, I, k, i, j), pg_hS(G, P, I, k, i, j)};
+        for (int q = 0; q < 2; q++) {
+          pts++;
+          if (memcmp(&ref[q], &got[q], sizeof(double)) != 0) {
+            if (bad < 5) fprintf(stderr, "face array %d k=%d i=%d j=%d ref=%.17g got=%.17g\n", q, k, i, j, ref[q], got[q]);
+            bad++;
+            maxabs = fmax(maxabs, fabs(ref[q] - got[q]));
+          }
+        }
+      }
+  // face thickness without ghosts (pressure RHS, divergence, averages)
+  or_average_work(&m);
+  for (int k = 0; k < G.L; k++) {
+    const size_t off = (size_t)k * fs;
+    for (size_t p = 0; p < fs; p++) {
+      u[off + p] = m.uw[k].d[p];
+      v[off + p] = m.vw[k].d[p];
+      h[off + p] = m.hw[k].d[p];
+    }
+  }
+  for (int k = 0; k < G.L; k++)
+    for (int i = 0; i < G.nx; i++)
+      for (int j = 0; j < G.ny; j++) {
+        const double ref[2] = {OF(&m.hwest[k], i + G.g, j + G.g), OF(&m.hsouth[k], i + G.g, j + G.g)};
+        const double got[2] = {pg_hW_ng(G, P, h, u, k, i, j), pg_hS_ng(G, P, h, v, k, i, j)};
+        for (int q = 0; q < 2; q++) {
+          pts++;
+          if (memcmp(&ref[q], &got[q], sizeof(double)) != 0) {
+            if (bad < 5) fprintf(stderr, "face %d k=%d i=%d j=%d ref=%.17g got=%.17g\n", q, k, i, j, ref[q], got[q]);
             bad++;
             maxabs = fmax(maxabs, fabs(ref[q] - got[q]));
           }

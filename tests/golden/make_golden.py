@@ -59,6 +59,22 @@ def cases():
         # online averages (U_avg ... P_avg); windows that end inside a step
         "box24_avg": inputs.gyre(nx=24, ny=24, nt=8, A2=50.0, h0=20.0, savefreqAvg=750.0, **T),
         "chan32x16_avg": inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1, savefreqAvg=500.0, **T),
+        # thickness schemes (work.c:566-683, 865-1032; HK83 kinetic energy diagnostics.c:581-642)
+        "box16_thk_hk83": inputs.gyre(nx=16, ny=16, nt=6, momentumScheme=1, thicknessScheme=1,
+                                      savefreqEZ=600.0, savefreqAvg=900.0, **T),
+        "per16_thk_hk83": inputs.periodic_box(nx=16, ny=16, nt=6, momentumScheme=1, thicknessScheme=1,
+                                              savefreqEZ=600.0, **T),
+        "box16_thk_up3": inputs.gyre(nx=16, ny=16, nt=6, thicknessScheme=2, A2=50.0, savefreqAvg=900.0, **T),
+        "per16_thk_up3": inputs.periodic_box(nx=16, ny=16, nt=8, thicknessScheme=2, savefreqAvg=900.0, **T),
+        "chan32x16_thk_kt00": inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1, thicknessScheme=3,
+                                             savefreqEZ=200.0, savefreqAvg=500.0, **T),
+        "per16_thk_kt00": inputs.periodic_box(nx=16, ny=16, nt=8, thicknessScheme=3, KT00_sigma=1.0, **T),
+        "wallx16_thk_kt00": inputs.periodic_box(nx=16, ny=16, nt=6, thicknessScheme=3, useWallEW=1,
+                                                savefreqAvg=900.0, **T),
+        # Smagorinsky viscosity (work.c:1587-1610)
+        "per16_smag": inputs.periodic_box(nx=16, ny=16, nt=8, A2smag=4.0, A4smag=4.0, **T),
+        "box16_smag": inputs.gyre(nx=16, ny=16, nt=6, A2=10.0, A2smag=2.0, **T),
+        "chan32x16_smag_a4": inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1, A4=1e9, A4smag=3.0, **T),
     }
 
 

@@ -65,6 +65,23 @@ TRAJECTORY_CASES = {
     # per-point kernel everywhere
     "jets96x64_pointkernel": dict(kind="jets", nx=96, ny=64, nlay=3, dx=20e3, nt=6, solver=1, A2=200.0,
                                   H_rest=(500.0, 1500.0, 2000.0), quadDragCoeff=2e-3, tendency_kernel=1),
+    # thickness schemes and Smagorinsky viscosity (per-point kernel, scheme-aware
+    # RHS / divergence / averages face thickness, HK83 kinetic energy)
+    "gyre64_thk_hk83": dict(kind="gyre", nx=64, ny=64, nt=8, solver=1, momentumScheme=1, thicknessScheme=1,
+                            savefreqEZ=600.0, savefreqAvg=900.0),
+    "jets96x64_thk_kt00_nonorms": dict(kind="jets", nx=96, ny=64, nlay=3, dx=20e3, nt=8, solver=1,
+                                       H_rest=(500.0, 1500.0, 2000.0), thicknessScheme=3, KT00_sigma=1.2,
+                                       A2=100.0, pressureTiming=0, savefreqAvg=900.0),
+    "per40x24_thk_up3_a4smag": dict(kind="periodic_box", nx=40, ny=24, nt=8, solver=0, thicknessScheme=2,
+                                    A4=1e9, A4smag=2.0, savefreqAvg=900.0),
+    # periodic-x / wall-y with Smagorinsky: the strain averages read corner
+    # ghosts, one load behind (with and without the norms and the averages)
+    "chan64x32_thk_kt00_smag": dict(kind="channel", nx=64, ny=32, nt=8, dx=20e3, solver=1, thicknessScheme=3,
+                                    A2smag=3.0, savefreqEZ=400.0),
+    "chan64x32_up3_smag_nonorms": dict(kind="channel", nx=64, ny=32, nt=8, dx=20e3, solver=1, thicknessScheme=2,
+                                       A2smag=3.0, A4smag=2.0, pressureTiming=0),
+    "chan48x24_smag_avg_nonorms": dict(kind="channel", nx=48, ny=24, nt=8, dx=20e3, solver=0, SOR_rp=1.8,
+                                       A2=20.0, A2smag=2.0, pressureTiming=0, savefreqAvg=600.0),
 }
 
 

@@ -7,10 +7,12 @@ thickness, PV coefficients) and the per-solve iteration counts and norms.
 The oracle must reproduce every array bit for bit and every iteration count
 exactly.  The cases cover SOR / gathered MG / full MG / general (non 2:1)
 transfers, walls / periodic / mixed boundaries, odd grids, A2 and A4
-viscosity, linear and quadratic drag, h0, AB1/AB2/AB3 and all four
-momentum schemes; two cases also pin the energy / enstrophy diagnostics
-(EZdiags.dat, diagnostics.c:483-703) and two the online averages
-(U_avg ... P_avg, diagnostics.c:307-470).
+viscosity with constant and Smagorinsky coefficients, linear and quadratic
+drag, h0, AB1/AB2/AB3, all four momentum schemes and all four thickness
+schemes (AL81, HK83, UP3, KT00); several cases also pin the energy /
+enstrophy diagnostics (EZdiags.dat, diagnostics.c:483-703, including the
+HK83 kinetic energy) and the online averages (U_avg ... P_avg,
+diagnostics.c:307-470, whose face thickness follows the scheme).
 """
 import os
 

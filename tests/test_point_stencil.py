@@ -1,0 +1,63 @@
+"""The device stencil math (pawsim_amd/csrc/pg_point.h), compiled for the host.
+
+tests/cpu_point/point_vs_oracle.cpp runs the oracle for three steps on a
+golden input, then evaluates pg_tendency_point at every owned point of every
+layer and the no-ghost face thickness (pressure RHS, divergence, averages)
+at every owned face, and compares both with the oracle bit for bit.  The
+oracle itself is pinned to the reference by tests/test_oracle_golden.py, so
+this covers, without a GPU, the exact functions the HIP kernels evaluate:
+every momentum and thickness scheme, constant and Smagorinsky viscosity,
+drag, wind, and every boundary combination the golden inputs hold.
+"""
+import os
+import subprocess
+
+import pytest
+
+from conftest import REPO, golden_ids, load_golden
+
+SRC = os.path.join(REPO, "tests", "cpu_point", "point_vs_oracle.cpp")
+EXE = os.path.join(REPO, "tests", "cpu_point", "point_vs_oracle")
+HEADERS = [os.path.join(REPO, "pawsim_amd", "csrc", "pg_point.h"), os.path.join(REPO, "oracle", "oracle.h")]
+
+
+@pytest.fixture(scope="module")
+def point_exe(oracle_exe):
+    lib = os.path.join(REPO, "oracle", "_build", "liboracle.so")
+    if not os.path.exists(lib):
+        subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True)
+    deps = [SRC, lib, *HEADERS]
+    if not os.path.exists(EXE) or os.path.getmtime(EXE) < max(os.path.getmtime(p) for p in deps):
+        subprocess.run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-I" + os.path.join(REPO, "oracle"),
+                        "-I" + os.path.join(REPO, "pawsim_amd", "csrc"), SRC,
+                        "-L" + os.path.join(REPO, "oracle", "_build"), "-loracle",
+                        "-Wl,-rpath,$ORIGIN/../../oracle/_build", "-lm", "-o", EXE], check=True)
+    return EXE
+
+
+@pytest.mark.parametrize("name", golden_ids())
+def test_point_stencil_matches_oracle(name, point_exe, tmp_path):
+    _, setup = load_golden(name)
+    infile = setup.write(str(tmp_path / "in"))
+    r = subprocess.run([point_exe, infile, "3"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "mismatches=0" in r.stdout
+
+
+def _trajectory_cases():
+    from test_gpu_parity import TRAJECTORY_CASES
+    return TRAJECTORY_CASES
+
+
+@pytest.mark.parametrize("name", sorted(_trajectory_cases()))
+def test_point_stencil_matches_oracle_trajectory_inputs(name, point_exe, tmp_path):
+    """The larger inputs the GPU trajectory tests run (tile-sized grids, the
+    periodic-x / wall-y corners with and without the norms)."""
+    from pawsim_amd import inputs
+    spec = dict(_trajectory_cases()[name])
+    kind = spec.pop("kind")
+    spec.setdefault("pressureTiming", 1)
+    setup = getattr(inputs, kind)(name=name, **spec)
+    infile = setup.write(str(tmp_path / "in"))
+    r = subprocess.run([point_exe, infile, "2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
