@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--pi-guess", type=int, default=2,
                    help="initial guess of each solve: 0 previous pi, 1/2 linear/quadratic extrapolation")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--variant-steps", type=int, default=40,
+                   help="timed steps of each thickness-scheme / Smagorinsky variant (0: skip)")
     p.add_argument("--cpu-steps", type=int, default=12)
     p.add_argument("--pmc-probe", action="store_true",
                    help="only launch the dominant kernel and a device copy of known size (for rocprofv3 --pmc)")
@@ -75,14 +77,14 @@ def parse():
     return p.parse_args()
 
 
-def make_model(a, mixed=1, solver=None, tol=None):
+def make_model(a, mixed=1, solver=None, tol=None, **extra):
     import pawsim_amd
     from pawsim_amd import inputs
 
     setup = inputs.jets(nx=a.nx, ny=a.ny, nlay=a.nlay, nt=a.steps + a.warmup,
                         solver=a.solver if solver is None else solver, tol=a.tol if tol is None else tol,
                         A2=10.0, maxiters=200, mixed_precision=mixed, mg_nu1=a.mg_nu1, mg_nu2=a.mg_nu2,
-                        pi_guess=a.pi_guess)
+                        pi_guess=a.pi_guess, **extra)
     m = pawsim_amd.Model.from_setup(setup)
     if a.mg_tail:
         m.set_mg_tail(a.mg_tail)
@@ -157,6 +159,36 @@ def run_steps(a, m, stream):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     return {"n": n, "elapsed": elapsed, "solve_ms": float(np.mean(solve_ms)), "iters": float(np.mean(iters))}
+
+
+# Other thickness schemes and Smagorinsky viscosity on the same workload
+# (FP64 throughout), against AL81 on the tiled and on the per-point kernel.
+VARIANTS = {
+    "al81_tiled": {},
+    "al81_per_point": {"tendency_kernel": 1},
+    "thk_hk83": {"thicknessScheme": 1},
+    "thk_up3": {"thicknessScheme": 2},
+    "thk_kt00": {"thicknessScheme": 3},
+    "a2smag": {"A2smag": 2.0},
+}
+
+
+def scheme_variants(a, stream):
+    import argparse
+
+    import pawsim_amd
+
+    b = argparse.Namespace(**vars(a))
+    b.steps, b.warmup = a.variant_steps, 3
+    out = {}
+    for name, extra in VARIANTS.items():
+        _, m = make_model(b, mixed=0, solver=pawsim_amd.SOLVER_GPU_MG, **extra)
+        r = run_steps(b, m, stream)
+        m.close()
+        out[name] = {"steps_per_s": b.steps / r["elapsed"], "ms_per_step": 1e3 * r["elapsed"] / b.steps}
+        log(f"variant {name}: {out[name]['steps_per_s']:.1f} steps/s")
+    return {"workload": f"zonal jets {a.nx}x{a.ny}x{a.nlay}, FP64 stencils, FP64 GPU MG to rel. residual {a.tol:g}",
+            "steps": b.steps, "variants": out}
 
 
 def learned_vs_plain(a, stream):
@@ -386,6 +418,7 @@ def main():
                                    "frac": b / tt / HBM_PEAK} for k, (b, tt) in kernels.items()}}
     m.close()
 
+    variants = scheme_variants(a, stream) if a.variant_steps > 0 else None
     learned = learned_vs_plain(a, stream) if a.learn_samples > 0 else None
     ensemble = ensemble_run(a) if a.ens_members > 0 else None
 
@@ -411,6 +444,7 @@ def main():
                            "same_rhs_from_zero": rhs_solve, "learned": learned},
         "pcie_inclusive_steps_per_s": pcie_rate,
         "ensemble": ensemble,
+        "scheme_variants": variants,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -165,62 +165,75 @@ PG_HD double pg_minmod(double a, double b, double c)
 // mapped position, where the stencil reads the ghost-filled h (and u or v
 // for the upwind direction).  The second differences and slopes of UP3 / KT00 are
 // evaluated at their own positions from h directly.
-PG_HD double pg_hW(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+// The face-thickness stencils themselves, on any accessor H(i, j) of the
+// ghost-filled h and the face velocity uf (vf).  cmin: the lowest index of
+// the UP3 second difference / KT00 slope along the face normal that lies in
+// its computed range (below it the work array holds zero).
+template <class HA>
+PG_HD double pg_face_w(const PgPhys& P, const HA& H, double uf, int i, int j)
 {
-  pg_fmap(G, i, j);
   switch (P.thk) {
     case PG_THK_HK83:
-      return (1.0 / 3) * (PG_H(k, i, j) + PG_H(k, i - 1, j)
-                          + 0.25 * (PG_H(k, i, j + 1) + PG_H(k, i - 1, j + 1) + PG_H(k, i, j - 1) + PG_H(k, i - 1, j - 1)));
+      return (1.0 / 3) * (H(i, j) + H(i - 1, j) + 0.25 * (H(i, j + 1) + H(i - 1, j + 1) + H(i, j - 1) + H(i - 1, j - 1)));
     case PG_THK_UP3: {
-      const int c = (PG_U(k, i, j) > 0) ? i - 1 : i;
-      const double d2 = PG_H(k, c + 1, j) - 2 * PG_H(k, c, j) + PG_H(k, c - 1, j);
-      const double w = 0.5 * (PG_H(k, i, j) + PG_H(k, i - 1, j));
+      const int c = (uf > 0) ? i - 1 : i;
+      const double d2 = H(c + 1, j) - 2 * H(c, j) + H(c - 1, j);
+      const double w = 0.5 * (H(i, j) + H(i - 1, j));
       return w - (1 / 6.0) * d2;
     }
     case PG_THK_KT00: {
       const double s = P.kt00;
-      if (PG_U(k, i, j) > 0) {
+      if (uf > 0) {
         const int c = i - 1;
-        const double d = pg_minmod(s * (PG_H(k, c + 1, j) - PG_H(k, c, j)), 0.5 * (PG_H(k, c + 1, j) - PG_H(k, c - 1, j)),
-                                   s * (PG_H(k, c, j) - PG_H(k, c - 1, j)));
-        return PG_H(k, c, j) + 0.5 * d;
+        const double d = pg_minmod(s * (H(c + 1, j) - H(c, j)), 0.5 * (H(c + 1, j) - H(c - 1, j)), s * (H(c, j) - H(c - 1, j)));
+        return H(c, j) + 0.5 * d;
       }
-      const double d = pg_minmod(s * (PG_H(k, i + 1, j) - PG_H(k, i, j)), 0.5 * (PG_H(k, i + 1, j) - PG_H(k, i - 1, j)),
-                                 s * (PG_H(k, i, j) - PG_H(k, i - 1, j)));
-      return PG_H(k, i, j) - 0.5 * d;
+      const double d = pg_minmod(s * (H(i + 1, j) - H(i, j)), 0.5 * (H(i + 1, j) - H(i - 1, j)), s * (H(i, j) - H(i - 1, j)));
+      return H(i, j) - 0.5 * d;
     }
     default:
-      return 0.5 * (PG_H(k, i, j) + PG_H(k, i - 1, j));
+      return 0.5 * (H(i, j) + H(i - 1, j));
   }
 }
-PG_HD double pg_hS(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+template <class HA>
+PG_HD double pg_face_s(const PgPhys& P, const HA& H, double vf, int i, int j, int cmin)
 {
-  pg_fmap(G, i, j);
   switch (P.thk) {
     case PG_THK_HK83:
-      return (1.0 / 3) * (PG_H(k, i, j) + PG_H(k, i, j - 1)
-                          + 0.25 * (PG_H(k, i + 1, j) + PG_H(k, i - 1, j) + PG_H(k, i + 1, j - 1) + PG_H(k, i - 1, j - 1)));
+      return (1.0 / 3) * (H(i, j) + H(i, j - 1) + 0.25 * (H(i + 1, j) + H(i - 1, j) + H(i + 1, j - 1) + H(i - 1, j - 1)));
     case PG_THK_UP3: {
-      // row 0 of the second difference is outside its computed range and
-      // stays zero (a corner's row 1 would read it)
-      const int c = (PG_V(k, i, j) > 0) ? j - 1 : j;
-      const double d2 = (c >= 1) ? PG_H(k, i, c + 1) - 2 * PG_H(k, i, c) + PG_H(k, i, c - 1) : 0.0;
-      const double s = 0.5 * (PG_H(k, i, j) + PG_H(k, i, j - 1));
+      const int c = (vf > 0) ? j - 1 : j;
+      const double d2 = (c >= cmin) ? H(i, c + 1) - 2 * H(i, c) + H(i, c - 1) : 0.0;
+      const double s = 0.5 * (H(i, j) + H(i, j - 1));
       return s - (1 / 6.0) * d2;
     }
     case PG_THK_KT00: {
       // both states add the half slope (work.c:673-677)
       const double s = P.kt00;
-      const int c = (PG_V(k, i, j) > 0) ? j - 1 : j;
-      const double d = (c >= 1) ? pg_minmod(s * (PG_H(k, i, c + 1) - PG_H(k, i, c)),
-                                            0.5 * (PG_H(k, i, c + 1) - PG_H(k, i, c - 1)), s * (PG_H(k, i, c) - PG_H(k, i, c - 1)))
-                                : 0.0;
-      return PG_H(k, i, c) + 0.5 * d;
+      const int c = (vf > 0) ? j - 1 : j;
+      const double d = (c >= cmin) ? pg_minmod(s * (H(i, c + 1) - H(i, c)), 0.5 * (H(i, c + 1) - H(i, c - 1)),
+                                               s * (H(i, c) - H(i, c - 1)))
+                                   : 0.0;
+      return H(i, c) + 0.5 * d;
     }
     default:
-      return 0.5 * (PG_H(k, i, j) + PG_H(k, i, j - 1));
+      return 0.5 * (H(i, j) + H(i, j - 1));
   }
+}
+
+PG_HD double pg_hW(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  pg_fmap(G, i, j);
+  auto H = [&](int a, int b) { return PG_H(k, a, b); };
+  return pg_face_w(P, H, PG_U(k, i, j), i, j);
+}
+PG_HD double pg_hS(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  pg_fmap(G, i, j);
+  auto H = [&](int a, int b) { return PG_H(k, a, b); };
+  // row 0 of the second difference / slope is outside its computed range
+  // (a corner's row 1 would read it)
+  return pg_face_s(P, H, PG_V(k, i, j), i, j, 1);
 }
 
 // h_west / h_south of the pressure RHS, the corrected divergence and the
@@ -404,29 +417,41 @@ PG_HD double pg_DDS(const PgGrid& G, const PgIn& I, int k, int i, int j)
 // core range only: constants (work.c:1614-1631), or with Smagorinsky
 // (work.c:1587-1610) the larger of the constant and the factor times the
 // strain magnitude, DD_T averaged to q points or DD_S averaged to h points.
+// Strain magnitude at q points from DD_T at (i,j), (i-1,j), (i,j-1),
+// (i-1,j-1) and DD_S at (i,j); at h points from DD_T at (i,j) and DD_S at
+// (i,j), (i+1,j), (i,j+1), (i+1,j+1) (work.c:1597-1602).
+PG_HD double pg_ddq(double t00, double tm0, double t0m, double tmm, double s)
+{
+  const double t = 0.25 * (t00 + tm0 + t0m + tmm);
+  return sqrt(t * t + s * s);
+}
+PG_HD double pg_ddh(double t, double s00, double s10, double s01, double s11)
+{
+  const double s = 0.25 * (s00 + s10 + s01 + s11);
+  return sqrt(t * t + s * s);
+}
+// Laplacian coefficient for a strain magnitude (work.c:1603-1604, 1619-1620).
+PG_HD double pg_visc2(const PgPhys& P, double dd) { return P.useSmag ? fmax(P.A2, P.A2fac * dd) : P.A2; }
+
 PG_HD double pg_DDq(const PgGrid& G, const PgIn& I, int k, int i, int j)
 {
-  const double t = 0.25 * (pg_DDT(G, I, k, i, j) + pg_DDT(G, I, k, i - 1, j) + pg_DDT(G, I, k, i, j - 1)
-                           + pg_DDT(G, I, k, i - 1, j - 1));
-  const double s = pg_DDS(G, I, k, i, j);
-  return sqrt(t * t + s * s);
+  return pg_ddq(pg_DDT(G, I, k, i, j), pg_DDT(G, I, k, i - 1, j), pg_DDT(G, I, k, i, j - 1),
+                pg_DDT(G, I, k, i - 1, j - 1), pg_DDS(G, I, k, i, j));
 }
 PG_HD double pg_DDh(const PgGrid& G, const PgIn& I, int k, int i, int j)
 {
-  const double t = pg_DDT(G, I, k, i, j);
-  const double s = 0.25 * (pg_DDS(G, I, k, i, j) + pg_DDS(G, I, k, i + 1, j) + pg_DDS(G, I, k, i, j + 1)
-                           + pg_DDS(G, I, k, i + 1, j + 1));
-  return sqrt(t * t + s * s);
+  return pg_ddh(pg_DDT(G, I, k, i, j), pg_DDS(G, I, k, i, j), pg_DDS(G, I, k, i + 1, j), pg_DDS(G, I, k, i, j + 1),
+                pg_DDS(G, I, k, i + 1, j + 1));
 }
 PG_HD double pg_A2h(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
 {
   if (!pg_in_core(G, i, j)) return 0.0;
-  return P.useSmag ? fmax(P.A2, P.A2fac * pg_DDh(G, I, k, i, j)) : P.A2;
+  return pg_visc2(P, P.useSmag ? pg_DDh(G, I, k, i, j) : 0.0);
 }
 PG_HD double pg_A2q(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
 {
   if (!pg_in_core(G, i, j)) return 0.0;
-  return P.useSmag ? fmax(P.A2, P.A2fac * pg_DDq(G, I, k, i, j)) : P.A2;
+  return pg_visc2(P, P.useSmag ? pg_DDq(G, I, k, i, j) : 0.0);
 }
 PG_HD double pg_A4h(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
 {

@@ -26,17 +26,25 @@ constexpr int UH = TI + 2, UW = TJ + 1;   // h*u: origin (i0-1, j0-1)
 constexpr int VH = TI + 1, VW = TJ + 2;   // h*v: origin (i0-1, j0-1)
 constexpr int KH = TI + 1, KW = TJ + 1;   // KE, MM, DD_T, |u_bot|: origin (i0-1, j0-1)
 constexpr int DH = TI + 1, DW = TJ + 1;   // DD_S, h_q: origin (i0, j0)
+// Extended variant (other thickness schemes, Smagorinsky): h staged from
+// (i0-3, j0-3) for the UP3 / KT00 stencils, DD_T and DD_S both on
+// (-1..TI) x (-1..TJ) for the strain averages.
+constexpr int SH3 = TI + 5, SW3 = TJ + 5;
+constexpr int XH = TI + 2, XW = TJ + 2;
 
 __device__ __forceinline__ bool is_inner(const PgGrid& G, int i, int j)
 {
   return i - 2 >= G.g && i + 3 < G.g + G.nx && j - 2 >= G.g && j + 3 < G.g + G.ny;
 }
 
+template <bool EXT>
 __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn I, double* du, double* dv, double* dh)
 {
-  __shared__ double su[SH * SW], sv[SH * SW], sh[SH * SW];
+  __shared__ double su[SH * SW], sv[SH * SW], sh[EXT ? SH3 * SW3 : SH * SW];
   __shared__ double sq[QH * QW], suu[UH * UW], svv[VH * VW], ske[KH * KW];
-  __shared__ double sdt[KH * KW], sds[DH * DW], shq[DH * DW], sub[KH * KW];
+  __shared__ double sdt[EXT ? XH * XW : KH * KW], sds[EXT ? XH * XW : DH * DW], shq[DH * DW], sub[KH * KW];
+  // h accessor relative to the tile origin
+  constexpr int HO = EXT ? 3 : 2, HW = EXT ? SW3 : SW;
   extern __shared__ double smm[];  // [L][KH * KW]
   const int tid = threadIdx.x;
   const int i0 = G.g + blockIdx.y * TI, j0 = G.g + blockIdx.x * TJ;
@@ -76,13 +84,19 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
         const size_t p = lay + pg_at(G, i, j);
         su[idx] = I.u[p];
         sv[idx] = I.v[p];
-        sh[idx] = I.h[p];
+        if (!EXT) sh[idx] = I.h[p];
       }
     }
+    if (EXT)
+      for (int idx = tid; idx < SH3 * SW3; idx += NTH) {
+        const int i = i0 - 3 + idx / SW3, j = j0 - 3 + idx % SW3;
+        if (i < G.sx && j < G.sy) sh[idx] = I.h[lay + pg_at(G, i, j)];
+      }
     __syncthreads();
 #define U(a, b) su[((a) + 2) * SW + (b) + 2]
 #define V(a, b) sv[((a) + 2) * SW + (b) + 2]
-#define H(a, b) sh[((a) + 2) * SW + (b) + 2]
+#define H(a, b) sh[((a) + HO) * HW + (b) + HO]
+    auto HL = [&](int a, int b) { return H(a, b); };
     // q (work.c:1258-1270) on (-1..TI+1) x (-1..TJ+1)
     for (int idx = tid; idx < QH * QW; idx += NTH) {
       const int a = idx / QW - 1, b = idx % QW - 1;
@@ -95,11 +109,11 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
     // h*u (work.c:176-182) on (-1..TI) x (-1..TJ-1), h*v on (-1..TI-1) x (-1..TJ)
     for (int idx = tid; idx < UH * UW; idx += NTH) {
       const int a = idx / UW - 1, b = idx % UW - 1;
-      suu[idx] = U(a, b) * (0.5 * (H(a, b) + H(a - 1, b)));
+      suu[idx] = EXT ? U(a, b) * pg_face_w(P, HL, U(a, b), a, b) : U(a, b) * (0.5 * (H(a, b) + H(a - 1, b)));
     }
     for (int idx = tid; idx < VH * VW; idx += NTH) {
       const int a = idx / VW - 1, b = idx % VW - 1;
-      svv[idx] = V(a, b) * (0.5 * (H(a, b) + H(a, b - 1)));
+      svv[idx] = EXT ? V(a, b) * pg_face_s(P, HL, V(a, b), a, b, -(1 << 30)) : V(a, b) * (0.5 * (H(a, b) + H(a, b - 1)));
     }
     // KE (work.c:1446-1497) and DD_T (work.c:1584-1587) on (-1..TI-1) x (-1..TJ-1)
 #define SQ(x) ((x) * (x))
@@ -118,25 +132,33 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
                    / 6;
       else
         ske[idx] = (SQ(U(a + 1, b)) + SQ(U(a, b))) / 4 + (SQ(V(a, b + 1)) + SQ(V(a, b))) / 4;
-      if (P.useA2) sdt[idx] = (U(a + 1, b) - U(a, b)) / dx - (V(a, b + 1) - V(a, b)) / dy;
+      if (!EXT && P.useA2) sdt[idx] = (U(a + 1, b) - U(a, b)) / dx - (V(a, b + 1) - V(a, b)) / dy;
     }
 #undef SQ
-    // DD_S (work.c:1588-1589) and h_q on (0..TI) x (0..TJ)
-    if (P.useA2)
+    // DD_S (work.c:1588-1589) and h_q on (0..TI) x (0..TJ); the extended
+    // variant has DD_T and DD_S on (-1..TI) x (-1..TJ)
+    if (P.useA2) {
       for (int idx = tid; idx < DH * DW; idx += NTH) {
         const int a = idx / DW, b = idx % DW;
-        sds[idx] = (U(a, b) - U(a, b - 1)) / dy + (V(a, b) - V(a - 1, b)) / dx;
+        if (!EXT) sds[idx] = (U(a, b) - U(a, b - 1)) / dy + (V(a, b) - V(a - 1, b)) / dx;
         shq[idx] = 0.25 * (H(a, b) + H(a - 1, b - 1) + H(a - 1, b) + H(a, b - 1));
       }
+      if (EXT)
+        for (int idx = tid; idx < XH * XW; idx += NTH) {
+          const int a = idx / XW - 1, b = idx % XW - 1;
+          sdt[idx] = (U(a + 1, b) - U(a, b)) / dx - (V(a, b + 1) - V(a, b)) / dy;
+          sds[idx] = (U(a, b) - U(a, b - 1)) / dy + (V(a, b) - V(a - 1, b)) / dx;
+        }
+    }
     __syncthreads();
 #define Q(a, b) sq[((li) + (a) + 1) * QW + (lj) + (b) + 1]
 #define HU(a, b) suu[((li) + (a) + 1) * UW + (lj) + (b) + 1]
 #define HV(a, b) svv[((li) + (a) + 1) * VW + (lj) + (b) + 1]
 #define KE(a, b) ske[((li) + (a) + 1) * KW + (lj) + (b) + 1]
 #define MM(a, b) smm[k * (KH * KW) + ((li) + (a) + 1) * KW + (lj) + (b) + 1]
-#define DT(a, b) sdt[((li) + (a) + 1) * KW + (lj) + (b) + 1]
+#define DT(a, b) sdt[((li) + (a) + 1) * (EXT ? XW : KW) + (lj) + (b) + 1]
 #define UB(a, b) sub[((li) + (a) + 1) * KW + (lj) + (b) + 1]
-#define DS(a, b) sds[((li) + (a)) * DW + (lj) + (b)]
+#define DS(a, b) (EXT ? sds[((li) + (a) + 1) * XW + (lj) + (b) + 1] : sds[((li) + (a)) * DW + (lj) + (b)])
 #define HQ(a, b) shq[((li) + (a)) * DW + (lj) + (b)]
     for (int idx = tid; idx < TI * TJ; idx += NTH) {
       const int li = idx / TJ, lj = idx % TJ;
@@ -172,8 +194,16 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
         p0m = (-Q(1, 0) + Q(0, 0) + Q(0, -1) - Q(1, -1)) / 24;
         p00 = (-Q(1, 1) + Q(0, 1) + Q(0, 0) - Q(1, 0)) / 24;
       }
-      const double hwest = 0.5 * (H(li, lj) + H(li - 1, lj));
-      const double hsouth = 0.5 * (H(li, lj) + H(li, lj - 1));
+      const double hwest = EXT ? pg_face_w(P, HL, U(li, lj), li, lj) : 0.5 * (H(li, lj) + H(li - 1, lj));
+      const double hsouth = EXT ? pg_face_s(P, HL, V(li, lj), li, lj, -(1 << 30)) : 0.5 * (H(li, lj) + H(li, lj - 1));
+      // Laplacian coefficients A2_h, A2_q at offsets (a, b) from the point
+      // (work.c:1597-1620)
+      auto A2h = [&](int a, int b) {
+        return pg_visc2(P, P.useSmag ? pg_ddh(DT(a, b), DS(a, b), DS(a + 1, b), DS(a, b + 1), DS(a + 1, b + 1)) : 0.0);
+      };
+      auto A2q = [&](int a, int b) {
+        return pg_visc2(P, P.useSmag ? pg_ddq(DT(a, b), DT(a - 1, b), DT(a, b - 1), DT(a - 1, b - 1), DS(a, b)) : 0.0);
+      };
       const double fs = (k == 0) ? 1.0 : 0.0, fb = (k == G.L - 1) ? 1.0 : 0.0;
       const size_t q = pg_at(G, i, j);
       double ru, rv;
@@ -190,9 +220,12 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
           qf += 0.25 * Q(0, 0) * (HV(0, 0) + HV(-1, 0)) + 0.25 * Q(0, 1) * (HV(0, 1) + HV(-1, 1));
         double a2 = 0;
         const double a4 = 0, buoy = 0;
-        if (P.useA2)
+        if (P.useA2 && !EXT)
           a2 = ((P.A2 * H(li, lj) * DT(0, 0) - P.A2 * H(li - 1, lj) * DT(-1, 0)) / dx
                 + (P.A2 * HQ(0, 1) * DS(0, 1) - P.A2 * HQ(0, 0) * DS(0, 0)) / dy);
+        else if (P.useA2)
+          a2 = ((A2h(0, 0) * H(li, lj) * DT(0, 0) - A2h(-1, 0) * H(li - 1, lj) * DT(-1, 0)) / dx
+                + (A2q(0, 1) * HQ(0, 1) * DS(0, 1) - A2q(0, 0) * HQ(0, 0) * DS(0, 0)) / dy);
         ru = gM + gK + qf + ((hwest != 0) ? (a2 + a4 + buoy) / hwest : 0);
         if (P.useWind) {
           const double tx = 1.0 * I.taux[q] + 0.0 * I.taux[q];
@@ -221,9 +254,12 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
           qf -= 0.25 * Q(0, 0) * (HU(0, 0) + HU(0, -1)) + 0.25 * Q(1, 0) * (HU(1, 0) + HU(1, -1));
         double a2 = 0;
         const double a4 = 0, buoy = 0;
-        if (P.useA2)
+        if (P.useA2 && !EXT)
           a2 = ((P.A2 * HQ(1, 0) * DS(1, 0) - P.A2 * HQ(0, 0) * DS(0, 0)) / dx
                 - (P.A2 * H(li, lj) * DT(0, 0) - P.A2 * H(li, lj - 1) * DT(0, -1)) / dy);
+        else if (P.useA2)
+          a2 = ((A2q(1, 0) * HQ(1, 0) * DS(1, 0) - A2q(0, 0) * HQ(0, 0) * DS(0, 0)) / dx
+                - (A2h(0, 0) * H(li, lj) * DT(0, 0) - A2h(0, -1) * H(li, lj - 1) * DT(0, -1)) / dy);
         rv = gM + gK + qf + ((hsouth != 0) ? (a2 + a4 + buoy) / hsouth : 0);
         if (P.useWind) {
           const double ty = 1.0 * I.tauy[q] + 0.0 * I.tauy[q];
@@ -517,12 +553,12 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, Pg
 
 }  // namespace
 
-// True if the fast pair of kernels serves this configuration (constant
-// Laplacian viscosity and centred face thickness; everything else goes
-// through the per-point kernel).
+// True if the fast pair of kernels serves this configuration (no
+// biharmonic viscosity; the other thickness schemes and Smagorinsky
+// coefficients take the extended interior variant).
 bool pg_tendency_fast_ok(const pg_handle* h)
 {
-  return !h->P.useA4 && !h->P.useSmag && h->P.thk == PG_THK_AL81 && h->G.nx >= 8 && h->G.ny >= 8 && h->G.L <= 8;
+  return !h->P.useA4 && h->G.nx >= 8 && h->G.ny >= 8 && h->G.L <= 8;
 }
 
 hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double* dh)
@@ -531,7 +567,10 @@ hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double
   const PgGrid& G = h->G;
   const size_t dyn = (size_t)G.L * KH * KW * sizeof(double);
   dim3 grid((G.ny + TJ - 1) / TJ, (G.nx + TI - 1) / TI);
-  hipLaunchKernelGGL(k_tendency_inner, grid, dim3(NTH), dyn, h->stream, G, h->P, I, du, dv, dh);
+  if (h->P.thk != PG_THK_AL81 || h->P.useSmag)
+    hipLaunchKernelGGL(k_tendency_inner<true>, grid, dim3(NTH), dyn, h->stream, G, h->P, I, du, dv, dh);
+  else
+    hipLaunchKernelGGL(k_tendency_inner<false>, grid, dim3(NTH), dyn, h->stream, G, h->P, I, du, dv, dh);
   return hipGetLastError();
 }
 

@@ -65,8 +65,15 @@ TRAJECTORY_CASES = {
     # per-point kernel everywhere
     "jets96x64_pointkernel": dict(kind="jets", nx=96, ny=64, nlay=3, dx=20e3, nt=6, solver=1, A2=200.0,
                                   H_rest=(500.0, 1500.0, 2000.0), quadDragCoeff=2e-3, tendency_kernel=1),
-    # thickness schemes and Smagorinsky viscosity (per-point kernel, scheme-aware
-    # RHS / divergence / averages face thickness, HK83 kinetic energy)
+    # thickness schemes and Smagorinsky viscosity (extended tiled interior
+    # kernel, or the per-point kernel with A4; scheme-aware RHS / divergence /
+    # averages face thickness, HK83 kinetic energy)
+    "jets96x64_up3_smag_pointkernel": dict(kind="jets", nx=96, ny=64, nlay=3, dx=20e3, nt=6, solver=1,
+                                           H_rest=(500.0, 1500.0, 2000.0), thicknessScheme=2, A2=50.0,
+                                           A2smag=2.0, tendency_kernel=1),
+    "jets136x72_kt00_smag_tiles": dict(kind="jets", nx=136, ny=72, nlay=3, dx=20e3, nt=6, solver=1,
+                                       H_rest=(500.0, 1500.0, 2000.0), thicknessScheme=3, A2smag=2.5,
+                                       quadDragCoeff=2e-3, momentumScheme=3),
     "gyre64_thk_hk83": dict(kind="gyre", nx=64, ny=64, nt=8, solver=1, momentumScheme=1, thicknessScheme=1,
                             savefreqEZ=600.0, savefreqAvg=900.0),
     "jets96x64_thk_kt00_nonorms": dict(kind="jets", nx=96, ny=64, nlay=3, dx=20e3, nt=8, solver=1,
