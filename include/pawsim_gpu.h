@@ -22,6 +22,7 @@
 #ifndef PAWSIM_GPU_H
 #define PAWSIM_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -157,6 +158,20 @@ int pg_energy(pg_handle *h, double *KE, double *PE, double *Z);
  * U V H M W HU HV HUU HVV HUV per layer, W of layer Nlay, P, and resets them. */
 int pg_average_add(pg_handle *h, double weight);
 int pg_average_take(pg_handle *h, double scale, double *out);
+
+/* Full-fidelity checkpoint.  Replaces, as an option, the reference's
+ * restart (pawsim_state_read_restart, state.c:1106-1155, which reads only
+ * U/V/H and restarts AB at first order with pi = 0): the blob holds u, v, h
+ * and pi with their ghost cells, the tendency history in order, the
+ * multigrid cycle prediction, the extrapolated-guess history, the halo
+ * flags, the online-average sums and the model time, so that loading it
+ * into a handle built from the same configuration and stepping on (with the
+ * global step index in pg_step) reproduces the uninterrupted run bit for
+ * bit.  pg_checkpoint_bytes gives the size of the blob as the handle stands;
+ * load rejects a blob of another grid, layer count or precision mode. */
+size_t pg_checkpoint_bytes(pg_handle *h);
+int pg_checkpoint_save(pg_handle *h, void *buf, size_t bytes);
+int pg_checkpoint_load(pg_handle *h, const void *buf, size_t bytes);
 
 /* Model time (ctx->t) and stream control. */
 double pg_time(const pg_handle *h);

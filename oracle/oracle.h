@@ -36,12 +36,14 @@ enum { OR_SOLVER_SOR = 0, OR_SOLVER_MG = 1 };
 /* Parsed parameter file (config.c:42-122 defaults, 172-282 keys). */
 typedef struct or_config {
   uint Nlay, Nx, Ny, Nt, tsScheme, momScheme, thkScheme, maxiters, solver, tauNrecs;
+  uint startIdx;              /* save index a restart reads (config.c:188) */
   real Lx, Ly, dt, tmin, tmax, savefrequency, h0, hsml, hbbl, A2, A4, A2smag, A4smag;
   real linDrag, quadDrag, linDragSurf, quadDragSurf, pi_tol, SOR_rp, omega_MG;
   real savefreqEZ;            /* energy/enstrophy diagnostics interval (config.c:206) */
   real savefreqAvg;           /* online-average window (config.c:200) */
   real KT00_sigma;            /* KT00 limiter slope factor (config.c:73, 207) */
   bool wallEW, wallNS, useRL, useFullMG, pressureTiming, writeInit, useWind;
+  bool restart;               /* read U/V/H from the output directory (config.c:244) */
   bool unsupported;           /* some active feature outside the restated path */
   char unsupported_what[128];
   char inputDir[512];
@@ -125,6 +127,9 @@ void or_fill(or_field *f, const or_model *m, int kind);
 void or_fill_internal(or_field *f, const or_model *m);
 
 bool or_model_init(or_model *m, const char *infile, FILE *err);
+/* Restart: overwrite u, v, h with the outputs U{k}_n=startIdx.dat ... in
+ * outdir and refresh their halos (state.c:1106-1155). */
+bool or_model_restart(or_model *m, const char *outdir, FILE *err);
 void or_model_free(or_model *m);
 void or_prepare_work(or_model *m);
 void or_forcing_update(or_model *m);

@@ -104,7 +104,7 @@ bool or_write_EZ_if_due(or_model *m, const char *outdir)
     if (!Z) return false;
     or_energy(m, &KE, &PE, Z);
     snprintf(path, sizeof(path), "%s/EZdiags.dat", outdir);
-    f = fopen(path, (m->n_EZ > 1) ? "a" : "w");
+    f = fopen(path, (m->cfg.restart || m->n_EZ > 1) ? "a" : "w");
     if (!f) {
       free(Z);
       return false;

@@ -90,12 +90,13 @@ bool or_config_read(const char *fname, or_config *c, FILE *err)
     R("tol", c->pi_tol); R("SOR_rp", c->SOR_rp); R("SOR_rp_min", c->SOR_rp); R("omega_MG", c->omega_MG);
     B("useWallEW", c->wallEW); B("useWallNS", c->wallNS); B("useRL", c->useRL);
     B("use_fullMG", c->useFullMG); B("pressureTiming", c->pressureTiming);
-    B("writeInitDiagnostics", c->writeInit); B("use_MG", useMG);
+    B("writeInitDiagnostics", c->writeInit); B("use_MG", useMG); B("restart", c->restart);
+    U("startIdx", c->startIdx);
     S("hInitFile", c->hInit); S("uInitFile", c->uInit); S("vInitFile", c->vInit);
     S("hbFile", c->hbFile); S("hsFile", c->hsFile); S("OmegazFile", c->omegaFile);
     S("gFile", c->gFile); S("tauxFile", c->tauxFile); S("tauyFile", c->tauyFile);
     else if (!strcmp(key, "pressureSolver")) { c->solver = (uint)strtoul(val, NULL, 10); solverGiven = true; }
-    else if (!strcmp(key, "useTracer") || !strcmp(key, "useBuoyancy") || !strcmp(key, "restart")
+    else if (!strcmp(key, "useTracer") || !strcmp(key, "useBuoyancy")
              || !strcmp(key, "windFeedback") || !strcmp(key, "useRandomForcing") || !strcmp(key, "useTrad")) {
       if (atoi(val) != 0) flag_unsupported(c, key);
     }

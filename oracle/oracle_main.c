@@ -57,6 +57,11 @@ int main(int argc, char **argv)
     fprintf(stderr, "ERROR: Could not initialize oracle state\n");
     return 1;
   }
+  /* readParameterData: a restart overwrites u, v, h from the outputs (pawsim.c:251-254) */
+  if (m.cfg.restart && !or_model_restart(&m, argv[2], stderr)) {
+    fprintf(stderr, "ERROR: Could not read the restart state\n");
+    return 1;
+  }
   printf("PAWSIM parsed %u x %u x %u AWSIM input\n", m.cfg.Nx, m.cfg.Ny, m.cfg.Nlay);
   tall = now_s();
   /* timeSteppingPreliminaries (pawsim.c:266-304). */
@@ -71,8 +76,10 @@ int main(int argc, char **argv)
   or_forcing_update(&m);
   or_tendency_core(&m);
   if (m.cfg.writeInit && !or_write_init_diagnostics(&m, argv[2])) return 1;
-  if (!or_write_state(&m, argv[2], 0)) return 1;
-  m.n_saves = 1;
+  /* the first state written carries the restart index (pawsim.c:427, diagnostics.c:977-999) */
+  m.n_saves = m.cfg.restart ? m.cfg.startIdx : 0;
+  if (!or_write_state(&m, argv[2], m.n_saves)) return 1;
+  m.n_saves++;
   if (m.cfg.savefreqAvg > 0) {  /* pawsim.c:429-433 */
     m.n_avg = (uint)round(m.cfg.tmin / m.cfg.savefreqAvg) + 1;
     m.t_next_avg = m.cfg.tmin + m.cfg.savefreqAvg;

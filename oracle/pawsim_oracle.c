@@ -213,7 +213,8 @@ static bool set_time(or_model *m, FILE *err)
     if (err) fprintf(err, "Must specify exactly two of {Nt, tmax, dt}\n");
     return false;
   }
-  if (c->tmin < 0) c->tmin = 0;
+  /* before savefrequency takes its default (pawsim.c:157-160) */
+  if (c->tmin < 0) c->tmin = c->restart ? c->startIdx * c->savefrequency : 0;
   if (c->Nt == 0) {
     if (c->tmax <= c->tmin) return false;
     c->Nt = (uint)ceil((c->tmax - c->tmin) / c->dt);
@@ -323,6 +324,30 @@ bool or_model_init(or_model *m, const char *infile, FILE *err)
   for (k = 1; k < L; k++) m->geff[k] = m->geff[k - 1] + m->gg[k];
   for (k = 0; k < L; k++)
     if (m->gg[k] < 0) return false;
+  return true;
+}
+
+bool or_model_restart(or_model *m, const char *outdir, FILE *err)
+{
+  uint k, i, j, nx = m->cfg.Nx, ny = m->cfg.Ny;
+  static const char *names[3] = {"U", "V", "H"};
+  or_field *dst[3] = {m->u, m->v, m->h};
+  real *buf = malloc((size_t)nx * ny * sizeof(real));
+  char path[1024];
+  int f;
+  if (!buf) return false;
+  for (k = 0; k < m->L; k++)
+    for (f = 0; f < 3; f++) {
+      snprintf(path, sizeof(path), "%s/%s%u_n=%u.dat", outdir, names[f], k, m->cfg.startIdx);
+      if (!or_read_matrix(path, buf, nx, ny, 1, err)) {
+        free(buf);
+        return false;
+      }
+      for (i = 0; i < nx; i++)
+        for (j = 0; j < ny; j++) OF(&dst[f][k], i + m->g, j + m->g) = buf[(size_t)i * ny + j];
+    }
+  free(buf);
+  fill_prognostic(m, m->u, m->v, m->h);
   return true;
 }
 

@@ -28,7 +28,8 @@ PG_OK, PG_EARG, PG_EHIP, PG_ENOCONV, PG_ENODEV = 0, -1, -2, -3, -4
 ABI_SYMBOLS = (
     "pg_create", "pg_destroy", "pg_last_error", "pg_upload_state", "pg_download_state",
     "pg_upload_pi", "pg_download_pi", "pg_tendency", "pg_step", "pg_pressure_correct",
-    "pg_solve_pressure", "pg_refresh_halos", "pg_energy", "pg_average_add", "pg_average_take", "pg_time", "pg_set_stream", "pg_synchronize",
+    "pg_solve_pressure", "pg_refresh_halos", "pg_energy", "pg_average_add", "pg_average_take",
+    "pg_checkpoint_bytes", "pg_checkpoint_save", "pg_checkpoint_load", "pg_time", "pg_set_stream", "pg_synchronize",
     "pg_bench_kernel",
     "pg_net_param_count", "pg_net_create", "pg_net_destroy", "pg_net_set_params", "pg_net_get_params",
     "pg_net_forward", "pg_net_loss_grad", "pg_net_adam_step",
@@ -115,6 +116,10 @@ def load_library() -> ctypes.CDLL:
     lib.pg_energy.argtypes = [ctypes.c_void_p, d, d, d]
     lib.pg_average_add.argtypes = [ctypes.c_void_p, ctypes.c_double]
     lib.pg_average_take.argtypes = [ctypes.c_void_p, ctypes.c_double, d]
+    lib.pg_checkpoint_bytes.argtypes = [ctypes.c_void_p]
+    lib.pg_checkpoint_bytes.restype = ctypes.c_size_t
+    lib.pg_checkpoint_save.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    lib.pg_checkpoint_load.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     lib.pg_time.argtypes = [ctypes.c_void_p]
     lib.pg_time.restype = ctypes.c_double
     lib.pg_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -292,6 +297,19 @@ class Model:
         res["W_avg"] = np.concatenate([res["W_avg"], out[10 * L][None]], axis=0)
         res["P_avg"] = out[10 * L + 1].copy()
         return res
+
+    def checkpoint(self) -> bytes:
+        """Full-fidelity checkpoint blob (state with ghosts, AB history, pi,
+        solver and guess history, average sums, time)."""
+        n = self.lib.pg_checkpoint_bytes(self.h)
+        buf = ctypes.create_string_buffer(n)
+        self._check(self.lib.pg_checkpoint_save(self.h, buf, n), "pg_checkpoint_save")
+        return buf.raw
+
+    def restore(self, blob: bytes) -> None:
+        """Load a blob from checkpoint() of a model with the same configuration."""
+        buf = ctypes.create_string_buffer(blob, len(blob))
+        self._check(self.lib.pg_checkpoint_load(self.h, buf, len(blob)), "pg_checkpoint_load")
 
     def refresh_halos(self) -> None:
         self._check(self.lib.pg_refresh_halos(self.h), "pg_refresh_halos")

@@ -29,6 +29,10 @@ typedef struct drv_config {
   double savefreqEZ;
   double savefreqAvg;
   bool useRL, writeInit, useMG, solverGiven;
+  bool restart;             /* read U/V/H of save startIdx from the output directory (config.c:244) */
+  unsigned startIdx;
+  bool saveCheckpoint;      /* write checkpoint_n=N.bin with the state saves (no reference counterpart) */
+  bool restartCheckpoint;   /* restart from checkpoint_n=startIdx.bin instead of U/V/H */
   char unsupported[128];
   char dir[512];
   char hInit[256], uInit[256], vInit[256], hbFile[256], hsFile[256], omFile[256], gFile[256];
@@ -143,7 +147,11 @@ static bool read_config(const char *fname, drv_config *c)
     else if (!strcmp(key, "gFile")) copy_name(c->gFile, val);
     else if (!strcmp(key, "tauxFile")) copy_name(c->tauxFile, val);
     else if (!strcmp(key, "tauyFile")) copy_name(c->tauyFile, val);
-    else if (!strcmp(key, "useTracer") || !strcmp(key, "useBuoyancy") || !strcmp(key, "restart")
+    else if (!strcmp(key, "restart")) c->restart = iv != 0;
+    else if (!strcmp(key, "startIdx")) c->startIdx = (unsigned)uv;
+    else if (!strcmp(key, "saveCheckpoint")) c->saveCheckpoint = iv != 0;
+    else if (!strcmp(key, "restartFromCheckpoint")) c->restartCheckpoint = iv != 0;
+    else if (!strcmp(key, "useTracer") || !strcmp(key, "useBuoyancy")
              || !strcmp(key, "windFeedback") || !strcmp(key, "useRandomForcing") || !strcmp(key, "useTrad")) {
       if (iv != 0) note_unsupported(c, key);
     } else if (!strcmp(key, "FbaroXFile") || !strcmp(key, "FbaroYFile") || !strcmp(key, "wDiaFile")
@@ -179,13 +187,11 @@ static void resolve(const drv_config *c, const char *name, char *out, size_t n)
 
 /* Raw float64 reader: m x n (x fastest), optionally `layers` fastest
  * (defs.c:191-211, 291-309); output a[(layer*m + i)*n + j]. */
-static double *read_raw(const drv_config *c, const char *name, unsigned m, unsigned n, unsigned layers)
+static double *read_raw_path(const char *path, unsigned m, unsigned n, unsigned layers)
 {
-  char path[1024];
   size_t cnt = (size_t)m * n * layers;
   double *buf = malloc(cnt * sizeof(double)), *a = malloc(cnt * sizeof(double));
   FILE *f;
-  resolve(c, name, path, sizeof(path));
   f = fopen(path, "rb");
   if (!f || fread(buf, sizeof(double), cnt, f) != cnt) {
     fprintf(stderr, "ERROR: Could not read %zu values from %s\n", cnt, path);
@@ -200,6 +206,33 @@ static double *read_raw(const drv_config *c, const char *name, unsigned m, unsig
       for (unsigned k = 0; k < layers; k++) a[((size_t)k * m + i) * n + j] = buf[((size_t)j * m + i) * layers + k];
   free(buf);
   return a;
+}
+
+static double *read_raw(const drv_config *c, const char *name, unsigned m, unsigned n, unsigned layers)
+{
+  char path[1024];
+  resolve(c, name, path, sizeof(path));
+  return read_raw_path(path, m, n, layers);
+}
+
+/* pawsim_state_read_restart (state.c:1106-1155): U/V/H of save idx from
+ * the output directory overwrite the initial state ([L][Nx][Ny] each). */
+static bool read_restart(const char *dir, unsigned idx, unsigned Nx, unsigned Ny, unsigned L, double *u, double *v,
+                         double *hh)
+{
+  static const char *names[3] = {"U", "V", "H"};
+  double *dst[3] = {u, v, hh};
+  const size_t N = (size_t)Nx * Ny;
+  char path[1024];
+  for (unsigned k = 0; k < L; k++)
+    for (int f = 0; f < 3; f++) {
+      double *a;
+      snprintf(path, sizeof(path), "%s/%s%u_n=%u.dat", dir, names[f], k, idx);
+      if (!(a = read_raw_path(path, Nx, Ny, 1))) return false;
+      memcpy(dst[f] + k * N, a, N * sizeof(double));
+      free(a);
+    }
+  return true;
 }
 
 /* Write an Nx x Ny field (element (i,j) at a[i*Ny+j]) with x fastest. */
@@ -227,7 +260,82 @@ typedef struct model {
   unsigned n_avg;
   double t_next_avg;
   double *avg;  /* (10 Nlay + 2) Nx x Ny */
+  unsigned n_done;  /* steps taken since the start of the (uninterrupted) run */
 } model;
+
+/* Checkpoint file: this header (the driver's schedule) followed by the
+ * library's blob (pg_checkpoint_save). */
+typedef struct drv_checkpoint {
+  uint64_t magic;
+  uint32_t n_done, n_saves, n_avg, n_EZ;
+  double t, t_next_avg, t_next_EZ;
+  uint64_t blob_bytes;
+} drv_checkpoint;
+
+#define DRV_CK_MAGIC 0x31444b43475053ull /* "SPGCKD1" */
+
+static bool write_checkpoint(model *M, const char *dir, unsigned idx)
+{
+  char path[1024];
+  drv_checkpoint ck;
+  size_t n = pg_checkpoint_bytes(M->h);
+  void *blob = malloc(n);
+  FILE *f;
+  bool ok;
+  if (!blob || pg_checkpoint_save(M->h, blob, n) != PG_OK) {
+    free(blob);
+    return false;
+  }
+  memset(&ck, 0, sizeof ck);
+  ck.magic = DRV_CK_MAGIC;
+  ck.n_done = M->n_done;
+  ck.n_saves = M->n_saves;
+  ck.n_avg = M->n_avg;
+  ck.n_EZ = M->n_EZ;
+  ck.t = M->t;
+  ck.t_next_avg = M->t_next_avg;
+  ck.t_next_EZ = M->t_next_EZ;
+  ck.blob_bytes = n;
+  snprintf(path, sizeof(path), "%s/checkpoint_n=%u.bin", dir, idx);
+  f = fopen(path, "wb");
+  ok = f && fwrite(&ck, sizeof ck, 1, f) == 1 && fwrite(blob, 1, n, f) == n;
+  if (f) ok = (fclose(f) == 0) && ok;
+  free(blob);
+  return ok;
+}
+
+static bool read_checkpoint(model *M, const char *dir, unsigned idx)
+{
+  char path[1024];
+  drv_checkpoint ck;
+  void *blob = NULL;
+  FILE *f;
+  bool ok;
+  snprintf(path, sizeof(path), "%s/checkpoint_n=%u.bin", dir, idx);
+  f = fopen(path, "rb");
+  if (!f || fread(&ck, sizeof ck, 1, f) != 1 || ck.magic != DRV_CK_MAGIC || !(blob = malloc(ck.blob_bytes))
+      || fread(blob, 1, ck.blob_bytes, f) != ck.blob_bytes) {
+    fprintf(stderr, "ERROR: Could not read checkpoint %s\n", path);
+    if (f) fclose(f);
+    free(blob);
+    return false;
+  }
+  fclose(f);
+  ok = pg_checkpoint_load(M->h, blob, ck.blob_bytes) == PG_OK;
+  free(blob);
+  if (!ok) {
+    fprintf(stderr, "ERROR: %s: %s\n", path, pg_last_error(M->h));
+    return false;
+  }
+  M->n_done = ck.n_done;
+  M->n_saves = ck.n_saves;
+  M->n_avg = ck.n_avg;
+  M->n_EZ = ck.n_EZ;
+  M->t = ck.t;
+  M->t_next_avg = ck.t_next_avg;
+  M->t_next_EZ = ck.t_next_EZ;
+  return true;
+}
 
 /* pawsim_diagnostics_accumulate_averages + write_averages_if_due
  * (diagnostics.c:307-470): the window weight of this step, then at the end
@@ -274,7 +382,7 @@ static bool write_EZ_if_due(model *M, const char *dir)
     FILE *f;
     if (pg_energy(M->h, &KE, &PE, Z) != PG_OK) return false;
     snprintf(path, sizeof(path), "%s/EZdiags.dat", dir);
-    f = fopen(path, (M->n_EZ > 1) ? "a" : "w");
+    f = fopen(path, (M->c.restart || M->n_EZ > 1) ? "a" : "w");
     if (!f) return false;
     fprintf(f, "%le", M->t_next_EZ);
     fprintf(f, " %.50e %.50e %.50e ", KE, PE, KE + PE);
@@ -373,7 +481,8 @@ int main(int argc, char **argv)
     fprintf(stderr, "Must specify exactly two of {Nt, tmax, dt}\n");
     return 1;
   }
-  if (c->tmin < 0) c->tmin = 0;
+  /* before savefrequency takes its default (pawsim.c:157-160) */
+  if (c->tmin < 0) c->tmin = c->restart ? c->startIdx * c->savefrequency : 0;
   if (c->Nt == 0) c->Nt = (unsigned)ceil((c->tmax - c->tmin) / c->pg.dt);
   if (c->pg.dt == 0) c->pg.dt = (c->tmax - c->tmin) / c->Nt;
   if (c->savefrequency == 0) c->savefrequency = c->pg.dt;
@@ -417,33 +526,45 @@ int main(int argc, char **argv)
     fprintf(stderr, "ERROR: Could not initialize the GPU model: %s\n", M.h ? pg_last_error(M.h) : "");
     return 1;
   }
+  /* readParameterData: a restart overwrites U/V/H from the outputs (pawsim.c:251-254) */
+  if (c->restart && !c->restartCheckpoint && !read_restart(argv[2], c->startIdx, Nx, Ny, L, M.u, M.v, M.hh)) return 1;
   if (pg_upload_state(M.h, M.u, M.v, M.hh) != PG_OK) {
     fprintf(stderr, "ERROR: state upload failed: %s\n", pg_last_error(M.h));
     return 1;
   }
   printf("PAWSIM parsed %u x %u x %u AWSIM input\n", Nx, Ny, L);
   t_all = now_s();
-  /* preliminary correction (pawsim.c:282-295) */
-  it = correct(&M, "prelim", 0);
-  if (it < 0) return 1;
-  printf("PAWSIM pressure iterations: %d\n", it);
-  pg_refresh_halos(M.h);  /* work-array rebuild after the preliminary correction */
-  if (!write_state(&M, argv[2], 0)) {
-    fprintf(stderr, "Unable to write model initial state\n");
-    return 1;
+  if (c->restart && c->restartCheckpoint) {
+    /* continue the interrupted run exactly: state, history, pi, schedule */
+    if (!read_checkpoint(&M, argv[2], c->startIdx)) return 1;
+  } else {
+    /* preliminary correction (pawsim.c:282-295) */
+    it = correct(&M, "prelim", 0);
+    if (it < 0) return 1;
+    printf("PAWSIM pressure iterations: %d\n", it);
+    pg_refresh_halos(M.h);  /* work-array rebuild after the preliminary correction */
+    /* the first state written carries the restart index (pawsim.c:427, diagnostics.c:977-999) */
+    M.n_saves = c->restart ? c->startIdx : 0;
+    if (!write_state(&M, argv[2], M.n_saves)) {
+      fprintf(stderr, "Unable to write model initial state\n");
+      return 1;
+    }
+    M.n_saves++;
+    if (c->savefreqAvg > 0) { /* pawsim.c:429-433 */
+      M.n_avg = (unsigned)round(c->tmin / c->savefreqAvg) + 1;
+      M.t_next_avg = c->tmin + c->savefreqAvg;
+    }
+    if (c->savefreqEZ > 0) { /* pawsim.c:434-438 */
+      M.n_EZ = (unsigned)round(c->tmin / c->savefreqEZ) + 1;
+      M.t_next_EZ = c->tmin + c->savefreqEZ;
+    }
   }
-  M.n_saves = 1;
-  if (c->savefreqAvg > 0) { /* pawsim.c:429-433 */
-    M.n_avg = (unsigned)round(c->tmin / c->savefreqAvg) + 1;
-    M.t_next_avg = c->tmin + c->savefreqAvg;
-  }
-  if (c->savefreqEZ > 0) { /* pawsim.c:434-438 */
-    M.n_EZ = (unsigned)round(c->tmin / c->savefreqEZ) + 1;
-    M.t_next_EZ = c->tmin + c->savefreqEZ;
-  }
+  const unsigned n0 = M.n_done;
   for (unsigned n = 1; n <= c->Nt; n++) {
     double ts = now_s();
-    if (pg_step(M.h, n) != PG_OK) {
+    const unsigned saves_before = M.n_saves;
+    M.n_done = n0 + n;
+    if (pg_step(M.h, M.n_done) != PG_OK) {
       fprintf(stderr, "ERROR: PAWSIM time step failed at n=%u: %s\n", n, pg_last_error(M.h));
       return 1;
     }
@@ -469,6 +590,10 @@ int main(int argc, char **argv)
     /* pawsim.c:501-503 */
     if (!averages(&M, argv[2]) || !write_EZ_if_due(&M, argv[2])) {
       fprintf(stderr, "ERROR: PAWSIM diagnostic averaging failed at n=%u\n", n);
+      return 1;
+    }
+    if (c->saveCheckpoint && M.n_saves != saves_before && !write_checkpoint(&M, argv[2], M.n_saves - 1)) {
+      fprintf(stderr, "Unable to write checkpoint n=%u\n", M.n_saves - 1);
       return 1;
     }
   }

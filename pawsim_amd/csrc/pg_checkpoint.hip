@@ -1,0 +1,158 @@
+// pg_checkpoint.hip -- full-fidelity checkpoints of a handle.
+//
+// A restart of the reference (restart = 1, state.c:1106-1155) reads U, V, H
+// from a save and starts over: the Adams-Bashforth history restarts at AB1,
+// pi at zero and the preliminary correction runs again, so the restarted
+// trajectory differs from the uninterrupted one.  A checkpoint here holds
+// everything a step reads that the previous steps wrote, so that loading it
+// and stepping on gives the uninterrupted run bit for bit:
+//
+//   * u, v, h and pi with their ghost cells (the serial halo fill leaves
+//     corner ghosts one fill behind on periodic-x / wall-y domains, and the
+//     stencils read some of them);
+//   * the three tendency slots in history order (FP64, or FP32 in mixed
+//     precision);
+//   * the GPU multigrid's cycle prediction, the earlier solutions of the
+//     extrapolated initial guess and their count, and the halo / fused
+//     thickness-fix / corner flags;
+//   * the online-average sums when averaging is on, and the model time.
+//
+// Layout: a fixed header, then the arrays back to back (FP64 unless noted).
+// The learned preconditioner's parameters are not part of it (they have
+// their own get/set entry points).
+#include "pg_internal.h"
+
+#include <cstring>
+
+namespace {
+
+constexpr uint64_t CK_MAGIC = 0x314b43474d495350ull;  // "PSIMGCK1"
+constexpr int AV_PER_LAYER = 10;                       // online-average fields per layer (pg_diag.hip)
+
+struct CkHeader {
+  uint64_t magic;
+  uint32_t nx, ny, nlay, mixed;
+  uint64_t fsz;
+  double t;
+  int32_t mg_pred, pi_solves;
+  uint32_t has_pi_hist, has_avg;
+  uint32_t halo_dirty, thickness_done, corner_fill_due, reserved;
+};
+
+struct Part {
+  void* dev;
+  size_t bytes;
+};
+
+// The arrays of a checkpoint, in file order.
+std::vector<Part> parts(pg_handle* h, bool pi_hist, bool avg)
+{
+  const PgGrid& G = h->G;
+  const size_t lf = (size_t)G.L * h->fsz, N = (size_t)G.nx * G.ny;
+  std::vector<Part> p = {{h->u, lf * 8}, {h->v, lf * 8}, {h->h, lf * 8}, {h->pi, h->fsz * 8}};
+  for (int s = 0; s < 3; s++) {
+    const int b = h->slot[s];
+    if (h->mixed) {
+      p.push_back({h->fu32[b], lf * 4});
+      p.push_back({h->fv32[b], lf * 4});
+      p.push_back({h->fh32[b], lf * 4});
+    } else {
+      p.push_back({h->fu[b], lf * 8});
+      p.push_back({h->fv[b], lf * 8});
+      p.push_back({h->fh[b], lf * 8});
+    }
+  }
+  if (pi_hist) p.push_back({h->pi_hist, 2 * N * 8});
+  if (avg) p.push_back({h->avg, ((size_t)AV_PER_LAYER * G.L + 2) * N * 8});
+  return p;
+}
+
+size_t total(const std::vector<Part>& p)
+{
+  size_t n = sizeof(CkHeader);
+  for (const Part& q : p) n += q.bytes;
+  return n;
+}
+
+}  // namespace
+
+extern "C" size_t pg_checkpoint_bytes(pg_handle* h)
+{
+  if (!h) return 0;
+  return total(parts(h, h->pi_hist != nullptr, h->avg != nullptr));
+}
+
+extern "C" int pg_checkpoint_save(pg_handle* h, void* buf, size_t bytes)
+{
+  if (!h || !buf) return PG_EARG;
+  const std::vector<Part> p = parts(h, h->pi_hist != nullptr, h->avg != nullptr);
+  if (bytes < total(p)) {
+    h->err = "pg_checkpoint_save: buffer smaller than pg_checkpoint_bytes()";
+    return PG_EARG;
+  }
+  CkHeader hd{};
+  hd.magic = CK_MAGIC;
+  hd.nx = (uint32_t)h->G.nx;
+  hd.ny = (uint32_t)h->G.ny;
+  hd.nlay = (uint32_t)h->G.L;
+  hd.mixed = h->mixed ? 1 : 0;
+  hd.fsz = h->fsz;
+  hd.t = h->t;
+  hd.mg_pred = h->mg_pred;
+  hd.pi_solves = h->pi_solves;
+  hd.has_pi_hist = h->pi_hist != nullptr;
+  hd.has_avg = h->avg != nullptr;
+  hd.halo_dirty = h->halo_dirty;
+  hd.thickness_done = h->thickness_done;
+  hd.corner_fill_due = h->corner_fill_due;
+  char* out = static_cast<char*>(buf);
+  std::memcpy(out, &hd, sizeof hd);
+  out += sizeof hd;
+  for (const Part& q : p) {
+    PG_CHECK(h, hipMemcpyAsync(out, q.dev, q.bytes, hipMemcpyDeviceToHost, h->stream));
+    out += q.bytes;
+  }
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
+
+extern "C" int pg_checkpoint_load(pg_handle* h, const void* buf, size_t bytes)
+{
+  if (!h || !buf || bytes < sizeof(CkHeader)) return PG_EARG;
+  CkHeader hd;
+  std::memcpy(&hd, buf, sizeof hd);
+  if (hd.magic != CK_MAGIC || hd.nx != (uint32_t)h->G.nx || hd.ny != (uint32_t)h->G.ny || hd.nlay != (uint32_t)h->G.L
+      || hd.mixed != (h->mixed ? 1u : 0u) || hd.fsz != h->fsz) {
+    h->err = "pg_checkpoint_load: the checkpoint belongs to a different grid, layer count or precision mode";
+    return PG_EARG;
+  }
+  const size_t N = (size_t)h->G.nx * h->G.ny;
+  if (hd.has_pi_hist && !h->pi_hist) PG_CHECK(h, hipMalloc(&h->pi_hist, 2 * N * sizeof(double)));
+  if (hd.has_avg && !h->avg)
+    PG_CHECK(h, hipMalloc(&h->avg, ((size_t)AV_PER_LAYER * h->G.L + 2) * N * sizeof(double)));
+  h->slot[0] = 0;  // history comes back in order
+  h->slot[1] = 1;
+  h->slot[2] = 2;
+  const std::vector<Part> p = parts(h, hd.has_pi_hist != 0, hd.has_avg != 0);
+  if (bytes < total(p)) {
+    h->err = "pg_checkpoint_load: truncated checkpoint";
+    return PG_EARG;
+  }
+  const char* in = static_cast<const char*>(buf) + sizeof hd;
+  for (const Part& q : p) {
+    PG_CHECK(h, hipMemcpyAsync(q.dev, in, q.bytes, hipMemcpyHostToDevice, h->stream));
+    in += q.bytes;
+  }
+  if (!hd.has_avg && h->avg) {  // no averaging in progress at the checkpoint: empty sums
+    PG_CHECK(h, hipMemsetAsync(h->avg, 0, ((size_t)AV_PER_LAYER * h->G.L + 2) * N * sizeof(double), h->stream));
+  }
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  h->t = hd.t;
+  h->mg_pred = hd.mg_pred;
+  h->pi_solves = hd.has_pi_hist ? hd.pi_solves : 0;
+  h->halo_dirty = hd.halo_dirty != 0;
+  h->thickness_done = hd.thickness_done != 0;
+  h->corner_fill_due = hd.corner_fill_due != 0;
+  h->rhs_mean_pending = false;
+  return PG_OK;
+}

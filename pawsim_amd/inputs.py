@@ -305,6 +305,15 @@ def read_outputs(outdir: str, nx: int, ny: int, nlay: int, n: int, with_p: bool 
 AVG_PREFIXES = ("U_avg", "V_avg", "H_avg", "M_avg", "W_avg", "HU_avg", "HV_avg", "HUU_avg", "HVV_avg", "HUV_avg")
 
 
+def write_restart_files(outdir: str, idx: int, fields: Dict[str, np.ndarray]) -> None:
+    """Place U/V/H ([Nlay, Nx, Ny] each) as the outputs of save index idx,
+    the files a restart reads (state.c:1106-1155)."""
+    os.makedirs(outdir, exist_ok=True)
+    for name, arr in fields.items():
+        for k in range(arr.shape[0]):
+            write_2d(os.path.join(outdir, f"{name}{k}_n={idx}.dat"), arr[k])
+
+
 def read_averages(outdir: str, nx: int, ny: int, nlay: int) -> Dict[int, Dict[str, np.ndarray]]:
     """Online-average records in outdir: {n: {prefix: [layers, Nx, Ny], "P_avg": [Nx, Ny]}}.
     W_avg has Nlay + 1 layers (interfaces), the others Nlay."""

@@ -32,6 +32,15 @@ def load_golden(name):
     return z, load_setup(z)
 
 
+def prepare_outdir(z, outdir):
+    """A restart fixture's run reads U/V/H of its start index from its
+    output directory: place them there."""
+    if "restart_from" in z.files:
+        from pawsim_amd import inputs
+
+        inputs.write_restart_files(outdir, int(z["restart_from"]), {k: z[f"restart_in_{k}"] for k in ("U", "V", "H")})
+
+
 def run_exe(exe, infile, outdir, timeout=600):
     os.makedirs(outdir, exist_ok=True)
     r = subprocess.run([exe, infile, outdir], capture_output=True, text=True, timeout=timeout)

@@ -78,6 +78,17 @@ def cases():
     }
 
 
+def restart_cases():
+    """(setup, idx): the reference runs the setup, then restarts from its save
+    idx (restart = 1, startIdx = idx) for the remaining steps; the fixture
+    holds the restart's inputs, its U/V/H files and its outputs."""
+    return {
+        "box16_restart": (inputs.gyre(nx=16, ny=16, nt=6, savefreqEZ=600.0, **T), 3),
+        "chan32x16_restart_avg": (inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1, savefreqAvg=400.0,
+                                                 thicknessScheme=2, **T), 2),
+    }
+
+
 SOLVE_RE = re.compile(r"phase=(\w+) n=(\d+) solver=(\w+) iters=(\d+) walltime=\S+ s residual_max=(\S+) "
                       r"residual_rms=(\S+) div_max=(\S+) div_rms=(\S+)")
 
@@ -89,16 +100,31 @@ def parse_solves(stdout: str) -> np.ndarray:
     return np.array(rows, dtype=np.float64)
 
 
-def make_case(name: str, setup: inputs.Setup, outdir: str) -> str:
+def make_case(name: str, setup: inputs.Setup, outdir: str, restart_from: int = -1) -> str:
     work = tempfile.mkdtemp(prefix=f"golden_{name}_")
     try:
+        restart_in = {}
+        if restart_from >= 0:
+            first = os.path.join(work, "first")
+            os.makedirs(first)
+            subprocess.run([REF, setup.write(os.path.join(work, "in_first")), first], capture_output=True, text=True,
+                           check=True)
+            restart_in = {k: v for k, v in inputs.read_outputs(first, setup.nx, setup.ny, setup.nlay, restart_from,
+                                                               with_p=False).items()}
+            p = dict(setup.params)
+            p.update(restart=1, startIdx=restart_from, Nt=int(p["Nt"]) - restart_from)
+            setup = inputs.Setup(p, setup.hhb, setup.hhs, setup.omegaz, setup.gg, setup.h, u=setup.u, v=setup.v,
+                                 taux=setup.taux, tauy=setup.tauy, name=setup.name, notes=setup.notes)
         infile = setup.write(work)
         res = os.path.join(work, "out")
         os.makedirs(res)
+        if restart_in:
+            inputs.write_restart_files(res, restart_from, restart_in)
         proc = subprocess.run([REF, infile, res], capture_output=True, text=True, check=True)
         nt = int(setup.params["Nt"])
         nx, ny, nl = setup.nx, setup.ny, setup.nlay
-        saves = sorted({0, 1, 2, 3, nt})
+        s0 = max(restart_from, 0)
+        saves = sorted({s0, s0 + 1, s0 + 2, s0 + nt})
         arrays = {
             "hhb": setup.hhb, "omegaz": setup.omegaz, "gg": np.asarray(setup.gg), "h": setup.h,
             "saves": np.array(saves),
@@ -108,6 +134,10 @@ def make_case(name: str, setup: inputs.Setup, outdir: str) -> str:
             val = getattr(setup, key)
             if val is not None:
                 arrays[key] = val
+        if restart_in:
+            arrays["restart_from"] = np.array(restart_from)
+            for k, v in restart_in.items():
+                arrays[f"restart_in_{k}"] = v
         for n in saves:
             out = inputs.read_outputs(res, nx, ny, nl, n)
             for k, v in out.items():
@@ -151,6 +181,11 @@ def main() -> None:
         if only and name not in only:
             continue
         p = make_case(name, setup, HERE)
+        print(f"{p}: {os.path.getsize(p)} bytes")
+    for name, (setup, idx) in restart_cases().items():
+        if only and name not in only:
+            continue
+        p = make_case(name, setup, HERE, restart_from=idx)
         print(f"{p}: {os.path.getsize(p)} bytes")
 
 

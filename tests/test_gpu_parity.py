@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import pawsim_amd
-from conftest import ez_close, golden_ids, load_golden, parse_iters, read_ez, run_exe, same_bits
+from conftest import ez_close, golden_ids, load_golden, parse_iters, prepare_outdir, read_ez, run_exe, same_bits
 from pawsim_amd import inputs
 
 pytestmark = pytest.mark.gpu
@@ -22,6 +22,7 @@ def test_gpu_matches_reference_golden(name, tmp_path):
     z, setup = load_golden(name)
     infile = setup.write(str(tmp_path / "in"))
     out = str(tmp_path / "gpu")
+    prepare_outdir(z, out)
     print(f"[gpu parity] {name} {setup.nx}x{setup.ny}x{setup.nlay}", flush=True)
     stdout = run_exe(pawsim_amd.DRIVER_PATH, infile, out, timeout=300)
     for n in z["saves"]:

@@ -1,0 +1,108 @@
+"""Restarts on the GPU.
+
+* The reference's restart (restart = 1, startIdx; state.c:1106-1155) is
+  covered bit for bit by the restart golden fixtures in
+  test_gpu_parity.py::test_gpu_matches_reference_golden.
+* The full-fidelity checkpoint (pg_checkpoint_save / pg_checkpoint_load,
+  driver keys saveCheckpoint / restartFromCheckpoint) must continue an
+  interrupted run exactly: a run restarted from the checkpoint of save k
+  writes the same U/V/H/P, average files and EZ lines, bit for bit, as the
+  uninterrupted run from save k on.
+"""
+import glob
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import pawsim_amd
+from conftest import read_ez, run_exe, same_bits
+from pawsim_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # bit-replica SOR, AB3 (history and pi restored)
+    "gyre48_sor": dict(kind="gyre", nx=48, ny=48, nt=12, solver=0, savefreqEZ=600.0),
+    # mixed precision, GPU multigrid with the quadratic pi extrapolation
+    "jets96x64_mixed": dict(kind="jets", nx=96, ny=64, nlay=3, dx=20e3, nt=12, solver=4, tol=1e-8,
+                            H_rest=(500.0, 1500.0, 2000.0), mixed_precision=1, pi_guess=2, A2=50.0,
+                            savefreqAvg=1500.0),
+    # periodic-x / wall-y channel, UP3 + Smagorinsky without the norms (corner
+    # ghosts one fill behind), averages across the restart
+    "chan48x24_up3_smag": dict(kind="channel", nx=48, ny=24, nt=12, dx=20e3, solver=1, thicknessScheme=2,
+                               A2smag=2.0, pressureTiming=0, savefreqAvg=900.0, savefreqEZ=600.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_checkpoint_restart_continues_bitwise(name, tmp_path):
+    spec = dict(CASES[name])
+    kind = spec.pop("kind")
+    spec.setdefault("pressureTiming", 1)
+    k = 5
+    full = getattr(inputs, kind)(name=name, saveCheckpoint=1, **spec)
+    nt = int(full.params["Nt"])
+    print(f"[gpu restart] {name} continuous", flush=True)
+    out_full = str(tmp_path / "full")
+    run_exe(pawsim_amd.DRIVER_PATH, full.write(str(tmp_path / "in_full")), out_full, timeout=300)
+    # the restarted run starts from nothing but the checkpoint of save k
+    out_rs = str(tmp_path / "restart")
+    os.makedirs(out_rs)
+    shutil.copy(os.path.join(out_full, f"checkpoint_n={k}.bin"), out_rs)
+    rs = inputs.Setup(dict(full.params, restart=1, startIdx=k, restartFromCheckpoint=1, Nt=nt - k,
+                           saveCheckpoint=0), full.hhb, full.hhs, full.omegaz, full.gg, full.h, u=full.u, v=full.v,
+                      taux=full.taux, tauy=full.tauy, name=name)
+    print(f"[gpu restart] {name} from checkpoint {k}", flush=True)
+    run_exe(pawsim_amd.DRIVER_PATH, rs.write(str(tmp_path / "in_rs")), out_rs, timeout=300)
+    for n in range(k + 1, nt + 1):
+        a = inputs.read_outputs(out_full, full.nx, full.ny, full.nlay, n)
+        b = inputs.read_outputs(out_rs, full.nx, full.ny, full.nlay, n)
+        for key in a:
+            assert same_bits(a[key], b[key]), f"{key} at save {n}"
+    if "savefreqAvg" in spec:
+        af = inputs.read_averages(out_full, full.nx, full.ny, full.nlay)
+        ar = inputs.read_averages(out_rs, full.nx, full.ny, full.nlay)
+        assert ar, "no average window closed after the restart"
+        for n in ar:
+            for key in ar[n]:
+                assert same_bits(ar[n][key], af[n][key]), f"{key} average {n}"
+    if "savefreqEZ" in spec:
+        ef = read_ez(os.path.join(out_full, "EZdiags.dat"))
+        er = read_ez(os.path.join(out_rs, "EZdiags.dat"))
+        assert len(er) >= 1
+        assert same_bits(er, ef[-len(er):])
+    assert not glob.glob(os.path.join(out_rs, "checkpoint_n=*.bin"))[1:]  # only the one copied in
+
+
+def test_model_checkpoint_roundtrip():
+    """Model.checkpoint() / restore() through the C ABI: a second handle that
+    loads the blob and steps on matches the first handle bit for bit."""
+    setup = inputs.jets(nx=72, ny=56, nlay=3, dx=20e3, nt=20, solver=3, tol=1e-8, A2=50.0, pi_guess=2,
+                        H_rest=(500.0, 1500.0, 2000.0), name="ck_model")
+    a = pawsim_amd.Model.from_setup(setup)
+    a.pressure_correct()
+    a.refresh_halos()
+    for n in range(1, 7):
+        a.step(n)
+        a.pressure_correct()
+    blob = a.checkpoint()
+    b = pawsim_amd.Model.from_setup(setup)
+    b.restore(blob)
+    for n in range(7, 13):
+        a.step(n)
+        a.pressure_correct()
+        b.step(n)
+        b.pressure_correct()
+    sa, sb = a.state(), b.state()
+    for x, y in zip(sa, sb):
+        assert same_bits(x, y)
+    assert same_bits(a.pi(), b.pi())
+    # a blob of another grid is refused
+    other = pawsim_amd.Model.from_setup(inputs.jets(nx=64, ny=56, nlay=3, dx=20e3, nt=5, solver=3,
+                                                    H_rest=(500.0, 1500.0, 2000.0)))
+    with pytest.raises(RuntimeError):
+        other.restore(blob)
+    for m in (a, b, other):
+        m.close()
