@@ -72,6 +72,10 @@ typedef struct pg_config {
   double KT00_sigma;           /* KT00 limiter slope factor, > 0 when thicknessScheme = 3
                                   (config.c:73: the reference's default is 1.4) */
   double A2smag, A4smag;       /* Smagorinsky coefficients (work.c:1538-1610); 0 = off */
+  uint32_t wDiaNrecs;          /* records in pg_static.wDia (config.c:96, 231); 0 means 1 */
+  double wDiaPeriod;           /* period the records cycle over, > 0 with wDiaNrecs > 1 (config.c:92, 227) */
+  double convDiff;             /* convective diapycnal viscosity; without active buoyancy it only switches the
+                                  diapycnal viscosity on (config.c:93, 311-312) */
 } pg_config;
 
 /* Static inputs (state.c:1012-1047).  NULL hhs means a flat lid at 0;
@@ -83,6 +87,15 @@ typedef struct pg_static {
   const double *gg;      /* [Nlay] reduced gravities */
   const double *taux;    /* [Nx][Ny] kinematic wind stress or NULL */
   const double *tauy;
+  /* Restoring and diapycnal inputs (state.c:1027-1041), NULL when absent.
+   * Restoring is on when any time scale is given (config.c:305-307), the
+   * interface velocity when restoring or wDia is (308), the diapycnal
+   * viscosity when nu_u_dia, nu_v_dia or convDiff is (311-312). */
+  const double *uRelax, *vRelax, *hRelax, *eRelax; /* [Nlay][Nx][Ny] restoring targets */
+  const double *uTime, *vTime, *eTime;             /* [Nlay][Nx][Ny] time scales (<= 0: none) */
+  const double *hTime;                             /* [Nx][Ny] thickness time scale, all layers */
+  const double *wDia;                              /* [wDiaNrecs][Nlay+1][Nx][Ny] interface velocity */
+  const double *nu_u_dia, *nu_v_dia;               /* [Nlay][Nx][Ny] diapycnal viscosity at interfaces */
 } pg_static;
 
 /* Per-solve statistics (context.h:52-61 plus the relative residual). */

@@ -37,6 +37,9 @@ enum { OR_SOLVER_SOR = 0, OR_SOLVER_MG = 1 };
 typedef struct or_config {
   uint Nlay, Nx, Ny, Nt, tsScheme, momScheme, thkScheme, maxiters, solver, tauNrecs;
   uint startIdx;              /* save index a restart reads (config.c:188) */
+  uint wDiaNrecs;             /* records in wDiaFile (config.c:231) */
+  real wDiaPeriod, convDiff;  /* config.c:227-228 */
+  bool useRelax, useWDia, useDiaVisc;  /* derived from the files (config.c:305-312) */
   real Lx, Ly, dt, tmin, tmax, savefrequency, h0, hsml, hbbl, A2, A4, A2smag, A4smag;
   real linDrag, quadDrag, linDragSurf, quadDragSurf, pi_tol, SOR_rp, omega_MG;
   real savefreqEZ;            /* energy/enstrophy diagnostics interval (config.c:206) */
@@ -49,6 +52,9 @@ typedef struct or_config {
   char inputDir[512];
   char hInit[256], uInit[256], vInit[256], hbFile[256], hsFile[256];
   char omegaFile[256], gFile[256], tauxFile[256], tauyFile[256];
+  char uRelaxFile[256], vRelaxFile[256], hRelaxFile[256], eRelaxFile[256];
+  char uTimeFile[256], vTimeFile[256], hTimeFile[256], eTimeFile[256];
+  char wDiaFile[256], diaViscUFile[256], diaViscVFile[256];
 } or_config;
 
 /* Ghosted field, column-major in x: element (i,j) at d[i*sy+j] (field.c:37-71). */
@@ -82,6 +88,11 @@ typedef struct or_model {
   or_field DD_T, DD_S, DD4_T, DD4_S, A2_h, A2_q, A4s_h, A4s_q, UU4, VV4;
   or_field thk_x, thk_y;            /* d2hx/d2hy (UP3) or dh_dx/dh_dy (KT00) (work.c:630-667) */
   or_field pi, rhs;
+  /* restoring and diapycnal inputs (state.c:1027-1041) and work arrays */
+  or_field *uRelax, *vRelax, *hRelax, *eRelax, *uTime, *vTime, *eTime, *nu_u, *nu_v;  /* [L] */
+  or_field hTime;
+  or_field *wdia_ff;                /* [wDiaNrecs * (L+1)], record-major */
+  or_field *wdia, *wdia_u, *wdia_v, *Fdia_u, *Fdia_v;  /* [L+1] */
   /* pressure diagnostics (context.h:52-61) */
   real res_max, res_rms, div_max, div_rms;
   uint solves, iters_total, n_saves;

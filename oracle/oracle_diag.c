@@ -150,7 +150,8 @@ void or_averages_free(or_model *m)
 #define AVG(q, k) (&m->avg[(k) * AV_PER_LAYER + (q)])
 
 /* pawsim_diagnostics_accumulate_averages (diagnostics.c:307-372); no
- * tracer, no diapycnal velocity (its averages stay zero). */
+ * tracer.  W is the interface velocity the last tendency evaluation built
+ * (the averages reload the state but do not rebuild wdia). */
 bool or_accumulate_averages(or_model *m)
 {
   uint i, j, k;
@@ -173,7 +174,7 @@ bool or_accumulate_averages(or_model *m)
         OF(AVG(AV_V, k), i, j) += dtfac * OF(V, i, j);
         OF(AVG(AV_H, k), i, j) += dtfac * OF(H, i, j);
         OF(AVG(AV_M, k), i, j) += dtfac * (OF(&m->MM[k], i, j) + (m->cfg.useRL ? OF(&m->pi, i, j) : 0));
-        OF(AVG(AV_W, k), i, j) += dtfac * 0.0;
+        OF(AVG(AV_W, k), i, j) += dtfac * OF(&m->wdia[k], i, j);
         OF(AVG(AV_HU, k), i, j) += dtfac * OF(&m->hwest[k], i, j) * OF(U, i, j);
         OF(AVG(AV_HV, k), i, j) += dtfac * OF(&m->hsouth[k], i, j) * OF(V, i, j);
         OF(AVG(AV_HUU, k), i, j) += dtfac * OF(&m->hwest[k], i, j) * OF(U, i, j) * OF(U, i, j);
@@ -184,7 +185,7 @@ bool or_accumulate_averages(or_model *m)
   for (i = g; i < g + m->cfg.Nx; i++)
     for (j = g; j < g + m->cfg.Ny; j++) {
       const real dtfac = avg_fac * m->cfg.dt;
-      OF(&m->avg[AV_PER_LAYER * L], i, j) += dtfac * 0.0;
+      OF(&m->avg[AV_PER_LAYER * L], i, j) += dtfac * OF(&m->wdia[L], i, j);
       OF(&m->avg[AV_PER_LAYER * L + 1], i, j) += dtfac * OF(&m->pi, i, j);
     }
   return true;

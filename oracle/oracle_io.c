@@ -20,6 +20,7 @@ void or_config_defaults(or_config *c)
   c->momScheme = OR_MOM_TW81;
   c->thkScheme = OR_THK_AL81;
   c->KT00_sigma = 1.4;
+  c->wDiaNrecs = 1;
   c->tmin = -1;
   c->pi_tol = 1e-10;
   c->SOR_rp = 1.0;
@@ -95,21 +96,22 @@ bool or_config_read(const char *fname, or_config *c, FILE *err)
     S("hInitFile", c->hInit); S("uInitFile", c->uInit); S("vInitFile", c->vInit);
     S("hbFile", c->hbFile); S("hsFile", c->hsFile); S("OmegazFile", c->omegaFile);
     S("gFile", c->gFile); S("tauxFile", c->tauxFile); S("tauyFile", c->tauyFile);
+    S("uRelaxFile", c->uRelaxFile); S("vRelaxFile", c->vRelaxFile); S("hRelaxFile", c->hRelaxFile);
+    S("eRelaxFile", c->eRelaxFile); S("uTimeFile", c->uTimeFile); S("vTimeFile", c->vTimeFile);
+    S("hTimeFile", c->hTimeFile); S("eTimeFile", c->eTimeFile); S("wDiaFile", c->wDiaFile);
+    S("diaViscUFile", c->diaViscUFile); S("diaViscVFile", c->diaViscVFile);
+    U("wDiaNrecs", c->wDiaNrecs); R("wDiaPeriod", c->wDiaPeriod); R("convDiff", c->convDiff);
     else if (!strcmp(key, "pressureSolver")) { c->solver = (uint)strtoul(val, NULL, 10); solverGiven = true; }
     else if (!strcmp(key, "useTracer") || !strcmp(key, "useBuoyancy")
              || !strcmp(key, "windFeedback") || !strcmp(key, "useRandomForcing") || !strcmp(key, "useTrad")) {
       if (atoi(val) != 0) flag_unsupported(c, key);
     }
-    else if (!strcmp(key, "FbaroXFile") || !strcmp(key, "FbaroYFile") || !strcmp(key, "wDiaFile")
-             || !strcmp(key, "uTimeFile") || !strcmp(key, "vTimeFile") || !strcmp(key, "hTimeFile")
-             || !strcmp(key, "eTimeFile") || !strcmp(key, "diaViscUFile") || !strcmp(key, "diaViscVFile")
-             || !strcmp(key, "bFluxFile") || !strcmp(key, "diaDiffFile")) {
+    else if (!strcmp(key, "FbaroXFile") || !strcmp(key, "FbaroYFile")) {
       flag_unsupported(c, key);
     }
     else if (!strcmp(key, "savefreqEZ")) c->savefreqEZ = strtod(val, NULL);
     else if (!strcmp(key, "savefreqAvg")) c->savefreqAvg = strtod(val, NULL);
-    else if (!strcmp(key, "convDiff")
-             || !strcmp(key, "savefreqUMom") || !strcmp(key, "savefreqVMom") || !strcmp(key, "savefreqThic")
+    else if (!strcmp(key, "savefreqUMom") || !strcmp(key, "savefreqVMom") || !strcmp(key, "savefreqThic")
              || !strcmp(key, "savefreqEnergy") || !strcmp(key, "savefreqTracer")) {
       if (strtod(val, NULL) != 0) flag_unsupported(c, key);
     }
@@ -122,6 +124,11 @@ bool or_config_read(const char *fname, or_config *c, FILE *err)
   /* use_MG <-> pressureSolver mapping (config.c:289-297). */
   if (!solverGiven) c->solver = useMG ? OR_SOLVER_MG : OR_SOLVER_SOR;
   c->useWind = (c->tauxFile[0] != '\0') || (c->tauyFile[0] != '\0');
+  /* config.c:305-312 (no tracer: bTimeFile, bFluxFile and diaDiffFile act only on tracers) */
+  c->useRelax = c->uTimeFile[0] || c->vTimeFile[0] || c->hTimeFile[0] || c->eTimeFile[0];
+  c->useWDia = c->useRelax || c->wDiaFile[0];
+  c->useDiaVisc = c->diaViscUFile[0] || c->diaViscVFile[0] || (c->convDiff != 0);
+  if (c->wDiaNrecs == 0 || c->convDiff < 0) flag_unsupported(c, "wDiaNrecs / convDiff");
   /* Features outside the restated path. */
   if (c->thkScheme > OR_THK_KT00) flag_unsupported(c, "thicknessScheme");
   if (c->hsml > 0) flag_unsupported(c, "hsml");

@@ -265,14 +265,21 @@ bool or_model_init(or_model *m, const char *infile, FILE *err)
   WL(hFsurf_w); WL(hFsurf_s); WL(hFbot_w); WL(hFbot_s);
   WL(hhq); WL(zeta); WL(qq); WL(al); WL(be); WL(ga); WL(de); WL(ep); WL(ph); WL(la); WL(mu);
   WL(pp); WL(KE); WL(MM); WL(huu); WL(hvv);
-#undef WL
   m->eta = alloc_layers(L + 1, nx, ny, g);
+  WL(uRelax); WL(vRelax); WL(hRelax); WL(eRelax); WL(uTime); WL(vTime); WL(eTime); WL(nu_u); WL(nu_v);
+  m->wdia = alloc_layers(L + 1, nx, ny, g);
+  m->wdia_u = alloc_layers(L + 1, nx, ny, g);
+  m->wdia_v = alloc_layers(L + 1, nx, ny, g);
+  m->Fdia_u = alloc_layers(L + 1, nx, ny, g);
+  m->Fdia_v = alloc_layers(L + 1, nx, ny, g);
+  m->wdia_ff = alloc_layers(c->wDiaNrecs * (L + 1), nx, ny, g);
 #define W2(f) or_field_alloc(&m->f, nx, ny, g)
   W2(hhs_w); W2(hhb_w); W2(hhs_west); W2(hhs_south); W2(hhb_west); W2(hhb_south);
   W2(Hc); W2(Hw); W2(Hs); W2(taux_w); W2(tauy_w); W2(usq_bot); W2(vsq_bot); W2(uabs_bot);
   W2(DD_T); W2(DD_S); W2(DD4_T); W2(DD4_S); W2(A2_h); W2(A2_q); W2(A4s_h); W2(A4s_q); W2(thk_x); W2(thk_y);
-  W2(UU4); W2(VV4); W2(pi); W2(rhs);
+  W2(UU4); W2(VV4); W2(pi); W2(rhs); W2(hTime);
 #undef W2
+#undef WL
   m->gg = calloc(L, sizeof(real));
   m->geff = calloc(L, sizeof(real));
   for (k = 0; k < L; k++) m->gg[k] = 9.81;
@@ -312,6 +319,40 @@ bool or_model_init(or_model *m, const char *infile, FILE *err)
       || !read_into(m, c->hInit, m->h, L, err) || !read_into(m, c->tauxFile, &m->taux, 1, err)
       || !read_into(m, c->tauyFile, &m->tauy, 1, err))
     return false;
+  /* restoring and diapycnal inputs (state.c:1027-1041, without tracers) */
+  if (!read_into(m, c->uRelaxFile, m->uRelax, L, err) || !read_into(m, c->vRelaxFile, m->vRelax, L, err)
+      || !read_into(m, c->hRelaxFile, m->hRelax, L, err) || !read_into(m, c->eRelaxFile, m->eRelax, L, err)
+      || !read_into(m, c->uTimeFile, m->uTime, L, err) || !read_into(m, c->vTimeFile, m->vTime, L, err)
+      || !read_into(m, c->hTimeFile, &m->hTime, 1, err) || !read_into(m, c->eTimeFile, m->eTime, L, err)
+      || !read_into(m, c->diaViscUFile, m->nu_u, L, err) || !read_into(m, c->diaViscVFile, m->nu_v, L, err))
+    return false;
+  if (c->wDiaFile[0]) {
+    /* readMatrix4 [wDiaNrecs][Nlay+1][Nx][Ny], records fastest (defs.c:327-420);
+     * field r*(L+1)+k holds record r of interface k (state.c:815-870) */
+    uint R = c->wDiaNrecs, r;
+    or_field *tmp = alloc_layers(R * (L + 1), nx, ny, g);
+    if (!read_into(m, c->wDiaFile, tmp, R * (L + 1), err)) return false;
+    for (r = 0; r < R; r++)
+      for (k = 0; k <= L; k++) {
+        or_field t = m->wdia_ff[r * (L + 1) + k];
+        m->wdia_ff[r * (L + 1) + k] = tmp[k * R + r];
+        tmp[k * R + r] = t;
+      }
+    free_layers(tmp, R * (L + 1));
+  }
+  for (k = 0; k < c->wDiaNrecs * (L + 1); k++) or_fill(&m->wdia_ff[k], m, OR_HALO_SCALAR);
+  for (k = 0; k < L; k++) {
+    or_fill(&m->uRelax[k], m, OR_HALO_U);
+    or_fill(&m->vRelax[k], m, OR_HALO_V);
+    or_fill(&m->hRelax[k], m, OR_HALO_SCALAR);
+    or_fill(&m->eRelax[k], m, OR_HALO_SCALAR);
+    or_fill(&m->uTime[k], m, OR_HALO_U);
+    or_fill(&m->vTime[k], m, OR_HALO_V);
+    or_fill(&m->eTime[k], m, OR_HALO_SCALAR);
+    or_fill(&m->nu_u[k], m, OR_HALO_U);
+    or_fill(&m->nu_v[k], m, OR_HALO_V);
+  }
+  or_fill(&m->hTime, m, OR_HALO_SCALAR);
 
   fill_prognostic(m, m->u, m->v, m->h);
   or_fill(&m->hhb, m, OR_HALO_SCALAR);
@@ -377,6 +418,13 @@ void or_model_free(or_model *m)
   or_field_free(&m->thk_x); or_field_free(&m->thk_y);
   or_field_free(&m->A2_h); or_field_free(&m->A2_q); or_field_free(&m->A4s_h); or_field_free(&m->A4s_q);
   or_field_free(&m->UU4); or_field_free(&m->VV4); or_field_free(&m->pi); or_field_free(&m->rhs);
+  free_layers(m->uRelax, L); free_layers(m->vRelax, L); free_layers(m->hRelax, L); free_layers(m->eRelax, L);
+  free_layers(m->uTime, L); free_layers(m->vTime, L); free_layers(m->eTime, L);
+  free_layers(m->nu_u, L); free_layers(m->nu_v, L);
+  free_layers(m->wdia, L + 1); free_layers(m->wdia_u, L + 1); free_layers(m->wdia_v, L + 1);
+  free_layers(m->Fdia_u, L + 1); free_layers(m->Fdia_v, L + 1);
+  free_layers(m->wdia_ff, m->cfg.wDiaNrecs * (L + 1));
+  or_field_free(&m->hTime);
   free(m->gg);
   free(m->geff);
   or_mg_free();
@@ -897,14 +945,13 @@ static void calc_thickness(or_model *m)
 {
   uint k, i, j, g = m->g;
   const real dx = m->dx, dy = m->dy;
-  const real wk = 0, wk1 = 0;
   for (k = 0; k < m->L; k++) {
     or_field *huu = &m->huu[k], *hvv = &m->hvv[k];
     or_field_zero(&m->dth[k]);
     for (i = g; i < g + m->cfg.Nx; i++)
       for (j = g; j < g + m->cfg.Ny; j++) {
         real adv = (OF(huu, i, j) - OF(huu, i + 1, j)) / dx + (OF(hvv, i, j) - OF(hvv, i, j + 1)) / dy;
-        real rel = -(wk - wk1);
+        real rel = -(OF(&m->wdia[k], i, j) - OF(&m->wdia[k + 1], i, j));
         OF(&m->dth[k], i, j) = adv + rel;
       }
     or_fill(&m->dth[k], m, OR_HALO_SCALAR);
@@ -913,15 +960,144 @@ static void calc_thickness(or_model *m)
 
 /* tendency.c:49-75 (restoring, diapycnal and tracer stages are no-ops
  * for the restated configurations). */
+/* pawsim_restoring_calc_wdia (restoring.c:29-166): interface velocity from
+ * the prescribed records (interpolated in time) plus thickness and
+ * interface-height relaxation, and its averages to u and v points. */
+static void calc_wdia(or_model *m)
+{
+  uint i, j, g = m->g, L = m->L, nx = m->cfg.Nx, ny = m->cfg.Ny;
+  int k;
+  for (k = 0; k <= (int)L; k++) or_field_zero(&m->wdia[k]);
+  if (!m->cfg.useWDia) return;
+  if (m->cfg.wDiaFile[0]) {
+    uint n0 = 0, n1 = 0;
+    real w0 = 1, w1 = 0;
+    if (m->cfg.wDiaNrecs > 1 && m->cfg.wDiaPeriod > 0) {
+      real nflt = (fmod(m->t, m->cfg.wDiaPeriod) / m->cfg.wDiaPeriod) * m->cfg.wDiaNrecs;
+      uint nm1 = (uint)floor(nflt), np1 = (uint)ceil(nflt);
+      if (np1 == nm1) np1 = nm1 + 1;
+      n0 = nm1;
+      n1 = np1 % m->cfg.wDiaNrecs;
+      w0 = np1 - nflt;
+      w1 = nflt - nm1;
+    }
+    for (k = 0; k <= (int)L; k++) {
+      const or_field *f0 = &m->wdia_ff[n0 * (L + 1) + (uint)k], *f1 = &m->wdia_ff[n1 * (L + 1) + (uint)k];
+      for (i = 0; i < nx; i++)
+        for (j = 0; j < ny; j++)
+          OF(&m->wdia[k], i + g, j + g) = w0 * OF(f0, i + g, j + g) + w1 * OF(f1, i + g, j + g);
+    }
+  }
+  if (!m->cfg.useRelax)
+    for (k = 0; k <= (int)L; k++) or_fill(&m->wdia[k], m, OR_HALO_SCALAR);
+  if (m->cfg.useRelax)
+    for (i = 0; i < nx; i++)
+      for (j = 0; j < ny; j++) {
+        uint il = i + g, jl = j + g;
+        real wh = 0;  /* hTime is one field for all layers: the sum runs upward from the bottom */
+        for (k = (int)L - 1; k >= 0; k--) {
+          if (OF(&m->hTime, il, jl) > 0) {
+            wh += (OF(&m->hw[k], il, jl) - OF(&m->hRelax[k], il, jl)) / OF(&m->hTime, il, jl);
+            OF(&m->wdia[k], il, jl) += wh;
+          }
+          if (OF(&m->eTime[k], il, jl) > 0)
+            OF(&m->wdia[k], il, jl) += (OF(&m->eta[k], il, jl) - OF(&m->eRelax[k], il, jl)) / OF(&m->eTime[k], il, jl);
+        }
+      }
+  for (k = 0; k <= (int)L; k++) or_fill(&m->wdia[k], m, OR_HALO_SCALAR);
+  for (k = 0; k <= (int)L; k++) {
+    for (i = 0; i < nx; i++)
+      for (j = 0; j < ny; j++) {
+        uint il = i + g, jl = j + g;
+        OF(&m->wdia_u[k], il, jl) = 0.5 * (OF(&m->wdia[k], il, jl) + OF(&m->wdia[k], il - 1, jl));
+        OF(&m->wdia_v[k], il, jl) = 0.5 * (OF(&m->wdia[k], il, jl) + OF(&m->wdia[k], il, jl - 1));
+      }
+    or_fill(&m->wdia_u[k], m, OR_HALO_U);
+    or_fill(&m->wdia_v[k], m, OR_HALO_V);
+  }
+}
+
+/* pawsim_diapycnal_apply_momentum (diapycnal.c:72-133, 203-291) without
+ * tracers: upwind vertical advection of momentum by wdia and explicit
+ * diapycnal viscosity.  Without active buoyancy the interface reduced
+ * gravity is gg >= 0, so convDiff never enhances the viscosity; it only
+ * switches it on. */
+static void diapycnal_momentum(or_model *m)
+{
+  uint k, i, j, g = m->g, L = m->L, nx = m->cfg.Nx, ny = m->cfg.Ny;
+  for (k = 0; k <= L; k++) {
+    or_field_zero(&m->Fdia_u[k]);
+    or_field_zero(&m->Fdia_v[k]);
+  }
+  if (m->cfg.useDiaVisc && L >= 2)
+    for (k = 1; k < L; k++)
+      for (i = 0; i < nx; i++)
+        for (j = 0; j < ny; j++) {
+          uint il = i + g, jl = j + g;
+          real dz_u = 0.5 * (OF(&m->hwest[k], il, jl) + OF(&m->hwest[k - 1], il, jl));
+          real dz_v = 0.5 * (OF(&m->hsouth[k], il, jl) + OF(&m->hsouth[k - 1], il, jl));
+          real nu_u = OF(&m->nu_u[k], il, jl), nu_v = OF(&m->nu_v[k], il, jl);
+          if (dz_u != 0) OF(&m->Fdia_u[k], il, jl) = nu_u * (OF(&m->uw[k - 1], il, jl) - OF(&m->uw[k], il, jl)) / dz_u;
+          if (dz_v != 0) OF(&m->Fdia_v[k], il, jl) = nu_v * (OF(&m->vw[k - 1], il, jl) - OF(&m->vw[k], il, jl)) / dz_v;
+        }
+  if (!m->cfg.useWDia && !m->cfg.useDiaVisc) return;
+  for (k = 0; k < L; k++) {
+    for (i = 0; i < nx; i++)
+      for (j = 0; j < ny; j++) {
+        uint il = i + g, jl = j + g;
+        real hwest = OF(&m->hwest[k], il, jl), hsouth = OF(&m->hsouth[k], il, jl);
+        real uk = OF(&m->uw[k], il, jl), vk = OF(&m->vw[k], il, jl);
+        if (m->cfg.useWDia) {
+          real wu0 = OF(&m->wdia_u[k], il, jl), wu1 = OF(&m->wdia_u[k + 1], il, jl);
+          real wv0 = OF(&m->wdia_v[k], il, jl), wv1 = OF(&m->wdia_v[k + 1], il, jl);
+          real u_p = (k == 0) ? 0 : OF(&m->uw[k - 1], il, jl), u_m = (k == L - 1) ? 0 : OF(&m->uw[k + 1], il, jl);
+          real v_p = (k == 0) ? 0 : OF(&m->vw[k - 1], il, jl), v_m = (k == L - 1) ? 0 : OF(&m->vw[k + 1], il, jl);
+          real du_p = (wu0 > 0) ? 0 : u_p - uk, du_m = (wu1 > 0) ? uk - u_m : 0;
+          real dv_p = (wv0 > 0) ? 0 : v_p - vk, dv_m = (wv1 > 0) ? vk - v_m : 0;
+          if (hwest != 0) OF(&m->dtu[k], il, jl) += -(wu0 * du_p + wu1 * du_m) / hwest;
+          if (hsouth != 0) OF(&m->dtv[k], il, jl) += -(wv0 * dv_p + wv1 * dv_m) / hsouth;
+        }
+        if (m->cfg.useDiaVisc) {
+          if (hwest != 0) OF(&m->dtu[k], il, jl) += (OF(&m->Fdia_u[k], il, jl) - OF(&m->Fdia_u[k + 1], il, jl)) / hwest;
+          if (hsouth != 0) OF(&m->dtv[k], il, jl) += (OF(&m->Fdia_v[k], il, jl) - OF(&m->Fdia_v[k + 1], il, jl)) / hsouth;
+        }
+      }
+    or_fill(&m->dtu[k], m, OR_HALO_U);
+    or_fill(&m->dtv[k], m, OR_HALO_V);
+  }
+}
+
+/* pawsim_restoring_apply_momentum (restoring.c:172-220): Newtonian u, v
+ * restoring where the time scale is positive. */
+static void restoring_momentum(or_model *m)
+{
+  uint k, i, j, g = m->g;
+  if (!m->cfg.useRelax) return;
+  for (k = 0; k < m->L; k++) {
+    for (i = g; i < g + m->cfg.Nx; i++)
+      for (j = g; j < g + m->cfg.Ny; j++) {
+        if (OF(&m->uTime[k], i, j) > 0)
+          OF(&m->dtu[k], i, j) += -(OF(&m->uw[k], i, j) - OF(&m->uRelax[k], i, j)) / OF(&m->uTime[k], i, j);
+        if (OF(&m->vTime[k], i, j) > 0)
+          OF(&m->dtv[k], i, j) += -(OF(&m->vw[k], i, j) - OF(&m->vRelax[k], i, j)) / OF(&m->vTime[k], i, j);
+      }
+    or_fill(&m->dtu[k], m, OR_HALO_U);
+    or_fill(&m->dtv[k], m, OR_HALO_V);
+  }
+}
+
 void or_tendency_core(or_model *m)
 {
   calc_face_thickness(m);
   calc_eta(m);
+  calc_wdia(m);
   calc_pv(m);
   calc_pv_coefficients(m);
   calc_bernoulli(m);
   calc_momentum(m);
+  diapycnal_momentum(m);
   forcing_apply(m);
+  restoring_momentum(m);
   calc_thickness(m);
 }
 

@@ -52,11 +52,21 @@ class PgConfig(ctypes.Structure):
         ("pressureTiming", ctypes.c_int32), ("mg_nu1", ctypes.c_uint32), ("mg_nu2", ctypes.c_uint32),
         ("tendency_kernel", ctypes.c_int32), ("mixed_precision", ctypes.c_int32), ("pi_guess", ctypes.c_uint32),
         ("KT00_sigma", ctypes.c_double), ("A2smag", ctypes.c_double), ("A4smag", ctypes.c_double),
+        ("wDiaNrecs", ctypes.c_uint32), ("wDiaPeriod", ctypes.c_double), ("convDiff", ctypes.c_double),
     ]
 
 
+# pg_static fields after the geometry and wind, and the parameter-file key
+# of the input each holds (restoring and diapycnal inputs, state.c:1027-1041)
+STATIC_FILES = (("uRelax", "uRelaxFile"), ("vRelax", "vRelaxFile"), ("hRelax", "hRelaxFile"),
+                ("eRelax", "eRelaxFile"), ("uTime", "uTimeFile"), ("vTime", "vTimeFile"), ("eTime", "eTimeFile"),
+                ("hTime", "hTimeFile"), ("wDia", "wDiaFile"), ("nu_u_dia", "diaViscUFile"),
+                ("nu_v_dia", "diaViscVFile"))
+
+
 class PgStatic(ctypes.Structure):
-    _fields_ = [(n, ctypes.POINTER(ctypes.c_double)) for n in ("hhb", "hhs", "omega_z", "gg", "taux", "tauy")]
+    _fields_ = [(n, ctypes.POINTER(ctypes.c_double))
+                for n in ("hhb", "hhs", "omega_z", "gg", "taux", "tauy") + tuple(f for f, _ in STATIC_FILES)]
 
 
 class PgEnsembleStats(ctypes.Structure):
@@ -169,7 +179,7 @@ class Model:
     """One device model (a pg_handle).  Arrays are [Nlay, Nx, Ny] float64."""
 
     def __init__(self, params: dict, hhb, omegaz, gg, hhs=None, taux=None, tauy=None,
-                 solver: Optional[int] = None):
+                 solver: Optional[int] = None, files: Optional[dict] = None):
         self.lib = load_library()
         p = dict(params)
         c = PgConfig()
@@ -207,9 +217,14 @@ class Model:
         c.KT00_sigma = float(p.get("KT00_sigma", 1.4))  # config.c:73
         c.A2smag = float(p.get("A2smag", 0.0))
         c.A4smag = float(p.get("A4smag", 0.0))
+        c.wDiaNrecs = int(p.get("wDiaNrecs", 1))
+        c.wDiaPeriod = float(p.get("wDiaPeriod", 0.0))
+        c.convDiff = float(p.get("convDiff", 0.0))
         self.cfg = c
         self.shape = (c.Nlay, c.Nx, c.Ny)
-        self._keep = [_c64(hhb), _c64(hhs), _c64(omegaz), _c64(gg), _c64(taux), _c64(tauy)]
+        files = files or {}
+        self._keep = [_c64(hhb), _c64(hhs), _c64(omegaz), _c64(gg), _c64(taux), _c64(tauy)] + [
+            _c64(files.get(key)) for _, key in STATIC_FILES]
         st = PgStatic(*[_dptr(a) for a in self._keep])
         h = ctypes.c_void_p()
         rc = self.lib.pg_create(ctypes.byref(c), ctypes.byref(st), ctypes.byref(h))
@@ -222,7 +237,7 @@ class Model:
         params = dict(setup.params)
         params.update(overrides)
         m = cls(params, setup.hhb, setup.omegaz, setup.gg, hhs=setup.hhs, taux=setup.taux, tauy=setup.tauy,
-                solver=solver)
+                solver=solver, files=getattr(setup, "files", None))
         L, X, Y = m.shape
         z = np.zeros((L, X, Y))
         m.upload_state(setup.u if setup.u is not None else z, setup.v if setup.v is not None else z, setup.h)

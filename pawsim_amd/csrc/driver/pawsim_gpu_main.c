@@ -37,7 +37,15 @@ typedef struct drv_config {
   char dir[512];
   char hInit[256], uInit[256], vInit[256], hbFile[256], hsFile[256], omFile[256], gFile[256];
   char tauxFile[256], tauyFile[256];
+  /* restoring and diapycnal inputs (config.c:267-281), in pg_static order */
+  char rsFile[11][256];
 } drv_config;
+
+/* Parameter keys of drv_config.rsFile, the pg_static fields they fill:
+ * uRelax vRelax hRelax eRelax uTime vTime eTime hTime wDia nu_u_dia nu_v_dia. */
+static const char *const RS_KEYS[11] = {"uRelaxFile", "vRelaxFile", "hRelaxFile", "eRelaxFile", "uTimeFile",
+                                        "vTimeFile", "eTimeFile", "hTimeFile", "wDiaFile", "diaViscUFile",
+                                        "diaViscVFile"};
 
 static double now_s(void)
 {
@@ -59,6 +67,7 @@ static void defaults(drv_config *c)
   c->pg.momentumScheme = 3;
   c->pg.thicknessScheme = 0;
   c->pg.KT00_sigma = 1.4;
+  c->pg.wDiaNrecs = 1;
   c->pg.pi_tol = 1e-10;
   c->pg.SOR_rp = 1.0;
   c->pg.omega_MG = 2.0 / 3.0;
@@ -67,6 +76,13 @@ static void defaults(drv_config *c)
   c->tauNrecs = 1;
   c->useRL = true;
   strcpy(c->dir, ".");
+}
+
+static int rs_key(const char *key)
+{
+  for (int q = 0; q < 11; q++)
+    if (!strcmp(key, RS_KEYS[q])) return q;
+  return -1;
 }
 
 static void copy_name(char *dst, const char *src)
@@ -148,23 +164,26 @@ static bool read_config(const char *fname, drv_config *c)
     else if (!strcmp(key, "tauxFile")) copy_name(c->tauxFile, val);
     else if (!strcmp(key, "tauyFile")) copy_name(c->tauyFile, val);
     else if (!strcmp(key, "restart")) c->restart = iv != 0;
+    else if (rs_key(key) >= 0) copy_name(c->rsFile[rs_key(key)], val);
     else if (!strcmp(key, "startIdx")) c->startIdx = (unsigned)uv;
     else if (!strcmp(key, "saveCheckpoint")) c->saveCheckpoint = iv != 0;
     else if (!strcmp(key, "restartFromCheckpoint")) c->restartCheckpoint = iv != 0;
     else if (!strcmp(key, "useTracer") || !strcmp(key, "useBuoyancy")
              || !strcmp(key, "windFeedback") || !strcmp(key, "useRandomForcing") || !strcmp(key, "useTrad")) {
       if (iv != 0) note_unsupported(c, key);
-    } else if (!strcmp(key, "FbaroXFile") || !strcmp(key, "FbaroYFile") || !strcmp(key, "wDiaFile")
-               || !strcmp(key, "uTimeFile") || !strcmp(key, "vTimeFile") || !strcmp(key, "hTimeFile")
-               || !strcmp(key, "eTimeFile") || !strcmp(key, "diaViscUFile") || !strcmp(key, "diaViscVFile")
-               || !strcmp(key, "bFluxFile") || !strcmp(key, "diaDiffFile")) {
+    } else if (!strcmp(key, "FbaroXFile") || !strcmp(key, "FbaroYFile")) {
       note_unsupported(c, key);
+    } else if (!strcmp(key, "wDiaNrecs")) {
+      c->pg.wDiaNrecs = (uint32_t)uv;
+    } else if (!strcmp(key, "wDiaPeriod")) {
+      c->pg.wDiaPeriod = dv;
+    } else if (!strcmp(key, "convDiff")) {
+      c->pg.convDiff = dv;
     } else if (!strcmp(key, "savefreqEZ")) {
       c->savefreqEZ = dv;
     } else if (!strcmp(key, "savefreqAvg")) {
       c->savefreqAvg = dv;
-    } else if (!strcmp(key, "convDiff")
-               || !strcmp(key, "savefreqUMom") || !strcmp(key, "savefreqVMom") || !strcmp(key, "savefreqThic")
+    } else if (!strcmp(key, "savefreqUMom") || !strcmp(key, "savefreqVMom") || !strcmp(key, "savefreqThic")
                || !strcmp(key, "savefreqEnergy") || !strcmp(key, "savefreqTracer")) {
       if (dv != 0) note_unsupported(c, key);
     }
@@ -455,6 +474,7 @@ int main(int argc, char **argv)
 {
   model M;
   pg_static st;
+  memset(&st, 0, sizeof st);
   unsigned k, tcount;
   int it;
   double *hhb, *hhs = NULL, *om, *gg, *taux = NULL, *tauy = NULL, t_all, t_loop = 0;
@@ -513,6 +533,36 @@ int main(int argc, char **argv)
   M.u = c->uInit[0] ? read_raw(c, c->uInit, Nx, Ny, L) : calloc(L * N, sizeof(double));
   M.v = c->vInit[0] ? read_raw(c, c->vInit, Nx, Ny, L) : calloc(L * N, sizeof(double));
   M.hh = c->hInit[0] ? read_raw(c, c->hInit, Nx, Ny, L) : calloc(L * N, sizeof(double));
+  /* restoring and diapycnal inputs (state.c:1027-1041) */
+  double *rs[11] = {NULL};
+  for (int q = 0; q < 11; q++) {
+    if (!c->rsFile[q][0]) continue;
+    if (q == 7) {  /* hTime: one field */
+      rs[q] = read_raw(c, c->rsFile[q], Nx, Ny, 1);
+    } else if (q == 8) {  /* wDia: readMatrix4 [nrecs][Nlay+1][Nx][Ny], records fastest (defs.c:327-420) */
+      const unsigned R = c->pg.wDiaNrecs ? c->pg.wDiaNrecs : 1, K = L + 1;
+      double *a = read_raw(c, c->rsFile[q], Nx, Ny, R * K);
+      if (a && (rs[q] = malloc((size_t)R * K * N * sizeof(double))))
+        for (unsigned r = 0; r < R; r++)
+          for (unsigned kk = 0; kk < K; kk++)
+            memcpy(rs[q] + ((size_t)r * K + kk) * N, a + ((size_t)kk * R + r) * N, N * sizeof(double));
+      free(a);
+    } else {
+      rs[q] = read_raw(c, c->rsFile[q], Nx, Ny, L);
+    }
+    if (!rs[q]) return 1;
+  }
+  st.uRelax = rs[0];
+  st.vRelax = rs[1];
+  st.hRelax = rs[2];
+  st.eRelax = rs[3];
+  st.uTime = rs[4];
+  st.vTime = rs[5];
+  st.eTime = rs[6];
+  st.hTime = rs[7];
+  st.wDia = rs[8];
+  st.nu_u_dia = rs[9];
+  st.nu_v_dia = rs[10];
   M.pi = calloc(N, sizeof(double));
   M.zero = calloc(N, sizeof(double));
   if (!hhb || !om || !M.u || !M.v || !M.hh || (c->hsFile[0] && !hhs)) return 1;
@@ -602,5 +652,6 @@ int main(int argc, char **argv)
   pg_destroy(M.h);
   free(hhb); free(hhs); free(om); free(gg); free(taux); free(tauy);
   free(M.u); free(M.v); free(M.hh); free(M.pi); free(M.zero); free(M.avg);
+  for (int q = 0; q < 11; q++) free(rs[q]);
   return 0;
 }

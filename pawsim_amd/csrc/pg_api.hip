@@ -122,9 +122,10 @@ int validate(const pg_config* c, std::string& why)
   if (c->pi_guess && c->pressureSolver != PG_SOLVER_GPU_MG && c->pressureSolver != PG_SOLVER_GPU_MG_MIXED)
     return why = "pi_guess needs a GPU multigrid solver (3 or 4)", PG_EARG;
   if (c->mixed_precision && (c->A4 > 0 || c->Nlay > 8 || c->Nx < 8 || c->Ny < 8 || c->tendency_kernel != 0
-                             || c->thicknessScheme != 0 || c->A2smag > 0 || c->A4smag > 0))
+                             || c->thicknessScheme != 0 || c->A2smag > 0 || c->A4smag > 0 || c->convDiff != 0))
     return why = "mixed_precision needs A4 = 0, Nlay <= 8, Nx and Ny >= 8, the tiled tendency kernel, "
-                 "the AL81 thickness scheme and no Smagorinsky viscosity", PG_EARG;
+                 "the AL81 thickness scheme, no Smagorinsky viscosity and no restoring or diapycnal terms", PG_EARG;
+  if (c->convDiff < 0 || c->wDiaPeriod < 0) return why = "convDiff and wDiaPeriod must be non-negative", PG_EARG;
   return PG_OK;
 }
 
@@ -188,6 +189,11 @@ int run_solver(pg_handle* h, uint32_t solver, pg_solve_stats* st)
 }
 
 }  // namespace
+
+int pg_upload_owned(pg_handle* h, double* dst, const double* src, int ld, std::vector<double>& stage)
+{
+  return upload_owned(h, dst, src, ld, stage);
+}
 
 int pg_ensure_solver(pg_handle* h, uint32_t solver) { return ensure_solver(h, solver); }
 hipError_t pg_ensure_halos(pg_handle* h) { return ensure_halos(h); }
@@ -325,6 +331,10 @@ static int create_body(const pg_config* cfg, const pg_static* st, pg_handle** ou
   PG_CHECK(h, pg_fill(h, h->taux, 1, PG_HALO_SCALAR));
   PG_CHECK(h, pg_fill(h, h->tauy, 1, PG_HALO_SCALAR));
   PG_CHECK(h, pg_launch_static_geometry(h));
+  rc = pg_restoring_setup(h, st);
+  if (rc) return rc;
+  if (h->mixed && (P.useRelax || P.useWDia || P.useDiaVisc))
+    return fail(h, PG_EARG, "mixed_precision does not serve restoring or diapycnal terms");
   rc = ensure_solver(h, c.pressureSolver);
   if (rc) return rc;
   PG_CHECK(h, hipStreamSynchronize(h->stream));
@@ -351,6 +361,7 @@ void pg_destroy(pg_handle* h)
   if (h->ctl_host) (void)hipHostFree(h->ctl_host);
   if (h->mapped_host) (void)hipHostFree(h->mapped_host);
   if (h->avg) (void)hipFree(h->avg);
+  pg_restoring_free(h);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
   if (h->ev_join) (void)hipEventDestroy(h->ev_join);

@@ -132,14 +132,16 @@ __global__ __launch_bounds__(64) void k_energy_finish(const double* part, int nb
 // diagnostics.c:307-372): per owned cell, with the reference's expressions
 // and association, so the sums equal the serial ones bit for bit.  Sum
 // order in avg: per layer U V H M W HU HV HUU HVV HUV, then W of layer L,
-// then P; each an Nx x Ny array (j contiguous).  W stays zero (no
-// diapycnal velocity on this path).
+// then P; each an Nx x Ny array (j contiguous).  W is the interface
+// velocity the last tendency evaluation built (zero without restoring or
+// prescribed records), as in the reference, whose averages reload the state
+// but not wdia.
 // ---------------------------------------------------------------------------
 constexpr int AV_PER_LAYER = 10, AV_LMAX = 16;
 
 __global__ __launch_bounds__(256) void k_avg_accumulate(PgGrid G, PgPhys P, const double* u, const double* v,
                                                         const double* hh, const double* hhb, const double* pi,
-                                                        double* avg, double dtfac, int useRL)
+                                                        const double* wdia, double* avg, double dtfac, int useRL)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
   if (j >= G.ny) return;
@@ -170,12 +172,14 @@ __global__ __launch_bounds__(256) void k_avg_accumulate(PgGrid G, PgPhys P, cons
     a[1 * N] += dtfac * vk;
     a[2 * N] += dtfac * hk;
     a[3 * N] += dtfac * (mm + (useRL ? pi[c] : 0));
+    a[4 * N] += dtfac * (wdia ? wdia[pg_lay(G, k) + c] : 0.0);
     a[5 * N] += dtfac * hw * uk;
     a[6 * N] += dtfac * hs * vk;
     a[7 * N] += dtfac * hw * uk * uk;
     a[8 * N] += dtfac * hs * vk * vk;
     a[9 * N] += dtfac * hq * uq * vq;
   }
+  avg[(size_t)AV_PER_LAYER * L * N + q] += dtfac * (wdia ? wdia[pg_lay(G, L) + c] : 0.0);
   avg[((size_t)AV_PER_LAYER * L + 1) * N + q] += dtfac * pi[c];
 }
 
@@ -199,7 +203,7 @@ extern "C" int pg_average_add(pg_handle* h, double weight)
   PG_CHECK(h, pg_ensure_halos(h));
   h->corner_fill_due = true;  // the averages reload the work arrays (diagnostics.c:320)
   hipLaunchKernelGGL(k_avg_accumulate, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->P, h->u, h->v, h->h,
-                     h->hhb, h->pi, h->avg, weight, 1);
+                     h->hhb, h->pi, h->rs_wdia, h->avg, weight, 1);
   PG_CHECK(h, hipGetLastError());
   return PG_OK;
 }

@@ -59,6 +59,14 @@ struct pg_handle {
   bool halo_dirty = false;             // u, v, h halos are stale (refilled before the next reader)
   bool thickness_done = false;         // the last AB update also applied the rigid-lid thickness fix
   bool corner_fill_due = true;         // u, v corner ghosts lag the reference's work copies by one fill
+  // restoring / diapycnal inputs (pg_restoring.hip), ghosted, NULL when off
+  double *rs_uRelax = nullptr, *rs_vRelax = nullptr, *rs_hRelax = nullptr, *rs_eRelax = nullptr;
+  double *rs_uTime = nullptr, *rs_vTime = nullptr, *rs_eTime = nullptr, *rs_hTime = nullptr;
+  double *rs_wdia_ff = nullptr;        // [nrecs][L+1] prescribed interface velocity records
+  double *rs_wdia = nullptr;           // [L+1] interface velocity of the last tendency evaluation
+  double *rs_nu_u = nullptr, *rs_nu_v = nullptr;
+  int wdia_nrecs = 1;
+  bool wdia_file = false;
   bool rhs_mean_pending = false;       // ctl->mean is still to be removed from rhs (by the MG gather)
   double *hhb = nullptr, *hhs = nullptr, *omz = nullptr, *taux = nullptr, *tauy = nullptr;
   double *Hc = nullptr, *Hw = nullptr, *Hs = nullptr;  // ghosted static geometry
@@ -106,6 +114,11 @@ hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind);
 hipError_t pg_fill1(pg_handle* h, double* f);
 hipError_t pg_fill_prognostic(pg_handle* h);  // u, v, h halos in one launch
 hipError_t pg_ensure_halos(pg_handle* h);     // refresh u, v, h halos if a refresh was deferred
+int pg_upload_owned(pg_handle* h, double* dst, const double* src, int ld, std::vector<double>& stage);
+int pg_restoring_setup(pg_handle* h, const pg_static* st);
+void pg_restoring_free(pg_handle* h);
+hipError_t pg_launch_wdia(pg_handle* h);     // interface velocity at h->t (restoring.c:29-166)
+PgIn pg_tendency_inputs(const pg_handle* h);
 hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh);
 hipError_t pg_launch_tendency_slot(pg_handle* h, int s);  // into history slot s, per precision mode
 hipError_t pg_widen_slot(pg_handle* h, int s);            // FP32 history slot -> FP64 slot

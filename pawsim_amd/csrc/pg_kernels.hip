@@ -700,8 +700,10 @@ hipError_t pg_widen_slot(pg_handle* h, int s)
 
 hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh)
 {
+  hipError_t e = pg_launch_wdia(h);
+  if (e != hipSuccess) return e;
   if (h->fast_tendency && pg_tendency_fast_ok(h)) return pg_launch_tendency_fast(h, du, dv, dh);
-  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
+  const PgIn I = pg_tendency_inputs(h);
   hipLaunchKernelGGL(k_tendency, owned_grid(h->G, h->G.L), dim3(TJ, TI), 0, h->stream, h->G, h->P, I, du, dv, dh);
   return hipGetLastError();
 }

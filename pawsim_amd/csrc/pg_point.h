@@ -58,6 +58,7 @@ struct PgPhys {
   double A4;               // constant biharmonic coefficient (A4c = sqrt(A4))
   double A2fac, A4fac;     // Smagorinsky factors (work.c:1542-1543)
   double kt00;             // KT00 limiter slope factor
+  int useRelax, useWDia, useDiaVisc;  // restoring, interface velocity, diapycnal viscosity (config.c:305-312)
   double gg[PG_MAXL], geff[PG_MAXL];
 };
 
@@ -86,6 +87,14 @@ struct PgIn {
   const double* omz;   // [sx*sy], q-kind filled
   const double* taux;  // [sx*sy] or NULL
   const double* tauy;
+  // restoring / diapycnal inputs (NULL when off), ghost-filled:
+  const double* wdia = nullptr;    // [L+1][sx*sy] interface velocity of this evaluation, scalar kind
+  const double* uRelax = nullptr;  // [L][sx*sy] u kind
+  const double* uTime = nullptr;   // u kind
+  const double* vRelax = nullptr;  // v kind
+  const double* vTime = nullptr;   // v kind
+  const double* nu_u = nullptr;    // [L][sx*sy] diapycnal viscosity on u points, u kind
+  const double* nu_v = nullptr;    // v kind
 };
 
 PG_HD size_t pg_at(const PgGrid& G, int i, int j) { return (size_t)i * (size_t)G.sy + (size_t)j; }
@@ -506,6 +515,45 @@ PG_HD double pg_uabs_bot(const PgGrid& G, const PgIn& I, int i, int j)
   return sqrt(0.5 * (u0 * u0 + u1 * u1) + 0.5 * (v0 * v0 + v1 * v1));
 }
 
+// Diapycnal momentum terms (diapycnal.c:72-133, 203-291) added to the u
+// (dir 0) or v (dir 1) tendency r of layer k at the owned point (i, j):
+// upwind vertical advection by the interface velocity averaged to the face,
+// then the explicit diapycnal viscosity.  Without active buoyancy the
+// interface reduced gravity is gg >= 0, so convDiff adds nothing to nu.
+PG_HD double pg_diapycnal(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j, double r,
+                          double hface, int dir)
+{
+  const double* X = dir ? I.v : I.u;
+  const int di = dir ? 0 : 1, dj = dir ? 1 : 0;
+  const bool wall_face = dir ? (G.wally && j == G.g) : (G.wallx && i == G.g);
+  auto XV = [&](int kk) { return X[pg_lay(G, kk) + pg_at(G, i, j)]; };
+  // interface velocity on the face (restoring.c:144-166); the u / v halo
+  // fill zeroes it on a wall face
+  auto wface = [&](int kk) {
+    if (wall_face) return 0.0;
+    const double* W = I.wdia + pg_lay(G, kk);
+    return 0.5 * (W[pg_at(G, i, j)] + W[pg_at(G, i - di, j - dj)]);
+  };
+  const double xk = XV(k);
+  if (P.useWDia) {
+    const double w0 = wface(k), w1 = wface(k + 1);
+    const double x_p = (k == 0) ? 0 : XV(k - 1), x_m = (k == G.L - 1) ? 0 : XV(k + 1);
+    const double d_p = (w0 > 0) ? 0 : x_p - xk, d_m = (w1 > 0) ? xk - x_m : 0;
+    if (hface != 0) r += -(w0 * d_p + w1 * d_m) / hface;
+  }
+  if (P.useDiaVisc) {
+    const double* NU = dir ? I.nu_v : I.nu_u;
+    auto F = [&](int kk) {  // flux through interface kk (zero at the surface and the bottom)
+      if (kk < 1 || kk >= G.L) return 0.0;
+      const double dz = dir ? 0.5 * (pg_hS(G, P, I, kk, i, j) + pg_hS(G, P, I, kk - 1, i, j))
+                            : 0.5 * (pg_hW(G, P, I, kk, i, j) + pg_hW(G, P, I, kk - 1, i, j));
+      return (dz != 0) ? NU[pg_lay(G, kk) + pg_at(G, i, j)] * (XV(kk - 1) - XV(kk)) / dz : 0.0;
+    };
+    if (hface != 0) r += (F(k) - F(k + 1)) / hface;
+  }
+  return r;
+}
+
 // Tendencies of layer k at the owned point (i, j) (ghosted coordinates):
 // tendency.c:49-75 stage results, i.e. work.c:1667-1855 momentum,
 // forcing.c:230-393 forcing, work.c:1880-1890 thickness.
@@ -624,7 +672,7 @@ PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, in
                 - pg_A4q(G, P, I, k, i, j) * pg_hq(G, I, k, i, j) * pg_DD4S(G, P, I, k, i, j)) / dy);
     }
     ru = gM + gK + qf + ((hwest != 0) ? (a2 + a4 + buoy) / hwest : 0);
-    // forcing (forcing.c:244-358): wind on the top layer, drag on the bottom layer
+    if (P.useWDia || P.useDiaVisc) ru = pg_diapycnal(G, P, I, k, i, j, ru, hwest, 0);
     const double fs = (k == 0) ? 1.0 : 0.0, fb = (k == G.L - 1) ? 1.0 : 0.0;
     if (P.useWind) {
       const size_t p = pg_at(G, i, j);
@@ -639,6 +687,11 @@ PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, in
     if (P.useQuad && hwest != 0) {
       const double ua = 0.5 * (pg_uabs_bot(G, I, i, j) + pg_uabs_bot(G, I, i - 1, j));
       ru -= P.quad * ua * PG_U(k, i, j) * fb / hwest;
+    }
+    // restoring (restoring.c:172-220), last of the momentum terms
+    if (P.useRelax) {
+      const size_t q = pg_lay(G, k) + pg_at(G, i, j);
+      if (I.uTime[q] > 0) ru += -(PG_U(k, i, j) - I.uRelax[q]) / I.uTime[q];
     }
   }
 
@@ -671,6 +724,7 @@ PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, in
                 - pg_A4h(G, P, I, k, i, j - 1) * PG_H(k, i, j - 1) * pg_DD4T(G, P, I, k, i, j - 1)) / dy);
     }
     rv = gM + gK + qf + ((hsouth != 0) ? (a2 + a4 + buoy) / hsouth : 0);
+    if (P.useWDia || P.useDiaVisc) rv = pg_diapycnal(G, P, I, k, i, j, rv, hsouth, 1);
     const double fs = (k == 0) ? 1.0 : 0.0, fb = (k == G.L - 1) ? 1.0 : 0.0;
     if (P.useWind) {
       const size_t p = pg_at(G, i, j);
@@ -686,10 +740,20 @@ PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, in
       const double ua = 0.5 * (pg_uabs_bot(G, I, i, j) + pg_uabs_bot(G, I, i, j - 1));
       rv -= P.quad * ua * PG_V(k, i, j) * fb / hsouth;
     }
+    if (P.useRelax) {
+      const size_t q = pg_lay(G, k) + pg_at(G, i, j);
+      if (I.vTime[q] > 0) rv += -(PG_V(k, i, j) - I.vRelax[q]) / I.vTime[q];
+    }
   }
   *du = ru;
   *dv = rv;
-  // ---- thickness (work.c:1880-1890); the -(0 - 0) diapycnal term is the
-  // identity under round-to-nearest and is omitted ----
-  *dh = (huu_00 - huu_p0) / dx + (hvv_00 - hvv_0p) / dy;
+  // ---- thickness (work.c:1880-1890); without an interface velocity the
+  // -(0 - 0) diapycnal term is the identity under round-to-nearest ----
+  const double adv = (huu_00 - huu_p0) / dx + (hvv_00 - hvv_0p) / dy;
+  if (P.useWDia) {
+    const size_t q = pg_at(G, i, j);
+    *dh = adv + -(I.wdia[pg_lay(G, k) + q] - I.wdia[pg_lay(G, k + 1) + q]);
+  } else {
+    *dh = adv;
+  }
 }

@@ -1,0 +1,178 @@
+// pg_restoring.hip -- restoring and diapycnal inputs on the device.
+//
+// Inputs (state.c:1027-1041) are uploaded once into ghosted arrays filled
+// with the halo kind of each (state.c:1063-1077): u kind for the u-point
+// targets, time scales and viscosity, v kind for the v-point ones, scalar
+// for the rest.  At every tendency evaluation one kernel builds the
+// interface velocity wdia of restoring.c:29-142 for all Nlay+1 interfaces:
+// the prescribed records interpolated in time (weights from the model time
+// on the host, restoring.c:50-92), plus thickness restoring (one time scale
+// for all layers, summed upward from the bottom) and interface-height
+// restoring.  The tendency stencils read it (pg_point.h pg_diapycnal and the
+// thickness term); the online averages read the one the last evaluation
+// built, as the reference's do.
+#include "pg_internal.h"
+
+#include <cmath>
+
+namespace {
+
+struct WdiaArgs {
+  const double *h, *hhb, *ff, *hRelax, *eRelax, *eTime, *hTime;
+  double* wdia;
+  int n0, n1, file, relax;
+  double w0, w1;
+};
+
+__global__ __launch_bounds__(256) void k_wdia(PgGrid G, WdiaArgs A)
+{
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= G.ny) return;
+  const int L = G.L;
+  const size_t p = pg_at(G, i + G.g, j + G.g);
+  // interface heights bottom-up (work.c:1173-1203), eta[k] for k < L
+  double eta[PG_MAXL + 1];
+  eta[L] = A.hhb[p];
+  for (int k = L - 1; k >= 0; k--) eta[k] = eta[k + 1] + A.h[pg_lay(G, k) + p];
+  for (int k = 0; k <= L; k++) {
+    double w = 0;
+    if (A.file) {
+      const double f0 = A.ff[((size_t)A.n0 * (L + 1) + k) * ((size_t)G.sx * G.sy) + p];
+      const double f1 = A.ff[((size_t)A.n1 * (L + 1) + k) * ((size_t)G.sx * G.sy) + p];
+      w = A.w0 * f0 + A.w1 * f1;
+    }
+    A.wdia[pg_lay(G, k) + p] = w;
+  }
+  if (!A.relax) return;
+  const double hT = A.hTime[p];
+  double wh = 0;
+  for (int k = L - 1; k >= 0; k--) {
+    double w = A.wdia[pg_lay(G, k) + p];
+    if (hT > 0) {
+      wh += (A.h[pg_lay(G, k) + p] - A.hRelax[pg_lay(G, k) + p]) / hT;
+      w += wh;
+    }
+    const double eT = A.eTime[pg_lay(G, k) + p];
+    if (eT > 0) w += (eta[k] - A.eRelax[pg_lay(G, k) + p]) / eT;
+    A.wdia[pg_lay(G, k) + p] = w;
+  }
+}
+
+}  // namespace
+
+// Upload the restoring / diapycnal inputs present in st (pg_create).
+int pg_restoring_setup(pg_handle* h, const pg_static* st)
+{
+  const PgGrid& G = h->G;
+  const pg_config& c = h->cfg;
+  PgPhys& P = h->P;
+  P.useRelax = st->uTime || st->vTime || st->hTime || st->eTime;
+  P.useWDia = P.useRelax || st->wDia;
+  P.useDiaVisc = st->nu_u_dia || st->nu_v_dia || c.convDiff != 0;
+  h->wdia_nrecs = c.wDiaNrecs ? (int)c.wDiaNrecs : 1;
+  h->wdia_file = st->wDia != nullptr;
+  if (!P.useRelax && !P.useWDia && !P.useDiaVisc) return PG_OK;
+  const size_t lf = (size_t)G.L * h->fsz, N = (size_t)G.nx * G.ny;
+  std::vector<double> stage;
+  // layered field: allocate, zero, upload the owned cells of each layer, fill
+  auto layered = [&](double** dst, const double* src, int layers, int kind) -> int {
+    const size_t bytes = (size_t)layers * h->fsz * sizeof(double);
+    PG_CHECK(h, hipMalloc(dst, bytes));
+    PG_CHECK(h, hipMemsetAsync(*dst, 0, bytes, h->stream));
+    if (src) {
+      for (int k = 0; k < layers; k++) {
+        int rc = pg_upload_owned(h, *dst + (size_t)k * h->fsz, src + k * N, G.ny, stage);
+        if (rc) return rc;
+      }
+      PG_CHECK(h, pg_fill(h, *dst, layers, kind));
+    }
+    return PG_OK;
+  };
+  int rc = PG_OK;
+  if (P.useRelax) {
+    if (!rc) rc = layered(&h->rs_uRelax, st->uRelax, G.L, PG_HALO_U);
+    if (!rc) rc = layered(&h->rs_vRelax, st->vRelax, G.L, PG_HALO_V);
+    if (!rc) rc = layered(&h->rs_hRelax, st->hRelax, G.L, PG_HALO_SCALAR);
+    if (!rc) rc = layered(&h->rs_eRelax, st->eRelax, G.L, PG_HALO_SCALAR);
+    if (!rc) rc = layered(&h->rs_uTime, st->uTime, G.L, PG_HALO_U);
+    if (!rc) rc = layered(&h->rs_vTime, st->vTime, G.L, PG_HALO_V);
+    if (!rc) rc = layered(&h->rs_eTime, st->eTime, G.L, PG_HALO_SCALAR);
+    if (!rc) rc = layered(&h->rs_hTime, st->hTime, 1, PG_HALO_SCALAR);
+  }
+  if (P.useWDia) {
+    if (!rc) rc = layered(&h->rs_wdia, nullptr, G.L + 1, PG_HALO_SCALAR);
+    if (!rc && st->wDia) {
+      // [nrecs][Nlay+1][Nx][Ny]; field r*(L+1)+k is record r of interface k (state.c:815-870)
+      rc = layered(&h->rs_wdia_ff, st->wDia, h->wdia_nrecs * (G.L + 1), PG_HALO_SCALAR);
+    }
+  }
+  if (P.useDiaVisc) {
+    if (!rc) rc = layered(&h->rs_nu_u, st->nu_u_dia, G.L, PG_HALO_U);
+    if (!rc) rc = layered(&h->rs_nu_v, st->nu_v_dia, G.L, PG_HALO_V);
+  }
+  if (rc) return rc;
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  (void)lf;
+  return PG_OK;
+}
+
+void pg_restoring_free(pg_handle* h)
+{
+  double** all[] = {&h->rs_uRelax, &h->rs_vRelax, &h->rs_hRelax, &h->rs_eRelax, &h->rs_uTime, &h->rs_vTime,
+                    &h->rs_eTime,  &h->rs_hTime,  &h->rs_wdia,   &h->rs_wdia_ff, &h->rs_nu_u, &h->rs_nu_v};
+  for (double** p : all) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+}
+
+// Interface velocity of the evaluation at the model time h->t.
+hipError_t pg_launch_wdia(pg_handle* h)
+{
+  if (!h->P.useWDia) return hipSuccess;
+  const PgGrid& G = h->G;
+  WdiaArgs A{};
+  A.h = h->h;
+  A.hhb = h->hhb;
+  A.ff = h->rs_wdia_ff;
+  A.hRelax = h->rs_hRelax;
+  A.eRelax = h->rs_eRelax;
+  A.eTime = h->rs_eTime;
+  A.hTime = h->rs_hTime;
+  A.wdia = h->rs_wdia;
+  A.file = h->wdia_file ? 1 : 0;
+  A.relax = h->P.useRelax;
+  A.n0 = A.n1 = 0;
+  A.w0 = 1;
+  A.w1 = 0;
+  const double per = h->cfg.wDiaPeriod;
+  const int R = h->wdia_nrecs;
+  if (A.file && R > 1 && per > 0) {  // restoring.c:58-71
+    const double nflt = (std::fmod(h->t, per) / per) * R;
+    const unsigned nm1 = (unsigned)std::floor(nflt);
+    unsigned np1 = (unsigned)std::ceil(nflt);
+    if (np1 == nm1) np1 = nm1 + 1;
+    A.n0 = (int)nm1;
+    A.n1 = (int)(np1 % (unsigned)R);
+    A.w0 = np1 - nflt;
+    A.w1 = nflt - nm1;
+  }
+  hipLaunchKernelGGL(k_wdia, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, A);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return pg_fill(h, h->rs_wdia, G.L + 1, PG_HALO_SCALAR);
+}
+
+// Every input of the tendency stencils.
+PgIn pg_tendency_inputs(const pg_handle* h)
+{
+  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
+  I.wdia = h->rs_wdia;
+  I.uRelax = h->rs_uRelax;
+  I.uTime = h->rs_uTime;
+  I.vRelax = h->rs_vRelax;
+  I.vTime = h->rs_vTime;
+  I.nu_u = h->rs_nu_u;
+  I.nu_v = h->rs_nu_v;
+  return I;
+}

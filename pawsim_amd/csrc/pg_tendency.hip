@@ -554,16 +554,17 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, Pg
 }  // namespace
 
 // True if the fast pair of kernels serves this configuration (no
-// biharmonic viscosity; the other thickness schemes and Smagorinsky
-// coefficients take the extended interior variant).
+// biharmonic viscosity, restoring or diapycnal terms; the other thickness
+// schemes and Smagorinsky coefficients take the extended interior variant).
 bool pg_tendency_fast_ok(const pg_handle* h)
 {
-  return !h->P.useA4 && h->G.nx >= 8 && h->G.ny >= 8 && h->G.L <= 8;
+  const PgPhys& P = h->P;
+  return !P.useA4 && !P.useRelax && !P.useWDia && !P.useDiaVisc && h->G.nx >= 8 && h->G.ny >= 8 && h->G.L <= 8;
 }
 
 hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double* dh)
 {
-  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
+  const PgIn I = pg_tendency_inputs(h);
   const PgGrid& G = h->G;
   const size_t dyn = (size_t)G.L * KH * KW * sizeof(double);
   dim3 grid((G.ny + TJ - 1) / TJ, (G.nx + TI - 1) / TI);
@@ -580,7 +581,7 @@ hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double
 template <typename T>
 static hipError_t launch_edge_forked(pg_handle* h, T* du, T* dv, T* dh, hipError_t (*inner)(pg_handle*, T*, T*, T*))
 {
-  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
+  const PgIn I = pg_tendency_inputs(h);
   const PgGrid& G = h->G;
   const int total = 5 * G.ny + (G.nx - 5) * 5;
   if (!h->edge_fork) {
@@ -608,7 +609,7 @@ hipError_t pg_launch_tendency_fast(pg_handle* h, double* du, double* dv, double*
 
 hipError_t pg_launch_tendency_inner32(pg_handle* h, float* du, float* dv, float* dh)
 {
-  PgIn I{h->u, h->v, h->h, h->hhb, h->omz, h->taux, h->tauy};
+  const PgIn I = pg_tendency_inputs(h);
   const PgGrid& G = h->G;
   const size_t dyn = (size_t)G.L * KH * KW * sizeof(double);
   dim3 grid((G.ny + TJ - 1) / TJ, (G.nx + TI - 1) / TI);

@@ -74,6 +74,11 @@ def write_3d(path: str, a: np.ndarray) -> None:
     np.ascontiguousarray(np.asarray(a, dtype=np.float64).transpose(2, 1, 0)).tofile(path)
 
 
+def write_4d(path: str, a: np.ndarray) -> None:
+    """a[r, k, i, j] -> file with r fastest, then k, x, y (readMatrix4, defs.c:327-420)."""
+    np.ascontiguousarray(np.asarray(a, dtype=np.float64).transpose(3, 2, 1, 0)).tofile(path)
+
+
 def read_3d(path: str, nl: int, nx: int, ny: int) -> np.ndarray:
     return np.fromfile(path, dtype=np.float64, count=nl * nx * ny).reshape(ny, nx, nl).transpose(2, 1, 0).copy()
 
@@ -97,6 +102,9 @@ class Setup:
     tauy: Optional[np.ndarray] = None
     name: str = "case"
     notes: Dict[str, object] = field(default_factory=dict)
+    # further input files by parameter key (e.g. "uRelaxFile"): [Nx, Ny],
+    # [Nlay, Nx, Ny] or, for wDiaFile, [wDiaNrecs, Nlay + 1, Nx, Ny]
+    files: Dict[str, np.ndarray] = field(default_factory=dict)
 
     @property
     def nx(self) -> int:
@@ -137,6 +145,10 @@ class Setup:
         if self.tauy is not None:
             write_2d(os.path.join(directory, "tauy.dat"), self.tauy)
             p["tauyFile"] = "tauy.dat"
+        for key, arr in self.files.items():
+            fname = key[:-4] + ".dat"
+            {2: write_2d, 3: write_3d, 4: write_4d}[np.ndim(arr)](os.path.join(directory, fname), arr)
+            p[key] = fname
         path = os.path.join(directory, self.name + "_in")
         with open(path, "w") as f:
             for k, val in p.items():
@@ -290,6 +302,50 @@ def periodic_box(nx=16, ny=16, nlay=2, dx=20e3, nt=10, dt=300.0, solver=0, tol=1
         u[k] = 0.1 * np.sin(2 * np.pi * yy)[None, :] * (k + 1) / nlay
         v[k] = 0.05 * np.cos(2 * np.pi * xx)[:, None] * np.ones((1, ny))
     return Setup(p, hhb, None, omegaz, gg, h, u=u, v=v, name=name, notes={"seed": seed})
+
+
+def with_restoring(s: Setup, seed: int = 11, wdia_recs: int = 2, relax: bool = True, wdia: bool = True,
+                   visc: bool = True, **params) -> Setup:
+    """Add restoring and diapycnal inputs to a setup (restoring.c, diapycnal.c):
+    u/v restoring toward a sheared flow over part of the domain, thickness
+    restoring (one time scale for all layers) and interface-height restoring
+    (its own time scale per layer), a prescribed interface velocity with
+    `wdia_recs` records cycled over 4 time steps, and a diapycnal viscosity.
+    Each time scale is zero (no restoring) over part of the domain."""
+    L, nx, ny = s.nlay, s.nx, s.ny
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    day = 86400.0
+    rnd = uniform01(seed, 4 * L * nx * ny).reshape(4, L, nx, ny)
+    files: Dict[str, np.ndarray] = {}
+    p = dict(s.params)
+    if relax:
+        files["uRelaxFile"] = np.stack([0.1 * np.sin(2 * np.pi * Y) * (1 - k / L) for k in range(L)])
+        files["vRelaxFile"] = np.stack([0.05 * np.cos(2 * np.pi * X) for _ in range(L)])
+        files["uTimeFile"] = np.stack([np.where(X < 0.6, (5 + k) * day, 0.0) for k in range(L)])
+        files["vTimeFile"] = np.stack([np.where(Y > 0.3, (7 + k) * day, 0.0) for k in range(L)])
+        files["hRelaxFile"] = s.h + 5.0 * (rnd[0] - 0.5)
+        files["hTimeFile"] = np.where((X - 0.5) ** 2 + (Y - 0.5) ** 2 < 0.16, 3 * day, 0.0)
+        eta = np.empty((L, nx, ny))
+        acc = np.asarray(s.hhb, dtype=np.float64).copy()
+        for k in range(L - 1, -1, -1):
+            acc = acc + s.h[k]
+            eta[k] = acc
+        files["eRelaxFile"] = eta + 3.0 * (rnd[1] - 0.5)
+        files["eTimeFile"] = np.stack([np.where(X + Y > 0.8, (4 + 2 * k) * day, 0.0) for k in range(L)])
+    if wdia:
+        w = 1e-6 * (rnd[2].reshape(L, nx, ny)[None].repeat(wdia_recs, 0) - 0.5)
+        rec = np.zeros((wdia_recs, L + 1, nx, ny))
+        rec[:, 1:L] = w[:, 1:L] * (1 + np.arange(wdia_recs))[:, None, None, None]
+        files["wDiaFile"] = rec
+        p.update(wDiaNrecs=wdia_recs, wDiaPeriod=4 * float(p["dt"]))
+    if visc:
+        files["diaViscUFile"] = np.stack([np.where(X < 0.8, 2e-3 * (k > 0), 0.0) for k in range(L)])
+        files["diaViscVFile"] = np.stack([np.where(Y < 0.7, 1e-3 * (k > 0), 0.0) for k in range(L)])
+    p.update(params)
+    return Setup(p, s.hhb, s.hhs, s.omegaz, s.gg, s.h, u=s.u, v=s.v, taux=s.taux, tauy=s.tauy, name=s.name,
+                 notes=dict(s.notes, restoring=True), files=files)
 
 
 def read_outputs(outdir: str, nx: int, ny: int, nlay: int, n: int, with_p: bool = True) -> Dict[str, np.ndarray]:

@@ -41,6 +41,19 @@ def prepare_outdir(z, outdir):
         inputs.write_restart_files(outdir, int(z["restart_from"]), {k: z[f"restart_in_{k}"] for k in ("U", "V", "H")})
 
 
+def build_setup(name, spec):
+    """A setup from a test-case spec: kind (the inputs generator), its
+    arguments, and optionally restoring=dict(...) for inputs.with_restoring."""
+    from pawsim_amd import inputs
+
+    spec = dict(spec)
+    kind = spec.pop("kind")
+    rest = spec.pop("restoring", None)
+    spec.setdefault("pressureTiming", 1)
+    setup = getattr(inputs, kind)(name=name, **spec)
+    return inputs.with_restoring(setup, **rest) if rest is not None else setup
+
+
 def run_exe(exe, infile, outdir, timeout=600):
     os.makedirs(outdir, exist_ok=True)
     r = subprocess.run([exe, infile, outdir], capture_output=True, text=True, timeout=timeout)

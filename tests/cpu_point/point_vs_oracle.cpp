@@ -16,6 +16,7 @@ extern "C" {
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 
 int main(int argc, char** argv)
 {
@@ -50,6 +51,9 @@ int main(int argc, char** argv)
   P.thk = (int)m.cfg.thkScheme;
   P.kt00 = m.cfg.KT00_sigma;
   pg_set_viscosity(P, m.cfg.A2, m.cfg.A4, m.cfg.A2smag, m.cfg.A4smag, G.dx, G.dy);
+  P.useRelax = m.cfg.useRelax;
+  P.useWDia = m.cfg.useWDia;
+  P.useDiaVisc = m.cfg.useDiaVisc;
   P.useWind = m.cfg.useWind;
   P.useLin = m.cfg.linDrag > 0;
   P.useQuad = m.cfg.quadDrag > 0;
@@ -72,6 +76,23 @@ int main(int argc, char** argv)
     memcpy(h + k * fs, m.hw[k].d, fs * sizeof(double));
   }
   PgIn I{u, v, h, m.hhb.d, m.omz.d, m.taux.d, m.tauy.d};
+  // restoring / diapycnal inputs: the oracle's ghost-filled arrays, and the
+  // interface velocity its tendency built
+  auto stack = [&](const or_field* f, int n) {
+    double* a = (double*)malloc((size_t)n * fs * sizeof(double));
+    for (int k = 0; k < n; k++) memcpy(a + k * fs, f[k].d, fs * sizeof(double));
+    return a;
+  };
+  double* wdia = stack(m.wdia, G.L + 1);
+  double *uR = stack(m.uRelax, G.L), *uT = stack(m.uTime, G.L), *vR = stack(m.vRelax, G.L), *vT = stack(m.vTime, G.L);
+  double *nu = stack(m.nu_u, G.L), *nv = stack(m.nu_v, G.L);
+  I.wdia = wdia;
+  I.uRelax = uR;
+  I.uTime = uT;
+  I.vRelax = vR;
+  I.vTime = vT;
+  I.nu_u = nu;
+  I.nu_v = nv;
   long pts = 0, bad = 0;
   double maxabs = 0;
   for (int k = 0; k < G.L; k++)
@@ -131,6 +152,7 @@ int main(int argc, char** argv)
         }
       }
   printf("points=%ld mismatches=%ld maxabs=%g\n", pts, bad, maxabs);
+  for (double* a : {wdia, uR, uT, vR, vT, nu, nv}) free(a);
   free(u);
   free(v);
   free(h);

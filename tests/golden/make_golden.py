@@ -75,6 +75,14 @@ def cases():
         "per16_smag": inputs.periodic_box(nx=16, ny=16, nt=8, A2smag=4.0, A4smag=4.0, **T),
         "box16_smag": inputs.gyre(nx=16, ny=16, nt=6, A2=10.0, A2smag=2.0, **T),
         "chan32x16_smag_a4": inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1, A4=1e9, A4smag=3.0, **T),
+        # restoring and diapycnal momentum (restoring.c, diapycnal.c:72-133, 203-291)
+        "box16_restoring": inputs.with_restoring(inputs.gyre(nx=16, ny=16, nt=8, **T)),
+        "chan32x16_wdia_visc": inputs.with_restoring(inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1,
+                                                                    savefreqAvg=500.0, **T), relax=False),
+        "per16_relax_convdiff": inputs.with_restoring(inputs.periodic_box(nx=16, ny=16, nt=8, **T), wdia=False,
+                                                      visc=False, convDiff=1e-3),
+        "box16_wdia_3rec": inputs.with_restoring(inputs.gyre(nx=16, ny=16, nt=9, **T), relax=False, visc=False,
+                                                 wdia_recs=3),
     }
 
 
@@ -114,7 +122,8 @@ def make_case(name: str, setup: inputs.Setup, outdir: str, restart_from: int = -
             p = dict(setup.params)
             p.update(restart=1, startIdx=restart_from, Nt=int(p["Nt"]) - restart_from)
             setup = inputs.Setup(p, setup.hhb, setup.hhs, setup.omegaz, setup.gg, setup.h, u=setup.u, v=setup.v,
-                                 taux=setup.taux, tauy=setup.tauy, name=setup.name, notes=setup.notes)
+                                 taux=setup.taux, tauy=setup.tauy, name=setup.name, notes=setup.notes,
+                                 files=setup.files)
         infile = setup.write(work)
         res = os.path.join(work, "out")
         os.makedirs(res)
@@ -134,6 +143,8 @@ def make_case(name: str, setup: inputs.Setup, outdir: str, restart_from: int = -
             val = getattr(setup, key)
             if val is not None:
                 arrays[key] = val
+        for key, val in setup.files.items():
+            arrays[f"file_{key}"] = val
         if restart_in:
             arrays["restart_from"] = np.array(restart_from)
             for k, v in restart_in.items():
@@ -168,8 +179,9 @@ def load_setup(npz) -> inputs.Setup:
     """Rebuild the Setup stored in a fixture."""
     meta = json.loads(str(npz["meta"]))
     get = lambda k: npz[k] if k in npz.files else None  # noqa: E731
+    files = {k[5:]: npz[k] for k in npz.files if k.startswith("file_")}
     return inputs.Setup(dict(meta["params"]), npz["hhb"], None, npz["omegaz"], npz["gg"], npz["h"],
-                        u=get("u"), v=get("v"), taux=get("taux"), tauy=get("tauy"), name=meta["name"])
+                        u=get("u"), v=get("v"), taux=get("taux"), tauy=get("tauy"), name=meta["name"], files=files)
 
 
 def main() -> None:

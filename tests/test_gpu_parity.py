@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import pawsim_amd
-from conftest import ez_close, golden_ids, load_golden, parse_iters, prepare_outdir, read_ez, run_exe, same_bits
+from conftest import build_setup, ez_close, golden_ids, load_golden, parse_iters, prepare_outdir, read_ez, run_exe, same_bits
 from pawsim_amd import inputs
 
 pytestmark = pytest.mark.gpu
@@ -90,6 +90,14 @@ TRAJECTORY_CASES = {
                                        A2smag=3.0, A4smag=2.0, pressureTiming=0),
     "chan48x24_smag_avg_nonorms": dict(kind="channel", nx=48, ny=24, nt=8, dx=20e3, solver=0, SOR_rp=1.8,
                                        A2=20.0, A2smag=2.0, pressureTiming=0, savefreqAvg=600.0),
+    # restoring (u, v, thickness, interface heights), prescribed interface
+    # velocity records, diapycnal viscosity (per-point kernel)
+    "gyre64_restoring": dict(kind="gyre", nx=64, ny=64, nt=8, solver=0, restoring=dict(),
+                             savefreqAvg=900.0, savefreqEZ=600.0),
+    "chan64x32_wdia_visc_nonorms": dict(kind="channel", nx=64, ny=32, nt=8, dx=20e3, solver=1, pressureTiming=0,
+                                        savefreqAvg=500.0, restoring=dict(relax=False, wdia_recs=3)),
+    "per40x24_relax_mg": dict(kind="periodic_box", nx=40, ny=24, nt=8, solver=1, momentumScheme=1,
+                              thicknessScheme=2, restoring=dict(wdia=False)),
 }
 
 
@@ -97,9 +105,7 @@ TRAJECTORY_CASES = {
 def test_gpu_matches_oracle_every_step(name, tmp_path, oracle_exe):
     """Run GPU and oracle on the same input; every saved step must be bitwise equal."""
     spec = dict(TRAJECTORY_CASES[name])
-    kind = spec.pop("kind")
-    spec.setdefault("pressureTiming", 1)
-    setup = getattr(inputs, kind)(name=name, **spec)
+    setup = build_setup(name, spec)
     infile = setup.write(str(tmp_path / "in"))
     print(f"[gpu trajectory] {name}", flush=True)
     gout = run_exe(pawsim_amd.DRIVER_PATH, infile, str(tmp_path / "gpu"), timeout=300)

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 import pawsim_amd
-from conftest import read_ez, run_exe, same_bits
+from conftest import build_setup, read_ez, run_exe, same_bits
 from pawsim_amd import inputs
 
 pytestmark = pytest.mark.gpu
@@ -33,16 +33,18 @@ CASES = {
     # ghosts one fill behind), averages across the restart
     "chan48x24_up3_smag": dict(kind="channel", nx=48, ny=24, nt=12, dx=20e3, solver=1, thicknessScheme=2,
                                A2smag=2.0, pressureTiming=0, savefreqAvg=900.0, savefreqEZ=600.0),
+    # restoring with interface-velocity records cycling in time, whose
+    # averages (W) span the restart
+    "gyre48_restoring": dict(kind="gyre", nx=48, ny=48, nt=12, solver=1, savefreqAvg=1500.0,
+                             restoring=dict(wdia_recs=3)),
 }
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_checkpoint_restart_continues_bitwise(name, tmp_path):
     spec = dict(CASES[name])
-    kind = spec.pop("kind")
-    spec.setdefault("pressureTiming", 1)
     k = 5
-    full = getattr(inputs, kind)(name=name, saveCheckpoint=1, **spec)
+    full = build_setup(name, dict(spec, saveCheckpoint=1))
     nt = int(full.params["Nt"])
     print(f"[gpu restart] {name} continuous", flush=True)
     out_full = str(tmp_path / "full")
@@ -53,7 +55,7 @@ def test_checkpoint_restart_continues_bitwise(name, tmp_path):
     shutil.copy(os.path.join(out_full, f"checkpoint_n={k}.bin"), out_rs)
     rs = inputs.Setup(dict(full.params, restart=1, startIdx=k, restartFromCheckpoint=1, Nt=nt - k,
                            saveCheckpoint=0), full.hhb, full.hhs, full.omegaz, full.gg, full.h, u=full.u, v=full.v,
-                      taux=full.taux, tauy=full.tauy, name=name)
+                      taux=full.taux, tauy=full.tauy, name=name, files=full.files)
     print(f"[gpu restart] {name} from checkpoint {k}", flush=True)
     run_exe(pawsim_amd.DRIVER_PATH, rs.write(str(tmp_path / "in_rs")), out_rs, timeout=300)
     for n in range(k + 1, nt + 1):

@@ -53,11 +53,9 @@ def _trajectory_cases():
 def test_point_stencil_matches_oracle_trajectory_inputs(name, point_exe, tmp_path):
     """The larger inputs the GPU trajectory tests run (tile-sized grids, the
     periodic-x / wall-y corners with and without the norms)."""
-    from pawsim_amd import inputs
-    spec = dict(_trajectory_cases()[name])
-    kind = spec.pop("kind")
-    spec.setdefault("pressureTiming", 1)
-    setup = getattr(inputs, kind)(name=name, **spec)
+    from conftest import build_setup
+
+    setup = build_setup(name, _trajectory_cases()[name])
     infile = setup.write(str(tmp_path / "in"))
     r = subprocess.run([point_exe, infile, "2"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
