@@ -170,6 +170,10 @@ int pg_energy(pg_handle *h, double *KE, double *PE, double *Z);
  * scale, copies them to out[(10 Nlay + 2)][Nx][Ny] in the order
  * U V H M W HU HV HUU HVV HUV per layer, W of layer Nlay, P, and resets them. */
 int pg_average_add(pg_handle *h, double weight);
+/* Interface velocity of the last tendency evaluation, out[Nlay+1][Nx][Ny]
+ * (the W files of diagnostics.c:874-908; zero without restoring or
+ * prescribed records). */
+int pg_download_wdia(pg_handle *h, double *out);
 int pg_average_take(pg_handle *h, double scale, double *out);
 
 /* Full-fidelity checkpoint.  Replaces, as an option, the reference's

@@ -1534,9 +1534,7 @@ bool or_write_state(or_model *m, const char *outdir, uint n)
 {
   uint k;
   char name[256];
-  or_field zero;
   bool ok = true;
-  or_field_alloc(&zero, m->cfg.Nx, m->cfg.Ny, m->g);
   for (k = 0; k < m->L && ok; k++) {
     snprintf(name, sizeof(name), "U%u_n=%u.dat", k, n);
     ok = ok && write_named(outdir, name, &m->u[k]);
@@ -1544,14 +1542,14 @@ bool or_write_state(or_model *m, const char *outdir, uint n)
     ok = ok && write_named(outdir, name, &m->v[k]);
     snprintf(name, sizeof(name), "H%u_n=%u.dat", k, n);
     ok = ok && write_named(outdir, name, &m->h[k]);
-    snprintf(name, sizeof(name), "W%u_n=%u.dat", k, n); /* diapycnal velocity: identically zero */
-    ok = ok && write_named(outdir, name, &zero);
+    /* the interface velocity of the last tendency evaluation (diagnostics.c:884) */
+    snprintf(name, sizeof(name), "W%u_n=%u.dat", k, n);
+    ok = ok && write_named(outdir, name, &m->wdia[k]);
   }
   snprintf(name, sizeof(name), "W%u_n=%u.dat", m->L, n);
-  ok = ok && write_named(outdir, name, &zero);
+  ok = ok && write_named(outdir, name, &m->wdia[m->L]);
   snprintf(name, sizeof(name), "P_n=%u.dat", n);
   ok = ok && write_named(outdir, name, &m->pi);
-  or_field_free(&zero);
   return ok;
 }
 

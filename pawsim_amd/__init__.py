@@ -29,7 +29,7 @@ ABI_SYMBOLS = (
     "pg_create", "pg_destroy", "pg_last_error", "pg_upload_state", "pg_download_state",
     "pg_upload_pi", "pg_download_pi", "pg_tendency", "pg_step", "pg_pressure_correct",
     "pg_solve_pressure", "pg_refresh_halos", "pg_energy", "pg_average_add", "pg_average_take",
-    "pg_checkpoint_bytes", "pg_checkpoint_save", "pg_checkpoint_load", "pg_time", "pg_set_stream", "pg_synchronize",
+    "pg_checkpoint_bytes", "pg_checkpoint_save", "pg_checkpoint_load", "pg_download_wdia", "pg_time", "pg_set_stream", "pg_synchronize",
     "pg_bench_kernel",
     "pg_net_param_count", "pg_net_create", "pg_net_destroy", "pg_net_set_params", "pg_net_get_params",
     "pg_net_forward", "pg_net_loss_grad", "pg_net_adam_step",

@@ -271,7 +271,7 @@ static bool write_field(const char *dir, const char *name, const double *a, unsi
 typedef struct model {
   drv_config c;
   pg_handle *h;
-  double *u, *v, *hh, *pi, *zero, t;
+  double *u, *v, *hh, *pi, *zero, *wdia, t;
   unsigned n_saves, solves, iters_total;
   double solve_ms;
   unsigned n_EZ;
@@ -420,7 +420,9 @@ static bool write_state(model *M, const char *dir, unsigned n)
   const unsigned Nx = M->c.pg.Nx, Ny = M->c.pg.Ny, L = M->c.pg.Nlay;
   const size_t N = (size_t)Nx * Ny;
   char name[256];
-  if (pg_download_state(M->h, M->u, M->v, M->hh) || pg_download_pi(M->h, M->pi)) return false;
+  if (pg_download_state(M->h, M->u, M->v, M->hh) || pg_download_pi(M->h, M->pi)
+      || pg_download_wdia(M->h, M->wdia))
+    return false;
   for (unsigned k = 0; k < L; k++) {
     snprintf(name, sizeof(name), "U%u_n=%u.dat", k, n);
     if (!write_field(dir, name, M->u + k * N, Nx, Ny)) return false;
@@ -428,11 +430,12 @@ static bool write_state(model *M, const char *dir, unsigned n)
     if (!write_field(dir, name, M->v + k * N, Nx, Ny)) return false;
     snprintf(name, sizeof(name), "H%u_n=%u.dat", k, n);
     if (!write_field(dir, name, M->hh + k * N, Nx, Ny)) return false;
-    snprintf(name, sizeof(name), "W%u_n=%u.dat", k, n);  /* diapycnal velocity: zero on this path */
-    if (!write_field(dir, name, M->zero, Nx, Ny)) return false;
+    /* the interface velocity of the last tendency evaluation (diagnostics.c:884) */
+    snprintf(name, sizeof(name), "W%u_n=%u.dat", k, n);
+    if (!write_field(dir, name, M->wdia + k * N, Nx, Ny)) return false;
   }
   snprintf(name, sizeof(name), "W%u_n=%u.dat", L, n);
-  if (!write_field(dir, name, M->zero, Nx, Ny)) return false;
+  if (!write_field(dir, name, M->wdia + L * N, Nx, Ny)) return false;
   snprintf(name, sizeof(name), "P_n=%u.dat", n);
   return write_field(dir, name, M->pi, Nx, Ny);
 }
@@ -565,6 +568,7 @@ int main(int argc, char **argv)
   st.nu_v_dia = rs[10];
   M.pi = calloc(N, sizeof(double));
   M.zero = calloc(N, sizeof(double));
+  M.wdia = calloc((L + 1) * N, sizeof(double));
   if (!hhb || !om || !M.u || !M.v || !M.hh || (c->hsFile[0] && !hhs)) return 1;
   st.hhb = hhb;
   st.hhs = hhs;
@@ -593,6 +597,12 @@ int main(int argc, char **argv)
     if (it < 0) return 1;
     printf("PAWSIM pressure iterations: %d\n", it);
     pg_refresh_halos(M.h);  /* work-array rebuild after the preliminary correction */
+    /* the tendency core that follows it (pawsim.c:289-295) builds the
+     * interface velocity the first W files hold */
+    if (pg_tendency(M.h, NULL, NULL, NULL) != PG_OK) {
+      fprintf(stderr, "ERROR: %s\n", pg_last_error(M.h));
+      return 1;
+    }
     /* the first state written carries the restart index (pawsim.c:427, diagnostics.c:977-999) */
     M.n_saves = c->restart ? c->startIdx : 0;
     if (!write_state(&M, argv[2], M.n_saves)) {
@@ -651,7 +661,7 @@ int main(int argc, char **argv)
          c->Nt, t_loop, M.solve_ms, now_s() - t_all, M.solves ? (double)M.iters_total / M.solves : 0.0);
   pg_destroy(M.h);
   free(hhb); free(hhs); free(om); free(gg); free(taux); free(tauy);
-  free(M.u); free(M.v); free(M.hh); free(M.pi); free(M.zero); free(M.avg);
+  free(M.u); free(M.v); free(M.hh); free(M.pi); free(M.zero); free(M.wdia); free(M.avg);
   for (int q = 0; q < 11; q++) free(rs[q]);
   return 0;
 }

@@ -163,6 +163,26 @@ hipError_t pg_launch_wdia(pg_handle* h)
   return pg_fill(h, h->rs_wdia, G.L + 1, PG_HALO_SCALAR);
 }
 
+extern "C" int pg_download_wdia(pg_handle* h, double* out)
+{
+  if (!h || !out) return PG_EARG;
+  const PgGrid& G = h->G;
+  const size_t N = (size_t)G.nx * G.ny;
+  if (!h->rs_wdia) {
+    for (size_t p = 0; p < (G.L + 1) * N; p++) out[p] = 0;
+    return PG_OK;
+  }
+  std::vector<double> stage(h->fsz);
+  for (int k = 0; k <= G.L; k++) {
+    PG_CHECK(h, hipMemcpyAsync(stage.data(), h->rs_wdia + (size_t)k * h->fsz, h->fsz * sizeof(double),
+                               hipMemcpyDeviceToHost, h->stream));
+    PG_CHECK(h, hipStreamSynchronize(h->stream));
+    for (int i = 0; i < G.nx; i++)
+      for (int j = 0; j < G.ny; j++) out[k * N + (size_t)i * G.ny + j] = stage[pg_at(G, i + G.g, j + G.g)];
+  }
+  return PG_OK;
+}
+
 // Every input of the tendency stencils.
 PgIn pg_tendency_inputs(const pg_handle* h)
 {

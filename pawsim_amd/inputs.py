@@ -349,12 +349,14 @@ def with_restoring(s: Setup, seed: int = 11, wdia_recs: int = 2, relax: bool = T
 
 
 def read_outputs(outdir: str, nx: int, ny: int, nlay: int, n: int, with_p: bool = True) -> Dict[str, np.ndarray]:
-    """Load U/V/H (and P) for save index n as [Nlay, Nx, Ny] arrays."""
+    """Load U/V/H (and P, and the interface velocity W [Nlay+1, Nx, Ny]) for
+    save index n as [Nlay, Nx, Ny] arrays."""
     out = {}
     for name in ("U", "V", "H"):
         out[name] = np.stack([read_2d(os.path.join(outdir, f"{name}{k}_n={n}.dat"), nx, ny) for k in range(nlay)])
     if with_p:
         out["P"] = read_2d(os.path.join(outdir, f"P_n={n}.dat"), nx, ny)
+        out["W"] = np.stack([read_2d(os.path.join(outdir, f"W{k}_n={n}.dat"), nx, ny) for k in range(nlay + 1)])
     return out
 
 

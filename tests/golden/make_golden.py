@@ -133,7 +133,7 @@ def make_case(name: str, setup: inputs.Setup, outdir: str, restart_from: int = -
         nt = int(setup.params["Nt"])
         nx, ny, nl = setup.nx, setup.ny, setup.nlay
         s0 = max(restart_from, 0)
-        saves = sorted({s0, s0 + 1, s0 + 2, s0 + nt})
+        saves = sorted({s0, s0 + 1, s0 + 2, s0 + min(3, nt), s0 + nt})
         arrays = {
             "hhb": setup.hhb, "omegaz": setup.omegaz, "gg": np.asarray(setup.gg), "h": setup.h,
             "saves": np.array(saves),
