@@ -32,6 +32,7 @@ enum { OR_AB1 = 3, OR_AB2 = 4, OR_AB3 = 5 };
 enum { OR_MOM_AL81 = 0, OR_MOM_HK83 = 1, OR_MOM_S75E = 2, OR_MOM_TW81 = 3 };
 enum { OR_THK_AL81 = 0, OR_THK_HK83 = 1, OR_THK_UP3 = 2, OR_THK_KT00 = 3 };
 enum { OR_SOLVER_SOR = 0, OR_SOLVER_MG = 1 };
+enum { OR_TRC_AL81 = 0, OR_TRC_UP3 = 1, OR_TRC_KT00 = 2 };
 
 /* Parsed parameter file (config.c:42-122 defaults, 172-282 keys). */
 typedef struct or_config {
@@ -40,6 +41,9 @@ typedef struct or_config {
   uint wDiaNrecs;             /* records in wDiaFile (config.c:231) */
   real wDiaPeriod, convDiff;  /* config.c:227-228 */
   bool useRelax, useWDia, useDiaVisc;  /* derived from the files (config.c:305-312) */
+  bool useTracer, useBuoyancy, useDiaDiff;
+  uint trcScheme;             /* tracer advection (defs.h:41-43) */
+  real K2, K4;                /* tracer diffusivities (config.c:83-84) */
   real Lx, Ly, dt, tmin, tmax, savefrequency, h0, hsml, hbbl, A2, A4, A2smag, A4smag;
   real linDrag, quadDrag, linDragSurf, quadDragSurf, pi_tol, SOR_rp, omega_MG;
   real savefreqEZ;            /* energy/enstrophy diagnostics interval (config.c:206) */
@@ -55,6 +59,7 @@ typedef struct or_config {
   char uRelaxFile[256], vRelaxFile[256], hRelaxFile[256], eRelaxFile[256];
   char uTimeFile[256], vTimeFile[256], hTimeFile[256], eTimeFile[256];
   char wDiaFile[256], diaViscUFile[256], diaViscVFile[256];
+  char bInit[256], bRelaxFile[256], bTimeFile[256], diaDiffFile[256], bFluxFile[256];
 } or_config;
 
 /* Ghosted field, column-major in x: element (i,j) at d[i*sy+j] (field.c:37-71). */
@@ -93,6 +98,10 @@ typedef struct or_model {
   or_field hTime;
   or_field *wdia_ff;                /* [wDiaNrecs * (L+1)], record-major */
   or_field *wdia, *wdia_u, *wdia_v, *Fdia_u, *Fdia_v;  /* [L+1] */
+  /* tracer (buoyancy) b: state, tendency history [3][L], work copy and terms */
+  or_field *b, *tb, *bw, *dtb, *bRelax, *bTime, *kappa, *gprime;  /* [L] */
+  or_field *Fdia_b;                 /* [L+1] */
+  or_field Fsurf_b, hub, hvb, db_dx, db_dy, udb_dx, vdb_dy, d2bx, d2by, BB4, hz;
   /* pressure diagnostics (context.h:52-61) */
   real res_max, res_rms, div_max, div_rms;
   uint solves, iters_total, n_saves;

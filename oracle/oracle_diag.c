@@ -80,7 +80,7 @@ void or_energy(const or_model *m, double *KE, double *PE, double *Z)
                    * ((OF(H, i, j) + OF(H, ip1, j)) * SQ(OF(U, ip1, j)) + (OF(H, im1, j) + OF(H, i, j)) * SQ(OF(U, i, j))
                       + (OF(H, i, j) + OF(H, i, jp1)) * SQ(OF(V, i, jp1)) + (OF(H, i, jm1) + OF(H, i, j)) * SQ(OF(V, i, j)));
         }
-        gtot = m->geff[k];
+        gtot = (m->cfg.useTracer && m->cfg.useBuoyancy) ? m->geff[k] - OF(&m->b[k], i, j) : m->geff[k];
         local_KE += KEdens;
         local_PE += dx * dy * gtot * OF(H, i, j) * (eta[k] - 0.5 * OF(H, i, j));
         local_PE += dx * dy * gtot * P4(m->cfg.h0) / SQ(OF(H, i, j)) / 6;
@@ -127,9 +127,12 @@ bool or_write_EZ_if_due(or_model *m, const char *outdir)
  * then P (write_average_state, diagnostics.c:376-425). */
 enum { AV_U, AV_V, AV_H, AV_M, AV_W, AV_HU, AV_HV, AV_HUU, AV_HVV, AV_HUV, AV_PER_LAYER };
 
+/* the B averages (one per layer) follow P when there is a tracer */
+static uint av_fields(const or_model *m) { return AV_PER_LAYER * m->L + 2 + (m->cfg.useTracer ? m->L : 0); }
+
 static bool averages_alloc(or_model *m)
 {
-  uint q, n = AV_PER_LAYER * m->L + 2;
+  uint q, n = av_fields(m);
   if (m->avg) return true;
   m->avg = calloc(n, sizeof(or_field));
   if (!m->avg) return false;
@@ -140,7 +143,7 @@ static bool averages_alloc(or_model *m)
 
 void or_averages_free(or_model *m)
 {
-  uint q, n = AV_PER_LAYER * m->L + 2;
+  uint q, n = av_fields(m);
   if (!m->avg) return;
   for (q = 0; q < n; q++) or_field_free(&m->avg[q]);
   free(m->avg);
@@ -180,6 +183,7 @@ bool or_accumulate_averages(or_model *m)
         OF(AVG(AV_HUU, k), i, j) += dtfac * OF(&m->hwest[k], i, j) * OF(U, i, j) * OF(U, i, j);
         OF(AVG(AV_HVV, k), i, j) += dtfac * OF(&m->hsouth[k], i, j) * OF(V, i, j) * OF(V, i, j);
         OF(AVG(AV_HUV, k), i, j) += dtfac * hq * uq * vq;
+        if (m->cfg.useTracer) OF(&m->avg[AV_PER_LAYER * L + 2 + k], i, j) += dtfac * OF(&m->b[k], i, j);
       }
   }
   for (i = g; i < g + m->cfg.Nx; i++)
@@ -205,7 +209,7 @@ bool or_write_averages_if_due(or_model *m, const char *outdir)
   static const char *names[AV_PER_LAYER] = {"U_avg", "V_avg", "H_avg", "M_avg", "W_avg",
                                             "HU_avg", "HV_avg", "HUU_avg", "HVV_avg", "HUV_avg"};
   uint q, k, i, j;
-  const uint L = m->L, nf = AV_PER_LAYER * L + 2;
+  const uint L = m->L, nf = av_fields(m);
   if (m->cfg.savefreqAvg <= 0 || m->t < m->t_next_avg) return true;
   if (!averages_alloc(m)) return false;
   for (q = 0; q < nf; q++) {
@@ -213,9 +217,12 @@ bool or_write_averages_if_due(or_model *m, const char *outdir)
     for (i = f->g; i < f->g + f->nx; i++)
       for (j = f->g; j < f->g + f->ny; j++) OF(f, i, j) *= 1 / m->cfg.savefreqAvg;
   }
-  for (k = 0; k < L; k++)
+  for (k = 0; k < L; k++) {
     for (q = 0; q < AV_PER_LAYER; q++)
       if (!write_avg(outdir, names[q], (int)k, m->n_avg, AVG(q, k))) return false;
+    if (m->cfg.useTracer && !write_avg(outdir, "B_avg", (int)k, m->n_avg, &m->avg[AV_PER_LAYER * L + 2 + k]))
+      return false;
+  }
   if (!write_avg(outdir, "W_avg", (int)L, m->n_avg, &m->avg[AV_PER_LAYER * L])) return false;
   if (m->cfg.useRL && !write_avg(outdir, "P_avg", -1, m->n_avg, &m->avg[AV_PER_LAYER * L + 1])) return false;
   for (q = 0; q < nf; q++) or_field_zero(&m->avg[q]);

@@ -101,9 +101,12 @@ bool or_config_read(const char *fname, or_config *c, FILE *err)
     S("hTimeFile", c->hTimeFile); S("eTimeFile", c->eTimeFile); S("wDiaFile", c->wDiaFile);
     S("diaViscUFile", c->diaViscUFile); S("diaViscVFile", c->diaViscVFile);
     U("wDiaNrecs", c->wDiaNrecs); R("wDiaPeriod", c->wDiaPeriod); R("convDiff", c->convDiff);
+    S("bInitFile", c->bInit); S("bRelaxFile", c->bRelaxFile); S("bTimeFile", c->bTimeFile);
+    S("diaDiffFile", c->diaDiffFile); S("bFluxFile", c->bFluxFile);
+    B("useTracer", c->useTracer); B("useBuoyancy", c->useBuoyancy); U("tracerScheme", c->trcScheme);
+    R("K2", c->K2); R("K4", c->K4);
     else if (!strcmp(key, "pressureSolver")) { c->solver = (uint)strtoul(val, NULL, 10); solverGiven = true; }
-    else if (!strcmp(key, "useTracer") || !strcmp(key, "useBuoyancy")
-             || !strcmp(key, "windFeedback") || !strcmp(key, "useRandomForcing") || !strcmp(key, "useTrad")) {
+    else if (!strcmp(key, "windFeedback") || !strcmp(key, "useRandomForcing") || !strcmp(key, "useTrad")) {
       if (atoi(val) != 0) flag_unsupported(c, key);
     }
     else if (!strcmp(key, "FbaroXFile") || !strcmp(key, "FbaroYFile")) {
@@ -124,8 +127,12 @@ bool or_config_read(const char *fname, or_config *c, FILE *err)
   /* use_MG <-> pressureSolver mapping (config.c:289-297). */
   if (!solverGiven) c->solver = useMG ? OR_SOLVER_MG : OR_SOLVER_SOR;
   c->useWind = (c->tauxFile[0] != '\0') || (c->tauyFile[0] != '\0');
-  /* config.c:305-312 (no tracer: bTimeFile, bFluxFile and diaDiffFile act only on tracers) */
-  c->useRelax = c->uTimeFile[0] || c->vTimeFile[0] || c->hTimeFile[0] || c->eTimeFile[0];
+  /* config.c:305-312 */
+  c->useRelax = c->uTimeFile[0] || c->vTimeFile[0] || c->hTimeFile[0] || c->eTimeFile[0]
+                || (c->useTracer && c->bTimeFile[0]);
+  c->useDiaDiff = c->bFluxFile[0] || c->diaDiffFile[0] || (c->convDiff != 0);
+  if (c->useBuoyancy && !c->useTracer) flag_unsupported(c, "useBuoyancy without useTracer");
+  if (c->trcScheme > OR_TRC_KT00 || c->K2 < 0 || c->K4 < 0) flag_unsupported(c, "tracerScheme / K2 / K4");
   c->useWDia = c->useRelax || c->wDiaFile[0];
   c->useDiaVisc = c->diaViscUFile[0] || c->diaViscVFile[0] || (c->convDiff != 0);
   if (c->wDiaNrecs == 0 || c->convDiff < 0) flag_unsupported(c, "wDiaNrecs / convDiff");

@@ -147,15 +147,18 @@ void or_fill(or_field *f, const or_model *m, int kind)
   if (!m->per_x) wall_x(f, kind);
 }
 
-static void fill_prognostic(or_model *m, or_field *u, or_field *v, or_field *h)
+/* pawsim_state_exchange_prognostic_halos (state.c:929-950); b when there is a tracer. */
+static void fill_prognostic4(or_model *m, or_field *u, or_field *v, or_field *h, or_field *b)
 {
   uint k;
   for (k = 0; k < m->L; k++) {
     or_fill(&u[k], m, OR_HALO_U);
     or_fill(&v[k], m, OR_HALO_V);
     or_fill(&h[k], m, OR_HALO_SCALAR);
+    if (m->cfg.useTracer) or_fill(&b[k], m, OR_HALO_SCALAR);
   }
 }
+#define fill_prognostic(m, u, v, h) fill_prognostic4(m, u, v, h, (m)->b)
 
 /* ------------------------------------------------------------------ */
 /* model setup (pawsim.c:118-196, 355-470; state.c:957-1100)           */
@@ -267,6 +270,9 @@ bool or_model_init(or_model *m, const char *infile, FILE *err)
   WL(pp); WL(KE); WL(MM); WL(huu); WL(hvv);
   m->eta = alloc_layers(L + 1, nx, ny, g);
   WL(uRelax); WL(vRelax); WL(hRelax); WL(eRelax); WL(uTime); WL(vTime); WL(eTime); WL(nu_u); WL(nu_v);
+  WL(b); WL(bw); WL(dtb); WL(bRelax); WL(bTime); WL(kappa); WL(gprime);
+  m->tb = alloc_layers(3 * L, nx, ny, g);
+  m->Fdia_b = alloc_layers(L + 1, nx, ny, g);
   m->wdia = alloc_layers(L + 1, nx, ny, g);
   m->wdia_u = alloc_layers(L + 1, nx, ny, g);
   m->wdia_v = alloc_layers(L + 1, nx, ny, g);
@@ -278,6 +284,7 @@ bool or_model_init(or_model *m, const char *infile, FILE *err)
   W2(Hc); W2(Hw); W2(Hs); W2(taux_w); W2(tauy_w); W2(usq_bot); W2(vsq_bot); W2(uabs_bot);
   W2(DD_T); W2(DD_S); W2(DD4_T); W2(DD4_S); W2(A2_h); W2(A2_q); W2(A4s_h); W2(A4s_q); W2(thk_x); W2(thk_y);
   W2(UU4); W2(VV4); W2(pi); W2(rhs); W2(hTime);
+  W2(Fsurf_b); W2(hub); W2(hvb); W2(db_dx); W2(db_dy); W2(udb_dx); W2(vdb_dy); W2(d2bx); W2(d2by); W2(BB4); W2(hz);
 #undef W2
 #undef WL
   m->gg = calloc(L, sizeof(real));
@@ -340,6 +347,17 @@ bool or_model_init(or_model *m, const char *infile, FILE *err)
       }
     free_layers(tmp, R * (L + 1));
   }
+  if (c->useTracer
+      && (!read_into(m, c->bInit, m->b, L, err) || !read_into(m, c->bRelaxFile, m->bRelax, L, err)
+          || !read_into(m, c->bTimeFile, m->bTime, L, err) || !read_into(m, c->diaDiffFile, m->kappa, L, err)
+          || !read_into(m, c->bFluxFile, &m->Fsurf_b, 1, err)))
+    return false;
+  for (k = 0; k < L; k++) {
+    or_fill(&m->bRelax[k], m, OR_HALO_SCALAR);
+    or_fill(&m->bTime[k], m, OR_HALO_SCALAR);
+    or_fill(&m->kappa[k], m, OR_HALO_SCALAR);
+  }
+  or_fill(&m->Fsurf_b, m, OR_HALO_SCALAR);
   for (k = 0; k < c->wDiaNrecs * (L + 1); k++) or_fill(&m->wdia_ff[k], m, OR_HALO_SCALAR);
   for (k = 0; k < L; k++) {
     or_fill(&m->uRelax[k], m, OR_HALO_U);
@@ -371,14 +389,14 @@ bool or_model_init(or_model *m, const char *infile, FILE *err)
 bool or_model_restart(or_model *m, const char *outdir, FILE *err)
 {
   uint k, i, j, nx = m->cfg.Nx, ny = m->cfg.Ny;
-  static const char *names[3] = {"U", "V", "H"};
-  or_field *dst[3] = {m->u, m->v, m->h};
+  static const char *names[4] = {"U", "V", "H", "B"};
+  or_field *dst[4] = {m->u, m->v, m->h, m->b};
   real *buf = malloc((size_t)nx * ny * sizeof(real));
   char path[1024];
   int f;
   if (!buf) return false;
   for (k = 0; k < m->L; k++)
-    for (f = 0; f < 3; f++) {
+    for (f = 0; f < (m->cfg.useTracer ? 4 : 3); f++) {
       snprintf(path, sizeof(path), "%s/%s%u_n=%u.dat", outdir, names[f], k, m->cfg.startIdx);
       if (!or_read_matrix(path, buf, nx, ny, 1, err)) {
         free(buf);
@@ -425,6 +443,12 @@ void or_model_free(or_model *m)
   free_layers(m->Fdia_u, L + 1); free_layers(m->Fdia_v, L + 1);
   free_layers(m->wdia_ff, m->cfg.wDiaNrecs * (L + 1));
   or_field_free(&m->hTime);
+  free_layers(m->b, L); free_layers(m->bw, L); free_layers(m->dtb, L); free_layers(m->bRelax, L);
+  free_layers(m->bTime, L); free_layers(m->kappa, L); free_layers(m->gprime, L); free_layers(m->tb, 3 * L);
+  free_layers(m->Fdia_b, L + 1);
+  or_field_free(&m->Fsurf_b); or_field_free(&m->hub); or_field_free(&m->hvb); or_field_free(&m->db_dx);
+  or_field_free(&m->db_dy); or_field_free(&m->udb_dx); or_field_free(&m->vdb_dy); or_field_free(&m->d2bx);
+  or_field_free(&m->d2by); or_field_free(&m->BB4); or_field_free(&m->hz);
   free(m->gg);
   free(m->geff);
   or_mg_free();
@@ -456,6 +480,10 @@ static void load_state(or_model *m)
     or_fill(&m->uw[k], m, OR_HALO_U);
     or_fill(&m->vw[k], m, OR_HALO_V);
     or_fill(&m->hw[k], m, OR_HALO_SCALAR);
+    if (m->cfg.useTracer) {
+      copy_interior(&m->bw[k], &m->b[k]);
+      or_fill(&m->bw[k], m, OR_HALO_SCALAR);
+    }
   }
 }
 
@@ -685,7 +713,11 @@ static void calc_bernoulli(or_model *m)
     for (i = 1; i < h->sx - 1; i++)
       for (j = 1; j < h->sy - 1; j++) {
         if (k == 0) OF(pp, i, j) = 0; /* rigid lid: barotropic pressure solved separately */
-        else OF(pp, i, j) = OF(&m->pp[k - 1], i, j) + m->gg[k] * OF(&m->eta[k], i, j);
+        else {
+          OF(pp, i, j) = OF(&m->pp[k - 1], i, j) + m->gg[k] * OF(&m->eta[k], i, j);
+          if (m->cfg.useTracer && m->cfg.useBuoyancy)  /* work.c:1445-1451 */
+            OF(pp, i, j) += OF(&m->eta[k], i, j) * (OF(&m->bw[k - 1], i, j) - OF(&m->bw[k], i, j));
+        }
         if (sch == OR_MOM_HK83) {
           OF(KE, i, j) = (SQ(OF(u, i + 1, j)) + SQ(OF(u, i, j))
                           + 0.25 * (SQ(OF(u, i + 1, j + 1)) + SQ(OF(u, i, j + 1)) + SQ(OF(u, i + 1, j - 1)) + SQ(OF(u, i, j - 1)))
@@ -747,6 +779,19 @@ static void calc_momentum(or_model *m)
     or_field_zero(&m->dtu[k]);
     or_field_zero(&m->dtv[k]);
     calc_mass_fluxes(m, k);
+    if (m->cfg.useTracer && m->cfg.useBuoyancy) {
+      /* calc_layer_active_buoyancy_work (work.c:188-213): buoyancy gradient
+       * and h times the layer mid-depth on [1,s-1), zero elsewhere */
+      or_field_zero(&m->hz);
+      or_field_zero(&m->db_dx);
+      or_field_zero(&m->db_dy);
+      for (i = 1; i < m->hz.sx - 1; i++)
+        for (j = 1; j < m->hz.sy - 1; j++) {
+          OF(&m->db_dx, i, j) = (OF(&m->bw[k], i, j) - OF(&m->bw[k], i - 1, j)) / dx;
+          OF(&m->db_dy, i, j) = (OF(&m->bw[k], i, j) - OF(&m->bw[k], i, j - 1)) / dy;
+          OF(&m->hz, i, j) = OF(h, i, j) * 0.5 * (OF(&m->eta[k], i, j) + OF(&m->eta[k + 1], i, j));
+        }
+    }
     if (useA2 || useA4) {
       or_field_zero(&m->DD_T); or_field_zero(&m->DD_S); or_field_zero(&m->A2_h); or_field_zero(&m->A2_q);
       or_field_zero(&m->A4s_h); or_field_zero(&m->A4s_q); or_field_zero(&m->UU4); or_field_zero(&m->VV4);
@@ -822,6 +867,8 @@ static void calc_momentum(or_model *m)
         if (useA4)
           a4 = -((OF(&m->A4s_h, i, j) * OF(h, i, j) * OF(&m->DD4_T, i, j) - OF(&m->A4s_h, i - 1, j) * OF(h, i - 1, j) * OF(&m->DD4_T, i - 1, j)) / dx
                  + (OF(&m->A4s_q, i, j + 1) * OF(hq, i, j + 1) * OF(&m->DD4_S, i, j + 1) - OF(&m->A4s_q, i, j) * OF(hq, i, j) * OF(&m->DD4_S, i, j)) / dy);
+        if (m->cfg.useTracer && m->cfg.useBuoyancy && hwest != 0)  /* work.c:1745-1749 */
+          buoy = -0.5 * (OF(&m->hz, i, j) + OF(&m->hz, i - 1, j)) * OF(&m->db_dx, i, j);
         rhs = gM + gK + qf + ((hwest != 0) ? (a2 + a4 + buoy) / hwest : 0);
         OF(&m->dtu[k], i, j) = rhs;
 
@@ -846,6 +893,9 @@ static void calc_momentum(or_model *m)
         if (useA4)
           a4 = -((OF(&m->A4s_q, i + 1, j) * OF(hq, i + 1, j) * OF(&m->DD4_S, i + 1, j) - OF(&m->A4s_q, i, j) * OF(hq, i, j) * OF(&m->DD4_S, i, j)) / dx
                  - (OF(&m->A4s_h, i, j) * OF(h, i, j) * OF(&m->DD4_T, i, j) - OF(&m->A4s_h, i, j - 1) * OF(h, i, j - 1) * OF(&m->DD4_T, i, j - 1)) / dy);
+        buoy = 0;
+        if (m->cfg.useTracer && m->cfg.useBuoyancy && hsouth != 0)  /* work.c:1829-1833 */
+          buoy = -0.5 * (OF(&m->hz, i, j) + OF(&m->hz, i, j - 1)) * OF(&m->db_dy, i, j);
         rhs = gM + gK + qf + ((hsouth != 0) ? (a2 + a4 + buoy) / hsouth : 0);
         OF(&m->dtv[k], i, j) = rhs;
       }
@@ -1017,14 +1067,57 @@ static void calc_wdia(or_model *m)
   }
 }
 
-/* pawsim_diapycnal_apply_momentum (diapycnal.c:72-133, 203-291) without
- * tracers: upwind vertical advection of momentum by wdia and explicit
- * diapycnal viscosity.  Without active buoyancy the interface reduced
- * gravity is gg >= 0, so convDiff never enhances the viscosity; it only
- * switches it on. */
+/* calc_gprime (diapycnal.c:27-66): interface reduced gravity, with the
+ * buoyancy jump when the tracer is active; zero without convDiff. */
+static void calc_gprime(or_model *m)
+{
+  uint k, i, j, g = m->g, L = m->L;
+  for (k = 0; k < L; k++) or_field_zero(&m->gprime[k]);
+  if (m->cfg.convDiff == 0 || L < 2) return;
+  for (k = 1; k < L; k++) {
+    for (i = 0; i < m->cfg.Nx; i++)
+      for (j = 0; j < m->cfg.Ny; j++) {
+        uint il = i + g, jl = j + g;
+        OF(&m->gprime[k], il, jl) = m->gg[k];
+        if (m->cfg.useTracer && m->cfg.useBuoyancy)
+          OF(&m->gprime[k], il, jl) += OF(&m->bw[k - 1], il, jl) - OF(&m->bw[k], il, jl);
+      }
+    or_fill(&m->gprime[k], m, OR_HALO_SCALAR);
+  }
+}
+
+/* pawsim_diapycnal_prepare_tracer_fluxes (diapycnal.c:140-197): surface
+ * buoyancy flux, diapycnal diffusion with the convective enhancement. */
+static void tracer_fluxes(or_model *m)
+{
+  uint k, i, j, g = m->g, L = m->L;
+  calc_gprime(m);
+  for (k = 0; k <= L; k++) or_field_zero(&m->Fdia_b[k]);
+  if (!m->cfg.useTracer || !m->cfg.useDiaDiff) return;
+  for (i = 0; i < m->cfg.Nx; i++)
+    for (j = 0; j < m->cfg.Ny; j++) {
+      uint il = i + g, jl = j + g;
+      OF(&m->Fdia_b[0], il, jl) = OF(&m->Fsurf_b, il, jl);
+      OF(&m->Fdia_b[L], il, jl) = 0;
+      for (k = 1; k < L; k++) {
+        real db = OF(&m->bw[k - 1], il, jl) - OF(&m->bw[k], il, jl);
+        real dz = 0.5 * (OF(&m->hw[k], il, jl) + OF(&m->hw[k - 1], il, jl));
+        real kap = OF(&m->kappa[k], il, jl);
+        if (m->cfg.useBuoyancy) db += m->gg[k];
+        if (m->cfg.convDiff != 0 && OF(&m->gprime[k], il, jl) < 0) kap += m->cfg.convDiff;
+        if (dz != 0) OF(&m->Fdia_b[k], il, jl) = kap * db / dz;
+      }
+    }
+  for (k = 0; k <= L; k++) or_fill(&m->Fdia_b[k], m, OR_HALO_SCALAR);
+}
+
+/* pawsim_diapycnal_apply_momentum (diapycnal.c:72-133, 203-291): upwind
+ * vertical advection of momentum by wdia and explicit diapycnal viscosity,
+ * enhanced by convDiff where an adjacent interface is statically unstable. */
 static void diapycnal_momentum(or_model *m)
 {
   uint k, i, j, g = m->g, L = m->L, nx = m->cfg.Nx, ny = m->cfg.Ny;
+  calc_gprime(m);
   for (k = 0; k <= L; k++) {
     or_field_zero(&m->Fdia_u[k]);
     or_field_zero(&m->Fdia_v[k]);
@@ -1037,6 +1130,10 @@ static void diapycnal_momentum(or_model *m)
           real dz_u = 0.5 * (OF(&m->hwest[k], il, jl) + OF(&m->hwest[k - 1], il, jl));
           real dz_v = 0.5 * (OF(&m->hsouth[k], il, jl) + OF(&m->hsouth[k - 1], il, jl));
           real nu_u = OF(&m->nu_u[k], il, jl), nu_v = OF(&m->nu_v[k], il, jl);
+          if (m->cfg.convDiff != 0) {
+            if (OF(&m->gprime[k], il, jl) < 0 || OF(&m->gprime[k], il - 1, jl) < 0) nu_u += m->cfg.convDiff;
+            if (OF(&m->gprime[k], il, jl) < 0 || OF(&m->gprime[k], il, jl - 1) < 0) nu_v += m->cfg.convDiff;
+          }
           if (dz_u != 0) OF(&m->Fdia_u[k], il, jl) = nu_u * (OF(&m->uw[k - 1], il, jl) - OF(&m->uw[k], il, jl)) / dz_u;
           if (dz_v != 0) OF(&m->Fdia_v[k], il, jl) = nu_v * (OF(&m->vw[k - 1], il, jl) - OF(&m->vw[k], il, jl)) / dz_v;
         }
@@ -1086,11 +1183,106 @@ static void restoring_momentum(or_model *m)
   }
 }
 
+/* pawsim_work_calc_tracer_tendency (work.c:1947-2144): advection (AL81
+ * gradient form, UP3 or KT00 flux form), vertical advection by wdia,
+ * diapycnal diffusion, K2 / K4 diffusion, restoring. */
+static void tracer_tendency(or_model *m)
+{
+  uint k, i, j, g = m->g, L = m->L;
+  const real dx = m->dx, dy = m->dy, dxsq = dx * dx, dysq = dy * dy, sig = m->cfg.KT00_sigma;
+  const uint sch = m->cfg.trcScheme;
+  if (!m->cfg.useTracer) return;
+  for (k = 0; k < L; k++) {
+    or_field *b = &m->bw[k], *u = &m->uw[k], *v = &m->vw[k], *hW = &m->hwest[k], *hS = &m->hsouth[k];
+    or_field *huu = &m->huu[k], *hvv = &m->hvv[k], *DX = &m->db_dx, *DY = &m->db_dy;
+    or_field_zero(&m->dtb[k]);
+    or_field_zero(&m->hub);
+    or_field_zero(&m->hvb);
+    for (i = 1; i < b->sx - 1; i++)
+      for (j = 1; j < b->sy - 1; j++) {
+        if (sch == OR_TRC_AL81 || sch == OR_TRC_UP3) {
+          OF(DX, i, j) = (OF(b, i, j) - OF(b, i - 1, j)) / dx;
+          OF(DY, i, j) = (OF(b, i, j) - OF(b, i, j - 1)) / dy;
+          OF(&m->udb_dx, i, j) = OF(u, i, j) * OF(DX, i, j);
+          OF(&m->vdb_dy, i, j) = OF(v, i, j) * OF(DY, i, j);
+        }
+        if (m->cfg.K4 > 0)
+          OF(&m->BB4, i, j) = ((OF(hW, i + 1, j) * (OF(b, i + 1, j) - OF(b, i, j)) - OF(hW, i, j) * (OF(b, i, j) - OF(b, i - 1, j))) / dxsq
+                               + (OF(hS, i, j + 1) * (OF(b, i, j + 1) - OF(b, i, j)) - OF(hS, i, j) * (OF(b, i, j) - OF(b, i, j - 1))) / dysq)
+                              / OF(&m->hw[k], i, j);
+      }
+    if (sch == OR_TRC_UP3) {
+      for (i = 1; i < b->sx - 1; i++)
+        for (j = 1; j < b->sy - 1; j++) {
+          OF(&m->d2bx, i, j) = OF(b, i + 1, j) - 2 * OF(b, i, j) + OF(b, i - 1, j);
+          OF(&m->d2by, i, j) = OF(b, i, j + 1) - 2 * OF(b, i, j) + OF(b, i, j - 1);
+        }
+      for (i = 1; i < b->sx - 1; i++)
+        for (j = 1; j < b->sy - 1; j++) {
+          OF(&m->hvb, i, j) = 0.5 * (OF(b, i, j) + OF(b, i, j - 1))
+                              - (1 / 6.0) * ((OF(v, i, j) > 0) ? OF(&m->d2by, i, j - 1) : OF(&m->d2by, i, j));
+          OF(&m->hvb, i, j) *= OF(hvv, i, j);
+          OF(&m->hub, i, j) = 0.5 * (OF(b, i, j) + OF(b, i - 1, j))
+                              - (1 / 6.0) * ((OF(u, i, j) > 0) ? OF(&m->d2bx, i - 1, j) : OF(&m->d2bx, i, j));
+          OF(&m->hub, i, j) *= OF(huu, i, j);
+        }
+    } else if (sch == OR_TRC_KT00) {
+      for (i = 1; i < b->sx - 1; i++)
+        for (j = 1; j < b->sy - 1; j++) {
+          OF(DX, i, j) = or_minmod(sig * (OF(b, i + 1, j) - OF(b, i, j)), 0.5 * (OF(b, i + 1, j) - OF(b, i - 1, j)),
+                                   sig * (OF(b, i, j) - OF(b, i - 1, j))) / dx;
+          OF(DY, i, j) = or_minmod(sig * (OF(b, i, j + 1) - OF(b, i, j)), 0.5 * (OF(b, i, j + 1) - OF(b, i, j - 1)),
+                                   sig * (OF(b, i, j) - OF(b, i, j - 1))) / dy;
+        }
+      for (i = 1; i < b->sx - 1; i++)
+        for (j = 1; j < b->sy - 1; j++) {
+          real bxm = OF(b, i - 1, j) + 0.5 * OF(DX, i - 1, j) * dx, bxp = OF(b, i, j) - 0.5 * OF(DX, i, j) * dx;
+          real bym = OF(b, i, j - 1) + 0.5 * OF(DY, i, j - 1) * dy, byp = OF(b, i, j) + 0.5 * OF(DY, i, j) * dy;
+          OF(&m->hub, i, j) = ((OF(u, i, j) > 0) ? bxm : bxp) * OF(huu, i, j);
+          OF(&m->hvb, i, j) = ((OF(v, i, j) > 0) ? bym : byp) * OF(hvv, i, j);
+        }
+    }
+    for (i = g; i < g + m->cfg.Nx; i++)
+      for (j = g; j < g + m->cfg.Ny; j++) {
+        real rhs = 0, adv = 0, hh = OF(&m->hw[k], i, j), bk = OF(b, i, j);
+        if (sch == OR_TRC_AL81)
+          adv = -0.5 * (OF(&m->udb_dx, i + 1, j) + OF(&m->udb_dx, i, j) + OF(&m->vdb_dy, i, j + 1) + OF(&m->vdb_dy, i, j));
+        else if (hh != 0)
+          adv = -((OF(&m->hub, i + 1, j) - OF(&m->hub, i, j)) / dx + (OF(&m->hvb, i, j + 1) - OF(&m->hvb, i, j)) / dy) / hh
+                + bk * ((OF(huu, i + 1, j) - OF(huu, i, j)) / dx + (OF(hvv, i, j + 1) - OF(hvv, i, j)) / dy) / hh;
+        rhs += adv;
+        if (m->cfg.useWDia && hh != 0) {
+          real w0 = OF(&m->wdia[k], i, j), w1 = OF(&m->wdia[k + 1], i, j);
+          real b_p = (k == 0) ? 0 : OF(&m->bw[k - 1], i, j), b_m = (k == L - 1) ? 0 : OF(&m->bw[k + 1], i, j);
+          real db_p = (w0 > 0) ? 0 : b_p - bk, db_m = (w1 > 0) ? bk - b_m : 0;
+          if (m->cfg.useBuoyancy) {
+            db_p += (w0 > 0) ? 0 : m->gg[k];
+            db_m += ((k + 1 < L) && (w1 > 0)) ? m->gg[k + 1] : 0;
+          }
+          rhs += -(w0 * db_p + w1 * db_m) / hh;
+        }
+        if (m->cfg.useDiaDiff && hh != 0) rhs += (OF(&m->Fdia_b[k], i, j) - OF(&m->Fdia_b[k + 1], i, j)) / hh;
+        if (m->cfg.K2 > 0 && hh != 0)
+          rhs += m->cfg.K2 * ((OF(hW, i + 1, j) * (OF(b, i + 1, j) - bk) - OF(hW, i, j) * (bk - OF(b, i - 1, j))) / dxsq
+                              + (OF(hS, i, j + 1) * (OF(b, i, j + 1) - bk) - OF(hS, i, j) * (bk - OF(b, i, j - 1))) / dysq) / hh;
+        if (m->cfg.K4 > 0 && hh != 0) {
+          or_field *B4 = &m->BB4;
+          rhs += -m->cfg.K4 * ((OF(hW, i + 1, j) * (OF(B4, i + 1, j) - OF(B4, i, j)) - OF(hW, i, j) * (OF(B4, i, j) - OF(B4, i - 1, j))) / dxsq
+                               + (OF(hS, i, j + 1) * (OF(B4, i, j + 1) - OF(B4, i, j)) - OF(hS, i, j) * (OF(B4, i, j) - OF(B4, i, j - 1))) / dysq) / hh;
+        }
+        if (OF(&m->bTime[k], i, j) > 0) rhs += -(bk - OF(&m->bRelax[k], i, j)) / OF(&m->bTime[k], i, j);
+        OF(&m->dtb[k], i, j) = rhs;
+      }
+    or_fill(&m->dtb[k], m, OR_HALO_SCALAR);
+  }
+}
+
 void or_tendency_core(or_model *m)
 {
   calc_face_thickness(m);
   calc_eta(m);
   calc_wdia(m);
+  tracer_fluxes(m);
   calc_pv(m);
   calc_pv_coefficients(m);
   calc_bernoulli(m);
@@ -1099,6 +1291,7 @@ void or_tendency_core(or_model *m)
   forcing_apply(m);
   restoring_momentum(m);
   calc_thickness(m);
+  tracer_tendency(m);
 }
 
 /* tendency.c:81-115: evaluate into tendency slot `slot` (0 = tendency). */
@@ -1110,6 +1303,7 @@ void or_tendency_eval(or_model *m, int slot)
     or_field_zero(&m->tu[slot * m->L + k]);
     or_field_zero(&m->tv[slot * m->L + k]);
     or_field_zero(&m->th[slot * m->L + k]);
+    or_field_zero(&m->tb[slot * m->L + k]);
   }
   load_state(m);
   static_geometry(m);
@@ -1118,8 +1312,9 @@ void or_tendency_eval(or_model *m, int slot)
     copy_interior(&m->tu[slot * m->L + k], &m->dtu[k]);
     copy_interior(&m->tv[slot * m->L + k], &m->dtv[k]);
     copy_interior(&m->th[slot * m->L + k], &m->dth[k]);
+    if (m->cfg.useTracer) copy_interior(&m->tb[slot * m->L + k], &m->dtb[k]);
   }
-  fill_prognostic(m, &m->tu[slot * m->L], &m->tv[slot * m->L], &m->th[slot * m->L]);
+  fill_prognostic4(m, &m->tu[slot * m->L], &m->tv[slot * m->L], &m->th[slot * m->L], &m->tb[slot * m->L]);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1145,6 +1340,7 @@ static void ab_update(or_model *m, int order)
     ab_field(&m->u[k], &m->tu[k], &m->tu[L + k], &m->tu[2 * L + k], order, m->cfg.dt);
     ab_field(&m->v[k], &m->tv[k], &m->tv[L + k], &m->tv[2 * L + k], order, m->cfg.dt);
     ab_field(&m->h[k], &m->th[k], &m->th[L + k], &m->th[2 * L + k], order, m->cfg.dt);
+    if (m->cfg.useTracer) ab_field(&m->b[k], &m->tb[k], &m->tb[L + k], &m->tb[2 * L + k], order, m->cfg.dt);
   }
 }
 
@@ -1164,6 +1360,8 @@ static void shift_history(or_model *m)
     copy_full(&m->tu[L + k], &m->tu[k]);
     copy_full(&m->tv[L + k], &m->tv[k]);
     copy_full(&m->th[L + k], &m->th[k]);
+    copy_full(&m->tb[2 * L + k], &m->tb[L + k]);
+    copy_full(&m->tb[L + k], &m->tb[k]);
   }
 }
 
@@ -1545,6 +1743,10 @@ bool or_write_state(or_model *m, const char *outdir, uint n)
     /* the interface velocity of the last tendency evaluation (diagnostics.c:884) */
     snprintf(name, sizeof(name), "W%u_n=%u.dat", k, n);
     ok = ok && write_named(outdir, name, &m->wdia[k]);
+    if (m->cfg.useTracer) {
+      snprintf(name, sizeof(name), "B%u_n=%u.dat", k, n);
+      ok = ok && write_named(outdir, name, &m->b[k]);
+    }
   }
   snprintf(name, sizeof(name), "W%u_n=%u.dat", m->L, n);
   ok = ok && write_named(outdir, name, &m->wdia[m->L]);

@@ -348,19 +348,56 @@ def with_restoring(s: Setup, seed: int = 11, wdia_recs: int = 2, relax: bool = T
                  notes=dict(s.notes, restoring=True), files=files)
 
 
+def with_tracer(s: Setup, scheme: int = 0, buoyancy: bool = False, K2: float = 0.0, K4: float = 0.0,
+                relax: bool = False, diadiff: bool = False, unstable: bool = False, seed: int = 13,
+                **params) -> Setup:
+    """Add a tracer b (a buoyancy anomaly when `buoyancy`) to a setup
+    (work.c:1947-2144, diapycnal.c:140-197): a smooth initial field with
+    noise, optionally restoring toward a shifted field over part of the
+    domain, a surface flux and interior diapycnal diffusivity, and
+    (`unstable`) a top-layer anomaly large enough to invert the
+    stratification in a patch, so that convDiff has something to enhance."""
+    L, nx, ny = s.nlay, s.nx, s.ny
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    rnd = uniform01(seed, L * nx * ny).reshape(L, nx, ny)
+    b = np.stack([2e-3 * np.sin(2 * np.pi * (X + 0.3 * k)) * np.cos(np.pi * Y) + 2e-4 * (rnd[k] - 0.5)
+                  for k in range(L)])
+    if unstable:
+        b[0] -= 0.05 * np.exp(-((X - 0.4) ** 2 + (Y - 0.6) ** 2) / 0.02)
+    files = dict(s.files)
+    files["bInitFile"] = b
+    day = 86400.0
+    if relax:
+        files["bRelaxFile"] = np.roll(b, 3, axis=1)
+        files["bTimeFile"] = np.stack([np.where(Y < 0.5, (2 + k) * day, 0.0) for k in range(L)])
+    if diadiff:
+        files["diaDiffFile"] = np.stack([np.where(X > 0.2, 1e-4 * (k > 0), 0.0) for k in range(L)])
+        files["bFluxFile"] = 1e-8 * np.cos(2 * np.pi * X)
+    p = dict(s.params)
+    p.update(useTracer=1, useBuoyancy=int(buoyancy), tracerScheme=scheme, K2=K2, K4=K4)
+    p.update(params)
+    return Setup(p, s.hhb, s.hhs, s.omegaz, s.gg, s.h, u=s.u, v=s.v, taux=s.taux, tauy=s.tauy, name=s.name,
+                 notes=dict(s.notes, tracer=True), files=files)
+
+
 def read_outputs(outdir: str, nx: int, ny: int, nlay: int, n: int, with_p: bool = True) -> Dict[str, np.ndarray]:
     """Load U/V/H (and P, and the interface velocity W [Nlay+1, Nx, Ny]) for
     save index n as [Nlay, Nx, Ny] arrays."""
     out = {}
     for name in ("U", "V", "H"):
         out[name] = np.stack([read_2d(os.path.join(outdir, f"{name}{k}_n={n}.dat"), nx, ny) for k in range(nlay)])
+    if os.path.exists(os.path.join(outdir, f"B0_n={n}.dat")):
+        out["B"] = np.stack([read_2d(os.path.join(outdir, f"B{k}_n={n}.dat"), nx, ny) for k in range(nlay)])
     if with_p:
         out["P"] = read_2d(os.path.join(outdir, f"P_n={n}.dat"), nx, ny)
         out["W"] = np.stack([read_2d(os.path.join(outdir, f"W{k}_n={n}.dat"), nx, ny) for k in range(nlay + 1)])
     return out
 
 
-AVG_PREFIXES = ("U_avg", "V_avg", "H_avg", "M_avg", "W_avg", "HU_avg", "HV_avg", "HUU_avg", "HVV_avg", "HUV_avg")
+AVG_PREFIXES = ("U_avg", "V_avg", "H_avg", "M_avg", "W_avg", "HU_avg", "HV_avg", "HUU_avg", "HVV_avg", "HUV_avg",
+                "B_avg")
 
 
 def write_restart_files(outdir: str, idx: int, fields: Dict[str, np.ndarray]) -> None:
@@ -381,6 +418,8 @@ def read_averages(outdir: str, nx: int, ny: int, nlay: int) -> Dict[int, Dict[st
             found[int(name[len("U_avg0_n="):-4])] = {}
     for n in found:
         for pre in AVG_PREFIXES:
+            if pre == "B_avg" and not os.path.exists(os.path.join(outdir, f"B_avg0_n={n}.dat")):
+                continue
             layers = nlay + 1 if pre == "W_avg" else nlay
             found[n][pre] = np.stack([read_2d(os.path.join(outdir, f"{pre}{k}_n={n}.dat"), nx, ny)
                                       for k in range(layers)])

@@ -38,7 +38,8 @@ def prepare_outdir(z, outdir):
     if "restart_from" in z.files:
         from pawsim_amd import inputs
 
-        inputs.write_restart_files(outdir, int(z["restart_from"]), {k: z[f"restart_in_{k}"] for k in ("U", "V", "H")})
+        keys = [k for k in ("U", "V", "H", "B") if f"restart_in_{k}" in z.files]
+        inputs.write_restart_files(outdir, int(z["restart_from"]), {k: z[f"restart_in_{k}"] for k in keys})
 
 
 def build_setup(name, spec):

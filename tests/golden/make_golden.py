@@ -83,6 +83,17 @@ def cases():
                                                       visc=False, convDiff=1e-3),
         "box16_wdia_3rec": inputs.with_restoring(inputs.gyre(nx=16, ny=16, nt=9, **T), relax=False, visc=False,
                                                  wdia_recs=3),
+        # tracers and active buoyancy (work.c:188-213, 1445-1451, 1745-1833, 1947-2144; diapycnal.c)
+        "box16_tracer_al81": inputs.with_tracer(inputs.gyre(nx=16, ny=16, nt=6, savefreqAvg=900.0, **T), K2=50.0),
+        "per16_tracer_up3_k4": inputs.with_tracer(inputs.periodic_box(nx=16, ny=16, nt=6, **T), scheme=1, K4=1e8,
+                                                  relax=True),
+        "chan32x16_tracer_kt00": inputs.with_tracer(inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1,
+                                                                   savefreqEZ=200.0, **T), scheme=2, diadiff=True),
+        "box16_buoyancy": inputs.with_tracer(inputs.gyre(nx=16, ny=16, nt=6, savefreqEZ=600.0, **T), buoyancy=True,
+                                             K2=20.0, diadiff=True),
+        "box16_buoyancy_convdiff": inputs.with_tracer(
+            inputs.with_restoring(inputs.gyre(nx=16, ny=16, nt=6, **T), relax=False, wdia_recs=2, convDiff=5e-3),
+            buoyancy=True, scheme=1, diadiff=True, relax=True, unstable=True),
     }
 
 
@@ -94,6 +105,7 @@ def restart_cases():
         "box16_restart": (inputs.gyre(nx=16, ny=16, nt=6, savefreqEZ=600.0, **T), 3),
         "chan32x16_restart_avg": (inputs.channel(nx=32, ny=16, nt=6, dx=20e3, solver=1, savefreqAvg=400.0,
                                                  thicknessScheme=2, **T), 2),
+        "box16_restart_tracer": (inputs.with_tracer(inputs.gyre(nx=16, ny=16, nt=6, **T), buoyancy=True, K2=20.0), 3),
     }
 
 
@@ -118,7 +130,7 @@ def make_case(name: str, setup: inputs.Setup, outdir: str, restart_from: int = -
             subprocess.run([REF, setup.write(os.path.join(work, "in_first")), first], capture_output=True, text=True,
                            check=True)
             restart_in = {k: v for k, v in inputs.read_outputs(first, setup.nx, setup.ny, setup.nlay, restart_from,
-                                                               with_p=False).items()}
+                                                               with_p=False).items()}  # U, V, H (and B)
             p = dict(setup.params)
             p.update(restart=1, startIdx=restart_from, Nt=int(p["Nt"]) - restart_from)
             setup = inputs.Setup(p, setup.hhb, setup.hhs, setup.omegaz, setup.gg, setup.h, u=setup.u, v=setup.v,
