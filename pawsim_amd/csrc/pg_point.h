@@ -40,6 +40,7 @@
 
 enum { PG_MOM_AL81 = 0, PG_MOM_HK83 = 1, PG_MOM_S75E = 2, PG_MOM_TW81 = 3 };
 enum { PG_THK_AL81 = 0, PG_THK_HK83 = 1, PG_THK_UP3 = 2, PG_THK_KT00 = 3 };
+enum { PG_TRC_AL81 = 0, PG_TRC_UP3 = 1, PG_TRC_KT00 = 2 };
 
 // Geometry and physics parameters, passed to kernels by value.
 struct PgGrid {
@@ -59,6 +60,8 @@ struct PgPhys {
   double A2fac, A4fac;     // Smagorinsky factors (work.c:1542-1543)
   double kt00;             // KT00 limiter slope factor
   int useRelax, useWDia, useDiaVisc;  // restoring, interface velocity, diapycnal viscosity (config.c:305-312)
+  int useTracer, useBuoyancy, useDiaDiff, trc;  // tracer b, b as buoyancy anomaly, diapycnal diffusion, scheme
+  double K2, K4, convDiff;             // tracer diffusivities, convective diapycnal enhancement
   double gg[PG_MAXL], geff[PG_MAXL];
 };
 
@@ -95,6 +98,12 @@ struct PgIn {
   const double* vTime = nullptr;   // v kind
   const double* nu_u = nullptr;    // [L][sx*sy] diapycnal viscosity on u points, u kind
   const double* nu_v = nullptr;    // v kind
+  // tracer: state and inputs (ghost-filled, scalar kind), NULL without a tracer
+  const double* b = nullptr;       // [L][sx*sy]
+  const double* bRelax = nullptr;  // [L][sx*sy]
+  const double* bTime = nullptr;   // [L][sx*sy]
+  const double* kappa = nullptr;   // [L][sx*sy] diapycnal diffusivity at interfaces
+  const double* Fsurf = nullptr;   // [sx*sy] surface buoyancy flux
 };
 
 PG_HD size_t pg_at(const PgGrid& G, int i, int j) { return (size_t)i * (size_t)G.sy + (size_t)j; }
@@ -318,13 +327,56 @@ PG_HD double pg_MM(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i
   eta[G.L] = I.hhb[p];
   for (int m = G.L - 1; m >= 1; m--) eta[m] = eta[m + 1] + I.h[pg_lay(G, m) + p];
   double pp = 0;
-  for (int m = 1; m <= k; m++) pp = pp + P.gg[m] * eta[m];
+  for (int m = 1; m <= k; m++) {
+    pp = pp + P.gg[m] * eta[m];
+    // active buoyancy jump across interface m (work.c:1445-1451)
+    if (P.useBuoyancy) pp += eta[m] * (I.b[pg_lay(G, m - 1) + p] - I.b[pg_lay(G, m) + p]);
+  }
   double mm = pp;
   if (P.h0 != 0) {
     const double hk = I.h[pg_lay(G, k) + p];
     mm -= P.geff[k] * (P.h0 * P.h0 * P.h0 * P.h0) / (hk * hk * hk) / 3;
   }
   return mm;
+}
+
+// Interface height m at (i, j), scalar kind (work.c:1173-1203).
+PG_HD double pg_eta(const PgGrid& G, const PgIn& I, int m, int i, int j)
+{
+  const size_t p = pg_at(G, pg_smap_x(G, i), pg_smap_y(G, j));
+  double e = I.hhb[p];
+  for (int q = G.L - 1; q >= m; q--) e = e + I.h[pg_lay(G, q) + p];
+  return e;
+}
+
+// Active-buoyancy work of layer k (work.c:188-213), computed on [1, s-1):
+// h times the layer mid-depth, and the buoyancy gradient along x / y.
+PG_HD double pg_hz(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0;
+  return PG_H(k, i, j) * 0.5 * (pg_eta(G, I, k, i, j) + pg_eta(G, I, k + 1, i, j));
+}
+PG_HD double pg_dbx(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0;
+  return (I.b[pg_lay(G, k) + pg_at(G, i, j)] - I.b[pg_lay(G, k) + pg_at(G, i - 1, j)]) / G.dx;
+}
+PG_HD double pg_dby(const PgGrid& G, const PgIn& I, int k, int i, int j)
+{
+  if (!pg_in_core(G, i, j)) return 0;
+  return (I.b[pg_lay(G, k) + pg_at(G, i, j)] - I.b[pg_lay(G, k) + pg_at(G, i, j - 1)]) / G.dy;
+}
+
+// Interface reduced gravity of interface k >= 1 (diapycnal.c:27-66), scalar
+// kind: gg plus the buoyancy jump with an active tracer; zero (never
+// negative) without convDiff.
+PG_HD double pg_gprime(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  if (P.convDiff == 0 || G.L < 2 || k < 1) return 0;
+  const size_t p = pg_at(G, pg_smap_x(G, i), pg_smap_y(G, j));
+  double gp = P.gg[k];
+  if (P.useBuoyancy) gp += I.b[pg_lay(G, k - 1) + p] - I.b[pg_lay(G, k) + p];
+  return gp;
 }
 
 // Kinetic energy in the Bernoulli function (work.c:1446-1497), scalar kind.
@@ -518,8 +570,7 @@ PG_HD double pg_uabs_bot(const PgGrid& G, const PgIn& I, int i, int j)
 // Diapycnal momentum terms (diapycnal.c:72-133, 203-291) added to the u
 // (dir 0) or v (dir 1) tendency r of layer k at the owned point (i, j):
 // upwind vertical advection by the interface velocity averaged to the face,
-// then the explicit diapycnal viscosity.  Without active buoyancy the
-// interface reduced gravity is gg >= 0, so convDiff adds nothing to nu.
+// then the explicit diapycnal viscosity.
 PG_HD double pg_diapycnal(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j, double r,
                           double hface, int dir)
 {
@@ -547,7 +598,12 @@ PG_HD double pg_diapycnal(const PgGrid& G, const PgPhys& P, const PgIn& I, int k
       if (kk < 1 || kk >= G.L) return 0.0;
       const double dz = dir ? 0.5 * (pg_hS(G, P, I, kk, i, j) + pg_hS(G, P, I, kk - 1, i, j))
                             : 0.5 * (pg_hW(G, P, I, kk, i, j) + pg_hW(G, P, I, kk - 1, i, j));
-      return (dz != 0) ? NU[pg_lay(G, kk) + pg_at(G, i, j)] * (XV(kk - 1) - XV(kk)) / dz : 0.0;
+      double nu = NU[pg_lay(G, kk) + pg_at(G, i, j)];
+      // convective enhancement where the interface is statically unstable
+      // on either side of the face (diapycnal.c:103-115)
+      if (P.convDiff != 0 && (pg_gprime(G, P, I, kk, i, j) < 0 || pg_gprime(G, P, I, kk, i - di, j - dj) < 0))
+        nu += P.convDiff;
+      return (dz != 0) ? nu * (XV(kk - 1) - XV(kk)) / dz : 0.0;
     };
     if (hface != 0) r += (F(k) - F(k + 1)) / hface;
   }
@@ -658,7 +714,9 @@ PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, in
       qf += 0.25 * pg_qq(G, I, k, i, j) * (hvv_00 + hvv_m0) + 0.25 * pg_qq(G, I, k, i, j + 1) * (hvv_0p + hvv_mp);
     }
     double a2 = 0, a4 = 0;
-    const double buoy = 0;
+    double buoy = 0;
+    if (P.useBuoyancy && hwest != 0)  // work.c:1745-1749
+      buoy = -0.5 * (pg_hz(G, I, k, i, j) + pg_hz(G, I, k, i - 1, j)) * pg_dbx(G, I, k, i, j);
     if (P.useA2) {
       a2 = ((pg_A2h(G, P, I, k, i, j) * PG_H(k, i, j) * pg_DDT(G, I, k, i, j)
              - pg_A2h(G, P, I, k, i - 1, j) * PG_H(k, i - 1, j) * pg_DDT(G, I, k, i - 1, j)) / dx
@@ -710,7 +768,9 @@ PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, in
       qf -= 0.25 * pg_qq(G, I, k, i, j) * (huu_00 + huu_0m) + 0.25 * pg_qq(G, I, k, i + 1, j) * (huu_p0 + huu_pm);
     }
     double a2 = 0, a4 = 0;
-    const double buoy = 0;
+    double buoy = 0;
+    if (P.useBuoyancy && hsouth != 0)  // work.c:1829-1833
+      buoy = -0.5 * (pg_hz(G, I, k, i, j) + pg_hz(G, I, k, i, j - 1)) * pg_dby(G, I, k, i, j);
     if (P.useA2) {
       a2 = ((pg_A2q(G, P, I, k, i + 1, j) * pg_hq(G, I, k, i + 1, j) * pg_DDS(G, I, k, i + 1, j)
              - pg_A2q(G, P, I, k, i, j) * pg_hq(G, I, k, i, j) * pg_DDS(G, I, k, i, j)) / dx
@@ -757,3 +817,111 @@ PG_HD void pg_tendency_point(const PgGrid& G, const PgPhys& P, const PgIn& I, in
     *dh = adv;
   }
 }
+
+// Tracer tendency of layer k at the owned point (i, j) (work.c:1947-2144):
+// advection (AL81 gradient form, or the UP3 / KT00 flux form divided by h
+// plus b times the mass-flux divergence), vertical advection by the
+// interface velocity (with the reduced-gravity jump for an active
+// buoyancy), diapycnal diffusion (diapycnal.c:140-197), K2 and K4
+// diffusion, restoring.  The work arrays (gradients, fluxes, BB4) are
+// computed on [1, s-1) and never refreshed, so each is the formula at its
+// position.
+PG_HD double pg_tracer_point(const PgGrid& G, const PgPhys& P, const PgIn& I, int k, int i, int j)
+{
+  const double dx = G.dx, dy = G.dy, dxsq = dx * dx, dysq = dy * dy;
+  const size_t lay = pg_lay(G, k);
+  auto B = [&](int a, int c) { return I.b[lay + pg_at(G, a, c)]; };
+  auto UU = [&](int a, int c) { return I.u[lay + pg_at(G, a, c)]; };
+  auto VV = [&](int a, int c) { return I.v[lay + pg_at(G, a, c)]; };
+  const double s = P.kt00;
+  auto DBX = [&](int a, int c) {
+    if (!pg_in_core(G, a, c)) return 0.0;
+    if (P.trc == PG_TRC_KT00)
+      return pg_minmod(s * (B(a + 1, c) - B(a, c)), 0.5 * (B(a + 1, c) - B(a - 1, c)), s * (B(a, c) - B(a - 1, c))) / dx;
+    return (B(a, c) - B(a - 1, c)) / dx;
+  };
+  auto DBY = [&](int a, int c) {
+    if (!pg_in_core(G, a, c)) return 0.0;
+    if (P.trc == PG_TRC_KT00)
+      return pg_minmod(s * (B(a, c + 1) - B(a, c)), 0.5 * (B(a, c + 1) - B(a, c - 1)), s * (B(a, c) - B(a, c - 1))) / dy;
+    return (B(a, c) - B(a, c - 1)) / dy;
+  };
+  auto D2X = [&](int a, int c) { return pg_in_core(G, a, c) ? B(a + 1, c) - 2 * B(a, c) + B(a - 1, c) : 0.0; };
+  auto D2Y = [&](int a, int c) { return pg_in_core(G, a, c) ? B(a, c + 1) - 2 * B(a, c) + B(a, c - 1) : 0.0; };
+  auto HUB = [&](int a, int c) {
+    if (!pg_in_core(G, a, c)) return 0.0;
+    double f;
+    if (P.trc == PG_TRC_UP3)
+      f = 0.5 * (B(a, c) + B(a - 1, c)) - (1 / 6.0) * ((UU(a, c) > 0) ? D2X(a - 1, c) : D2X(a, c));
+    else
+      f = (UU(a, c) > 0) ? B(a - 1, c) + 0.5 * DBX(a - 1, c) * dx : B(a, c) - 0.5 * DBX(a, c) * dx;
+    return f * pg_huu(G, P, I, k, a, c);
+  };
+  auto HVB = [&](int a, int c) {
+    if (!pg_in_core(G, a, c)) return 0.0;
+    double f;
+    if (P.trc == PG_TRC_UP3)
+      f = 0.5 * (B(a, c) + B(a, c - 1)) - (1 / 6.0) * ((VV(a, c) > 0) ? D2Y(a, c - 1) : D2Y(a, c));
+    else  // both KT00 states add the half slope (work.c:2037-2038)
+      f = (VV(a, c) > 0) ? B(a, c - 1) + 0.5 * DBY(a, c - 1) * dy : B(a, c) + 0.5 * DBY(a, c) * dy;
+    return f * pg_hvv(G, P, I, k, a, c);
+  };
+  auto BB4 = [&](int a, int c) {
+    if (!pg_in_core(G, a, c)) return 0.0;
+    return ((pg_hW(G, P, I, k, a + 1, c) * (B(a + 1, c) - B(a, c)) - pg_hW(G, P, I, k, a, c) * (B(a, c) - B(a - 1, c))) / dxsq
+            + (pg_hS(G, P, I, k, a, c + 1) * (B(a, c + 1) - B(a, c)) - pg_hS(G, P, I, k, a, c) * (B(a, c) - B(a, c - 1)))
+                  / dysq)
+           / PG_H(k, a, c);
+  };
+  const size_t p = pg_at(G, i, j);
+  const double hh = PG_H(k, i, j), bk = B(i, j);
+  double rhs = 0, adv = 0;
+  if (P.trc == PG_TRC_AL81) {
+    adv = -0.5 * (UU(i + 1, j) * DBX(i + 1, j) + UU(i, j) * DBX(i, j) + VV(i, j + 1) * DBY(i, j + 1) + VV(i, j) * DBY(i, j));
+  } else if (hh != 0) {
+    adv = -((HUB(i + 1, j) - HUB(i, j)) / dx + (HVB(i, j + 1) - HVB(i, j)) / dy) / hh
+          + bk
+                * ((pg_huu(G, P, I, k, i + 1, j) - pg_huu(G, P, I, k, i, j)) / dx
+                   + (pg_hvv(G, P, I, k, i, j + 1) - pg_hvv(G, P, I, k, i, j)) / dy)
+                / hh;
+  }
+  rhs += adv;
+  if (P.useWDia && hh != 0) {
+    const double w0 = I.wdia[pg_lay(G, k) + p], w1 = I.wdia[pg_lay(G, k + 1) + p];
+    const double b_p = (k == 0) ? 0 : I.b[pg_lay(G, k - 1) + p], b_m = (k == G.L - 1) ? 0 : I.b[pg_lay(G, k + 1) + p];
+    double db_p = (w0 > 0) ? 0 : b_p - bk, db_m = (w1 > 0) ? bk - b_m : 0;
+    if (P.useBuoyancy) {
+      db_p += (w0 > 0) ? 0 : P.gg[k];
+      db_m += ((k + 1 < G.L) && (w1 > 0)) ? P.gg[k + 1] : 0;
+    }
+    rhs += -(w0 * db_p + w1 * db_m) / hh;
+  }
+  if (P.useDiaDiff && hh != 0) {
+    auto FB = [&](int kk) {  // diapycnal buoyancy flux through interface kk
+      if (kk == 0) return I.Fsurf[p];
+      if (kk >= G.L) return 0.0;
+      double db = I.b[pg_lay(G, kk - 1) + p] - I.b[pg_lay(G, kk) + p];
+      const double dz = 0.5 * (I.h[pg_lay(G, kk) + p] + I.h[pg_lay(G, kk - 1) + p]);
+      double kap = I.kappa[pg_lay(G, kk) + p];
+      if (P.useBuoyancy) db += P.gg[kk];
+      if (P.convDiff != 0 && pg_gprime(G, P, I, kk, i, j) < 0) kap += P.convDiff;
+      return (dz != 0) ? kap * db / dz : 0.0;
+    };
+    rhs += (FB(k) - FB(k + 1)) / hh;
+  }
+  if (P.K2 > 0 && hh != 0)
+    rhs += P.K2
+           * ((pg_hW(G, P, I, k, i + 1, j) * (B(i + 1, j) - bk) - pg_hW(G, P, I, k, i, j) * (bk - B(i - 1, j))) / dxsq
+              + (pg_hS(G, P, I, k, i, j + 1) * (B(i, j + 1) - bk) - pg_hS(G, P, I, k, i, j) * (bk - B(i, j - 1))) / dysq)
+           / hh;
+  if (P.K4 > 0 && hh != 0)
+    rhs += -P.K4
+           * ((pg_hW(G, P, I, k, i + 1, j) * (BB4(i + 1, j) - BB4(i, j)) - pg_hW(G, P, I, k, i, j) * (BB4(i, j) - BB4(i - 1, j)))
+                  / dxsq
+              + (pg_hS(G, P, I, k, i, j + 1) * (BB4(i, j + 1) - BB4(i, j)) - pg_hS(G, P, I, k, i, j) * (BB4(i, j) - BB4(i, j - 1)))
+                    / dysq)
+           / hh;
+  if (I.bTime && I.bTime[lay + p] > 0) rhs += -(bk - I.bRelax[lay + p]) / I.bTime[lay + p];
+  return rhs;
+}
+

@@ -54,6 +54,13 @@ int main(int argc, char** argv)
   P.useRelax = m.cfg.useRelax;
   P.useWDia = m.cfg.useWDia;
   P.useDiaVisc = m.cfg.useDiaVisc;
+  P.useTracer = m.cfg.useTracer;
+  P.useBuoyancy = m.cfg.useTracer && m.cfg.useBuoyancy;
+  P.useDiaDiff = m.cfg.useDiaDiff;
+  P.trc = (int)m.cfg.trcScheme;
+  P.K2 = m.cfg.K2;
+  P.K4 = m.cfg.K4;
+  P.convDiff = m.cfg.convDiff;
   P.useWind = m.cfg.useWind;
   P.useLin = m.cfg.linDrag > 0;
   P.useQuad = m.cfg.quadDrag > 0;
@@ -93,6 +100,15 @@ int main(int argc, char** argv)
   I.vTime = vT;
   I.nu_u = nu;
   I.nu_v = nv;
+  double *bw = stack(m.bw, G.L), *bR = stack(m.bRelax, G.L), *bT = stack(m.bTime, G.L), *kap = stack(m.kappa, G.L);
+  double* fs_b = stack(&m.Fsurf_b, 1);
+  if (m.cfg.useTracer) {
+    I.b = bw;
+    I.bRelax = bR;
+    I.bTime = bT;
+    I.kappa = kap;
+    I.Fsurf = fs_b;
+  }
   long pts = 0, bad = 0;
   double maxabs = 0;
   for (int k = 0; k < G.L; k++)
@@ -100,9 +116,10 @@ int main(int argc, char** argv)
       for (int j = G.g; j < G.g + G.ny; j++) {
         double du, dv, dh;
         pg_tendency_point(G, P, I, k, i, j, &du, &dv, &dh);
-        const double ref[3] = {OF(&m.tu[k], i, j), OF(&m.tv[k], i, j), OF(&m.th[k], i, j)};
-        const double got[3] = {du, dv, dh};
-        for (int q = 0; q < 3; q++) {
+        const double db = m.cfg.useTracer ? pg_tracer_point(G, P, I, k, i, j) : 0.0;
+        const double ref[4] = {OF(&m.tu[k], i, j), OF(&m.tv[k], i, j), OF(&m.th[k], i, j), OF(&m.tb[k], i, j)};
+        const double got[4] = {du, dv, dh, db};
+        for (int q = 0; q < (m.cfg.useTracer ? 4 : 3); q++) {
           pts++;
           if (memcmp(&ref[q], &got[q], sizeof(double)) != 0 && !(ref[q] == 0 && got[q] == 0)) {
             if (bad < 5) fprintf(stderr, "field %d k=%d i=%d j=%d ref=%.17g got=%.17g\n", q, k, i, j, ref[q], got[q]);
@@ -152,7 +169,7 @@ int main(int argc, char** argv)
         }
       }
   printf("points=%ld mismatches=%ld maxabs=%g\n", pts, bad, maxabs);
-  for (double* a : {wdia, uR, uT, vR, vT, nu, nv}) free(a);
+  for (double* a : {wdia, uR, uT, vR, vT, nu, nv, bw, bR, bT, kap, fs_b}) free(a);
   free(u);
   free(v);
   free(h);
