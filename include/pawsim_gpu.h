@@ -74,8 +74,12 @@ typedef struct pg_config {
   double A2smag, A4smag;       /* Smagorinsky coefficients (work.c:1538-1610); 0 = off */
   uint32_t wDiaNrecs;          /* records in pg_static.wDia (config.c:96, 231); 0 means 1 */
   double wDiaPeriod;           /* period the records cycle over, > 0 with wDiaNrecs > 1 (config.c:92, 227) */
-  double convDiff;             /* convective diapycnal viscosity; without active buoyancy it only switches the
-                                  diapycnal viscosity on (config.c:93, 311-312) */
+  double convDiff;             /* convective diapycnal viscosity / diffusivity where an interface is statically
+                                  unstable; it also switches both on (config.c:93, 309-312) */
+  int32_t useTracer;           /* prognostic tracer b per layer (config.c:117, 250) */
+  int32_t useBuoyancy;         /* b is a buoyancy anomaly acting on the flow (needs useTracer) */
+  uint32_t tracerScheme;       /* 0 AL81, 1 UP3, 2 KT00 (defs.h:41-43) */
+  double K2, K4;               /* tracer Laplacian / biharmonic diffusivities (config.c:83-84) */
 } pg_config;
 
 /* Static inputs (state.c:1012-1047).  NULL hhs means a flat lid at 0;
@@ -96,6 +100,10 @@ typedef struct pg_static {
   const double *hTime;                             /* [Nx][Ny] thickness time scale, all layers */
   const double *wDia;                              /* [wDiaNrecs][Nlay+1][Nx][Ny] interface velocity */
   const double *nu_u_dia, *nu_v_dia;               /* [Nlay][Nx][Ny] diapycnal viscosity at interfaces */
+  /* Tracer inputs (state.c:1031-1039), used with useTracer, NULL when absent. */
+  const double *bRelax, *bTime;                    /* [Nlay][Nx][Ny] tracer restoring target and time scale */
+  const double *kappa_dia;                         /* [Nlay][Nx][Ny] diapycnal diffusivity at interfaces */
+  const double *Fsurf_b;                           /* [Nx][Ny] surface buoyancy flux */
 } pg_static;
 
 /* Per-solve statistics (context.h:52-61 plus the relative residual). */
@@ -125,6 +133,10 @@ const char *pg_last_error(const pg_handle *h);
  * (pawsim_state_exchange_prognostic_halos, state.c:929-950). */
 int pg_upload_state(pg_handle *h, const double *u, const double *v, const double *hh);
 int pg_download_state(pg_handle *h, double *u, double *v, double *hh);
+/* The tracer b ([Nlay][Nx][Ny]) of a handle with useTracer; upload also
+ * refreshes its halos (state.c:948). */
+int pg_upload_tracer(pg_handle *h, const double *b);
+int pg_download_tracer(pg_handle *h, double *b);
 int pg_upload_pi(pg_handle *h, const double *pi);
 int pg_download_pi(pg_handle *h, double *pi);
 

@@ -29,7 +29,8 @@ ABI_SYMBOLS = (
     "pg_create", "pg_destroy", "pg_last_error", "pg_upload_state", "pg_download_state",
     "pg_upload_pi", "pg_download_pi", "pg_tendency", "pg_step", "pg_pressure_correct",
     "pg_solve_pressure", "pg_refresh_halos", "pg_energy", "pg_average_add", "pg_average_take",
-    "pg_checkpoint_bytes", "pg_checkpoint_save", "pg_checkpoint_load", "pg_download_wdia", "pg_time", "pg_set_stream", "pg_synchronize",
+    "pg_checkpoint_bytes", "pg_checkpoint_save", "pg_checkpoint_load", "pg_download_wdia",
+    "pg_upload_tracer", "pg_download_tracer", "pg_time", "pg_set_stream", "pg_synchronize",
     "pg_bench_kernel",
     "pg_net_param_count", "pg_net_create", "pg_net_destroy", "pg_net_set_params", "pg_net_get_params",
     "pg_net_forward", "pg_net_loss_grad", "pg_net_adam_step",
@@ -53,6 +54,8 @@ class PgConfig(ctypes.Structure):
         ("tendency_kernel", ctypes.c_int32), ("mixed_precision", ctypes.c_int32), ("pi_guess", ctypes.c_uint32),
         ("KT00_sigma", ctypes.c_double), ("A2smag", ctypes.c_double), ("A4smag", ctypes.c_double),
         ("wDiaNrecs", ctypes.c_uint32), ("wDiaPeriod", ctypes.c_double), ("convDiff", ctypes.c_double),
+        ("useTracer", ctypes.c_int32), ("useBuoyancy", ctypes.c_int32), ("tracerScheme", ctypes.c_uint32),
+        ("K2", ctypes.c_double), ("K4", ctypes.c_double),
     ]
 
 
@@ -61,7 +64,8 @@ class PgConfig(ctypes.Structure):
 STATIC_FILES = (("uRelax", "uRelaxFile"), ("vRelax", "vRelaxFile"), ("hRelax", "hRelaxFile"),
                 ("eRelax", "eRelaxFile"), ("uTime", "uTimeFile"), ("vTime", "vTimeFile"), ("eTime", "eTimeFile"),
                 ("hTime", "hTimeFile"), ("wDia", "wDiaFile"), ("nu_u_dia", "diaViscUFile"),
-                ("nu_v_dia", "diaViscVFile"))
+                ("nu_v_dia", "diaViscVFile"), ("bRelax", "bRelaxFile"), ("bTime", "bTimeFile"),
+                ("kappa_dia", "diaDiffFile"), ("Fsurf_b", "bFluxFile"))
 
 
 class PgStatic(ctypes.Structure):
@@ -115,6 +119,8 @@ def load_library() -> ctypes.CDLL:
     lib.pg_last_error.argtypes = [ctypes.c_void_p]
     lib.pg_last_error.restype = ctypes.c_char_p
     lib.pg_upload_state.argtypes = [ctypes.c_void_p, d, d, d]
+    lib.pg_upload_tracer.argtypes = [ctypes.c_void_p, d]
+    lib.pg_download_tracer.argtypes = [ctypes.c_void_p, d]
     lib.pg_download_state.argtypes = [ctypes.c_void_p, d, d, d]
     lib.pg_upload_pi.argtypes = [ctypes.c_void_p, d]
     lib.pg_download_pi.argtypes = [ctypes.c_void_p, d]
@@ -220,6 +226,11 @@ class Model:
         c.wDiaNrecs = int(p.get("wDiaNrecs", 1))
         c.wDiaPeriod = float(p.get("wDiaPeriod", 0.0))
         c.convDiff = float(p.get("convDiff", 0.0))
+        c.useTracer = int(p.get("useTracer", 0))
+        c.useBuoyancy = int(p.get("useBuoyancy", 0))
+        c.tracerScheme = int(p.get("tracerScheme", 0))
+        c.K2 = float(p.get("K2", 0.0))
+        c.K4 = float(p.get("K4", 0.0))
         self.cfg = c
         self.shape = (c.Nlay, c.Nx, c.Ny)
         files = files or {}
@@ -241,6 +252,8 @@ class Model:
         L, X, Y = m.shape
         z = np.zeros((L, X, Y))
         m.upload_state(setup.u if setup.u is not None else z, setup.v if setup.v is not None else z, setup.h)
+        if m.cfg.useTracer:
+            m.upload_tracer(getattr(setup, "files", {}).get("bInitFile", z))
         return m
 
     def _check(self, rc: int, what: str, allow_noconv: bool = False) -> int:
@@ -303,7 +316,8 @@ class Model:
     def average_take(self, scale: float) -> dict:
         """Scaled sums {"U_avg": [L, Nx, Ny], ..., "W_avg": [L + 1, Nx, Ny], "P_avg": [Nx, Ny]}; resets them."""
         L, nx, ny = self.shape
-        out = np.empty((10 * L + 2, nx, ny))
+        nb = L if self.cfg.useTracer else 0
+        out = np.empty((10 * L + 2 + nb, nx, ny))
         self._check(self.lib.pg_average_take(self.h, scale, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
                     "pg_average_take")
         names = ("U_avg", "V_avg", "H_avg", "M_avg", "W_avg", "HU_avg", "HV_avg", "HUU_avg", "HVV_avg", "HUV_avg")
@@ -311,7 +325,18 @@ class Model:
         res = {n: per[:, q].copy() for q, n in enumerate(names)}
         res["W_avg"] = np.concatenate([res["W_avg"], out[10 * L][None]], axis=0)
         res["P_avg"] = out[10 * L + 1].copy()
+        if nb:
+            res["B_avg"] = out[10 * L + 2:].copy()
         return res
+
+    def upload_tracer(self, b) -> None:
+        b = _c64(b)
+        self._check(self.lib.pg_upload_tracer(self.h, _dptr(b)), "pg_upload_tracer")
+
+    def tracer(self) -> np.ndarray:
+        b = np.empty(self.shape)
+        self._check(self.lib.pg_download_tracer(self.h, _dptr(b)), "pg_download_tracer")
+        return b
 
     def checkpoint(self) -> bytes:
         """Full-fidelity checkpoint blob (state with ghosts, AB history, pi,

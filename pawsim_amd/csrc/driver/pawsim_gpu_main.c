@@ -38,14 +38,17 @@ typedef struct drv_config {
   char hInit[256], uInit[256], vInit[256], hbFile[256], hsFile[256], omFile[256], gFile[256];
   char tauxFile[256], tauyFile[256];
   /* restoring and diapycnal inputs (config.c:267-281), in pg_static order */
-  char rsFile[11][256];
+  char rsFile[16][256];
 } drv_config;
 
 /* Parameter keys of drv_config.rsFile, the pg_static fields they fill:
- * uRelax vRelax hRelax eRelax uTime vTime eTime hTime wDia nu_u_dia nu_v_dia. */
-static const char *const RS_KEYS[11] = {"uRelaxFile", "vRelaxFile", "hRelaxFile", "eRelaxFile", "uTimeFile",
-                                        "vTimeFile", "eTimeFile", "hTimeFile", "wDiaFile", "diaViscUFile",
-                                        "diaViscVFile"};
+ * uRelax vRelax hRelax eRelax uTime vTime eTime hTime wDia nu_u_dia nu_v_dia
+ * bRelax bTime kappa_dia Fsurf_b, and the initial tracer. */
+#define N_RS 16
+static const char *const RS_KEYS[N_RS] = {"uRelaxFile", "vRelaxFile", "hRelaxFile", "eRelaxFile", "uTimeFile",
+                                          "vTimeFile", "eTimeFile", "hTimeFile", "wDiaFile", "diaViscUFile",
+                                          "diaViscVFile", "bRelaxFile", "bTimeFile", "diaDiffFile", "bFluxFile",
+                                          "bInitFile"};
 
 static double now_s(void)
 {
@@ -80,7 +83,7 @@ static void defaults(drv_config *c)
 
 static int rs_key(const char *key)
 {
-  for (int q = 0; q < 11; q++)
+  for (int q = 0; q < N_RS; q++)
     if (!strcmp(key, RS_KEYS[q])) return q;
   return -1;
 }
@@ -168,8 +171,12 @@ static bool read_config(const char *fname, drv_config *c)
     else if (!strcmp(key, "startIdx")) c->startIdx = (unsigned)uv;
     else if (!strcmp(key, "saveCheckpoint")) c->saveCheckpoint = iv != 0;
     else if (!strcmp(key, "restartFromCheckpoint")) c->restartCheckpoint = iv != 0;
-    else if (!strcmp(key, "useTracer") || !strcmp(key, "useBuoyancy")
-             || !strcmp(key, "windFeedback") || !strcmp(key, "useRandomForcing") || !strcmp(key, "useTrad")) {
+    else if (!strcmp(key, "useTracer")) c->pg.useTracer = iv != 0;
+    else if (!strcmp(key, "useBuoyancy")) c->pg.useBuoyancy = iv != 0;
+    else if (!strcmp(key, "tracerScheme")) c->pg.tracerScheme = (uint32_t)uv;
+    else if (!strcmp(key, "K2")) c->pg.K2 = dv;
+    else if (!strcmp(key, "K4")) c->pg.K4 = dv;
+    else if (!strcmp(key, "windFeedback") || !strcmp(key, "useRandomForcing") || !strcmp(key, "useTrad")) {
       if (iv != 0) note_unsupported(c, key);
     } else if (!strcmp(key, "FbaroXFile") || !strcmp(key, "FbaroYFile")) {
       note_unsupported(c, key);
@@ -237,14 +244,14 @@ static double *read_raw(const drv_config *c, const char *name, unsigned m, unsig
 /* pawsim_state_read_restart (state.c:1106-1155): U/V/H of save idx from
  * the output directory overwrite the initial state ([L][Nx][Ny] each). */
 static bool read_restart(const char *dir, unsigned idx, unsigned Nx, unsigned Ny, unsigned L, double *u, double *v,
-                         double *hh)
+                         double *hh, double *b)
 {
-  static const char *names[3] = {"U", "V", "H"};
-  double *dst[3] = {u, v, hh};
+  static const char *names[4] = {"U", "V", "H", "B"};
+  double *dst[4] = {u, v, hh, b};
   const size_t N = (size_t)Nx * Ny;
   char path[1024];
   for (unsigned k = 0; k < L; k++)
-    for (int f = 0; f < 3; f++) {
+    for (int f = 0; f < (b ? 4 : 3); f++) {
       double *a;
       snprintf(path, sizeof(path), "%s/%s%u_n=%u.dat", dir, names[f], k, idx);
       if (!(a = read_raw_path(path, Nx, Ny, 1))) return false;
@@ -271,7 +278,7 @@ static bool write_field(const char *dir, const char *name, const double *a, unsi
 typedef struct model {
   drv_config c;
   pg_handle *h;
-  double *u, *v, *hh, *pi, *zero, *wdia, t;
+  double *u, *v, *hh, *b, *pi, *zero, *wdia, t;
   unsigned n_saves, solves, iters_total;
   double solve_ms;
   unsigned n_EZ;
@@ -373,7 +380,8 @@ static bool averages(model *M, const char *dir)
     if (pg_average_add(M->h, avg_fac * dt) != PG_OK) return false;
   }
   if (M->t < M->t_next_avg) return true;
-  if (!M->avg && !(M->avg = malloc((10 * (size_t)L + 2) * N * sizeof(double)))) return false;
+  const size_t nb = M->c.pg.useTracer ? L : 0;  /* B averages follow P */
+  if (!M->avg && !(M->avg = malloc((10 * (size_t)L + 2 + nb) * N * sizeof(double)))) return false;
   if (pg_average_take(M->h, 1 / freq, M->avg) != PG_OK) return false;
   for (unsigned k = 0; k < L; k++)
     for (unsigned q = 0; q < 10; q++) {
@@ -384,6 +392,10 @@ static bool averages(model *M, const char *dir)
   if (!write_field(dir, name, M->avg + (size_t)10 * L * N, Nx, Ny)) return false;
   snprintf(name, sizeof(name), "P_avg_n=%u.dat", M->n_avg);  /* rigid lid (useRL) */
   if (!write_field(dir, name, M->avg + ((size_t)10 * L + 1) * N, Nx, Ny)) return false;
+  for (unsigned k = 0; k < nb; k++) {
+    snprintf(name, sizeof(name), "B_avg%u_n=%u.dat", k, M->n_avg);
+    if (!write_field(dir, name, M->avg + ((size_t)10 * L + 2 + k) * N, Nx, Ny)) return false;
+  }
   M->n_avg++;
   M->t_next_avg = M->n_avg * freq;
   return true;
@@ -421,7 +433,7 @@ static bool write_state(model *M, const char *dir, unsigned n)
   const size_t N = (size_t)Nx * Ny;
   char name[256];
   if (pg_download_state(M->h, M->u, M->v, M->hh) || pg_download_pi(M->h, M->pi)
-      || pg_download_wdia(M->h, M->wdia))
+      || pg_download_wdia(M->h, M->wdia) || (M->c.pg.useTracer && pg_download_tracer(M->h, M->b)))
     return false;
   for (unsigned k = 0; k < L; k++) {
     snprintf(name, sizeof(name), "U%u_n=%u.dat", k, n);
@@ -433,6 +445,10 @@ static bool write_state(model *M, const char *dir, unsigned n)
     /* the interface velocity of the last tendency evaluation (diagnostics.c:884) */
     snprintf(name, sizeof(name), "W%u_n=%u.dat", k, n);
     if (!write_field(dir, name, M->wdia + k * N, Nx, Ny)) return false;
+    if (M->c.pg.useTracer) {
+      snprintf(name, sizeof(name), "B%u_n=%u.dat", k, n);
+      if (!write_field(dir, name, M->b + k * N, Nx, Ny)) return false;
+    }
   }
   snprintf(name, sizeof(name), "W%u_n=%u.dat", L, n);
   if (!write_field(dir, name, M->wdia + L * N, Nx, Ny)) return false;
@@ -537,10 +553,11 @@ int main(int argc, char **argv)
   M.v = c->vInit[0] ? read_raw(c, c->vInit, Nx, Ny, L) : calloc(L * N, sizeof(double));
   M.hh = c->hInit[0] ? read_raw(c, c->hInit, Nx, Ny, L) : calloc(L * N, sizeof(double));
   /* restoring and diapycnal inputs (state.c:1027-1041) */
-  double *rs[11] = {NULL};
-  for (int q = 0; q < 11; q++) {
+  double *rs[N_RS] = {NULL};
+  for (int q = 0; q < N_RS; q++) {
     if (!c->rsFile[q][0]) continue;
-    if (q == 7) {  /* hTime: one field */
+    if (q >= 11 && !c->pg.useTracer) continue;  /* tracer inputs are read only with a tracer (state.c:1020-1039) */
+    if (q == 7 || q == 14) {  /* hTime, Fsurf_b: one field */
       rs[q] = read_raw(c, c->rsFile[q], Nx, Ny, 1);
     } else if (q == 8) {  /* wDia: readMatrix4 [nrecs][Nlay+1][Nx][Ny], records fastest (defs.c:327-420) */
       const unsigned R = c->pg.wDiaNrecs ? c->pg.wDiaNrecs : 1, K = L + 1;
@@ -566,6 +583,11 @@ int main(int argc, char **argv)
   st.wDia = rs[8];
   st.nu_u_dia = rs[9];
   st.nu_v_dia = rs[10];
+  st.bRelax = rs[11];
+  st.bTime = rs[12];
+  st.kappa_dia = rs[13];
+  st.Fsurf_b = rs[14];
+  M.b = rs[15] ? rs[15] : calloc(L * N, sizeof(double));  /* the initial tracer, zero without bInitFile */
   M.pi = calloc(N, sizeof(double));
   M.zero = calloc(N, sizeof(double));
   M.wdia = calloc((L + 1) * N, sizeof(double));
@@ -581,8 +603,11 @@ int main(int argc, char **argv)
     return 1;
   }
   /* readParameterData: a restart overwrites U/V/H from the outputs (pawsim.c:251-254) */
-  if (c->restart && !c->restartCheckpoint && !read_restart(argv[2], c->startIdx, Nx, Ny, L, M.u, M.v, M.hh)) return 1;
-  if (pg_upload_state(M.h, M.u, M.v, M.hh) != PG_OK) {
+  if (c->restart && !c->restartCheckpoint
+      && !read_restart(argv[2], c->startIdx, Nx, Ny, L, M.u, M.v, M.hh, c->pg.useTracer ? M.b : NULL))
+    return 1;
+  if (pg_upload_state(M.h, M.u, M.v, M.hh) != PG_OK
+      || (c->pg.useTracer && pg_upload_tracer(M.h, M.b) != PG_OK)) {
     fprintf(stderr, "ERROR: state upload failed: %s\n", pg_last_error(M.h));
     return 1;
   }
@@ -662,6 +687,7 @@ int main(int argc, char **argv)
   pg_destroy(M.h);
   free(hhb); free(hhs); free(om); free(gg); free(taux); free(tauy);
   free(M.u); free(M.v); free(M.hh); free(M.pi); free(M.zero); free(M.wdia); free(M.avg);
-  for (int q = 0; q < 11; q++) free(rs[q]);
+  for (int q = 0; q < N_RS - 1; q++) free(rs[q]);
+  free(M.b);
   return 0;
 }

@@ -126,6 +126,11 @@ int validate(const pg_config* c, std::string& why)
     return why = "mixed_precision needs A4 = 0, Nlay <= 8, Nx and Ny >= 8, the tiled tendency kernel, "
                  "the AL81 thickness scheme, no Smagorinsky viscosity and no restoring or diapycnal terms", PG_EARG;
   if (c->convDiff < 0 || c->wDiaPeriod < 0) return why = "convDiff and wDiaPeriod must be non-negative", PG_EARG;
+  if (c->useBuoyancy && !c->useTracer) return why = "useBuoyancy requires useTracer (config.c:421-425)", PG_EARG;
+  if (c->tracerScheme > 2 || c->K2 < 0 || c->K4 < 0) return why = "unknown tracerScheme or negative K2 / K4", PG_EARG;
+  if (c->useTracer && c->tracerScheme == 2 && !(c->KT00_sigma > 0))
+    return why = "KT00_sigma must be positive with the KT00 tracer scheme (the reference's default is 1.4)", PG_EARG;
+  if (c->mixed_precision && c->useTracer) return why = "mixed_precision does not serve a tracer", PG_EARG;
   return PG_OK;
 }
 

@@ -7,7 +7,7 @@
 // everything a step reads that the previous steps wrote, so that loading it
 // and stepping on gives the uninterrupted run bit for bit:
 //
-//   * u, v, h and pi with their ghost cells (the serial halo fill leaves
+//   * u, v, h, pi and the tracer b with their ghost cells (the serial halo fill leaves
 //     corner ghosts one fill behind on periodic-x / wall-y domains, and the
 //     stencils read some of them);
 //   * the three tendency slots in history order (FP64, or FP32 in mixed
@@ -50,6 +50,7 @@ std::vector<Part> parts(pg_handle* h, bool pi_hist, bool avg)
   const PgGrid& G = h->G;
   const size_t lf = (size_t)G.L * h->fsz, N = (size_t)G.nx * G.ny;
   std::vector<Part> p = {{h->u, lf * 8}, {h->v, lf * 8}, {h->h, lf * 8}, {h->pi, h->fsz * 8}};
+  if (h->b) p.push_back({h->b, lf * 8});
   for (int s = 0; s < 3; s++) {
     const int b = h->slot[s];
     if (h->mixed) {
@@ -61,9 +62,10 @@ std::vector<Part> parts(pg_handle* h, bool pi_hist, bool avg)
       p.push_back({h->fv[b], lf * 8});
       p.push_back({h->fh[b], lf * 8});
     }
+    if (h->b) p.push_back({h->fb[b], lf * 8});
   }
   if (pi_hist) p.push_back({h->pi_hist, 2 * N * 8});
-  if (avg) p.push_back({h->avg, ((size_t)AV_PER_LAYER * G.L + 2) * N * 8});
+  if (avg) p.push_back({h->avg, ((size_t)AV_PER_LAYER * G.L + 2 + (h->b ? G.L : 0)) * N * 8});
   return p;
 }
 
@@ -128,8 +130,8 @@ extern "C" int pg_checkpoint_load(pg_handle* h, const void* buf, size_t bytes)
   }
   const size_t N = (size_t)h->G.nx * h->G.ny;
   if (hd.has_pi_hist && !h->pi_hist) PG_CHECK(h, hipMalloc(&h->pi_hist, 2 * N * sizeof(double)));
-  if (hd.has_avg && !h->avg)
-    PG_CHECK(h, hipMalloc(&h->avg, ((size_t)AV_PER_LAYER * h->G.L + 2) * N * sizeof(double)));
+  const size_t navg = ((size_t)AV_PER_LAYER * h->G.L + 2 + (h->b ? h->G.L : 0)) * N;
+  if (hd.has_avg && !h->avg) PG_CHECK(h, hipMalloc(&h->avg, navg * sizeof(double)));
   h->slot[0] = 0;  // history comes back in order
   h->slot[1] = 1;
   h->slot[2] = 2;
@@ -144,7 +146,7 @@ extern "C" int pg_checkpoint_load(pg_handle* h, const void* buf, size_t bytes)
     in += q.bytes;
   }
   if (!hd.has_avg && h->avg) {  // no averaging in progress at the checkpoint: empty sums
-    PG_CHECK(h, hipMemsetAsync(h->avg, 0, ((size_t)AV_PER_LAYER * h->G.L + 2) * N * sizeof(double), h->stream));
+    PG_CHECK(h, hipMemsetAsync(h->avg, 0, navg * sizeof(double), h->stream));
   }
   PG_CHECK(h, hipStreamSynchronize(h->stream));
   h->t = hd.t;

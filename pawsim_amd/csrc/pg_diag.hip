@@ -25,7 +25,7 @@ namespace {
 constexpr int EJ = 64, EI = 4;
 
 __global__ __launch_bounds__(EJ* EI) void k_energy(PgGrid G, PgPhys P, const double* u, const double* v, const double* hh,
-                                                   const double* hhb, const double* omz, double* part)
+                                                   const double* hhb, const double* omz, const double* b, double* part)
 {
   const int k = blockIdx.z;
   const int gj = blockIdx.x * EJ + threadIdx.x, gi = blockIdx.y * EI + threadIdx.y;
@@ -82,7 +82,8 @@ __global__ __launch_bounds__(EJ* EI) void k_energy(PgGrid G, PgPhys P, const dou
     const size_t p = pg_at(G, i, j);
     double eta = hhb[p];
     for (int m = G.L - 1; m >= k; m--) eta = eta + hh[pg_lay(G, m) + p];
-    const double gtot = P.geff[k], hk = H(i, j);
+    // active buoyancy lowers the layer's reduced gravity (diagnostics.c:653-655)
+    const double gtot = P.useBuoyancy ? P.geff[k] - b[lay + pg_at(G, i, j)] : P.geff[k], hk = H(i, j);
     pe = dx * dy * gtot * hk * (eta - 0.5 * hk);
     pe = pe + dx * dy * gtot * (P.h0 * P.h0 * P.h0 * P.h0) / (hk * hk) / 6;
   }
@@ -141,7 +142,8 @@ constexpr int AV_PER_LAYER = 10, AV_LMAX = 16;
 
 __global__ __launch_bounds__(256) void k_avg_accumulate(PgGrid G, PgPhys P, const double* u, const double* v,
                                                         const double* hh, const double* hhb, const double* pi,
-                                                        const double* wdia, double* avg, double dtfac, int useRL)
+                                                        const double* wdia, const double* b, double* avg, double dtfac,
+                                                        int useRL)
 {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
   if (j >= G.ny) return;
@@ -157,7 +159,10 @@ __global__ __launch_bounds__(256) void k_avg_accumulate(PgGrid G, PgPhys P, cons
   double pp = 0;
   for (int k = 0; k < L; k++) {
     // Montgomery potential MM_B (work.c:1437-1504)
-    if (k > 0) pp = pp + P.gg[k] * eta[k];
+    if (k > 0) {
+      pp = pp + P.gg[k] * eta[k];
+      if (P.useBuoyancy) pp += eta[k] * (b[pg_lay(G, k - 1) + c] - b[pg_lay(G, k) + c]);
+    }
     double mm = pp;
     const double hk = H(k, gi, gj);
     if (P.h0 != 0) mm -= P.geff[k] * (P.h0 * P.h0 * P.h0 * P.h0) / (hk * hk * hk) / 3;
@@ -178,6 +183,7 @@ __global__ __launch_bounds__(256) void k_avg_accumulate(PgGrid G, PgPhys P, cons
     a[7 * N] += dtfac * hw * uk * uk;
     a[8 * N] += dtfac * hs * vk * vk;
     a[9 * N] += dtfac * hq * uq * vq;
+    if (b) avg[((size_t)AV_PER_LAYER * L + 2 + k) * N + q] += dtfac * b[pg_lay(G, k) + c];
   }
   avg[(size_t)AV_PER_LAYER * L * N + q] += dtfac * (wdia ? wdia[pg_lay(G, L) + c] : 0.0);
   avg[((size_t)AV_PER_LAYER * L + 1) * N + q] += dtfac * pi[c];
@@ -195,7 +201,7 @@ extern "C" int pg_average_add(pg_handle* h, double weight)
   if (!h) return PG_EARG;
   const PgGrid& G = h->G;
   if (G.L > AV_LMAX) return PG_EARG;
-  const size_t n = ((size_t)AV_PER_LAYER * G.L + 2) * G.nx * G.ny;
+  const size_t n = ((size_t)AV_PER_LAYER * G.L + 2 + (h->b ? G.L : 0)) * G.nx * G.ny;
   if (!h->avg) {
     PG_CHECK(h, hipMalloc(&h->avg, n * sizeof(double)));
     PG_CHECK(h, hipMemsetAsync(h->avg, 0, n * sizeof(double), h->stream));
@@ -203,7 +209,7 @@ extern "C" int pg_average_add(pg_handle* h, double weight)
   PG_CHECK(h, pg_ensure_halos(h));
   h->corner_fill_due = true;  // the averages reload the work arrays (diagnostics.c:320)
   hipLaunchKernelGGL(k_avg_accumulate, dim3((G.ny + 255) / 256, G.nx), dim3(256), 0, h->stream, G, h->P, h->u, h->v, h->h,
-                     h->hhb, h->pi, h->rs_wdia, h->avg, weight, 1);
+                     h->hhb, h->pi, h->rs_wdia, h->b, h->avg, weight, 1);
   PG_CHECK(h, hipGetLastError());
   return PG_OK;
 }
@@ -212,7 +218,7 @@ extern "C" int pg_average_take(pg_handle* h, double scale, double* out)
 {
   if (!h || !out) return PG_EARG;
   const PgGrid& G = h->G;
-  const size_t n = ((size_t)AV_PER_LAYER * G.L + 2) * G.nx * G.ny;
+  const size_t n = ((size_t)AV_PER_LAYER * G.L + 2 + (h->b ? G.L : 0)) * G.nx * G.ny;
   if (!h->avg) {  // nothing accumulated yet: zero sums
     for (size_t p = 0; p < n; p++) out[p] = 0;
     return PG_OK;
@@ -242,7 +248,8 @@ extern "C" int pg_energy(pg_handle* h, double* KE, double* PE, double* Z)
   }
   PG_CHECK(h, pg_ensure_halos(h));
   double* out = h->diag + (size_t)nb * G.L * 3;
-  hipLaunchKernelGGL(k_energy, grid, dim3(EJ, EI), 0, h->stream, G, h->P, h->u, h->v, h->h, h->hhb, h->omz, h->diag);
+  hipLaunchKernelGGL(k_energy, grid, dim3(EJ, EI), 0, h->stream, G, h->P, h->u, h->v, h->h, h->hhb, h->omz, h->b,
+                     h->diag);
   hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(64), 0, h->stream, h->diag, nb, G.L, out);
   PG_CHECK(h, hipGetLastError());
   std::vector<double> res(2 + G.L);

@@ -67,6 +67,11 @@ struct pg_handle {
   double *rs_nu_u = nullptr, *rs_nu_v = nullptr;
   int wdia_nrecs = 1;
   bool wdia_file = false;
+  // tracer b (useTracer): state, FP64 tendency history slots (same rotation
+  // as u, v, h), restoring / diffusion inputs
+  double* b = nullptr;
+  double* fb[3] = {};
+  double *rs_bRelax = nullptr, *rs_bTime = nullptr, *rs_kappa = nullptr, *rs_Fsurf = nullptr;
   bool rhs_mean_pending = false;       // ctl->mean is still to be removed from rhs (by the MG gather)
   double *hhb = nullptr, *hhs = nullptr, *omz = nullptr, *taux = nullptr, *tauy = nullptr;
   double *Hc = nullptr, *Hw = nullptr, *Hs = nullptr;  // ghosted static geometry
@@ -119,6 +124,7 @@ int pg_restoring_setup(pg_handle* h, const pg_static* st);
 void pg_restoring_free(pg_handle* h);
 hipError_t pg_launch_wdia(pg_handle* h);     // interface velocity at h->t (restoring.c:29-166)
 PgIn pg_tendency_inputs(const pg_handle* h);
+hipError_t pg_launch_tracer(pg_handle* h, double* db);  // tracer tendency (work.c:1947-2144)
 hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh);
 hipError_t pg_launch_tendency_slot(pg_handle* h, int s);  // into history slot s, per precision mode
 hipError_t pg_widen_slot(pg_handle* h, int s);            // FP32 history slot -> FP64 slot

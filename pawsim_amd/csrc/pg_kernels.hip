@@ -119,10 +119,10 @@ __global__ __launch_bounds__(TJ* TI) void k_tendency(PgGrid G, PgPhys P, PgIn I,
 // Adams-Bashforth update of u, v, h in place (integrator.c:15-70).
 // ---------------------------------------------------------------------------
 struct AbArgs {
-  double* x[3];
-  const double* f0[3];
-  const double* f1[3];
-  const double* f2[3];
+  double* x[4];  // u, v, h and, with a tracer, b (integrator.c:75-125)
+  const double* f0[4];
+  const double* f1[4];
+  const double* f2[4];
   double c0, c1, c2;
   int order;
 };
@@ -673,6 +673,10 @@ hipError_t pg_fill(pg_handle* h, double* base, int nfields, int kind)
 hipError_t pg_fill_prognostic(pg_handle* h)
 {
   const PgGrid& G = h->G;
+  if (h->P.useTracer) {  // b with the prognostic fields (state.c:948)
+    hipError_t e = pg_fill(h, h->b, G.L, PG_HALO_SCALAR);
+    if (e != hipSuccess) return e;
+  }
   if (!(!G.wallx && G.wally) && G.nx > 2 * G.g && G.ny > 2 * G.g) {
     const int T = (2 * G.g + 1) * (G.sy + G.nx - 1);
     hipLaunchKernelGGL(k_fill_direct, dim3((T + 255) / 256, 3 * G.L), dim3(256), 0, h->stream, h->u, h->v, h->h, h->fsz, G);
@@ -686,7 +690,9 @@ hipError_t pg_fill_prognostic(pg_handle* h)
 hipError_t pg_launch_tendency_slot(pg_handle* h, int s)
 {
   if (h->mixed) return pg_launch_tendency32(h, h->fu32[s], h->fv32[s], h->fh32[s]);
-  return pg_launch_tendency(h, h->fu[s], h->fv[s], h->fh[s]);
+  hipError_t e = pg_launch_tendency(h, h->fu[s], h->fv[s], h->fh[s]);
+  if (e == hipSuccess && h->P.useTracer) e = pg_launch_tracer(h, h->fb[s]);
+  return e;
 }
 
 hipError_t pg_widen_slot(pg_handle* h, int s)
@@ -735,9 +741,10 @@ hipError_t pg_launch_ab(pg_handle* h, int order, bool fuse_thickness)
     return hipGetLastError();
   }
   AbArgs A;
-  double* xs[3] = {h->u, h->v, h->h};
-  double** fs[3] = {h->fu, h->fv, h->fh};
-  for (int f = 0; f < 3; f++) {
+  double* xs[4] = {h->u, h->v, h->h, h->b};
+  double** fs[4] = {h->fu, h->fv, h->fh, h->fb};
+  const int nf = h->P.useTracer ? 4 : 3;
+  for (int f = 0; f < nf; f++) {
     A.x[f] = xs[f];
     A.f0[f] = fs[f][h->slot[0]];
     A.f1[f] = fs[f][h->slot[1]];
@@ -752,7 +759,7 @@ hipError_t pg_launch_ab(pg_handle* h, int order, bool fuse_thickness)
   } else {
     A.c0 = 23 * dt / 12; A.c1 = 4 * dt / 3; A.c2 = 5 * dt / 12;
   }
-  hipLaunchKernelGGL(k_ab, owned_grid(h->G, 3 * h->G.L), dim3(TJ, TI), 0, h->stream, h->G, A);
+  hipLaunchKernelGGL(k_ab, owned_grid(h->G, nf * h->G.L), dim3(TJ, TI), 0, h->stream, h->G, A);
   return hipGetLastError();
 }
 

@@ -58,6 +58,14 @@ __global__ __launch_bounds__(256) void k_wdia(PgGrid G, WdiaArgs A)
   }
 }
 
+// Tracer tendency at every owned point of every layer (work.c:1947-2144).
+__global__ __launch_bounds__(256) void k_tracer(PgGrid G, PgPhys P, PgIn I, double* db)
+{
+  const int j = G.g + blockIdx.x * blockDim.x + threadIdx.x, i = G.g + blockIdx.y, k = blockIdx.z;
+  if (j >= G.g + G.ny) return;
+  db[pg_lay(G, k) + pg_at(G, i, j)] = pg_tracer_point(G, P, I, k, i, j);
+}
+
 }  // namespace
 
 // Upload the restoring / diapycnal inputs present in st (pg_create).
@@ -69,9 +77,20 @@ int pg_restoring_setup(pg_handle* h, const pg_static* st)
   P.useRelax = st->uTime || st->vTime || st->hTime || st->eTime;
   P.useWDia = P.useRelax || st->wDia;
   P.useDiaVisc = st->nu_u_dia || st->nu_v_dia || c.convDiff != 0;
+  P.useTracer = c.useTracer != 0;
+  P.useBuoyancy = c.useTracer && c.useBuoyancy;
+  P.trc = (int)c.tracerScheme;
+  P.K2 = c.K2;
+  P.K4 = c.K4;
+  P.convDiff = c.convDiff;
+  if (P.useTracer) {  // config.c:305-310 with a tracer
+    P.useRelax = P.useRelax || st->bTime;
+    P.useWDia = P.useWDia || P.useRelax;
+    P.useDiaDiff = st->Fsurf_b || st->kappa_dia || c.convDiff != 0;
+  }
   h->wdia_nrecs = c.wDiaNrecs ? (int)c.wDiaNrecs : 1;
   h->wdia_file = st->wDia != nullptr;
-  if (!P.useRelax && !P.useWDia && !P.useDiaVisc) return PG_OK;
+  if (!P.useRelax && !P.useWDia && !P.useDiaVisc && !P.useTracer) return PG_OK;
   const size_t lf = (size_t)G.L * h->fsz, N = (size_t)G.nx * G.ny;
   std::vector<double> stage;
   // layered field: allocate, zero, upload the owned cells of each layer, fill
@@ -110,6 +129,14 @@ int pg_restoring_setup(pg_handle* h, const pg_static* st)
     if (!rc) rc = layered(&h->rs_nu_u, st->nu_u_dia, G.L, PG_HALO_U);
     if (!rc) rc = layered(&h->rs_nu_v, st->nu_v_dia, G.L, PG_HALO_V);
   }
+  if (P.useTracer) {
+    if (!rc) rc = layered(&h->b, nullptr, G.L, PG_HALO_SCALAR);
+    for (int s = 0; s < 3 && !rc; s++) rc = layered(&h->fb[s], nullptr, G.L, PG_HALO_SCALAR);
+    if (!rc) rc = layered(&h->rs_bRelax, st->bRelax, G.L, PG_HALO_SCALAR);
+    if (!rc) rc = layered(&h->rs_bTime, st->bTime, G.L, PG_HALO_SCALAR);
+    if (!rc) rc = layered(&h->rs_kappa, st->kappa_dia, G.L, PG_HALO_SCALAR);
+    if (!rc) rc = layered(&h->rs_Fsurf, st->Fsurf_b, 1, PG_HALO_SCALAR);
+  }
   if (rc) return rc;
   PG_CHECK(h, hipStreamSynchronize(h->stream));
   (void)lf;
@@ -119,7 +146,9 @@ int pg_restoring_setup(pg_handle* h, const pg_static* st)
 void pg_restoring_free(pg_handle* h)
 {
   double** all[] = {&h->rs_uRelax, &h->rs_vRelax, &h->rs_hRelax, &h->rs_eRelax, &h->rs_uTime, &h->rs_vTime,
-                    &h->rs_eTime,  &h->rs_hTime,  &h->rs_wdia,   &h->rs_wdia_ff, &h->rs_nu_u, &h->rs_nu_v};
+                    &h->rs_eTime,  &h->rs_hTime,  &h->rs_wdia,   &h->rs_wdia_ff, &h->rs_nu_u, &h->rs_nu_v,
+                    &h->b,         &h->fb[0],     &h->fb[1],     &h->fb[2],      &h->rs_bRelax, &h->rs_bTime,
+                    &h->rs_kappa,  &h->rs_Fsurf};
   for (double** p : all) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
@@ -194,5 +223,60 @@ PgIn pg_tendency_inputs(const pg_handle* h)
   I.vTime = h->rs_vTime;
   I.nu_u = h->rs_nu_u;
   I.nu_v = h->rs_nu_v;
+  I.b = h->b;
+  I.bRelax = h->rs_bRelax;
+  I.bTime = h->rs_bTime;
+  I.kappa = h->rs_kappa;
+  I.Fsurf = h->rs_Fsurf;
   return I;
+}
+
+// Tracer tendency into db (after the momentum and thickness tendencies).
+hipError_t pg_launch_tracer(pg_handle* h, double* db)
+{
+  if (!h->P.useTracer) return hipSuccess;
+  const PgGrid& G = h->G;
+  hipLaunchKernelGGL(k_tracer, dim3((G.ny + 255) / 256, G.nx, G.L), dim3(256), 0, h->stream, G, h->P,
+                     pg_tendency_inputs(h), db);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return pg_fill(h, db, G.L, PG_HALO_SCALAR);
+}
+
+extern "C" int pg_upload_tracer(pg_handle* h, const double* b)
+{
+  if (!h || !b) return PG_EARG;
+  if (!h->b) {
+    h->err = "pg_upload_tracer: the handle has no tracer (pg_config.useTracer = 0)";
+    return PG_EARG;
+  }
+  std::vector<double> stage;
+  const size_t N = (size_t)h->G.nx * h->G.ny;
+  for (int k = 0; k < h->G.L; k++) {
+    int rc = pg_upload_owned(h, h->b + pg_lay(h->G, k), b + k * N, h->G.ny, stage);
+    if (rc) return rc;
+  }
+  PG_CHECK(h, pg_fill(h, h->b, h->G.L, PG_HALO_SCALAR));
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
+
+extern "C" int pg_download_tracer(pg_handle* h, double* b)
+{
+  if (!h || !b) return PG_EARG;
+  if (!h->b) {
+    h->err = "pg_download_tracer: the handle has no tracer (pg_config.useTracer = 0)";
+    return PG_EARG;
+  }
+  const PgGrid& G = h->G;
+  const size_t N = (size_t)G.nx * G.ny;
+  std::vector<double> stage(h->fsz);
+  for (int k = 0; k < G.L; k++) {
+    PG_CHECK(h, hipMemcpyAsync(stage.data(), h->b + pg_lay(G, k), h->fsz * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream));
+    PG_CHECK(h, hipStreamSynchronize(h->stream));
+    for (int i = 0; i < G.nx; i++)
+      for (int j = 0; j < G.ny; j++) b[k * N + (size_t)i * G.ny + j] = stage[pg_at(G, i + G.g, j + G.g)];
+  }
+  return PG_OK;
 }

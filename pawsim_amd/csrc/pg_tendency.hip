@@ -554,12 +554,14 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, Pg
 }  // namespace
 
 // True if the fast pair of kernels serves this configuration (no
-// biharmonic viscosity, restoring or diapycnal terms; the other thickness
+// biharmonic viscosity, restoring, diapycnal terms or active buoyancy; a
+// passive tracer has its own per-point kernel; the other thickness
 // schemes and Smagorinsky coefficients take the extended interior variant).
 bool pg_tendency_fast_ok(const pg_handle* h)
 {
   const PgPhys& P = h->P;
-  return !P.useA4 && !P.useRelax && !P.useWDia && !P.useDiaVisc && h->G.nx >= 8 && h->G.ny >= 8 && h->G.L <= 8;
+  return !P.useA4 && !P.useRelax && !P.useWDia && !P.useDiaVisc && !P.useBuoyancy && h->G.nx >= 8 && h->G.ny >= 8
+         && h->G.L <= 8;
 }
 
 hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double* dh)

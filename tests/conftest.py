@@ -44,15 +44,19 @@ def prepare_outdir(z, outdir):
 
 def build_setup(name, spec):
     """A setup from a test-case spec: kind (the inputs generator), its
-    arguments, and optionally restoring=dict(...) for inputs.with_restoring."""
+    arguments, and optionally restoring=dict(...) for inputs.with_restoring
+    and tracer=dict(...) for inputs.with_tracer."""
     from pawsim_amd import inputs
 
     spec = dict(spec)
     kind = spec.pop("kind")
     rest = spec.pop("restoring", None)
+    trc = spec.pop("tracer", None)
     spec.setdefault("pressureTiming", 1)
     setup = getattr(inputs, kind)(name=name, **spec)
-    return inputs.with_restoring(setup, **rest) if rest is not None else setup
+    if rest is not None:
+        setup = inputs.with_restoring(setup, **rest)
+    return inputs.with_tracer(setup, **trc) if trc is not None else setup
 
 
 def run_exe(exe, infile, outdir, timeout=600):

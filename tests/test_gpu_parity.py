@@ -98,6 +98,16 @@ TRAJECTORY_CASES = {
                                         savefreqAvg=500.0, restoring=dict(relax=False, wdia_recs=3)),
     "per40x24_relax_mg": dict(kind="periodic_box", nx=40, ny=24, nt=8, solver=1, momentumScheme=1,
                               thicknessScheme=2, restoring=dict(wdia=False)),
+    # tracers: passive (tiled momentum kernel, per-point tracer kernel) and
+    # active buoyancy with convective enhancement
+    "jets136x72_tracer_kt00": dict(kind="jets", nx=136, ny=72, nlay=3, dx=20e3, nt=6, solver=1,
+                                   H_rest=(500.0, 1500.0, 2000.0), savefreqAvg=900.0,
+                                   tracer=dict(scheme=2, K2=30.0, diadiff=True, relax=True)),
+    "gyre64_buoyancy_up3_nonorms": dict(kind="gyre", nx=64, ny=64, nt=8, solver=1, pressureTiming=0,
+                                        savefreqEZ=600.0, tracer=dict(scheme=1, buoyancy=True, K4=1e9)),
+    "chan64x32_buoyancy_convdiff": dict(kind="channel", nx=64, ny=32, nt=8, dx=20e3, solver=1,
+                                        restoring=dict(relax=False, convDiff=5e-3),
+                                        tracer=dict(buoyancy=True, diadiff=True, unstable=True)),
 }
 
 
