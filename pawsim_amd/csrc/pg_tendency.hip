@@ -298,14 +298,17 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
 
 // Edge ring: owned points that are not inner, through the per-point path
 // (FP64 arithmetic; stored as T).
+// One thread per point, layer and component (grid.y = 3 Nlay: u, v, h):
+// each component's chain of dependent loads is about half the whole
+// point's, and the inlined copies keep only what their component reads.
+// Two waves per SIMD (up to 256 VGPRs): with a 96-VGPR cap the point path
+// spilled 1.4 KB per lane to scratch.  Measured at 1024^2 x 4 layers beside
+// the mixed-precision interior kernel: 106 us for both, against 155 us for
+// one thread per point under the 96-VGPR cap.
 template <typename T>
-// At most 96 VGPRs (5 waves per SIMD), so the ring's workgroups fit beside
-// the interior kernel's waves on the same compute units and run concurrently
-// with it (measured: 131 us for both at 1024^2 x 4 layers, against 148 us
-// with the unconstrained allocation of 215 VGPRs).
-__global__ __launch_bounds__(256, 5) void k_tendency_edge(PgGrid G, PgPhys P, PgIn I, T* du, T* dv, T* dh)
+__global__ __launch_bounds__(256, 2) void k_tendency_edge(PgGrid G, PgPhys P, PgIn I, T* du, T* dv, T* dh)
 {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x, comp = blockIdx.y / G.L, k = blockIdx.y % G.L;
   const int nx = G.nx, ny = G.ny, g = G.g;
   const int band = 5 * ny, total = band + (nx - 5) * 5;
   if (e >= total) return;
@@ -320,11 +323,17 @@ __global__ __launch_bounds__(256, 5) void k_tendency_edge(PgGrid G, PgPhys P, Pg
     j = (c < 2) ? g + c : g + ny - 3 + (c - 2);
   }
   double a, b, c;
-  pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
   const size_t p = pg_lay(G, k) + pg_at(G, i, j);
-  du[p] = (T)a;
-  dv[p] = (T)b;
-  dh[p] = (T)c;
+  if (comp == 0) {
+    pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
+    du[p] = (T)a;
+  } else if (comp == 1) {
+    pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
+    dv[p] = (T)b;
+  } else {
+    pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
+    dh[p] = (T)c;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -336,15 +345,24 @@ __global__ __launch_bounds__(256, 5) void k_tendency_edge(PgGrid G, PgPhys P, Pg
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
-__global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, PgIn I, float* du, float* dv, float* dh)
+// Each thread owns fixed slots of the staged (i0-2, j0-2) region.  The next
+// layer's u, v, h (FP64) are loaded into registers while the current layer
+// is computed, and the Montgomery column is carried in the same slots, so
+// the per-layer global loads overlap the stencil work of the layer before.
+// Tile of the mixed-precision kernel: TIF x TJ points, NTF threads.  16
+// rows (against 8) cut the re-read of the 2-row halo from 50% to 25%; at
+// 63.6 KB of LDS two workgroups share a compute unit, four waves per SIMD.
+constexpr int TIF = 16, NTF = 512;
+constexpr int SHF = TIF + 4, QHF = TIF + 3, UHF = TIF + 2, VHF = TIF + 1, KHF = TIF + 1, DHF = TIF + 1;
+__global__ __launch_bounds__(NTF, 4) void k_tendency_inner32(PgGrid G, PgPhys P, PgIn I, float* du, float* dv, float* dh)
 {
 #pragma clang fp contract(fast)
-  __shared__ float su[SH * SW], sv[SH * SW], sh[SH * SW];
-  __shared__ float sq[QH * QW], suu[UH * UW], svv[VH * VW], ske[KH * KW];
-  __shared__ float sdt[KH * KW], sds[DH * DW], shq[DH * DW], sub[KH * KW];
-  __shared__ double smm[KH * KW];  // Montgomery potential of the current layer, FP64
+  __shared__ float su[SHF * SW], sv[SHF * SW], sh[SHF * SW];
+  __shared__ float sq[QHF * QW], suu[UHF * UW], svv[VHF * VW], ske[KHF * KW];
+  __shared__ float sdt[KHF * KW], sds[DHF * DW], shq[DHF * DW], sub[KHF * KW];
+  __shared__ double smm[SHF * SW];  // Montgomery potential of the current layer, FP64
   const int tid = threadIdx.x;
-  const int i0 = G.g + blockIdx.y * TI, j0 = G.g + blockIdx.x * TJ;
+  const int i0 = G.g + blockIdx.y * TIF, j0 = G.g + blockIdx.x * TJ;
   const bool tw = P.mom == PG_MOM_TW81, al = (P.mom == PG_MOM_AL81 || P.mom == PG_MOM_HK83);
   const double rdx_d = 1.0 / G.dx, rdy_d = 1.0 / G.dy;
   const float rdx = (float)rdx_d, rdy = (float)rdy_d, r24 = 1.0f / 24.0f;
@@ -354,70 +372,89 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, Pg
   // (work.c:1410-1512) is built layer by layer from a running column in
   // registers: eta_1 = hhb + sum_{m>=1} h_m bottom-up, then eta_{k+1} =
   // eta_k - h_k.  Only the current layer's MM is in LDS.
-  constexpr int KR = (KH * KW + NTH - 1) / NTH;
-  double eta_r[KR], pp_r[KR];
+  constexpr int SR = (SHF * SW + NTF - 1) / NTF;
+  int sp[SR];  // offset of each staged slot in a layer, -1 outside the region or the array
+  double eta_r[SR], pp_r[SR], pu[SR], pv[SR], ph[SR];
 #pragma unroll
-  for (int r = 0; r < KR; r++) {
-    const int idx = tid + r * NTH;
-    eta_r[r] = 0;
-    pp_r[r] = 0;
-    if (idx >= KH * KW) continue;
-    const int i = i0 - 1 + idx / KW, j = j0 - 1 + idx % KW;
-    if (i >= G.sx - 1 || j >= G.sy - 1) continue;
-    const size_t p = pg_at(G, i, j);
+  for (int r = 0; r < SR; r++) {
+    const int idx = tid + r * NTF;
+    const int i = i0 - 2 + idx / SW, j = j0 - 2 + idx % SW;
+    sp[r] = (idx < SHF * SW && i < G.sx && j < G.sy) ? (int)pg_at(G, i, j) : -1;
+    eta_r[r] = pp_r[r] = pu[r] = pv[r] = ph[r] = 0;
+    if (sp[r] < 0) continue;
+    const size_t p = (size_t)sp[r];
     double eta = I.hhb[p];
     for (int m = G.L - 1; m >= 1; m--) eta = eta + I.h[pg_lay(G, m) + p];
     eta_r[r] = eta;
-    if (P.useQuad) sub[idx] = (float)pg_uabs_bot(G, I, i, j);
+    pu[r] = I.u[p];
+    pv[r] = I.v[p];
+    ph[r] = I.h[p];
+  }
+  if (P.useQuad)
+    for (int idx = tid; idx < KHF * KW; idx += NTF) {
+      const int i = i0 - 1 + idx / KW, j = j0 - 1 + idx % KW;
+      if (i < G.sx - 1 && j < G.sy - 1) sub[idx] = (float)pg_uabs_bot(G, I, i, j);
+    }
+  // 2 Omega_z at the q points of the tile, once for all layers
+  constexpr int QR = (QHF * QW + NTF - 1) / NTF;
+  float omz_r[QR];
+#pragma unroll
+  for (int r = 0; r < QR; r++) {
+    const int idx = tid + r * NTF;
+    const int i = i0 + idx / QW - 1, j = j0 + idx % QW - 1;
+    omz_r[r] = (idx < QHF * QW && i < G.sx - 2 && j < G.sy - 2) ? 2.0f * (float)I.omz[pg_at(G, i, j)] : 0.0f;
   }
 
   for (int k = 0; k < G.L; k++) {
     const size_t lay = pg_lay(G, k);
 #pragma unroll
-    for (int r = 0; r < KR; r++) {
-      const int idx = tid + r * NTH;
-      if (idx >= KH * KW) continue;
-      const int i = i0 - 1 + idx / KW, j = j0 - 1 + idx % KW;
-      if (i >= G.sx - 1 || j >= G.sy - 1) continue;
-      const size_t p = pg_at(G, i, j);
-      const double hk = I.h[lay + p];
+    for (int r = 0; r < SR; r++) {
+      if (sp[r] < 0) continue;
+      const int idx = tid + r * NTF;
+      const double hk = ph[r];
       if (k >= 1) pp_r[r] = pp_r[r] + P.gg[k] * eta_r[r];
       double mm = pp_r[r];
       if (P.h0 != 0) mm -= P.geff[k] * (P.h0 * P.h0 * P.h0 * P.h0) / (hk * hk * hk) / 3;
       smm[idx] = mm;
       if (k >= 1) eta_r[r] = eta_r[r] - hk;
+      su[idx] = (float)pu[r];
+      sv[idx] = (float)pv[r];
+      sh[idx] = (float)hk;
     }
-    for (int idx = tid; idx < SH * SW; idx += NTH) {
-      const int i = i0 - 2 + idx / SW, j = j0 - 2 + idx % SW;
-      if (i < G.sx && j < G.sy) {
-        const size_t p = lay + pg_at(G, i, j);
-        su[idx] = (float)I.u[p];
-        sv[idx] = (float)I.v[p];
-        sh[idx] = (float)I.h[p];
+    if (k + 1 < G.L) {  // next layer, in flight during this one
+      const size_t nl = pg_lay(G, k + 1);
+#pragma unroll
+      for (int r = 0; r < SR; r++) {
+        if (sp[r] < 0) continue;
+        pu[r] = I.u[nl + sp[r]];
+        pv[r] = I.v[nl + sp[r]];
+        ph[r] = I.h[nl + sp[r]];
       }
     }
     __syncthreads();
 #define U(a, b) su[((a) + 2) * SW + (b) + 2]
 #define V(a, b) sv[((a) + 2) * SW + (b) + 2]
 #define H(a, b) sh[((a) + 2) * SW + (b) + 2]
-    for (int idx = tid; idx < QH * QW; idx += NTH) {
+#pragma unroll
+    for (int r = 0; r < QR; r++) {
+      const int idx = tid + r * NTF;
       const int a = idx / QW - 1, b = idx % QW - 1;
       const int i = i0 + a, j = j0 + b;
-      if (i >= G.sx - 2 || j >= G.sy - 2) continue;
+      if (idx >= QHF * QW || i >= G.sx - 2 || j >= G.sy - 2) continue;
       const float zeta = (U(a, b - 1) - U(a, b)) * rdy + (V(a, b) - V(a - 1, b)) * rdx;
       const float hq = 0.25f * (H(a, b) + H(a - 1, b - 1) + H(a - 1, b) + H(a, b - 1));
-      sq[idx] = (2.0f * (float)I.omz[pg_at(G, i, j)] + zeta) * rcpf(hq);
+      sq[idx] = (omz_r[r] + zeta) * rcpf(hq);
     }
-    for (int idx = tid; idx < UH * UW; idx += NTH) {
+    for (int idx = tid; idx < UHF * UW; idx += NTF) {
       const int a = idx / UW - 1, b = idx % UW - 1;
       suu[idx] = U(a, b) * (0.5f * (H(a, b) + H(a - 1, b)));
     }
-    for (int idx = tid; idx < VH * VW; idx += NTH) {
+    for (int idx = tid; idx < VHF * VW; idx += NTF) {
       const int a = idx / VW - 1, b = idx % VW - 1;
       svv[idx] = V(a, b) * (0.5f * (H(a, b) + H(a, b - 1)));
     }
 #define SQ(x) ((x) * (x))
-    for (int idx = tid; idx < KH * KW; idx += NTH) {
+    for (int idx = tid; idx < KHF * KW; idx += NTF) {
       const int a = idx / KW - 1, b = idx % KW - 1;
       if (tw)
         ske[idx] = 0.25f * ((2.0f / 3.0f) * SQ(U(a, b)) + (1.0f / 3.0f) * SQ(0.5f * (U(a - 1, b) + U(a + 1, b)))
@@ -436,7 +473,7 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, Pg
     }
 #undef SQ
     if (P.useA2)
-      for (int idx = tid; idx < DH * DW; idx += NTH) {
+      for (int idx = tid; idx < DHF * DW; idx += NTF) {
         const int a = idx / DW, b = idx % DW;
         sds[idx] = (U(a, b) - U(a, b - 1)) * rdy + (V(a, b) - V(a - 1, b)) * rdx;
         shq[idx] = 0.25f * (H(a, b) + H(a - 1, b - 1) + H(a - 1, b) + H(a, b - 1));
@@ -446,12 +483,12 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, Pg
 #define HU(a, b) suu[((li) + (a) + 1) * UW + (lj) + (b) + 1]
 #define HV(a, b) svv[((li) + (a) + 1) * VW + (lj) + (b) + 1]
 #define KE(a, b) ske[((li) + (a) + 1) * KW + (lj) + (b) + 1]
-#define MM(a, b) smm[((li) + (a) + 1) * KW + (lj) + (b) + 1]
+#define MM(a, b) smm[((li) + (a) + 2) * SW + (lj) + (b) + 2]
 #define DT(a, b) sdt[((li) + (a) + 1) * KW + (lj) + (b) + 1]
 #define UB(a, b) sub[((li) + (a) + 1) * KW + (lj) + (b) + 1]
 #define DS(a, b) sds[((li) + (a)) * DW + (lj) + (b)]
 #define HQ(a, b) shq[((li) + (a)) * DW + (lj) + (b)]
-    for (int idx = tid; idx < TI * TJ; idx += NTH) {
+    for (int idx = tid; idx < TIF * TJ; idx += NTF) {
       const int li = idx / TJ, lj = idx % TJ;
       const int i = i0 + li, j = j0 + lj;
       if (i >= G.g + G.nx || j >= G.g + G.ny || !is_inner(G, i, j)) continue;
@@ -487,7 +524,7 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, Pg
       const float hwest = 0.5f * (H(li, lj) + H(li - 1, lj));
       const float hsouth = 0.5f * (H(li, lj) + H(li, lj - 1));
       const float rhw = (hwest != 0) ? rcpf(hwest) : 0.0f, rhs_ = (hsouth != 0) ? rcpf(hsouth) : 0.0f;
-      const float fs = (k == 0) ? 1.0f : 0.0f, fb = (k == G.L - 1) ? 1.0f : 0.0f;
+      const float fb = (k == G.L - 1) ? 1.0f : 0.0f;
       const size_t q = pg_at(G, i, j);
       float ru, rv;
       {
@@ -506,8 +543,8 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, Pg
           a2 = (A2 * H(li, lj) * DT(0, 0) - A2 * H(li - 1, lj) * DT(-1, 0)) * rdx
                + (A2 * HQ(0, 1) * DS(0, 1) - A2 * HQ(0, 0) * DS(0, 0)) * rdy;
         ru = gM + gK + qf + a2 * rhw;
-        if (P.useWind) ru += (float)I.taux[q] * fs * rhw;
-        if (P.useLin) ru += -lin * U(li, lj) * fb * rhw;
+        if (P.useWind && k == 0) ru += (float)I.taux[q] * rhw;
+        if (P.useLin && k == G.L - 1) ru += -lin * U(li, lj) * rhw;
         if (P.useQuad) ru -= quad * (0.5f * (UB(0, 0) + UB(-1, 0))) * U(li, lj) * fb * rhw;
       }
       {
@@ -526,8 +563,8 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner32(PgGrid G, PgPhys P, Pg
           a2 = (A2 * HQ(1, 0) * DS(1, 0) - A2 * HQ(0, 0) * DS(0, 0)) * rdx
                - (A2 * H(li, lj) * DT(0, 0) - A2 * H(li, lj - 1) * DT(0, -1)) * rdy;
         rv = gM + gK + qf + a2 * rhs_;
-        if (P.useWind) rv += (float)I.tauy[q] * fs * rhs_;
-        if (P.useLin) rv += -lin * V(li, lj) * fb * rhs_;
+        if (P.useWind && k == 0) rv += (float)I.tauy[q] * rhs_;
+        if (P.useLin && k == G.L - 1) rv += -lin * V(li, lj) * rhs_;
         if (P.useQuad) rv -= quad * (0.5f * (UB(0, 0) + UB(0, -1))) * V(li, lj) * fb * rhs_;
       }
       const size_t p = lay + q;
@@ -589,13 +626,13 @@ static hipError_t launch_edge_forked(pg_handle* h, T* du, T* dv, T* dh, hipError
   if (!h->edge_fork) {
     hipError_t e = inner(h, du, dv, dh);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_tendency_edge<T>, dim3((total + 255) / 256, G.L), dim3(256), 0, h->stream, G, h->P, I, du, dv, dh);
+    hipLaunchKernelGGL(k_tendency_edge<T>, dim3((total + 255) / 256, 3 * G.L), dim3(256), 0, h->stream, G, h->P, I, du, dv, dh);
     return hipGetLastError();
   }
   hipError_t e = hipEventRecord(h->ev_fork, h->stream);
   if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->ev_fork, 0);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_tendency_edge<T>, dim3((total + 255) / 256, G.L), dim3(256), 0, h->side, G, h->P, I, du, dv, dh);
+  hipLaunchKernelGGL(k_tendency_edge<T>, dim3((total + 255) / 256, 3 * G.L), dim3(256), 0, h->side, G, h->P, I, du, dv, dh);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->side);
   if (e != hipSuccess) return e;
@@ -613,10 +650,8 @@ hipError_t pg_launch_tendency_inner32(pg_handle* h, float* du, float* dv, float*
 {
   const PgIn I = pg_tendency_inputs(h);
   const PgGrid& G = h->G;
-  const size_t dyn = (size_t)G.L * KH * KW * sizeof(double);
-  dim3 grid((G.ny + TJ - 1) / TJ, (G.nx + TI - 1) / TI);
-  (void)dyn;
-  hipLaunchKernelGGL(k_tendency_inner32, grid, dim3(NTH), 0, h->stream, G, h->P, I, du, dv, dh);
+  dim3 grid((G.ny + TJ - 1) / TJ, (G.nx + TIF - 1) / TIF);
+  hipLaunchKernelGGL(k_tendency_inner32, grid, dim3(NTF), 0, h->stream, G, h->P, I, du, dv, dh);
   return hipGetLastError();
 }
 
