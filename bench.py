@@ -77,7 +77,7 @@ def parse():
     return p.parse_args()
 
 
-def make_model(a, mixed=1, solver=None, tol=None, **extra):
+def make_model(a, mixed=1, solver=None, tol=None, restoring=None, tracer=None, **extra):
     import pawsim_amd
     from pawsim_amd import inputs
 
@@ -85,6 +85,10 @@ def make_model(a, mixed=1, solver=None, tol=None, **extra):
                         solver=a.solver if solver is None else solver, tol=a.tol if tol is None else tol,
                         A2=10.0, maxiters=200, mixed_precision=mixed, mg_nu1=a.mg_nu1, mg_nu2=a.mg_nu2,
                         pi_guess=a.pi_guess, **extra)
+    if restoring is not None:
+        setup = inputs.with_restoring(setup, **restoring)
+    if tracer is not None:
+        setup = inputs.with_tracer(setup, **tracer)
     m = pawsim_amd.Model.from_setup(setup)
     if a.mg_tail:
         m.set_mg_tail(a.mg_tail)
@@ -161,8 +165,9 @@ def run_steps(a, m, stream):
     return {"n": n, "elapsed": elapsed, "solve_ms": float(np.mean(solve_ms)), "iters": float(np.mean(iters))}
 
 
-# Other thickness schemes and Smagorinsky viscosity on the same workload
-# (FP64 throughout), against AL81 on the tiled and on the per-point kernel.
+# Other thickness schemes, Smagorinsky viscosity, restoring / diapycnal
+# terms, tracers and active buoyancy on the same workload (FP64 throughout),
+# against AL81 on the tiled and on the per-point kernel.
 VARIANTS = {
     "al81_tiled": {},
     "al81_per_point": {"tendency_kernel": 1},
@@ -170,6 +175,9 @@ VARIANTS = {
     "thk_up3": {"thicknessScheme": 2},
     "thk_kt00": {"thicknessScheme": 3},
     "a2smag": {"A2smag": 2.0},
+    "restoring_diapycnal": {"restoring": {}},
+    "tracer_kt00": {"tracer": {"scheme": 2, "K2": 30.0, "diadiff": True, "relax": True}},
+    "buoyancy_al81": {"tracer": {"buoyancy": True}},
 }
 
 
@@ -178,17 +186,59 @@ def scheme_variants(a, stream):
 
     import pawsim_amd
 
+    import numpy as np
+
     b = argparse.Namespace(**vars(a))
     b.steps, b.warmup = a.variant_steps, 3
     out = {}
     for name, extra in VARIANTS.items():
         _, m = make_model(b, mixed=0, solver=pawsim_amd.SOLVER_GPU_MG, **extra)
         r = run_steps(b, m, stream)
+        finite = all(bool(np.isfinite(x).all()) for x in m.state())
         m.close()
-        out[name] = {"steps_per_s": b.steps / r["elapsed"], "ms_per_step": 1e3 * r["elapsed"] / b.steps}
+        out[name] = {"steps_per_s": b.steps / r["elapsed"], "ms_per_step": 1e3 * r["elapsed"] / b.steps,
+                     "vcycles_per_step": r["iters"], "finite": finite}
         log(f"variant {name}: {out[name]['steps_per_s']:.1f} steps/s")
     return {"workload": f"zonal jets {a.nx}x{a.ny}x{a.nlay}, FP64 stencils, FP64 GPU MG to rel. residual {a.tol:g}",
             "steps": b.steps, "variants": out}
+
+
+def diagnostics_and_restart(a, stream):
+    """Cost of the calls around the step on the headline model: one energy /
+    enstrophy line (pg_energy, with its host read-back of Nlay + 2 doubles),
+    one online-average accumulation (pg_average_add), and a full-fidelity
+    checkpoint saved to and loaded from host memory (PCIe-bound)."""
+    import argparse
+
+    import torch
+
+    b = argparse.Namespace(**vars(a))
+    b.steps, b.warmup = 0, 3
+    _, m = make_model(b, mixed=1)
+    run_steps(b, m, stream)
+
+    def per_call(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        m.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / reps
+
+    out = {"energy_ms": per_call(m.energy, 50), "average_add_ms": per_call(lambda: m.average_add(0.5), 50)}
+    t0 = time.perf_counter()
+    blob = m.checkpoint()
+    out["checkpoint_save_ms"] = 1e3 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    m.restore(blob)
+    out["checkpoint_load_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["checkpoint_bytes"] = len(blob)
+    m.close()
+    log(f"energy {out['energy_ms']:.3f} ms, average_add {out['average_add_ms']:.3f} ms, checkpoint "
+        f"{out['checkpoint_bytes'] / 2**20:.0f} MiB save {out['checkpoint_save_ms']:.0f} ms load "
+        f"{out['checkpoint_load_ms']:.0f} ms")
+    return out
 
 
 def learned_vs_plain(a, stream):
@@ -419,6 +469,7 @@ def main():
     m.close()
 
     variants = scheme_variants(a, stream) if a.variant_steps > 0 else None
+    diag = diagnostics_and_restart(a, stream)
     learned = learned_vs_plain(a, stream) if a.learn_samples > 0 else None
     ensemble = ensemble_run(a) if a.ens_members > 0 else None
 
@@ -445,6 +496,7 @@ def main():
         "pcie_inclusive_steps_per_s": pcie_rate,
         "ensemble": ensemble,
         "scheme_variants": variants,
+        "diagnostics_and_restart": diag,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

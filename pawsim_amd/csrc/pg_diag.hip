@@ -140,11 +140,14 @@ __global__ __launch_bounds__(64) void k_energy_finish(const double* part, int nb
 // ---------------------------------------------------------------------------
 constexpr int AV_PER_LAYER = 10, AV_LMAX = 16;
 
-__global__ __launch_bounds__(256) void k_avg_accumulate(PgGrid G, PgPhys P, const double* u, const double* v,
+__global__ __launch_bounds__(256) void k_avg_accumulate(PgGrid G, PgPhys Pk, const double* u, const double* v,
                                                         const double* hh, const double* hhb, const double* pi,
                                                         const double* wdia, const double* b, double* avg, double dtfac,
                                                         int useRL)
 {
+  __shared__ PgPhys P;  // indexed per layer: from LDS, not a private copy of the argument
+  if (threadIdx.x == 0) P = Pk;
+  __syncthreads();
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
   if (j >= G.ny) return;
   const int L = G.L, gi = i + G.g, gj = j + G.g;

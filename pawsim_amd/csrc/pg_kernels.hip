@@ -98,21 +98,36 @@ __global__ __launch_bounds__(1024) void k_fill(FillSet S, size_t stride, PgGrid 
 }
 
 // ---------------------------------------------------------------------------
-// Tendency: one thread per (layer, owned point); the stage results of
-// tendency.c:49-75 are evaluated on the fly by pg_tendency_point.
+// Tendency: one thread per (component, layer, owned point), grid.z =
+// 3 Nlay (u, v, h); the stage results of tendency.c:49-75 are evaluated on
+// the fly by pg_tendency_point, whose inlined copies keep only what their
+// component reads (the whole point in one thread needed more than 255
+// VGPRs and spilled to scratch).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(TJ* TI) void k_tendency(PgGrid G, PgPhys P, PgIn I, double* du, double* dv, double* dh)
+__global__ __launch_bounds__(TJ* TI, 2) void k_tendency(PgGrid G, PgPhys Pk, PgIn I, double* du, double* dv, double* dh)
 {
+  // The stencils index the per-layer arrays of PgPhys with the layer: read
+  // from a by-value kernel argument that way, the whole struct was copied
+  // into private (scratch) memory per lane; an LDS copy per workgroup is not.
+  __shared__ PgPhys P;
+  if (threadIdx.x == 0 && threadIdx.y == 0) P = Pk;
+  __syncthreads();
   const int j = G.g + blockIdx.x * TJ + threadIdx.x;
   const int i = G.g + blockIdx.y * TI + threadIdx.y;
-  const int k = blockIdx.z;
+  const int comp = blockIdx.z / G.L, k = blockIdx.z % G.L;
   if (i >= G.g + G.nx || j >= G.g + G.ny) return;
   double a, b, c;
-  pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
   const size_t p = pg_lay(G, k) + pg_at(G, i, j);
-  du[p] = a;
-  dv[p] = b;
-  dh[p] = c;
+  if (comp == 0) {
+    pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
+    du[p] = a;
+  } else if (comp == 1) {
+    pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
+    dv[p] = b;
+  } else {
+    pg_tendency_point(G, P, I, k, i, j, &a, &b, &c);
+    dh[p] = c;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -710,7 +725,7 @@ hipError_t pg_launch_tendency(pg_handle* h, double* du, double* dv, double* dh)
   if (e != hipSuccess) return e;
   if (h->fast_tendency && pg_tendency_fast_ok(h)) return pg_launch_tendency_fast(h, du, dv, dh);
   const PgIn I = pg_tendency_inputs(h);
-  hipLaunchKernelGGL(k_tendency, owned_grid(h->G, h->G.L), dim3(TJ, TI), 0, h->stream, h->G, h->P, I, du, dv, dh);
+  hipLaunchKernelGGL(k_tendency, owned_grid(h->G, 3 * h->G.L), dim3(TJ, TI), 0, h->stream, h->G, h->P, I, du, dv, dh);
   return hipGetLastError();
 }
 

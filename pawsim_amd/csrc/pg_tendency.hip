@@ -306,8 +306,11 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
 // the mixed-precision interior kernel: 106 us for both, against 155 us for
 // one thread per point under the 96-VGPR cap.
 template <typename T>
-__global__ __launch_bounds__(256, 2) void k_tendency_edge(PgGrid G, PgPhys P, PgIn I, T* du, T* dv, T* dh)
+__global__ __launch_bounds__(256, 2) void k_tendency_edge(PgGrid G, PgPhys Pk, PgIn I, T* du, T* dv, T* dh)
 {
+  __shared__ PgPhys P;  // the stencils index its per-layer arrays; from LDS, not a private copy
+  if (threadIdx.x == 0) P = Pk;
+  __syncthreads();
   const int e = blockIdx.x * blockDim.x + threadIdx.x, comp = blockIdx.y / G.L, k = blockIdx.y % G.L;
   const int nx = G.nx, ny = G.ny, g = G.g;
   const int band = 5 * ny, total = band + (nx - 5) * 5;
