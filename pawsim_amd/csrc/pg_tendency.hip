@@ -37,7 +37,10 @@ __device__ __forceinline__ bool is_inner(const PgGrid& G, int i, int j)
   return i - 2 >= G.g && i + 3 < G.g + G.nx && j - 2 >= G.g && j + 3 < G.g + G.ny;
 }
 
-template <bool EXT>
+// XTRA: restoring, diapycnal and active-buoyancy terms (their own
+// instantiation, so that the plain kernel keeps its register budget and the
+// edge ring still fits beside it).
+template <bool EXT, bool XTRA>
 __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn I, double* du, double* dv, double* dh)
 {
   __shared__ double su[SH * SW], sv[SH * SW], sh[EXT ? SH3 * SW3 : SH * SW];
@@ -66,6 +69,8 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
     double pp = 0;
     for (int k = 0; k < G.L; k++) {
       if (k >= 1) pp = pp + P.gg[k] * smm[k * (KH * KW) + idx];
+      // active buoyancy jump across interface k (work.c:1445-1451)
+      if (XTRA && k >= 1 && P.useBuoyancy) pp += smm[k * (KH * KW) + idx] * (I.b[pg_lay(G, k - 1) + p] - I.b[pg_lay(G, k) + p]);
       double mm = pp;
       if (P.h0 != 0) {
         const double hk = I.h[pg_lay(G, k) + p];
@@ -218,8 +223,10 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
           qf += a00 * HV(0, 1) + b00 * HV(-1, 1) + c00 * HV(-1, 0) + d00 * HV(0, 0) - e00 * HU(1, 0) + em0 * HU(-1, 0);
         else
           qf += 0.25 * Q(0, 0) * (HV(0, 0) + HV(-1, 0)) + 0.25 * Q(0, 1) * (HV(0, 1) + HV(-1, 1));
-        double a2 = 0;
-        const double a4 = 0, buoy = 0;
+        double a2 = 0, buoy = 0;
+        const double a4 = 0;
+        if (XTRA && P.useBuoyancy && hwest != 0)  // work.c:1745-1749
+          buoy = -0.5 * (pg_hz(G, I, k, i, j) + pg_hz(G, I, k, i - 1, j)) * pg_dbx(G, I, k, i, j);
         if (P.useA2 && !EXT)
           a2 = ((P.A2 * H(li, lj) * DT(0, 0) - P.A2 * H(li - 1, lj) * DT(-1, 0)) / dx
                 + (P.A2 * HQ(0, 1) * DS(0, 1) - P.A2 * HQ(0, 0) * DS(0, 0)) / dy);
@@ -227,6 +234,7 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
           a2 = ((A2h(0, 0) * H(li, lj) * DT(0, 0) - A2h(-1, 0) * H(li - 1, lj) * DT(-1, 0)) / dx
                 + (A2q(0, 1) * HQ(0, 1) * DS(0, 1) - A2q(0, 0) * HQ(0, 0) * DS(0, 0)) / dy);
         ru = gM + gK + qf + ((hwest != 0) ? (a2 + a4 + buoy) / hwest : 0);
+        if (XTRA && (P.useWDia || P.useDiaVisc)) ru = pg_diapycnal(G, P, I, k, i, j, ru, hwest, 0);
         if (P.useWind) {
           const double tx = 1.0 * I.taux[q] + 0.0 * I.taux[q];
           const double r = tx * fs;
@@ -240,6 +248,7 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
           const double ua = 0.5 * (UB(0, 0) + UB(-1, 0));
           ru -= P.quad * ua * U(li, lj) * fb / hwest;
         }
+        if (XTRA && P.useRelax && I.uTime[lay + q] > 0) ru += -(U(li, lj) - I.uRelax[lay + q]) / I.uTime[lay + q];
       }
       {  // v momentum (work.c:1758-1836)
         const double gM = -(MM(0, 0) - MM(0, -1)) / dy;
@@ -252,8 +261,10 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
           qf += p0m * HV(0, -1) - p00 * HV(0, 1) - cp0 * HU(1, 0) - d00 * HU(0, 0) - a0m * HU(0, -1) - bpm * HU(1, -1);
         else
           qf -= 0.25 * Q(0, 0) * (HU(0, 0) + HU(0, -1)) + 0.25 * Q(1, 0) * (HU(1, 0) + HU(1, -1));
-        double a2 = 0;
-        const double a4 = 0, buoy = 0;
+        double a2 = 0, buoy = 0;
+        const double a4 = 0;
+        if (XTRA && P.useBuoyancy && hsouth != 0)  // work.c:1829-1833
+          buoy = -0.5 * (pg_hz(G, I, k, i, j) + pg_hz(G, I, k, i, j - 1)) * pg_dby(G, I, k, i, j);
         if (P.useA2 && !EXT)
           a2 = ((P.A2 * HQ(1, 0) * DS(1, 0) - P.A2 * HQ(0, 0) * DS(0, 0)) / dx
                 - (P.A2 * H(li, lj) * DT(0, 0) - P.A2 * H(li, lj - 1) * DT(0, -1)) / dy);
@@ -261,6 +272,7 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
           a2 = ((A2q(1, 0) * HQ(1, 0) * DS(1, 0) - A2q(0, 0) * HQ(0, 0) * DS(0, 0)) / dx
                 - (A2h(0, 0) * H(li, lj) * DT(0, 0) - A2h(0, -1) * H(li, lj - 1) * DT(0, -1)) / dy);
         rv = gM + gK + qf + ((hsouth != 0) ? (a2 + a4 + buoy) / hsouth : 0);
+        if (XTRA && (P.useWDia || P.useDiaVisc)) rv = pg_diapycnal(G, P, I, k, i, j, rv, hsouth, 1);
         if (P.useWind) {
           const double ty = 1.0 * I.tauy[q] + 0.0 * I.tauy[q];
           const double r = ty * fs;
@@ -274,11 +286,14 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
           const double ua = 0.5 * (UB(0, 0) + UB(0, -1));
           rv -= P.quad * ua * V(li, lj) * fb / hsouth;
         }
+        if (XTRA && P.useRelax && I.vTime[lay + q] > 0) rv += -(V(li, lj) - I.vRelax[lay + q]) / I.vTime[lay + q];
       }
       const size_t p = lay + q;
       du[p] = ru;
       dv[p] = rv;
-      dh[p] = (HU(0, 0) - HU(1, 0)) / dx + (HV(0, 0) - HV(0, 1)) / dy;
+      const double adv = (HU(0, 0) - HU(1, 0)) / dx + (HV(0, 0) - HV(0, 1)) / dy;
+      // thickness change by the interface velocity (work.c:1880-1890)
+      dh[p] = (XTRA && P.useWDia) ? adv + -(I.wdia[lay + q] - I.wdia[pg_lay(G, k + 1) + q]) : adv;
     }
 #undef Q
 #undef HU
@@ -594,14 +609,16 @@ __global__ __launch_bounds__(NTF, 4) void k_tendency_inner32(PgGrid G, PgPhys P,
 }  // namespace
 
 // True if the fast pair of kernels serves this configuration (no
-// biharmonic viscosity, restoring, diapycnal terms or active buoyancy; a
-// passive tracer has its own per-point kernel; the other thickness
-// schemes and Smagorinsky coefficients take the extended interior variant).
+// biharmonic viscosity; the tracer has its own per-point kernel; the other
+// thickness schemes and Smagorinsky coefficients take the extended interior
+// variant; restoring, diapycnal and buoyancy terms read their inputs point
+// by point from global memory through the same functions as the per-point
+// path).  The mixed-precision kernel serves none of the latter, which
+// pg_create refuses with mixed precision.
 bool pg_tendency_fast_ok(const pg_handle* h)
 {
   const PgPhys& P = h->P;
-  return !P.useA4 && !P.useRelax && !P.useWDia && !P.useDiaVisc && !P.useBuoyancy && h->G.nx >= 8 && h->G.ny >= 8
-         && h->G.L <= 8;
+  return !P.useA4 && h->G.nx >= 8 && h->G.ny >= 8 && h->G.L <= 8;
 }
 
 hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double* dh)
@@ -610,10 +627,12 @@ hipError_t pg_launch_tendency_inner(pg_handle* h, double* du, double* dv, double
   const PgGrid& G = h->G;
   const size_t dyn = (size_t)G.L * KH * KW * sizeof(double);
   dim3 grid((G.ny + TJ - 1) / TJ, (G.nx + TI - 1) / TI);
-  if (h->P.thk != PG_THK_AL81 || h->P.useSmag)
-    hipLaunchKernelGGL(k_tendency_inner<true>, grid, dim3(NTH), dyn, h->stream, G, h->P, I, du, dv, dh);
-  else
-    hipLaunchKernelGGL(k_tendency_inner<false>, grid, dim3(NTH), dyn, h->stream, G, h->P, I, du, dv, dh);
+  const PgPhys& P = h->P;
+  const bool ext = P.thk != PG_THK_AL81 || P.useSmag;
+  const bool xtra = P.useRelax || P.useWDia || P.useDiaVisc || P.useBuoyancy;
+  auto kern = ext ? (xtra ? k_tendency_inner<true, true> : k_tendency_inner<true, false>)
+                  : (xtra ? k_tendency_inner<false, true> : k_tendency_inner<false, false>);
+  hipLaunchKernelGGL(kern, grid, dim3(NTH), dyn, h->stream, G, h->P, I, du, dv, dh);
   return hipGetLastError();
 }
 

@@ -108,6 +108,10 @@ TRAJECTORY_CASES = {
     "chan64x32_buoyancy_convdiff": dict(kind="channel", nx=64, ny=32, nt=8, dx=20e3, solver=1,
                                         restoring=dict(relax=False, convDiff=5e-3),
                                         tracer=dict(buoyancy=True, diadiff=True, unstable=True)),
+    # the same terms through the per-point kernel everywhere
+    "gyre64_restoring_buoyancy_pointkernel": dict(kind="gyre", nx=64, ny=64, nt=8, solver=1, tendency_kernel=1,
+                                                  restoring=dict(wdia_recs=3),
+                                                  tracer=dict(buoyancy=True, scheme=1, K2=20.0)),
 }
 
 
