@@ -108,21 +108,43 @@ __global__ __launch_bounds__(EJ* EI) void k_energy(PgGrid G, PgPhys P, const dou
   }
 }
 
-// out[0] = KE, out[1] = PE, out[2 + k] = Z_k; layer sums first, then over layers
-__global__ __launch_bounds__(64) void k_energy_finish(const double* part, int nb, int L, double* out)
+// out[0] = KE, out[1] = PE, out[2 + k] = Z_k; layer sums first, then over
+// layers.  One workgroup of 1024 threads: strided per-thread sums of the
+// block partials, then a fixed tree in LDS (a deterministic order for a
+// given partial count).
+__global__ __launch_bounds__(1024) void k_energy_finish(const double* part, int nb, int L, double* out)
 {
+  __shared__ double red[3][1024];
+  const int t = threadIdx.x;
   double KE = 0, PE = 0;
   for (int k = 0; k < L; k++) {
-    double s[3] = {0, 0, 0};
-    for (int b = threadIdx.x; b < nb; b += 64)
-      for (int c = 0; c < 3; c++) s[c] += part[((size_t)k * nb + b) * 3 + c];
-    for (int o = 32; o > 0; o >>= 1)
-      for (int c = 0; c < 3; c++) s[c] += __shfl_xor(s[c], o, 64);
-    KE += s[0];
-    PE += s[1];
-    if (threadIdx.x == 0) out[2 + k] = s[2];
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int b = t; b < nb; b += 1024) {
+      const double* q = part + ((size_t)k * nb + b) * 3;
+      s0 += q[0];
+      s1 += q[1];
+      s2 += q[2];
+    }
+    red[0][t] = s0;
+    red[1][t] = s1;
+    red[2][t] = s2;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+      if (t < o) {
+        red[0][t] += red[0][t + o];
+        red[1][t] += red[1][t + o];
+        red[2][t] += red[2][t + o];
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      KE += red[0][0];
+      PE += red[1][0];
+      out[2 + k] = red[2][0];
+    }
+    __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  if (t == 0) {
     out[0] = KE;
     out[1] = PE;
   }
@@ -253,7 +275,7 @@ extern "C" int pg_energy(pg_handle* h, double* KE, double* PE, double* Z)
   double* out = h->diag + (size_t)nb * G.L * 3;
   hipLaunchKernelGGL(k_energy, grid, dim3(EJ, EI), 0, h->stream, G, h->P, h->u, h->v, h->h, h->hhb, h->omz, h->b,
                      h->diag);
-  hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(64), 0, h->stream, h->diag, nb, G.L, out);
+  hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(1024), 0, h->stream, h->diag, nb, G.L, out);
   PG_CHECK(h, hipGetLastError());
   std::vector<double> res(2 + G.L);
   PG_CHECK(h, hipMemcpyAsync(res.data(), out, res.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));

@@ -81,22 +81,63 @@ __global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn
     if (P.useQuad) sub[idx] = pg_uabs_bot(G, I, i, j);
   }
 
-  for (int k = 0; k < G.L; k++) {
-    const size_t lay = pg_lay(G, k);
-    for (int idx = tid; idx < SH * SW; idx += NTH) {
-      const int i = i0 - 2 + idx / SW, j = j0 - 2 + idx % SW;
-      if (i < G.sx && j < G.sy) {
-        const size_t p = lay + pg_at(G, i, j);
-        su[idx] = I.u[p];
-        sv[idx] = I.v[p];
-        if (!EXT) sh[idx] = I.h[p];
+  // u, v, h of the next layer are loaded into registers (fixed slots per
+  // thread) while the current layer is computed.
+  constexpr int SR = (SH * SW + NTH - 1) / NTH, HR = EXT ? (SH3 * SW3 + NTH - 1) / NTH : SR;
+  double pu[SR], pv[SR], ph[HR];
+  auto uv_slot = [&](int r, int& idx, size_t& p) {
+    idx = tid + r * NTH;
+    const int i = i0 - 2 + idx / SW, j = j0 - 2 + idx % SW;
+    const bool ok = idx < SH * SW && i < G.sx && j < G.sy;
+    p = ok ? pg_at(G, i, j) : 0;
+    return ok;
+  };
+  auto h_slot = [&](int r, int& idx, size_t& p) {
+    if (!EXT) return uv_slot(r, idx, p);
+    idx = tid + r * NTH;
+    const int i = i0 - 3 + idx / SW3, j = j0 - 3 + idx % SW3;
+    const bool ok = idx < SH3 * SW3 && i < G.sx && j < G.sy;
+    p = ok ? pg_at(G, i, j) : 0;
+    return ok;
+  };
+  auto load_layer = [&](int kk) {
+    const size_t l = pg_lay(G, kk);
+#pragma unroll
+    for (int r = 0; r < SR; r++) {
+      int idx;
+      size_t p;
+      if (uv_slot(r, idx, p)) {
+        pu[r] = I.u[l + p];
+        pv[r] = I.v[l + p];
       }
     }
-    if (EXT)
-      for (int idx = tid; idx < SH3 * SW3; idx += NTH) {
-        const int i = i0 - 3 + idx / SW3, j = j0 - 3 + idx % SW3;
-        if (i < G.sx && j < G.sy) sh[idx] = I.h[lay + pg_at(G, i, j)];
+#pragma unroll
+    for (int r = 0; r < HR; r++) {
+      int idx;
+      size_t p;
+      if (h_slot(r, idx, p)) ph[r] = I.h[l + p];
+    }
+  };
+  load_layer(0);
+
+  for (int k = 0; k < G.L; k++) {
+    const size_t lay = pg_lay(G, k);
+#pragma unroll
+    for (int r = 0; r < SR; r++) {
+      int idx;
+      size_t p;
+      if (uv_slot(r, idx, p)) {
+        su[idx] = pu[r];
+        sv[idx] = pv[r];
       }
+    }
+#pragma unroll
+    for (int r = 0; r < HR; r++) {
+      int idx;
+      size_t p;
+      if (h_slot(r, idx, p)) sh[idx] = ph[r];
+    }
+    if (k + 1 < G.L) load_layer(k + 1);
     __syncthreads();
 #define U(a, b) su[((a) + 2) * SW + (b) + 2]
 #define V(a, b) sv[((a) + 2) * SW + (b) + 2]
