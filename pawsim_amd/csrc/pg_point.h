@@ -351,10 +351,16 @@ PG_HD double pg_eta(const PgGrid& G, const PgIn& I, int m, int i, int j)
 
 // Active-buoyancy work of layer k (work.c:188-213), computed on [1, s-1):
 // h times the layer mid-depth, and the buoyancy gradient along x / y.
+// One bottom-up pass gives both interfaces: eta_{k+1} = hhb + h_{L-1} + ...
+// + h_{k+1} and eta_k = eta_{k+1} + h_k, the sums pg_eta forms.
 PG_HD double pg_hz(const PgGrid& G, const PgIn& I, int k, int i, int j)
 {
   if (!pg_in_core(G, i, j)) return 0;
-  return PG_H(k, i, j) * 0.5 * (pg_eta(G, I, k, i, j) + pg_eta(G, I, k + 1, i, j));
+  const size_t p = pg_at(G, pg_smap_x(G, i), pg_smap_y(G, j));
+  double e1 = I.hhb[p];
+  for (int q = G.L - 1; q >= k + 1; q--) e1 = e1 + I.h[pg_lay(G, q) + p];
+  const double e0 = e1 + I.h[pg_lay(G, k) + p];
+  return PG_H(k, i, j) * 0.5 * (e0 + e1);
 }
 PG_HD double pg_dbx(const PgGrid& G, const PgIn& I, int k, int i, int j)
 {
