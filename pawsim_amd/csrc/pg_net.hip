@@ -92,6 +92,10 @@ __global__ __launch_bounds__(256) void k_conv(const bf16_t* __restrict__ in, int
 // wave per (group of NTW n tiles, co tile, chunk): A = dZ (co x pixels) is
 // loaded once per 16 pixels and reused for every n tile of the group.
 constexpr int NTW = 4;
+// Waves of one weight-gradient launch (chunks x tiles): eight per SIMD, so
+// that the gathers of some waves overlap the MFMAs of others.  Each wave
+// writes at most 16 x 64 partials.
+constexpr int WGRAD_WAVES = 8192;
 
 __global__ __launch_bounds__(64) void k_conv_wgrad(const bf16_t* __restrict__ X, int Cin, const bf16_t* __restrict__ dZ,
                                                    int Cz, int Cout, int B, int H, int W, int chunk, int NP, int MP,
@@ -323,7 +327,7 @@ static hipError_t launch_wgrad(pg_net* N, int l, const bf16_t* X, const bf16_t* 
   const int K9 = 9 * Cin, NP = ceil_to(K9 + 1, 16), MP = ceil_to(Cout, 16);
   const int ngroups = (NP / 16 + NTW - 1) / NTW;
   const int groups = ngroups * (MP / 16);
-  int nch = 1024 / groups;
+  int nch = WGRAD_WAVES / groups;
   if (nch < 1) nch = 1;
   const int chunk = ceil_to((npx + nch - 1) / nch, 16);
   nch = (npx + chunk - 1) / chunk;
@@ -487,7 +491,7 @@ int pg_net_alloc(pg_net* N, int H, int W, int B, hipStream_t stream)
   ok &= alloc((void**)&N->dy4, n0 * 4 * 2);
   ok &= alloc((void**)&N->y, n0 * 4);
   ok &= alloc((void**)&N->t, n0 * 4);
-  N->npart = (size_t)4096 * 256;
+  N->npart = (size_t)WGRAD_WAVES * 16 * 16 * NTW;
   ok &= alloc((void**)&N->part, N->npart * 4);
   ok &= alloc((void**)&N->part2, (size_t)(65536 + 4096) * 4);
   ok &= alloc((void**)&N->loss, 64);
