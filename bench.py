@@ -366,14 +366,20 @@ def main():
     # FP64 path (the reference's arithmetic, FP64 GPU multigrid): the error
     # reference for the mixed-precision run and the plain-MG solve time.
     _, m64 = make_model(a, mixed=0, solver=pawsim_amd.SOLVER_GPU_MG)
+    m64.set_stream(stream.cuda_stream)
+    mass64 = [m64.mass()]
     r64 = run_steps(a, m64, stream)
+    mass64.append(m64.mass())
     log(f"FP64 path: {a.steps / r64['elapsed']:.1f} steps/s")
     s64 = m64.state()
     m64.close()
 
     # Headline: mixed precision (FP32 stencils, FP32 V-cycles in FP64 refinement).
     setup, m = make_model(a, mixed=1)
+    m.set_stream(stream.cuda_stream)
+    mass32 = [m.mass()]
     r32 = run_steps(a, m, stream)
+    mass32.append(m.mass())
     log(f"mixed precision: {a.steps / r32['elapsed']:.1f} steps/s")
     n = r32["n"]
     s32 = m.state()
@@ -497,6 +503,10 @@ def main():
         "ensemble": ensemble,
         "scheme_variants": variants,
         "diagnostics_and_restart": diag,
+        # per-layer volume (pg_mass, FP64 device sums) after the timed steps
+        # against before them: max_k |M_end - M_0| / M_0
+        "mass_drift": {"fp64": float(np.max(np.abs(mass64[1] - mass64[0]) / mass64[0])),
+                       "mixed": float(np.max(np.abs(mass32[1] - mass32[0]) / mass32[0]))},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -169,9 +169,15 @@ int pg_solve_pressure(pg_handle *h, uint32_t solver, double *pi, const double *r
 /* Total kinetic energy, potential energy and per-layer potential enstrophy
  * (Z: Nlay values) of the current state in FP64: the values of one
  * EZdiags.dat line (pawsim_diagnostics_write_EZ_if_due,
- * diagnostics.c:483-703; AL81 thickness, no active buoyancy).  Tree-ordered
+ * diagnostics.c:483-703; HK83 kinetic-energy weights with that thickness
+ * scheme, total reduced gravity g - b with active buoyancy).  Tree-ordered
  * sums: equal to the reference's serial sums up to rounding. */
 int pg_energy(pg_handle *h, double *KE, double *PE, double *Z);
+/* Volume of each layer, M[k] = dx dy sum over owned cells of h_k (Nlay
+ * values), in FP64 with a fixed summation order.  No counterpart in the
+ * reference: the mass diagnostic that accompanies pg_energy, to check
+ * volume conservation of a run (mixed precision in particular). */
+int pg_mass(pg_handle *h, double *M);
 /* Online averages: pawsim_diagnostics_accumulate_averages and the
  * normalise/write/reset of pawsim_diagnostics_write_averages_if_due
  * (diagnostics.c:307-470).  pg_average_add adds weight times the current

@@ -150,6 +150,34 @@ __global__ __launch_bounds__(1024) void k_energy_finish(const double* part, int 
   }
 }
 
+// Layer volume partials: part[k * gridDim.x + block] = sum of h_k over the
+// block's owned cells (grid-stride over the Nx x Ny owned cells).
+__global__ __launch_bounds__(256) void k_mass(PgGrid G, const double* hh, double* part)
+{
+  const int k = blockIdx.y;
+  const size_t N = (size_t)G.nx * G.ny, lay = pg_lay(G, k);
+  double s = 0;
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < N; q += (size_t)gridDim.x * 256) {
+    const int i = (int)(q / G.ny), j = (int)(q % G.ny);
+    s += hh[lay + pg_at(G, i + G.g, j + G.g)];
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// out[k] = dxdy * sum of the layer's block partials, in index order.
+__global__ __launch_bounds__(64) void k_mass_finish(const double* part, int nb, double dxdy, double* out)
+{
+  const int k = blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += 64) s += part[(size_t)k * nb + b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (threadIdx.x == 0) out[k] = dxdy * s;
+}
+
 // ---------------------------------------------------------------------------
 // Online averages (pawsim_diagnostics_accumulate_averages,
 // diagnostics.c:307-372): per owned cell, with the reference's expressions
@@ -253,6 +281,29 @@ extern "C" int pg_average_take(pg_handle* h, double scale, double* out)
   PG_CHECK(h, hipGetLastError());
   PG_CHECK(h, hipMemcpyAsync(out, h->avg, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   PG_CHECK(h, hipMemsetAsync(h->avg, 0, n * sizeof(double), h->stream));
+  PG_CHECK(h, hipStreamSynchronize(h->stream));
+  return PG_OK;
+}
+
+extern "C" int pg_mass(pg_handle* h, double* M)
+{
+  if (!h || !M) return PG_EARG;
+  const PgGrid& G = h->G;
+  constexpr int NB = 256;  // blocks per layer
+  const size_t need = (size_t)NB * G.L + G.L;
+  if (h->ndiag < need) {
+    (void)hipFree(h->diag);
+    h->diag = nullptr;
+    h->ndiag = 0;
+    PG_CHECK(h, hipMalloc(&h->diag, need * sizeof(double)));
+    h->ndiag = need;
+  }
+  PG_CHECK(h, pg_ensure_halos(h));
+  double* out = h->diag + (size_t)NB * G.L;
+  hipLaunchKernelGGL(k_mass, dim3(NB, G.L), dim3(256), 0, h->stream, G, h->h, h->diag);
+  hipLaunchKernelGGL(k_mass_finish, dim3(G.L), dim3(64), 0, h->stream, h->diag, NB, G.dx * G.dy, out);
+  PG_CHECK(h, hipGetLastError());
+  PG_CHECK(h, hipMemcpyAsync(M, out, G.L * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   PG_CHECK(h, hipStreamSynchronize(h->stream));
   return PG_OK;
 }

@@ -166,6 +166,27 @@ def test_model_energy_matches_oracle(tmp_path, oracle_exe):
     assert ez_close(np.array(rows), ref)
 
 
+def test_model_mass_matches_host_sum():
+    """Model.mass() (device FP64 reduction) against the host sum of the
+    downloaded thickness, after steps of an FP64 and a mixed-precision run."""
+    for mixed in (0, 1):
+        setup = inputs.jets(nx=136, ny=72, nlay=3, dx=20e3, nt=4, solver=4 if mixed else 1, tol=1e-8,
+                            H_rest=(500.0, 1500.0, 2000.0), mixed_precision=mixed, name="mass_model")
+        m = pawsim_amd.Model.from_setup(setup)
+        m.pressure_correct()
+        m.refresh_halos()
+        for n in range(1, 4):
+            m.step(n)
+            m.pressure_correct()
+        got = m.mass()
+        h = m.state()[2]
+        dx, dy = setup.params["Lx"] / setup.nx, setup.params["Ly"] / setup.ny
+        ref = h.sum(axis=(1, 2)) * (dx * dy)
+        assert got.shape == (3,)
+        assert np.all(np.abs(got - ref) <= 1e-13 * np.abs(ref)), (mixed, got, ref)
+        m.close()
+
+
 MIXED_CASES = {
     # closed basin with topography, TW81 + A2 + quadratic drag
     "jets128x4": dict(kind="jets", nx=128, ny=128, nlay=4, dx=20e3, nt=60, A2=50.0, quadDragCoeff=2e-3),
