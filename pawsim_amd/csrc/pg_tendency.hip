@@ -41,7 +41,8 @@ __device__ __forceinline__ bool is_inner(const PgGrid& G, int i, int j)
 // instantiation, so that the plain kernel keeps its register budget and the
 // edge ring still fits beside it).
 template <bool EXT, bool XTRA>
-__global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn I, double* du, double* dv, double* dh)
+__global__ __launch_bounds__(NTH) void k_tendency_inner(PgGrid G, PgPhys P, PgIn I, double* __restrict__ du, double* __restrict__ dv,
+                                                 double* __restrict__ dh)
 {
   __shared__ double su[SH * SW], sv[SH * SW], sh[EXT ? SH3 * SW3 : SH * SW];
   __shared__ double sq[QH * QW], suu[UH * UW], svv[VH * VW], ske[KH * KW];
@@ -413,7 +414,8 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 // 63.6 KB of LDS two workgroups share a compute unit, four waves per SIMD.
 constexpr int TIF = 16, NTF = 512;
 constexpr int SHF = TIF + 4, QHF = TIF + 3, UHF = TIF + 2, VHF = TIF + 1, KHF = TIF + 1, DHF = TIF + 1;
-__global__ __launch_bounds__(NTF, 4) void k_tendency_inner32(PgGrid G, PgPhys P, PgIn I, float* du, float* dv, float* dh)
+__global__ __launch_bounds__(NTF, 4) void k_tendency_inner32(PgGrid G, PgPhys P, PgIn I, float* __restrict__ du, float* __restrict__ dv,
+                                                          float* __restrict__ dh)
 {
 #pragma clang fp contract(fast)
   __shared__ float su[SHF * SW], sv[SHF * SW], sh[SHF * SW];
